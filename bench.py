@@ -1,0 +1,157 @@
+"""Benchmark: 3-party oblivious radix sort of 2^24 int64 keys on one MI355X.
+
+One step = one `radix_sort` as the reference's micro_sorting times it
+(bench/micro/micro_sorting.cpp:31-38): permutation pre-generation (reserve of
+66 sharded permutations with gen_perm) plus the 64 bit iterations and the
+unpadding tail.  Shares are device-resident before the timed region (secret
+sharing and the final open are outside it).  Keys: Zipf(s=1.1) over [0, 2^20)
+from splitmix64 (seed 0x0A11CE03), secret-shared by party 0.
+
+Prints ONE JSON line.  Per-kernel device time comes from HIP events recorded
+on the library's own stream (orqg_timer_*), in a separate pass after the
+timed steps so that the event synchronisation does not perturb the timing.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def zipf_keys(n, s=1.1, domain=1 << 20, seed=0x0A11CE03):
+    from orq_amd import splitmix64
+
+    ranks = np.arange(1, domain + 1, dtype=np.float64)
+    cdf = np.cumsum(ranks ** (-s))
+    cdf /= cdf[-1]
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    return np.searchsorted(cdf, u, side="right").astype(np.uint64)  # rank - 1 in [0, domain)
+
+
+def cpu_baseline(n_sample):
+    """Oracle radix sort (single thread, 3 simulated parties) on a bounded sample."""
+    from oracle import oracle as O
+    from orq_amd import keys_from_seed
+
+    keys = zipf_keys(n_sample)
+    ctx = O.Ctx(keys_from_seed(0x0A11CE03))
+    v = ctx.secret_share(keys, 8)
+    t0 = time.perf_counter()
+    ctx.radix_sort(v)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n_sample / dt / 1e6,
+        "unit": "Mrows/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle radix_sort of {n_sample} int64 Zipf keys incl. permutation "
+                  f"generation, 1 thread, {dt:.2f} s",
+    }
+
+
+# Algorithmic HBM bytes per row for the sort's round kernel (k_round2, L=3 words
+# per share): per row it reads idx (4 B), gathers v (36 B) and perm (12 B),
+# writes v (36 B) and perm (12 B).
+ROUND_BYTES_PER_ROW = 4 + 36 + 12 + 36 + 12
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 14)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: all three parties share "
+                                   "one device; --gpus must be 1"}))
+        sys.exit(2)
+
+    import orq_amd
+
+    n = args.n
+    keys_plain = zipf_keys(n)
+    ctx = orq_amd.Ctx(orq_amd.keys_from_seed(0x0A11CE03))
+    base = ctx.secret_share(keys_plain, 8)
+    work = ctx.vec(n, 8)
+
+    def one_step():
+        orq_amd.lib().orqg_vec_copy(ctx.h, base.h, work.h)
+        t0 = time.perf_counter()
+        perm = ctx.radix_sort(work)
+        ctx.sync()
+        t = time.perf_counter() - t0
+        perm.free()
+        return t
+
+    for _ in range(args.warmup):
+        one_step()
+    times = [one_step() for _ in range(args.steps)]
+    t_step = sum(times) / len(times)
+
+    # correctness of the last step (outside the timed region)
+    got = ctx.open(work).view(np.int64)
+    ok = bool(np.array_equal(got, np.sort(keys_plain.view(np.int64), kind="stable")))
+
+    # per-kernel-family device time (events on the library stream)
+    ctx.timer_enable(True)
+    ctx.timer_reset()
+    orq_amd.lib().orqg_vec_copy(ctx.h, base.h, work.h)
+    t0 = time.perf_counter()
+    ctx.radix_sort(work).free()
+    ctx.sync()
+    t_timed = time.perf_counter() - t0
+    fams = {}
+    for f in ["gen_perm", "sort_b2a", "sort_perm", "sort_round", "sort_tail"]:
+        ms, cnt = ctx.timer(f)
+        fams[f] = {"ms": round(ms, 3), "launch_groups": cnt}
+    ctx.timer_enable(False)
+    online_ms = sum(fams[f]["ms"] for f in ["sort_b2a", "sort_perm", "sort_round", "sort_tail"])
+
+    # roofline of the round kernel family: 64 bits x 3 rounds, n rows each
+    r_ms, r_cnt = fams["sort_round"]["ms"], fams["sort_round"]["launch_groups"]
+    per_launch_ms = r_ms / max(r_cnt, 1)
+    round_bytes = 3 * n * ROUND_BYTES_PER_ROW + n * (12 + 36 + 36)  # 3 gathers + open/scatter
+    achieved = round_bytes / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
+
+    line = {
+        "metric": "Oblivious radix sort Mrows/s @2^24x64b 3PC",
+        "value": round(n / t_step / 1e6, 4),
+        "unit": "Mrows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic: Zipf(s=1.1) keys over [0,2^20), splitmix64 seed 0x0A11CE03",
+        "config": {"workload": f"3PC oblivious radix_sort, {n} rows x int64 key, all 3 parties "
+                               "on one GPU, incl. gen_perm of 66 sharded permutations",
+                   "rows": n, "key_bits": 64, "parties": 3, "batch_size": -12},
+        "correct": ok,
+        "online_ms": round(online_ms, 2),
+        "gen_perm_ms": fams["gen_perm"]["ms"],
+        "families_ms": fams,
+        "timed_pass_ms": round(t_timed * 1e3, 2),
+        "roofline": {"kernel": "sort_round (k_round2 x3 + k_open_scatter_pad) per bit",
+                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

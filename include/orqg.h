@@ -1,0 +1,160 @@
+/*
+ * orqg.h -- C ABI of the MI355X execution backend for ORQ's 3-party
+ * replicated (honest-majority) oblivious operators.
+ *
+ * All three parties live on one GPU.  A shared vector stores the three
+ * distinct replicated shares s_0, s_1, s_2 once, element-major
+ * ([n][3] elements); party p's view is (s_p, s_{p+1}).  Every message the
+ * reference would send between parties becomes a device-buffer handoff whose
+ * byte count is appended to the context's message log.
+ *
+ * ORQ itself is a header-only C++ template library with no FFI; the entry
+ * points below replace the per-party protocol primitives and operators cited
+ * beside each declaration (paths relative to the ORQ source tree).
+ *
+ * Conventions: every function returns ORQG_OK (0) or a negative ORQG_E_*
+ * code; orqg_last_error() gives the message.  Calls are stream-ordered on the
+ * context's HIP stream; functions that return host data synchronize.
+ * A context is single-owner (one host thread at a time), like a reference
+ * worker that owns its Protocol, PRGs and Communicator.
+ */
+#ifndef ORQG_H_
+#define ORQG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ORQG_OK = 0,
+  ORQG_E_ARG = -1,    /* bad argument, size or width */
+  ORQG_E_OOM = -2,    /* device allocation failed */
+  ORQG_E_BUDGET = -3, /* request exceeds the configured HBM budget */
+  ORQG_E_DEVICE = -4, /* HIP runtime error */
+  ORQG_E_PARITY = -5, /* a device-side consistency check failed */
+};
+
+enum { ORQG_BSHARED = 1, ORQG_ASHARED = 0 }; /* include/core/containers/encoding.h:8 */
+enum { ORQG_ASC = 0, ORQG_DESC = 1 };
+
+/* PRG roles of the 10 AES-256 keys (include/backend/common/rand_setup.h:21-146):
+ *   0..2  localPRG of party p (secret sharing)
+ *   3..5  pairwise key K_p: "next" PRG of party p and "prev" PRG of party p+1
+ *   6..9  group keys {0,1}, {0,2}, {1,2}, {0,1,2}                           */
+#define ORQG_NUM_KEYS 10
+
+typedef struct {
+  int device;                        /* HIP device ordinal */
+  uint8_t keys[ORQG_NUM_KEYS][32];   /* see roles above */
+  long batch_size;                   /* runtime batch size; <0 = |b| equal batches (ref default -12) */
+  size_t hbm_budget;                 /* bytes; 0 = no limit */
+} orqg_config;
+
+typedef struct orqg_ctx orqg_ctx;
+typedef struct orqg_vec orqg_vec;
+
+/* ---- context --------------------------------------------------------- */
+/* replaces replicated_3pc::orq_init / orq_runtime_init
+ * (include/backend/common/setup.h:7-87) with deterministic keys           */
+int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out);
+int orqg_ctx_destroy(orqg_ctx *ctx);
+const char *orqg_last_error(orqg_ctx *ctx);
+int orqg_sync(orqg_ctx *ctx);
+/* the context's HIP stream (hipStream_t), for callers that order their own work */
+void *orqg_stream(orqg_ctx *ctx);
+/* current nonce of PRG `role` (0..9, see above): AESPRGAlgorithm::nonce
+ * (include/core/random/prg/prg_algorithm.h:200-207)                         */
+int orqg_prg_nonce(orqg_ctx *ctx, int role, uint64_t *nonce);
+
+/* ---- raw device buffers (plumbing for FFI callers and tests) ---------- */
+int orqg_dev_alloc(orqg_ctx *ctx, size_t bytes, void **dev_out);
+int orqg_dev_free(void *dev);
+/* stream-ordered copy device -> host, then synchronize */
+int orqg_dev_to_host(orqg_ctx *ctx, void *host, const void *dev, size_t bytes);
+
+/* ---- shared vectors (EVector / A-/BSharedVector storage) -------------- */
+/* width: 4 (int32) or 8 (int64) bytes per share element
+ * (include/core/containers/e_vector.h:130-533)                              */
+int orqg_vec_create(orqg_ctx *ctx, size_t n, int width, int encoding, orqg_vec **out);
+int orqg_vec_free(orqg_vec *v);
+size_t orqg_vec_size(const orqg_vec *v);
+int orqg_vec_width(const orqg_vec *v);
+/* device pointer to the [n][3] share array */
+void *orqg_vec_data(orqg_vec *v);
+/* host I/O of the three distinct shares, host layout [3][n] (s_0, s_1, s_2) */
+int orqg_vec_write(orqg_ctx *ctx, orqg_vec *v, const void *host_shares);
+int orqg_vec_read(orqg_ctx *ctx, const orqg_vec *v, void *host_shares);
+int orqg_vec_copy(orqg_ctx *ctx, const orqg_vec *src, orqg_vec *dst);
+
+/* ---- randomness ------------------------------------------------------ */
+/* AESPRGAlgorithm::fillBytes of PRG `role` into device memory, advancing its
+ * nonce (prg_algorithm.h:161-231)                                            */
+int orqg_prg_keystream(orqg_ctx *ctx, int role, size_t bytes, void *dev_out);
+
+/* ---- protocol primitives (all three parties per call) ---------------- */
+/* Replicated_3PC::secret_share_b/_a from party 0 (replicated_3pc.h:401-448) */
+int orqg_secret_share(orqg_ctx *ctx, const void *host_plain, orqg_vec *v);
+/* open_shares_b/_a (replicated_3pc.h:310-330); host_out gets n elements    */
+int orqg_open(orqg_ctx *ctx, const orqg_vec *v, void *host_out);
+/* public_share (replicated_3pc.h:456-470)                                    */
+int orqg_public_share(orqg_ctx *ctx, const void *host_plain, orqg_vec *v);
+/* xor_b / add_a / sub_a (protocol.h:561-649)                                 */
+int orqg_xor_b(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+int orqg_add_a(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+int orqg_sub_a(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+/* not_b (replicated_3pc.h:199)                                               */
+int orqg_not_b(orqg_ctx *ctx, const orqg_vec *x, orqg_vec *z);
+/* and_b (replicated_3pc.h:166-191)                                           */
+int orqg_and_b(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+/* multiply_a (replicated_3pc.h:44-67)                                        */
+int orqg_mul_a(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+/* b2a_bit (replicated_3pc.h:220-251): y = A-sharing of the LSB of x          */
+int orqg_b2a_bit(orqg_ctx *ctx, const orqg_vec *x, orqg_vec *y);
+/* Protocol::reshare for group 0..2 = {0,1},{0,2},{1,2} (protocol.h:500-541) */
+int orqg_reshare(orqg_ctx *ctx, orqg_vec *v, int group);
+
+/* ---- permutations ---------------------------------------------------- */
+/* PermutationManager::reserve (permutation_manager.h:102-181): generate
+ * `count` honest-majority sharded permutations of size n with gen_perm
+ * (hm_sharded_permutation_generator.h:70-108) and queue them               */
+int orqg_reserve_perms(orqg_ctx *ctx, size_t n, size_t count);
+size_t orqg_perm_queue_size(orqg_ctx *ctx);
+/* local permutation of queued sharded permutation `index` for group 0..2
+ * (host copy, for tests)                                                    */
+int orqg_peek_perm(orqg_ctx *ctx, size_t index, int group, int32_t *host_out);
+/* oblivious_apply_sharded_perm / _inverse_ (shuffle.h:349-412,598-660) with
+ * the next queued permutation                                               */
+int orqg_apply_sharded_perm(orqg_ctx *ctx, orqg_vec *x, int inverse);
+/* oblivious_apply_elementwise_perm (shuffle.h:687-706); perm: width-4 vector */
+int orqg_apply_elementwise_perm(orqg_ctx *ctx, orqg_vec *x, const orqg_vec *perm);
+
+/* ---- operators ------------------------------------------------------- */
+/* radix_sort (include/core/operators/radixsort.h:205-229): sorts the
+ * B-shared vector v in place on its low `bits` bits (bits == 8*width sorts
+ * signed), returns the A-shared sorting permutation in perm_out (width 4). */
+int orqg_radix_sort(orqg_ctx *ctx, orqg_vec *v, int order, int bits, orqg_vec *perm_out);
+
+/* ---- message log (Communicator byte accounting, communicator.h:14-183) - */
+typedef struct {
+  char op[24];
+  int from, to;
+  uint64_t bytes;
+} orqg_msg;
+size_t orqg_msglog_count(orqg_ctx *ctx);
+int orqg_msglog_get(orqg_ctx *ctx, size_t i, orqg_msg *out);
+int orqg_msglog_clear(orqg_ctx *ctx);
+
+/* ---- timing (HIP events on the context stream) ------------------------ */
+/* per-kernel-family device time accumulated since the last reset, in ms;
+ * family: "aes_genperm", "sort_round", ...; returns 0 if unknown          */
+int orqg_timer_enable(orqg_ctx *ctx, int enable);
+int orqg_timer_get(orqg_ctx *ctx, const char *family, double *ms, uint64_t *launches);
+int orqg_timer_reset(orqg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ORQG_H_ */

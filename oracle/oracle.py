@@ -1,0 +1,213 @@
+"""ctypes binding of the CPU oracle (liborqoracle.so).
+
+Test infrastructure only: the tests, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of ``bench.py`` import this module; the GPU library never
+does.  The shared library is a restatement of ORQ's 3-party replicated
+protocol path (see orq_oracle.cpp for the per-function reference citations).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborqoracle.so")
+_lib = None
+
+_DT = {4: np.uint32, 8: np.uint64}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, sz, u64, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int
+        L.oc_create.restype = vp
+        L.oc_create.argtypes = [ctypes.c_char_p, ctypes.c_long, i32]
+        L.oc_destroy.argtypes = [vp]
+        L.oc_prg_nonce.restype = u64
+        L.oc_prg_nonce.argtypes = [vp, i32, i32]
+        L.oc_keystream.restype = u64
+        L.oc_keystream.argtypes = [ctypes.c_char_p, u64, vp, sz]
+        L.oc_gen_perm.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), vp, sz, ctypes.POINTER(u64)]
+        L.oc_vec_new.restype = vp
+        L.oc_vec_new.argtypes = [sz, i32]
+        L.oc_vec_free.argtypes = [vp]
+        L.oc_vec_set.argtypes = [vp, i32, i32, vp]
+        L.oc_vec_get.argtypes = [vp, i32, i32, vp]
+        for name, args in [
+            ("oc_secret_share", [vp, vp, vp, i32]),
+            ("oc_open", [vp, vp, i32, vp]),
+            ("oc_and_b", [vp, vp, vp, vp]),
+            ("oc_mul_a", [vp, vp, vp, vp]),
+            ("oc_b2a_bit", [vp, vp, vp]),
+            ("oc_reshare", [vp, vp, i32, i32]),
+            ("oc_reserve", [vp, sz, sz]),
+            ("oc_peek_perm", [vp, sz, i32, i32, vp]),
+            ("oc_apply_sharded", [vp, vp, i32, i32]),
+            ("oc_apply_elementwise", [vp, vp, i32, vp, i32]),
+            ("oc_radix_sort", [vp, vp, i32, i32, vp]),
+        ]:
+            f = getattr(L, name)
+            f.restype = i32
+            f.argtypes = args
+        L.oc_batches.restype = sz
+        L.oc_batches.argtypes = [sz, ctypes.c_long, vp, sz]
+        L.oc_queue_size.restype = sz
+        L.oc_queue_size.argtypes = [vp]
+        L.oc_msglog_count.restype = sz
+        L.oc_msglog_count.argtypes = [vp]
+        L.oc_msglog_get.argtypes = [vp, sz, ctypes.c_char_p, ctypes.POINTER(i32),
+                                    ctypes.POINTER(i32), ctypes.POINTER(u64)]
+        L.oc_msglog_clear.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def keystream(key: bytes, nonce: int, nbytes: int):
+    out = np.zeros(nbytes, dtype=np.uint8)
+    nxt = lib().oc_keystream(key, nonce, _ptr(out), nbytes)
+    return out, nxt
+
+
+def gen_perm(key: bytes, nonce: int, n: int):
+    out = np.zeros(n, dtype=np.int32)
+    nn = ctypes.c_uint64(nonce)
+    draws = ctypes.c_uint64(0)
+    lib().oc_gen_perm(key, ctypes.byref(nn), _ptr(out), n, ctypes.byref(draws))
+    return out, nn.value, draws.value
+
+
+def batches(n, batch_size=-12):
+    out = np.zeros(4096, dtype=np.uint64)
+    k = lib().oc_batches(n, batch_size, _ptr(out), out.size)
+    return [int(x) for x in out[:k]]
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed")
+
+
+class Vec:
+    """A replicated vector: party p holds (s_p, s_{p+1})."""
+
+    def __init__(self, n, width):
+        self.n, self.width = n, width
+        self.h = lib().oc_vec_new(n, width)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oc_vec_free(self.h)
+            self.h = None
+
+    def set(self, party, idx, arr):
+        a = np.ascontiguousarray(arr, dtype=_DT[self.width])
+        assert a.size == self.n
+        lib().oc_vec_set(self.h, party, idx, _ptr(a))
+
+    def get(self, party, idx):
+        a = np.zeros(self.n, dtype=_DT[self.width])
+        lib().oc_vec_get(self.h, party, idx, _ptr(a))
+        return a
+
+    def shares(self):
+        """The three distinct shares (s_0, s_1, s_2) as seen by their first holder."""
+        return [self.get(p, 0) for p in range(3)]
+
+    def set_shares(self, s):
+        for p in range(3):
+            self.set(p, 0, s[p])
+            self.set(p, 1, s[(p + 1) % 3])
+
+
+class Ctx:
+    def __init__(self, keys: bytes, batch_size=-12, gen_everyone=False):
+        assert len(keys) == 320
+        self.h = lib().oc_create(keys, batch_size, int(gen_everyone))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oc_destroy(self.h)
+            self.h = None
+
+    def nonce(self, party, role):
+        return lib().oc_prg_nonce(self.h, party, role)
+
+    def secret_share(self, data, width, arith=False):
+        a = np.ascontiguousarray(data, dtype=_DT[width])
+        v = Vec(a.size, width)
+        _check(lib().oc_secret_share(self.h, _ptr(a), v.h, int(arith)), "secret_share")
+        return v
+
+    def open(self, v, arith=False):
+        out = np.zeros(v.n, dtype=_DT[v.width])
+        _check(lib().oc_open(self.h, v.h, int(arith), _ptr(out)), "open")
+        return out
+
+    def and_b(self, x, y):
+        z = Vec(x.n, x.width)
+        _check(lib().oc_and_b(self.h, x.h, y.h, z.h), "and_b")
+        return z
+
+    def mul_a(self, x, y):
+        z = Vec(x.n, x.width)
+        _check(lib().oc_mul_a(self.h, x.h, y.h, z.h), "mul_a")
+        return z
+
+    def b2a_bit(self, x):
+        y = Vec(x.n, x.width)
+        _check(lib().oc_b2a_bit(self.h, x.h, y.h), "b2a_bit")
+        return y
+
+    def reshare(self, v, group, arith=False):
+        _check(lib().oc_reshare(self.h, v.h, group, int(arith)), "reshare")
+
+    def reserve(self, n, count):
+        _check(lib().oc_reserve(self.h, n, count), "reserve")
+
+    def queue_size(self):
+        return lib().oc_queue_size(self.h)
+
+    def peek_perm(self, index, party, group, n):
+        out = np.zeros(n, dtype=np.int32)
+        _check(lib().oc_peek_perm(self.h, index, party, group, _ptr(out)), "peek_perm")
+        return out
+
+    def apply_sharded(self, x, arith=False, inverse=False):
+        _check(lib().oc_apply_sharded(self.h, x.h, int(arith), int(inverse)), "apply_sharded")
+
+    def apply_elementwise(self, x, perm, x_arith=False, perm_arith=True):
+        _check(lib().oc_apply_elementwise(self.h, x.h, int(x_arith), perm.h, int(perm_arith)),
+               "apply_elementwise")
+
+    def radix_sort(self, v, bits=None, desc=False):
+        if bits is None:
+            bits = 8 * v.width
+        perm = Vec(v.n, 4)
+        _check(lib().oc_radix_sort(self.h, v.h, bits, int(desc), perm.h), "radix_sort")
+        return perm
+
+    def msglog(self):
+        L = lib()
+        out = []
+        op = ctypes.create_string_buffer(24)
+        f, t, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        for i in range(L.oc_msglog_count(self.h)):
+            L.oc_msglog_get(self.h, i, op, ctypes.byref(f), ctypes.byref(t), ctypes.byref(b))
+            out.append((op.value.decode(), f.value, t.value, b.value))
+        return out
+
+    def clear_msglog(self):
+        lib().oc_msglog_clear(self.h)

@@ -1,0 +1,900 @@
+// CPU restatement of ORQ's 3-party replicated (honest-majority) protocol path:
+// AES PRG keystream, zero sharings, boolean/arithmetic primitives, b2a_bit,
+// honest-majority sharded permutations, elementwise-permutation application
+// and oblivious radix sort.
+//
+// THIS IS TEST INFRASTRUCTURE (the oracle). Only the tests, the smoke check and
+// the CPU-baseline leg of the benchmark load it; the GPU library never does.
+//
+// Every party is simulated separately: party p holds the replicated pair
+// (s_p, s_{p+1}) of every shared vector and its own copies of every PRG object,
+// exactly as one reference process would.  Messages are copies between party
+// structs and are logged with their byte counts.  Runtime batching follows the
+// reference's single-worker schedule (threads = 1, batch_size < 0 => |b| equal
+// batches of 256-multiples), because every batch makes its own PRG calls and
+// therefore owns its own nonces.
+//
+// Parity pins (see tests/test_oracle.py):
+//   AES keystream ........ AES-GCM spec test case 14 + a second KAT; OpenSSL CTR
+//   zero sharings ........ XOR / sum over parties is 0
+//   and/mul/b2a_bit/open . plaintext closed forms on seeded inputs
+//   gen_perm ............. brute-force enumeration on tiny n; bijection
+//   sharded perms ........ apply then inverse = identity, reshare keeps values
+//   radix_sort ........... plaintext stable sort (signed order); perm applies
+#include <openssl/evp.h>
+
+#include <algorithm>
+#include <cassert>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+namespace {
+
+// ---------------------------------------------------------------- PRG ------
+// AESPRGAlgorithm (include/core/random/prg/prg_algorithm.h:139-231):
+// fillBytes splits a request into <=1 MiB chunks (:161-172); each chunk is one
+// AES-256-GCM encryption of zeros under IV = LE64(nonce) || 0^4, after which
+// the nonce increments (:214-231).  GCM's keystream starts at counter block
+// IV || BE32(2), so a chunk equals AES-256-CTR from that block.
+constexpr size_t kMaxAesQuery = 1u << 20;
+
+struct Prg {
+  uint8_t key[32];
+  uint64_t nonce = 0;
+  EVP_CIPHER_CTX *ctx = nullptr;
+
+  Prg() {}
+  explicit Prg(const uint8_t *k) { std::memcpy(key, k, 32); }
+  Prg(const Prg &o) : nonce(o.nonce), ctx(nullptr) { std::memcpy(key, o.key, 32); }
+  Prg &operator=(const Prg &o) {
+    std::memcpy(key, o.key, 32);
+    nonce = o.nonce;
+    return *this;
+  }
+  ~Prg() {
+    if (ctx) EVP_CIPHER_CTX_free(ctx);
+  }
+
+  // one aesGenerateValues call (prg_algorithm.h:214-231)
+  void chunk(uint8_t *dst, size_t bytes) {
+    static const uint8_t zeros[4096] = {0};
+    uint8_t iv[16] = {0};
+    for (int b = 0; b < 8; b++) iv[b] = (uint8_t)(nonce >> (8 * b));
+    iv[15] = 2;
+    if (!ctx) ctx = EVP_CIPHER_CTX_new();
+    if (EVP_EncryptInit_ex(ctx, EVP_aes_256_ctr(), nullptr, key, iv) != 1)
+      throw std::runtime_error("EVP init failed");
+    size_t done = 0;
+    while (done < bytes) {
+      int len = (int)std::min(bytes - done, sizeof(zeros));
+      int out = 0;
+      EVP_EncryptUpdate(ctx, dst + done, &out, zeros, len);
+      done += (size_t)len;
+    }
+    nonce++;
+  }
+
+  // fillBytes (prg_algorithm.h:161-172)
+  void fill(uint8_t *dst, size_t bytes) {
+    size_t idx = 0;
+    while (idx < bytes) {
+      size_t b = std::min(kMaxAesQuery, bytes - idx);
+      chunk(dst + idx, b);
+      idx += b;
+    }
+  }
+
+  // PRGAlgorithm::getNext(Vector<T>) -> getNextBuffered with a 1 MiB buffer
+  // (prg_algorithm.h:56-62, 87-104): one fillBytes per buffer-full of elements.
+  template <typename T>
+  void getNext(T *dst, size_t n) {
+    const size_t per = kMaxAesQuery / sizeof(T);
+    size_t idx = 0;
+    while (idx < n) {
+      size_t m = std::min(per, n - idx);
+      fill(reinterpret_cast<uint8_t *>(dst + idx), m * sizeof(T));
+      idx += m;
+    }
+  }
+};
+
+// ------------------------------------------------------- batch schedule ---
+// RunTime::getThreadBatchBoundaries (include/backend/common/runtime.h:198-247)
+// with one worker gives the whole range to thread 0; Task splits it into
+// batches of compute_equal_batches(...) elements when batch_size < 0
+// (include/backend/common/task.h:34-44, 89-94).
+struct Batch {
+  size_t s, e;
+};
+
+std::vector<Batch> batches_for(size_t n, long batch_size) {
+  std::vector<Batch> out;
+  if (n == 0) return out;
+  size_t bs;
+  if (batch_size < 0) {
+    size_t nb = (size_t)(-batch_size);
+    size_t chunks = (n + 255) / 256;
+    bs = (chunks + nb - 1) / nb * 256;
+  } else {
+    bs = (size_t)batch_size;
+  }
+  for (size_t i = 0; i < n; i += bs) out.push_back({i, std::min(i + bs, n)});
+  return out;
+}
+
+// --------------------------------------------------------- message log ---
+struct Msg {
+  char op[24];
+  int from, to;
+  uint64_t bytes;
+};
+
+// ------------------------------------------------------------- parties ---
+// Groups in the order generateGroups(3,2,1) yields them
+// (include/core/protocols/protocol.h:309-375): {0,1},{0,2},{1,2}; index 3 is
+// the everyone group {0,1,2}.
+const int kGroups[4][3] = {{0, 1, -1}, {0, 2, -1}, {1, 2, -1}, {0, 1, 2}};
+bool in_group(int g, int p) {
+  for (int k = 0; k < 3; k++)
+    if (kGroups[g][k] == p) return true;
+  return false;
+}
+
+struct Party {
+  int id;
+  Prg local;     // localPRG: secret sharing (replicated_3pc.h:338-392)
+  Prg prev;      // commonPRGManager->get(-1): key of party id-1
+  Prg next;      // commonPRGManager->get(+1): this party's own pairwise key
+  Prg group[4];  // group PRGs, valid only where in_group(g, id)
+};
+
+// Key roles (include/backend/common/rand_setup.h:90-146): party p generates
+// local_key_p, sends it to p+1 and uses it for relative rank +1 (next); the key
+// received from p-1 is used for relative rank -1 (prev).  Group keys come from
+// the lowest member (:21-78).  Here they are given explicitly:
+//   keys[0..2]  localPRG of party p
+//   keys[3..5]  pairwise key K_p (next of p, prev of p+1)
+//   keys[6..9]  group keys {0,1},{0,2},{1,2},{0,1,2}
+}  // namespace
+
+struct oc_ctx {
+  Party P[3];
+  long batch_size = -12;
+  int gen_everyone = 0;  // also run gen_perm for the unused {0,1,2} group
+  std::vector<Msg> log;
+  // PermutationManager queue (permutation_manager.h:102-181): each entry is
+  // one honest-majority sharded permutation, per party a map group->perm.
+  struct Sharded {
+    size_t n;
+    std::vector<int32_t> perm[3][4];  // [party][group]
+  };
+  std::deque<Sharded> queue;
+  size_t stored_size = 0;
+};
+
+struct oc_vec {
+  size_t n;
+  int width;  // bytes per element: 4, 8 or 16
+  std::vector<uint8_t> sh[3][2];  // party p: (s_p, s_{p+1})
+};
+
+namespace {
+
+// Byte accounting per (op, from, to) in first-seen order; the reference's
+// Communicator counts bytes per send (communicator.h:22-33).
+void log_msg(oc_ctx *c, const char *op, int from, int to, uint64_t bytes) {
+  if (bytes == 0) return;
+  for (auto &m : c->log)
+    if (m.from == from && m.to == to && std::strcmp(m.op, op) == 0) {
+      m.bytes += bytes;
+      return;
+    }
+  Msg m;
+  std::snprintf(m.op, sizeof(m.op), "%s", op);
+  m.from = from;
+  m.to = to;
+  m.bytes = bytes;
+  c->log.push_back(m);
+}
+
+template <typename T>
+T *S(oc_vec *v, int p, int i) {
+  return reinterpret_cast<T *>(v->sh[p][i].data());
+}
+
+oc_vec *new_vec(size_t n, int width) {
+  oc_vec *v = new oc_vec;
+  v->n = n;
+  v->width = width;
+  for (int p = 0; p < 3; p++)
+    for (int i = 0; i < 2; i++) v->sh[p][i].assign(n * width, 0);
+  return v;
+}
+
+oc_vec *clone_vec(const oc_vec *x) { return new oc_vec(*x); }
+
+// ZeroSharingGenerator::getNextBinary / getNextArithmetic on a Vector
+// (include/core/random/correlation/zero_sharing_generator.h:102-147).
+template <typename T>
+void zero_binary(Party &pt, T *out, size_t n) {
+  std::vector<T> a(n), b(n);
+  pt.prev.getNext(a.data(), n);
+  pt.next.getNext(b.data(), n);
+  for (size_t i = 0; i < n; i++) out[i] = a[i] ^ b[i];
+}
+template <typename T>
+void zero_arith(Party &pt, T *out, size_t n) {
+  std::vector<T> b(n);
+  pt.prev.getNext(out, n);
+  pt.next.getNext(b.data(), n);
+  for (size_t i = 0; i < n; i++) out[i] -= b[i];
+}
+
+// ---------------------------------------------------------- primitives ---
+// Replicated_3PC::and_b (include/core/protocols/replicated_3pc.h:166-191) and
+// multiply_a (:44-67), evaluated batch by batch as RunTime::eval_protocol_2arg
+// does (runtime.h:690-702).  z may alias x or y.
+template <typename T, bool kArith>
+void mult_batch(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z, size_t s, size_t e,
+                const char *op) {
+  size_t nb = e - s;
+  std::vector<T> local[3];
+  for (int p = 0; p < 3; p++) {
+    local[p].resize(nb);
+    if (kArith)
+      zero_arith<T>(c->P[p], local[p].data(), nb);
+    else
+      zero_binary<T>(c->P[p], local[p].data(), nb);
+    const T *x0 = S<T>(x, p, 0) + s, *x1 = S<T>(x, p, 1) + s;
+    const T *y0 = S<T>(y, p, 0) + s, *y1 = S<T>(y, p, 1) + s;
+    for (size_t i = 0; i < nb; i++) {
+      if (kArith)
+        local[p][i] += x0[i] * y0[i] + x0[i] * y1[i] + x1[i] * y0[i];
+      else
+        local[p][i] ^= (x0[i] & y0[i]) ^ (x0[i] & y1[i]) ^ (x1[i] & y0[i]);
+    }
+  }
+  // exchangeShares(local, remote, 2, 1): send to p-1, receive from p+1
+  for (int p = 0; p < 3; p++) {
+    std::memcpy(S<T>(z, p, 0) + s, local[p].data(), nb * sizeof(T));
+    std::memcpy(S<T>(z, p, 1) + s, local[(p + 1) % 3].data(), nb * sizeof(T));
+    log_msg(c, op, p, (p + 2) % 3, nb * sizeof(T));
+  }
+}
+
+template <typename T, bool kArith>
+void mult(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z, const char *op) {
+  for (auto b : batches_for(x->n, c->batch_size)) mult_batch<T, kArith>(c, x, y, z, b.s, b.e, op);
+}
+
+// Replicated_3PC::b2a_bit (replicated_3pc.h:220-251), per batch.
+template <typename T>
+void b2a_bit_batch(oc_ctx *c, oc_vec *x, oc_vec *y, size_t s, size_t e) {
+  size_t nb = e - s;
+  oc_vec *s0 = new_vec(nb, sizeof(T)), *s1 = new_vec(nb, sizeof(T));
+  std::vector<T> send0[3], send1[3];
+  for (int p = 0; p < 3; p++) {
+    T *a0 = S<T>(s0, p, 0), *a1 = S<T>(s0, p, 1), *b0 = S<T>(s1, p, 0);
+    const T *x0 = S<T>(x, p, 0) + s, *x1 = S<T>(x, p, 1) + s;
+    std::vector<T> lsb0(nb), lsb1(nb);
+    for (size_t i = 0; i < nb; i++) {
+      lsb0[i] = x0[i] & 1;
+      lsb1[i] = x1[i] & 1;
+      if (p == 0) lsb1[i] ^= lsb0[i];
+    }
+    zero_arith<T>(c->P[p], a0, nb);
+    zero_arith<T>(c->P[p], b0, nb);
+    for (size_t i = 0; i < nb; i++) {
+      if (p == 0) a0[i] += lsb1[i];
+      if (p == 1) b0[i] += lsb1[i];
+    }
+    (void)a1;
+    send0[p].assign(a0, a0 + nb);
+    send1[p].assign(b0, b0 + nb);
+  }
+  for (int p = 0; p < 3; p++) {
+    std::memcpy(S<T>(s0, p, 1), send0[(p + 1) % 3].data(), nb * sizeof(T));
+    std::memcpy(S<T>(s1, p, 1), send1[(p + 1) % 3].data(), nb * sizeof(T));
+    log_msg(c, "b2a_bit", p, (p + 2) % 3, 2 * nb * sizeof(T));
+  }
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *a = S<T>(s0, p, k), *b = S<T>(s1, p, k);
+      for (size_t i = 0; i < nb; i++) a[i] -= b[i];
+    }
+  // y = s0 * s0 (multiply_a on the same batch)
+  oc_vec *out = new_vec(nb, sizeof(T));
+  mult_batch<T, true>(c, s0, s0, out, 0, nb, "b2a_bit");
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++)
+      std::memcpy(S<T>(y, p, k) + s, S<T>(out, p, k), nb * sizeof(T));
+  delete s0;
+  delete s1;
+  delete out;
+}
+
+template <typename T>
+void b2a_bit(oc_ctx *c, oc_vec *x, oc_vec *y) {
+  for (auto b : batches_for(x->n, c->batch_size)) b2a_bit_batch<T>(c, x, y, b.s, b.e);
+}
+
+// open_shares_a / open_shares_b (replicated_3pc.h:310-330)
+template <typename T, bool kArith>
+std::vector<T> open_vec(oc_ctx *c, oc_vec *x) {
+  std::vector<T> out(x->n);
+  for (auto b : batches_for(x->n, c->batch_size)) {
+    for (int p = 0; p < 3; p++) log_msg(c, "open", p, (p + 2) % 3, (b.e - b.s) * sizeof(T));
+  }
+  // every party reconstructs the same value; compute party 0's view and
+  // check the others agree.
+  for (int p = 0; p < 3; p++) {
+    const T *a = S<T>(x, p, 0), *b = S<T>(x, p, 1), *r = S<T>(x, (p + 1) % 3, 1);
+    for (size_t i = 0; i < x->n; i++) {
+      T v = kArith ? (T)(a[i] + b[i] + r[i]) : (T)(a[i] ^ b[i] ^ r[i]);
+      if (p == 0)
+        out[i] = v;
+      else if (out[i] != v)
+        throw std::runtime_error("open: parties disagree (inconsistent replicated shares)");
+    }
+  }
+  return out;
+}
+
+// public_share (replicated_3pc.h:456-470): P0 (x,0), P1 (0,0), P2 (0,x).
+template <typename T>
+void public_share(oc_vec *v, const T *x) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) std::fill(v->sh[p][k].begin(), v->sh[p][k].end(), 0);
+  std::memcpy(S<T>(v, 0, 0), x, v->n * sizeof(T));
+  std::memcpy(S<T>(v, 2, 1), x, v->n * sizeof(T));
+}
+
+// secret_share_b / secret_share_a from party 0 (replicated_3pc.h:401-448),
+// batched by eval_protocol_1arg_alloc (runtime.h:584-603).
+template <typename T, bool kArith>
+void secret_share(oc_ctx *c, const T *data, oc_vec *v) {
+  for (auto b : batches_for(v->n, c->batch_size)) {
+    size_t nb = b.e - b.s;
+    std::vector<T> s1(nb), s2(nb), s3(nb);
+    c->P[0].local.getNext(s1.data(), nb);
+    c->P[0].local.getNext(s2.data(), nb);
+    for (size_t i = 0; i < nb; i++)
+      s3[i] = kArith ? (T)(data[b.s + i] - s1[i] - s2[i]) : (T)(data[b.s + i] ^ s1[i] ^ s2[i]);
+    const std::vector<T> *parts[3][2] = {{&s1, &s2}, {&s2, &s3}, {&s3, &s1}};
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++)
+        std::memcpy(S<T>(v, p, k) + b.s, parts[p][k]->data(), nb * sizeof(T));
+    log_msg(c, "secret_share", 0, 1, 2 * nb * sizeof(T));
+    log_msg(c, "secret_share", 0, 2, 2 * nb * sizeof(T));
+  }
+}
+
+// Protocol::reshare (include/core/protocols/protocol.h:500-541) for 3PC,
+// per batch.  Members draw rand[0], rand[1] from the group PRG, rand[2] is
+// dependent (zero_sharing_generator.h:159-183); party p masks its shares
+// (p, p+1) with rand[p], rand[p+1].  The outsider c then receives share c
+// from c-1 and share c+1 from c+1 (generate_send/recv_assignment :108-209).
+template <typename T, bool kArith>
+void reshare(oc_ctx *c, oc_vec *v, int g) {
+  int outsider = -1;
+  for (int p = 0; p < 3; p++)
+    if (!in_group(g, p)) outsider = p;
+  for (auto b : batches_for(v->n, c->batch_size)) {
+    size_t nb = b.e - b.s;
+    for (int p = 0; p < 3; p++) {
+      if (p == outsider) continue;
+      std::vector<T> r[3];
+      for (int k = 0; k < 3; k++) r[k].assign(nb, 0);
+      for (int k = 0; k < 2; k++) {
+        c->P[p].group[g].getNext(r[k].data(), nb);
+        for (size_t i = 0; i < nb; i++) {
+          if (kArith)
+            r[2][i] -= r[k][i];
+          else
+            r[2][i] ^= r[k][i];
+        }
+      }
+      for (int k = 0; k < 2; k++) {
+        T *d = S<T>(v, p, k) + b.s;
+        const T *m = r[(p + k) % 3].data();
+        for (size_t i = 0; i < nb; i++) d[i] = kArith ? (T)(d[i] + m[i]) : (T)(d[i] ^ m[i]);
+      }
+    }
+    int o = outsider, pm = (o + 2) % 3, pp = (o + 1) % 3;
+    std::memcpy(S<T>(v, o, 0) + b.s, S<T>(v, pm, 1) + b.s, nb * sizeof(T));
+    std::memcpy(S<T>(v, o, 1) + b.s, S<T>(v, pp, 0) + b.s, nb * sizeof(T));
+    log_msg(c, "reshare", pm, o, nb * sizeof(T));
+    log_msg(c, "reshare", pp, o, nb * sizeof(T));
+  }
+}
+
+// ------------------------------------------------------- permutations ---
+// gen_perm (include/core/random/permutations/hm_sharded_permutation_generator.h:70-108)
+// Fisher-Yates driven by a 2^20-entry uint32 buffer refilled eagerly from the
+// group PRG, rejection-sampled against the fixed mask 2^bit_width(n)-1.
+void gen_perm(std::vector<int32_t> &perm, Prg &g, uint64_t *draws_out) {
+  size_t size = perm.size();
+  std::iota(perm.begin(), perm.end(), 0);
+  const size_t buf = kMaxAesQuery;  // elements (uint32), i.e. 4 MiB
+  std::vector<uint32_t> random(buf);
+  g.getNext(random.data(), buf);
+  int bits = 0;
+  for (size_t t = size; t; t >>= 1) bits++;
+  uint32_t mask = (uint32_t)((1ull << bits) - 1);
+  size_t rand_index = 0;
+  uint64_t draws = 0;
+  for (size_t i = 0; i < size; i++) {
+    uint32_t rnd = (uint32_t)size;
+    while ((size_t)rnd > size - i - 1) {
+      rnd = random[rand_index] & mask;
+      rand_index++;
+      draws++;
+      if (rand_index == buf) {
+        g.getNext(random.data(), buf);
+        rand_index = 0;
+      }
+    }
+    std::swap(perm[i], perm[i + rnd]);
+  }
+  if (draws_out) *draws_out = draws;
+}
+
+// PermutationManager::reserve -> RunTime::generate_permutations ->
+// HMShardedPermutationGenerator::generateBatch (hm_sharded_permutation_generator.h:178-194):
+// for each permutation, for each group containing the party (in group order),
+// gen_perm from that group's PRG.
+void reserve(oc_ctx *c, size_t n, size_t count) {
+  if (!c->queue.empty() && c->stored_size != n) c->queue.clear();
+  c->stored_size = n;
+  if (count <= c->queue.size()) return;
+  count -= c->queue.size();
+  for (size_t k = 0; k < count; k++) {
+    oc_ctx::Sharded sp;
+    sp.n = n;
+    for (int p = 0; p < 3; p++)
+      for (int g = 0; g < 4; g++) {
+        if (!in_group(g, p)) continue;
+        if (g == 3 && !c->gen_everyone) continue;
+        sp.perm[p][g].resize(n);
+        gen_perm(sp.perm[p][g], c->P[p].group[g], nullptr);
+      }
+    c->queue.push_back(std::move(sp));
+  }
+}
+
+oc_ctx::Sharded next_perm(oc_ctx *c, size_t n) {
+  if (c->queue.empty() || c->stored_size != n) reserve(c, n, 1);
+  oc_ctx::Sharded s = std::move(c->queue.front());
+  c->queue.pop_front();
+  return s;
+}
+
+// local_apply_perm (include/core/operators/shuffle.h:113-133): new[pi[i]] = x[i]
+template <typename T>
+void scatter(T *x, const int32_t *pi, size_t n) {
+  std::vector<T> t(n);
+  for (size_t i = 0; i < n; i++) t[pi[i]] = x[i];
+  std::memcpy(x, t.data(), n * sizeof(T));
+}
+// local_apply_inverse_perm (shuffle.h:250-270): new[i] = x[pi[i]]
+template <typename T>
+void gather(T *x, const int32_t *pi, size_t n) {
+  std::vector<T> t(n);
+  for (size_t i = 0; i < n; i++) t[i] = x[pi[i]];
+  std::memcpy(x, t.data(), n * sizeof(T));
+}
+
+// hm_oblivious_apply_sharded_perm / _inverse_ (shuffle.h:349-412)
+template <typename T, bool kArith>
+void apply_sharded(oc_ctx *c, oc_vec *x, const oc_ctx::Sharded &sp, bool inverse) {
+  for (int gi = 0; gi < 3; gi++) {
+    int g = inverse ? 2 - gi : gi;
+    for (int p = 0; p < 3; p++) {
+      if (!in_group(g, p)) continue;
+      for (int k = 0; k < 2; k++) {
+        if (inverse)
+          gather<T>(S<T>(x, p, k), sp.perm[p][g].data(), x->n);
+        else
+          scatter<T>(S<T>(x, p, k), sp.perm[p][g].data(), x->n);
+      }
+    }
+    reshare<T, kArith>(c, x, g);
+  }
+}
+
+// oblivious_apply_elementwise_perm (shuffle.h:687-706) for an int32 A- or
+// B-shared permutation `perm` applied to x.
+template <typename T, bool kArith>
+void apply_elementwise(oc_ctx *c, oc_vec *x, oc_vec *perm, bool perm_arith) {
+  oc_vec *cp = clone_vec(perm);
+  oc_ctx::Sharded pi = next_perm(c, x->n);  // getNextPair: (pi, clone(pi))
+  apply_sharded<T, kArith>(c, x, pi, false);
+  std::vector<int32_t> opened;
+  if (perm_arith) {
+    apply_sharded<uint32_t, true>(c, cp, pi, false);
+    auto o = open_vec<uint32_t, true>(c, cp);
+    opened.assign(o.begin(), o.end());
+  } else {
+    apply_sharded<uint32_t, false>(c, cp, pi, false);
+    auto o = open_vec<uint32_t, false>(c, cp);
+    opened.assign(o.begin(), o.end());
+  }
+  for (size_t i = 0; i < x->n; i++)
+    if (opened[i] < 0 || (size_t)opened[i] >= x->n)
+      throw std::runtime_error("opened permutation out of range");
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) scatter<T>(S<T>(x, p, k), opened.data(), x->n);
+  delete cp;
+}
+
+// ElementwisePermutation::b2a (include/core/containers/permutation.h:222-258)
+void perm_b2a(oc_ctx *c, oc_vec *perm) {
+  oc_vec *cp = clone_vec(perm);
+  oc_ctx::Sharded pi = next_perm(c, perm->n);
+  apply_sharded<uint32_t, false>(c, cp, pi, false);
+  auto o = open_vec<uint32_t, false>(c, cp);
+  public_share<uint32_t>(perm, o.data());
+  apply_sharded<uint32_t, true>(c, perm, pi, true);
+  delete cp;
+}
+
+// ElementwisePermutation::invert (permutation.h:205-212)
+void perm_invert(oc_ctx *c, oc_vec *perm) {
+  oc_vec *inv = new_vec(perm->n, 4);
+  std::vector<uint32_t> id(perm->n);
+  std::iota(id.begin(), id.end(), 0u);
+  public_share<uint32_t>(inv, id.data());
+  apply_elementwise<uint32_t, true>(c, inv, perm, true);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) perm->sh[p][k] = inv->sh[p][k];
+  delete inv;
+}
+
+// multiply by the public constant -1, then add -1 (ElementwisePermutation::negate,
+// permutation.h:268-280; the result aliases the permutation's storage).
+void perm_negate(oc_ctx *c, oc_vec *perm) {
+  oc_vec *m1 = new_vec(perm->n, 4);
+  std::vector<uint32_t> neg(perm->n, 0xffffffffu);
+  public_share<uint32_t>(m1, neg.data());
+  mult<uint32_t, true>(c, perm, m1, perm, "mul_a");
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      uint32_t *a = S<uint32_t>(perm, p, k), *b = S<uint32_t>(m1, p, k);
+      for (size_t i = 0; i < perm->n; i++) a[i] += b[i];
+    }
+  delete m1;
+}
+
+void reverse_vec(oc_vec *v) {
+  size_t w = v->width;
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      auto &d = v->sh[p][k];
+      for (size_t i = 0; i < v->n / 2; i++)
+        std::swap_ranges(d.begin() + i * w, d.begin() + (i + 1) * w,
+                         d.begin() + (v->n - 1 - i) * w);
+    }
+}
+
+// ---------------------------------------------------------- radix sort ---
+// radix_sort (include/core/operators/radixsort.h:205-229) with pad_input /
+// remove_padding (include/core/operators/sorting.h:431-495).  K is the key
+// share type (uint32_t or uint64_t), PT the padded type (PadWidth, :409-411).
+template <typename K, typename PT, typename SPT>
+void radix_sort(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
+  const size_t n = v->n;
+  const bool full_width = (int)(sizeof(K) * 8) == bits;
+  // pad_input
+  oc_vec *pad = new_vec(n, sizeof(PT));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const K *src = S<K>(v, p, k);
+      PT *dst = S<PT>(pad, p, k);
+      for (size_t i = 0; i < n; i++) {
+        typedef typename std::make_signed<K>::type SK;
+        dst[i] = (PT)(SPT)(SK)src[i];  // sign-extending copy-cast
+        dst[i] <<= 32;
+      }
+    }
+  for (size_t i = 0; i < n; i++) {
+    uint32_t idx = desc ? (uint32_t)(-1 - (int64_t)i) : (uint32_t)i;
+    S<PT>(pad, 0, 0)[i] |= idx;
+    S<PT>(pad, 2, 1)[i] |= idx;
+  }
+  if (desc) reverse_vec(pad);
+
+  // radix_sort_body (radixsort.h:116-193)
+  reserve(c, n, (size_t)bits + 2);  // 1 permutation + (bits+1) pairs, pairs are singles in 3PC
+  oc_vec *vprime = new_vec(n, 4), *f1 = new_vec(n, 4), *s = new_vec(2 * n, 4);
+  oc_vec *s1 = new_vec(n, 4), *perm = new_vec(n, 4);
+  for (int i = 0; i < bits; i++) {
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++) {
+        const PT *src = S<PT>(pad, p, k);
+        uint32_t *dst = S<uint32_t>(vprime, p, k);
+        for (size_t j = 0; j < n; j++) {
+          PT sh = (PT)((SPT)src[j] >> (32 + i));  // arithmetic shift
+          if (full_width && i == bits - 1) sh = ~sh;  // inplace_invert (not_b)
+          dst[j] = (uint32_t)sh;                     // copy-cast to int32
+        }
+      }
+    b2a_bit<uint32_t>(c, vprime, f1);
+    // s0 = one - f1 ; s1 = f1 ; s[0] -= 1 ; prefix_sum over 2n
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++) {
+        uint32_t *sp = S<uint32_t>(s, p, k);
+        const uint32_t *f = S<uint32_t>(f1, p, k);
+        uint32_t one = ((p == 0 && k == 0) || (p == 2 && k == 1)) ? 1u : 0u;
+        for (size_t j = 0; j < n; j++) {
+          sp[j] = one - f[j];
+          sp[n + j] = f[j];
+        }
+        if (n > 0) sp[0] -= one;
+        uint32_t acc = 0;
+        for (size_t j = 0; j < 2 * n; j++) {
+          acc += sp[j];
+          sp[j] = acc;
+        }
+      }
+    // s1 -= s0 ; f1 *= s1 ; s0 += f1
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++) {
+        uint32_t *sp = S<uint32_t>(s, p, k), *d = S<uint32_t>(s1, p, k);
+        for (size_t j = 0; j < n; j++) d[j] = sp[n + j] - sp[j];
+      }
+    mult<uint32_t, true>(c, f1, s1, f1, "mul_a");
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++) {
+        uint32_t *sp = S<uint32_t>(s, p, k), *d = S<uint32_t>(perm, p, k);
+        const uint32_t *f = S<uint32_t>(f1, p, k);
+        for (size_t j = 0; j < n; j++) d[j] = sp[j] + f[j];
+      }
+    apply_elementwise<PT, false>(c, pad, perm, true);
+  }
+  delete vprime;
+  delete f1;
+  delete s;
+  delete s1;
+  delete perm;
+
+  if (desc) reverse_vec(pad);
+  // remove_padding: low 32 bits -> B-shared perm; (padded >> 32) -> v
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const PT *src = S<PT>(pad, p, k);
+      uint32_t *pd = S<uint32_t>(perm_out, p, k);
+      K *vd = S<K>(v, p, k);
+      for (size_t j = 0; j < n; j++) {
+        pd[j] = (uint32_t)src[j];
+        vd[j] = (K)(PT)((SPT)src[j] >> 32);
+      }
+    }
+  perm_b2a(c, perm_out);
+  if (desc) perm_negate(c, perm_out);
+  perm_invert(c, perm_out);
+  delete pad;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+oc_ctx *oc_create(const uint8_t *keys /* 10 x 32 bytes */, long batch_size, int gen_everyone) {
+  oc_ctx *c = new oc_ctx;
+  c->batch_size = batch_size;
+  c->gen_everyone = gen_everyone;
+  for (int p = 0; p < 3; p++) {
+    Party &pt = c->P[p];
+    pt.id = p;
+    pt.local = Prg(keys + 32 * p);
+    pt.next = Prg(keys + 32 * (3 + p));
+    pt.prev = Prg(keys + 32 * (3 + (p + 2) % 3));
+    for (int g = 0; g < 4; g++) pt.group[g] = Prg(keys + 32 * (6 + g));
+  }
+  return c;
+}
+
+void oc_destroy(oc_ctx *c) { delete c; }
+
+// nonce of one PRG object: role 0 local, 1 prev, 2 next, 3+g group g
+uint64_t oc_prg_nonce(oc_ctx *c, int party, int role) {
+  Party &pt = c->P[party];
+  if (role == 0) return pt.local.nonce;
+  if (role == 1) return pt.prev.nonce;
+  if (role == 2) return pt.next.nonce;
+  return pt.group[role - 3].nonce;
+}
+
+// AESPRGAlgorithm keystream for `bytes` bytes starting at `nonce`;
+// returns the nonce after the call.
+uint64_t oc_keystream(const uint8_t *key, uint64_t nonce, uint8_t *out, size_t bytes) {
+  Prg g(key);
+  g.nonce = nonce;
+  g.fill(out, bytes);
+  return g.nonce;
+}
+
+// gen_perm from a PRG at `*nonce` (advanced in place); `draws` gets the number
+// of uint32 words consumed before the last acceptance, inclusive.
+void oc_gen_perm(const uint8_t *key, uint64_t *nonce, int32_t *out, size_t n, uint64_t *draws) {
+  Prg g(key);
+  g.nonce = *nonce;
+  std::vector<int32_t> perm(n);
+  gen_perm(perm, g, draws);
+  std::memcpy(out, perm.data(), n * sizeof(int32_t));
+  *nonce = g.nonce;
+}
+
+oc_vec *oc_vec_new(size_t n, int width) { return new_vec(n, width); }
+void oc_vec_free(oc_vec *v) { delete v; }
+size_t oc_vec_size(oc_vec *v) { return v->n; }
+void oc_vec_set(oc_vec *v, int party, int idx, const void *src) {
+  std::memcpy(v->sh[party][idx].data(), src, v->n * v->width);
+}
+void oc_vec_get(oc_vec *v, int party, int idx, void *dst) {
+  std::memcpy(dst, v->sh[party][idx].data(), v->n * v->width);
+}
+
+#define OC_TRY(...)                                  \
+  try {                                              \
+    __VA_ARGS__;                                     \
+    return 0;                                        \
+  } catch (const std::exception &e) {                \
+    std::fprintf(stderr, "oracle: %s\n", e.what()); \
+    return -1;                                       \
+  }
+
+int oc_secret_share(oc_ctx *c, const void *data, oc_vec *v, int arith) {
+  OC_TRY({
+    if (v->width == 4)
+      arith ? secret_share<uint32_t, true>(c, (const uint32_t *)data, v)
+            : secret_share<uint32_t, false>(c, (const uint32_t *)data, v);
+    else if (v->width == 8)
+      arith ? secret_share<uint64_t, true>(c, (const uint64_t *)data, v)
+            : secret_share<uint64_t, false>(c, (const uint64_t *)data, v);
+    else
+      throw std::runtime_error("secret_share: width");
+  })
+}
+
+int oc_open(oc_ctx *c, oc_vec *v, int arith, void *out) {
+  OC_TRY({
+    if (v->width == 4) {
+      auto o = arith ? open_vec<uint32_t, true>(c, v) : open_vec<uint32_t, false>(c, v);
+      std::memcpy(out, o.data(), o.size() * 4);
+    } else if (v->width == 8) {
+      auto o = arith ? open_vec<uint64_t, true>(c, v) : open_vec<uint64_t, false>(c, v);
+      std::memcpy(out, o.data(), o.size() * 8);
+    } else
+      throw std::runtime_error("open: width");
+  })
+}
+
+int oc_and_b(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z) {
+  OC_TRY({
+    if (x->width == 4)
+      mult<uint32_t, false>(c, x, y, z, "and_b");
+    else if (x->width == 8)
+      mult<uint64_t, false>(c, x, y, z, "and_b");
+    else
+      throw std::runtime_error("and_b: width");
+  })
+}
+
+int oc_mul_a(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z) {
+  OC_TRY({
+    if (x->width == 4)
+      mult<uint32_t, true>(c, x, y, z, "mul_a");
+    else if (x->width == 8)
+      mult<uint64_t, true>(c, x, y, z, "mul_a");
+    else
+      throw std::runtime_error("mul_a: width");
+  })
+}
+
+int oc_b2a_bit(oc_ctx *c, oc_vec *x, oc_vec *y) {
+  OC_TRY({
+    if (x->width == 4)
+      b2a_bit<uint32_t>(c, x, y);
+    else if (x->width == 8)
+      b2a_bit<uint64_t>(c, x, y);
+    else
+      throw std::runtime_error("b2a_bit: width");
+  })
+}
+
+int oc_reshare(oc_ctx *c, oc_vec *v, int group, int arith) {
+  OC_TRY({
+    if (v->width == 4)
+      arith ? reshare<uint32_t, true>(c, v, group) : reshare<uint32_t, false>(c, v, group);
+    else if (v->width == 8)
+      arith ? reshare<uint64_t, true>(c, v, group) : reshare<uint64_t, false>(c, v, group);
+    else
+      throw std::runtime_error("reshare: width");
+  })
+}
+
+int oc_reserve(oc_ctx *c, size_t n, size_t count) { OC_TRY(reserve(c, n, count)) }
+size_t oc_queue_size(oc_ctx *c) { return c->queue.size(); }
+
+// copy one queued sharded permutation's local permutation for a group as
+// party `party` sees it (index 0 = front of the queue)
+int oc_peek_perm(oc_ctx *c, size_t index, int party, int group, int32_t *out) {
+  OC_TRY({
+    if (index >= c->queue.size()) throw std::runtime_error("peek: index");
+    const auto &v = c->queue[index].perm[party][group];
+    if (v.empty()) throw std::runtime_error("peek: party not in group");
+    std::memcpy(out, v.data(), v.size() * 4);
+  })
+}
+
+int oc_apply_sharded(oc_ctx *c, oc_vec *x, int arith, int inverse) {
+  OC_TRY({
+    oc_ctx::Sharded pi = next_perm(c, x->n);
+    if (x->width == 4)
+      arith ? apply_sharded<uint32_t, true>(c, x, pi, inverse)
+            : apply_sharded<uint32_t, false>(c, x, pi, inverse);
+    else if (x->width == 8)
+      arith ? apply_sharded<uint64_t, true>(c, x, pi, inverse)
+            : apply_sharded<uint64_t, false>(c, x, pi, inverse);
+    else
+      throw std::runtime_error("apply_sharded: width");
+  })
+}
+
+int oc_apply_elementwise(oc_ctx *c, oc_vec *x, int x_arith, oc_vec *perm, int perm_arith) {
+  OC_TRY({
+    if (x->width == 4)
+      x_arith ? apply_elementwise<uint32_t, true>(c, x, perm, perm_arith)
+              : apply_elementwise<uint32_t, false>(c, x, perm, perm_arith);
+    else if (x->width == 8)
+      x_arith ? apply_elementwise<uint64_t, true>(c, x, perm, perm_arith)
+              : apply_elementwise<uint64_t, false>(c, x, perm, perm_arith);
+    else
+      throw std::runtime_error("apply_elementwise: width");
+  })
+}
+
+// radix_sort of a B-shared int32/int64 vector (in place); perm_out (width 4)
+// receives the A-shared sorting permutation.
+int oc_radix_sort(oc_ctx *c, oc_vec *v, int bits, int desc, oc_vec *perm_out) {
+  OC_TRY({
+    if (perm_out->n != v->n || perm_out->width != 4) throw std::runtime_error("radix: perm_out");
+    if (v->width == 4)
+      radix_sort<uint32_t, uint64_t, int64_t>(c, v, bits, desc, perm_out);
+    else if (v->width == 8)
+      radix_sort<uint64_t, u128, i128>(c, v, bits, desc, perm_out);
+    else
+      throw std::runtime_error("radix_sort: width");
+  })
+}
+
+// batch sizes of one runtime call over n elements (returns the batch count)
+size_t oc_batches(size_t n, long batch_size, uint64_t *sizes, size_t cap) {
+  auto b = batches_for(n, batch_size);
+  for (size_t i = 0; i < b.size() && i < cap; i++) sizes[i] = b[i].e - b[i].s;
+  return b.size();
+}
+
+size_t oc_msglog_count(oc_ctx *c) { return c->log.size(); }
+void oc_msglog_get(oc_ctx *c, size_t i, char *op, int *from, int *to, uint64_t *bytes) {
+  const Msg &m = c->log[i];
+  std::memcpy(op, m.op, sizeof(m.op));
+  *from = m.from;
+  *to = m.to;
+  *bytes = m.bytes;
+}
+void oc_msglog_clear(oc_ctx *c) { c->log.clear(); }
+
+}  // extern "C"

@@ -1,0 +1,322 @@
+"""orq_amd -- MI355X execution backend for ORQ's 3-party replicated oblivious operators.
+
+The compute lives in ``liborqg.so`` (hand-written HIP kernels for gfx950 behind
+the C ABI declared in ``include/orqg.h``).  This module is a thin ctypes
+binding used by the tests and the benchmark; it never computes anything on the
+CPU.  If the library is missing it raises instead of falling back.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liborqg.so")
+_lib = None
+
+ORQG_BSHARED, ORQG_ASHARED = 1, 0
+ASC, DESC = 0, 1
+_DT = {4: np.uint32, 8: np.uint64}
+
+ROLE_LOCAL = (0, 1, 2)
+ROLE_PAIR = (3, 4, 5)
+ROLE_GROUP = (6, 7, 8, 9)
+
+
+class OrqgError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int),
+        ("keys", (ctypes.c_uint8 * 32) * 10),
+        ("batch_size", ctypes.c_long),
+        ("hbm_budget", ctypes.c_size_t),
+    ]
+
+
+class _Msg(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_char * 24), ("from_", ctypes.c_int), ("to", ctypes.c_int),
+                ("bytes", ctypes.c_uint64)]
+
+
+EXPORTS = [
+    "orqg_ctx_create", "orqg_ctx_destroy", "orqg_last_error", "orqg_sync", "orqg_stream",
+    "orqg_prg_nonce", "orqg_dev_alloc", "orqg_dev_free", "orqg_dev_to_host", "orqg_vec_create", "orqg_vec_free", "orqg_vec_size", "orqg_vec_width",
+    "orqg_vec_data", "orqg_vec_write", "orqg_vec_read", "orqg_vec_copy", "orqg_prg_keystream",
+    "orqg_secret_share", "orqg_open", "orqg_public_share", "orqg_xor_b", "orqg_add_a",
+    "orqg_sub_a", "orqg_not_b", "orqg_and_b", "orqg_mul_a", "orqg_b2a_bit", "orqg_reshare",
+    "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
+    "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
+    "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OrqgError(f"{LIB_PATH} is missing: build it with `python orq_amd/build.py` "
+                            "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, i32, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
+        sig = {
+            "orqg_ctx_create": (i32, [ctypes.POINTER(_Config), ctypes.POINTER(vp)]),
+            "orqg_ctx_destroy": (i32, [vp]),
+            "orqg_last_error": (ctypes.c_char_p, [vp]),
+            "orqg_sync": (i32, [vp]),
+            "orqg_stream": (vp, [vp]),
+            "orqg_prg_nonce": (i32, [vp, i32, ctypes.POINTER(u64)]),
+            "orqg_dev_alloc": (i32, [vp, sz, ctypes.POINTER(vp)]),
+            "orqg_dev_free": (i32, [vp]),
+            "orqg_dev_to_host": (i32, [vp, vp, vp, sz]),
+            "orqg_vec_create": (i32, [vp, sz, i32, i32, ctypes.POINTER(vp)]),
+            "orqg_vec_free": (i32, [vp]),
+            "orqg_vec_size": (sz, [vp]),
+            "orqg_vec_width": (i32, [vp]),
+            "orqg_vec_data": (vp, [vp]),
+            "orqg_vec_write": (i32, [vp, vp, vp]),
+            "orqg_vec_read": (i32, [vp, vp, vp]),
+            "orqg_vec_copy": (i32, [vp, vp, vp]),
+            "orqg_prg_keystream": (i32, [vp, i32, sz, vp]),
+            "orqg_secret_share": (i32, [vp, vp, vp]),
+            "orqg_open": (i32, [vp, vp, vp]),
+            "orqg_public_share": (i32, [vp, vp, vp]),
+            "orqg_xor_b": (i32, [vp, vp, vp, vp]),
+            "orqg_add_a": (i32, [vp, vp, vp, vp]),
+            "orqg_sub_a": (i32, [vp, vp, vp, vp]),
+            "orqg_not_b": (i32, [vp, vp, vp]),
+            "orqg_and_b": (i32, [vp, vp, vp, vp]),
+            "orqg_mul_a": (i32, [vp, vp, vp, vp]),
+            "orqg_b2a_bit": (i32, [vp, vp, vp]),
+            "orqg_reshare": (i32, [vp, vp, i32]),
+            "orqg_reserve_perms": (i32, [vp, sz, sz]),
+            "orqg_perm_queue_size": (sz, [vp]),
+            "orqg_peek_perm": (i32, [vp, sz, i32, vp]),
+            "orqg_apply_sharded_perm": (i32, [vp, vp, i32]),
+            "orqg_apply_elementwise_perm": (i32, [vp, vp, vp]),
+            "orqg_radix_sort": (i32, [vp, vp, i32, i32, vp]),
+            "orqg_msglog_count": (sz, [vp]),
+            "orqg_msglog_get": (i32, [vp, sz, ctypes.POINTER(_Msg)]),
+            "orqg_msglog_clear": (i32, [vp]),
+            "orqg_timer_enable": (i32, [vp, i32]),
+            "orqg_timer_get": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(u64)]),
+            "orqg_timer_reset": (i32, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def splitmix64(seed, count):
+    """splitmix64 stream (Steele, Lea, Flood 2014) as a uint64 array."""
+    M = (1 << 64) - 1
+    idx = np.arange(1, count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & M) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def keys_from_seed(seed):
+    """The 10 AES-256 keys (local x3, pairwise x3, group x4) from a 64-bit seed."""
+    return splitmix64(seed, 40).astype("<u8").tobytes()
+
+
+class Vec:
+    """A device-resident replicated vector: shares s_0, s_1, s_2 stored [n][3]."""
+
+    def __init__(self, ctx, n, width, encoding=ORQG_BSHARED):
+        self.ctx, self.n, self.width = ctx, n, width
+        self.encoding = encoding
+        h = ctypes.c_void_p()
+        ctx._check(lib().orqg_vec_create(ctx.h, n, width, encoding, ctypes.byref(h)), "vec_create")
+        self.h = h
+        ctx._vecs.add(self)
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().orqg_vec_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def shares(self):
+        out = np.zeros((3, self.n), dtype=_DT[self.width])
+        self.ctx._check(lib().orqg_vec_read(self.ctx.h, self.h, _ptr(out)), "vec_read")
+        return [out[0], out[1], out[2]]
+
+    def set_shares(self, s):
+        a = np.ascontiguousarray(np.stack([np.asarray(x, dtype=_DT[self.width]) for x in s]))
+        self.ctx._check(lib().orqg_vec_write(self.ctx.h, self.h, _ptr(a)), "vec_write")
+
+
+class Ctx:
+    def __init__(self, keys: bytes, batch_size=-12, device=0, hbm_budget=0):
+        cfg = _Config()
+        cfg.device = device
+        ctypes.memmove(ctypes.addressof(cfg.keys), keys, 320)
+        cfg.batch_size = batch_size
+        cfg.hbm_budget = hbm_budget
+        h = ctypes.c_void_p()
+        rc = lib().orqg_ctx_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise OrqgError(f"orqg_ctx_create failed ({rc})")
+        self.h = h
+        import weakref
+
+        self._vecs = weakref.WeakSet()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orqg_ctx_destroy(self.h)
+            self.h = None
+            for v in list(self._vecs):
+                v.free()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().orqg_last_error(self.h)
+            raise OrqgError(f"{what}: {msg.decode() if msg else rc} (code {rc})")
+
+    def sync(self):
+        self._check(lib().orqg_sync(self.h), "sync")
+
+    def nonce(self, role):
+        v = ctypes.c_uint64()
+        self._check(lib().orqg_prg_nonce(self.h, role, ctypes.byref(v)), "prg_nonce")
+        return v.value
+
+    def vec(self, n, width, encoding=ORQG_BSHARED):
+        return Vec(self, n, width, encoding)
+
+    def secret_share(self, data, width, arith=False):
+        a = np.ascontiguousarray(data, dtype=_DT[width])
+        v = Vec(self, a.size, width, ORQG_ASHARED if arith else ORQG_BSHARED)
+        self._check(lib().orqg_secret_share(self.h, _ptr(a), v.h), "secret_share")
+        return v
+
+    def open(self, v):
+        out = np.zeros(v.n, dtype=_DT[v.width])
+        self._check(lib().orqg_open(self.h, v.h, _ptr(out)), "open")
+        return out
+
+    def public_share(self, data, width, arith=True):
+        a = np.ascontiguousarray(data, dtype=_DT[width])
+        v = Vec(self, a.size, width, ORQG_ASHARED if arith else ORQG_BSHARED)
+        self._check(lib().orqg_public_share(self.h, _ptr(a), v.h), "public_share")
+        return v
+
+    def _binop(self, fn, x, y, enc):
+        z = Vec(self, x.n, x.width, enc)
+        self._check(getattr(lib(), fn)(self.h, x.h, y.h, z.h), fn)
+        return z
+
+    def xor_b(self, x, y):
+        return self._binop("orqg_xor_b", x, y, ORQG_BSHARED)
+
+    def and_b(self, x, y):
+        return self._binop("orqg_and_b", x, y, ORQG_BSHARED)
+
+    def add_a(self, x, y):
+        return self._binop("orqg_add_a", x, y, ORQG_ASHARED)
+
+    def sub_a(self, x, y):
+        return self._binop("orqg_sub_a", x, y, ORQG_ASHARED)
+
+    def mul_a(self, x, y):
+        return self._binop("orqg_mul_a", x, y, ORQG_ASHARED)
+
+    def not_b(self, x):
+        z = Vec(self, x.n, x.width, ORQG_BSHARED)
+        self._check(lib().orqg_not_b(self.h, x.h, z.h), "not_b")
+        return z
+
+    def b2a_bit(self, x):
+        y = Vec(self, x.n, x.width, ORQG_ASHARED)
+        self._check(lib().orqg_b2a_bit(self.h, x.h, y.h), "b2a_bit")
+        return y
+
+    def reshare(self, v, group):
+        self._check(lib().orqg_reshare(self.h, v.h, group), "reshare")
+
+    def keystream(self, role, nbytes):
+        L = lib()
+        d = ctypes.c_void_p()
+        self._check(L.orqg_dev_alloc(self.h, max(nbytes, 16), ctypes.byref(d)), "dev_alloc")
+        try:
+            self._check(L.orqg_prg_keystream(self.h, role, nbytes, d), "keystream")
+            out = np.zeros(nbytes, dtype=np.uint8)
+            self._check(L.orqg_dev_to_host(self.h, _ptr(out), d, nbytes), "dev_to_host")
+        finally:
+            L.orqg_dev_free(d)
+        return out
+
+    def reserve(self, n, count):
+        self._check(lib().orqg_reserve_perms(self.h, n, count), "reserve_perms")
+
+    def queue_size(self):
+        return lib().orqg_perm_queue_size(self.h)
+
+    def peek_perm(self, index, group, n):
+        out = np.zeros(n, dtype=np.int32)
+        self._check(lib().orqg_peek_perm(self.h, index, group, _ptr(out)), "peek_perm")
+        return out
+
+    def apply_sharded(self, x, inverse=False):
+        self._check(lib().orqg_apply_sharded_perm(self.h, x.h, int(inverse)), "apply_sharded_perm")
+
+    def apply_elementwise(self, x, perm):
+        self._check(lib().orqg_apply_elementwise_perm(self.h, x.h, perm.h), "apply_elementwise")
+
+    def radix_sort(self, v, bits=None, desc=False):
+        if bits is None:
+            bits = 8 * v.width
+        perm = Vec(self, v.n, 4, ORQG_ASHARED)
+        self._check(lib().orqg_radix_sort(self.h, v.h, DESC if desc else ASC, bits, perm.h),
+                    "radix_sort")
+        return perm
+
+    def msglog(self):
+        L = lib()
+        out = []
+        m = _Msg()
+        for i in range(L.orqg_msglog_count(self.h)):
+            L.orqg_msglog_get(self.h, i, ctypes.byref(m))
+            out.append((m.op.decode(), m.from_, m.to, m.bytes))
+        return out
+
+    def clear_msglog(self):
+        lib().orqg_msglog_clear(self.h)
+
+    def timer_enable(self, on=True):
+        lib().orqg_timer_enable(self.h, int(on))
+
+    def timer(self, family):
+        ms, cnt = ctypes.c_double(), ctypes.c_uint64()
+        lib().orqg_timer_get(self.h, family.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        return ms.value, cnt.value
+
+    def timer_reset(self):
+        lib().orqg_timer_reset(self.h)

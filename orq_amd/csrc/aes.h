@@ -1,0 +1,146 @@
+// AES-256 encryption of counter blocks for the reference's PRG keystream.
+//
+// Keystream framing (include/core/random/prg/prg_algorithm.h:214-231): a PRG
+// call of B bytes is split into <=1 MiB chunks; chunk c is the AES-256-GCM
+// encryption of zeros under IV = LE64(nonce + c) || 0^4, i.e. AES-256-CTR
+// starting at counter block IV || BE32(2).
+//
+// State words are little-endian columns: w_c = s_{0,c} | s_{1,c}<<8 |
+// s_{2,c}<<16 | s_{3,c}<<24.  One full round is
+//   t_c = T0[b0(w_c)] ^ T1[b1(w_{c+1})] ^ T2[b2(w_{c+2})] ^ T3[b3(w_{c+3})] ^ rk
+// with T0[x] = (2S, S, S, 3S) and T1..T3 its byte rotations (FIPS-197 5.1,
+// table form).  The tables live in LDS, replicated COPIES times so that lane l
+// reads copy (l % COPIES): with 32 copies every lane of a half-wave hits its
+// own bank and table lookups are conflict-free.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace orqg {
+
+// ---- host-side tables and key schedule ------------------------------------
+struct AesTables {
+  uint8_t sbox[256];
+  uint32_t t0[256];
+};
+
+inline uint8_t aes_xtime(uint8_t x) { return (uint8_t)((x << 1) ^ ((x & 0x80) ? 0x1b : 0)); }
+
+// S-box from the multiplicative inverse in GF(2^8) and the affine map,
+// walking the multiplicative group with generator 3.
+inline void aes_make_tables(AesTables *t) {
+  uint8_t p = 1, q = 1;
+  do {
+    p = (uint8_t)(p ^ aes_xtime(p));  // p *= 3
+    q ^= (uint8_t)(q << 1);           // q /= 3
+    q ^= (uint8_t)(q << 2);
+    q ^= (uint8_t)(q << 4);
+    if (q & 0x80) q ^= 0x09;
+    uint8_t x = (uint8_t)(q ^ ((q << 1) | (q >> 7)) ^ ((q << 2) | (q >> 6)) ^
+                          ((q << 3) | (q >> 5)) ^ ((q << 4) | (q >> 4)));
+    t->sbox[p] = (uint8_t)(x ^ 0x63);
+  } while (p != 1);
+  t->sbox[0] = 0x63;
+  for (int i = 0; i < 256; i++) {
+    uint8_t s = t->sbox[i], s2 = aes_xtime(s), s3 = (uint8_t)(s2 ^ s);
+    t->t0[i] = (uint32_t)s2 | ((uint32_t)s << 8) | ((uint32_t)s << 16) | ((uint32_t)s3 << 24);
+  }
+}
+
+// AES-256 key expansion (FIPS-197 5.2) into 60 little-endian words.
+inline void aes256_expand(const AesTables &t, const uint8_t key[32], uint32_t rk[60]) {
+  for (int i = 0; i < 8; i++)
+    rk[i] = (uint32_t)key[4 * i] | ((uint32_t)key[4 * i + 1] << 8) |
+            ((uint32_t)key[4 * i + 2] << 16) | ((uint32_t)key[4 * i + 3] << 24);
+  uint8_t rcon = 1;
+  auto sub = [&](uint32_t w) {
+    return (uint32_t)t.sbox[w & 0xff] | ((uint32_t)t.sbox[(w >> 8) & 0xff] << 8) |
+           ((uint32_t)t.sbox[(w >> 16) & 0xff] << 16) | ((uint32_t)t.sbox[w >> 24] << 24);
+  };
+  for (int i = 8; i < 60; i++) {
+    uint32_t tmp = rk[i - 1];
+    if (i % 8 == 0) {
+      tmp = sub((tmp >> 8) | (tmp << 24)) ^ rcon;
+      rcon = aes_xtime(rcon);
+    } else if (i % 8 == 4) {
+      tmp = sub(tmp);
+    }
+    rk[i] = rk[i - 8] ^ tmp;
+  }
+}
+
+// ---- device side ------------------------------------------------------------
+// LDS table layout: entry (table j, index x, copy c) at word
+// ((j * 256 + x) * COPIES + c).  Table j = rotl(T0, 8 j).
+template <int COPIES>
+struct AesLds {
+  static constexpr int kWords = 4 * 256 * COPIES;
+  static constexpr int kBytes = kWords * 4;
+};
+
+template <int COPIES>
+__device__ inline void aes_load_tables(uint32_t *lds, const uint32_t *__restrict__ g_t0) {
+  for (int w = threadIdx.x; w < 4 * 256 * COPIES; w += blockDim.x) {
+    int c = w % COPIES;
+    int x = (w / COPIES) & 255;
+    int j = w / (256 * COPIES);
+    uint32_t v = g_t0[x];
+    (void)c;
+    lds[w] = j == 0 ? v : ((v << (8 * j)) | (v >> (32 - 8 * j)));
+  }
+  __syncthreads();
+}
+
+template <int COPIES>
+struct AesCtx {
+  const uint32_t *t;  // LDS tables, pre-offset by this lane's copy
+  __device__ explicit AesCtx(const uint32_t *lds) : t(lds + (threadIdx.x % COPIES)) {}
+  __device__ inline uint32_t T(int j, uint32_t x) const { return t[(j * 256 + x) * COPIES]; }
+};
+
+// Encrypt one block (4 LE words) under round keys rk[60] (wave-uniform).
+template <int COPIES>
+__device__ inline void aes256_encrypt(const AesCtx<COPIES> &A, const uint32_t *__restrict__ rk,
+                                      uint32_t &w0, uint32_t &w1, uint32_t &w2, uint32_t &w3) {
+  uint32_t s0 = w0 ^ rk[0], s1 = w1 ^ rk[1], s2 = w2 ^ rk[2], s3 = w3 ^ rk[3];
+#pragma unroll
+  for (int r = 1; r < 14; r++) {
+    uint32_t t0 = A.T(0, s0 & 255) ^ A.T(1, (s1 >> 8) & 255) ^ A.T(2, (s2 >> 16) & 255) ^
+                  A.T(3, s3 >> 24) ^ rk[4 * r + 0];
+    uint32_t t1 = A.T(0, s1 & 255) ^ A.T(1, (s2 >> 8) & 255) ^ A.T(2, (s3 >> 16) & 255) ^
+                  A.T(3, s0 >> 24) ^ rk[4 * r + 1];
+    uint32_t t2 = A.T(0, s2 & 255) ^ A.T(1, (s3 >> 8) & 255) ^ A.T(2, (s0 >> 16) & 255) ^
+                  A.T(3, s1 >> 24) ^ rk[4 * r + 2];
+    uint32_t t3 = A.T(0, s3 & 255) ^ A.T(1, (s0 >> 8) & 255) ^ A.T(2, (s1 >> 16) & 255) ^
+                  A.T(3, s2 >> 24) ^ rk[4 * r + 3];
+    s0 = t0;
+    s1 = t1;
+    s2 = t2;
+    s3 = t3;
+  }
+  // last round: SubBytes + ShiftRows + AddRoundKey; S[x] is byte 0 of T2,
+  // byte 1 of T3, byte 2 of T0 and byte 3 of T1.
+  w0 = ((A.T(2, s0 & 255) & 0xffu) | (A.T(3, (s1 >> 8) & 255) & 0xff00u) |
+        (A.T(0, (s2 >> 16) & 255) & 0xff0000u) | (A.T(1, s3 >> 24) & 0xff000000u)) ^
+       rk[56];
+  w1 = ((A.T(2, s1 & 255) & 0xffu) | (A.T(3, (s2 >> 8) & 255) & 0xff00u) |
+        (A.T(0, (s3 >> 16) & 255) & 0xff0000u) | (A.T(1, s0 >> 24) & 0xff000000u)) ^
+       rk[57];
+  w2 = ((A.T(2, s2 & 255) & 0xffu) | (A.T(3, (s3 >> 8) & 255) & 0xff00u) |
+        (A.T(0, (s0 >> 16) & 255) & 0xff0000u) | (A.T(1, s1 >> 24) & 0xff000000u)) ^
+       rk[58];
+  w3 = ((A.T(2, s3 & 255) & 0xffu) | (A.T(3, (s0 >> 8) & 255) & 0xff00u) |
+        (A.T(0, (s1 >> 16) & 255) & 0xff0000u) | (A.T(1, s2 >> 24) & 0xff000000u)) ^
+       rk[59];
+}
+
+// Keystream block `ctr` of nonce `nonce`: encrypt LE64(nonce) || 0^4 || BE32(ctr).
+template <int COPIES>
+__device__ inline uint4 aes_ctr_block(const AesCtx<COPIES> &A, const uint32_t *__restrict__ rk,
+                                      uint64_t nonce, uint32_t ctr) {
+  uint32_t w0 = (uint32_t)nonce, w1 = (uint32_t)(nonce >> 32), w2 = 0, w3 = __builtin_bswap32(ctr);
+  aes256_encrypt<COPIES>(A, rk, w0, w1, w2, w3);
+  return make_uint4(w0, w1, w2, w3);
+}
+
+}  // namespace orqg

@@ -1,0 +1,388 @@
+// extern "C" entry points of liborqg (declared in include/orqg.h) and the
+// context / vector / message-log plumbing behind them.
+#include <cstring>
+
+#include "internal.h"
+
+using namespace orqg;
+
+void *orqg_ctx::scratch_get(const std::string &name, size_t bytes) {
+  Scratch &s = scratch[name];
+  if (s.bytes < bytes) {
+    if (s.p) dev_free(s.p);
+    s.p = nullptr;
+    s.bytes = 0;
+    size_t want = bytes + bytes / 8;
+    s.p = dev_alloc(this, want);
+    s.bytes = want;
+  }
+  return s.p;
+}
+
+// Byte accounting per (op, from, to), first-seen order: the reference's
+// Communicator counts bytes per send (communicator.h:22-33).
+void orqg_ctx::log_msg(const char *op, int from, int to, uint64_t bytes) {
+  if (bytes == 0) return;
+  for (auto &m : log)
+    if (m.from == from && m.to == to && std::strcmp(m.op, op) == 0) {
+      m.bytes += bytes;
+      return;
+    }
+  orqg_msg m;
+  std::memset(&m, 0, sizeof(m));
+  std::strncpy(m.op, op, sizeof(m.op) - 1);
+  m.from = from;
+  m.to = to;
+  m.bytes = bytes;
+  log.push_back(m);
+}
+
+namespace orqg {
+static size_t g_live_bytes = 0;
+
+void *dev_alloc(orqg_ctx *c, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (c->budget && g_live_bytes + bytes > c->budget)
+    throw Error(ORQG_E_BUDGET, "allocation exceeds the configured HBM budget");
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(ORQG_E_OOM, "hipMalloc failed");
+  }
+  g_live_bytes += bytes;
+  return p;
+}
+void dev_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+}  // namespace orqg
+
+#define API_BEGIN(c) \
+  try {              \
+    (void)(c);
+#define API_END(c)                                      \
+  return ORQG_OK;                                       \
+  }                                                     \
+  catch (const orqg::Error &e) {                        \
+    if (c) (c)->err = e.what();                         \
+    return e.code;                                      \
+  }                                                     \
+  catch (const std::exception &e) {                     \
+    if (c) (c)->err = e.what();                         \
+    return ORQG_E_DEVICE;                               \
+  }
+
+extern "C" {
+
+int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
+  if (!cfg || !out) return ORQG_E_ARG;
+  orqg_ctx *c = new orqg_ctx;
+  try {
+    c->device = cfg->device;
+    c->batch_size = cfg->batch_size == 0 ? -12 : cfg->batch_size;
+    c->budget = cfg->hbm_budget;
+    std::memcpy(c->keys, cfg->keys, sizeof(c->keys));
+    ORQG_HIP(hipSetDevice(c->device));
+    ORQG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    ORQG_HIP(hipEventCreate(&c->ev0));
+    ORQG_HIP(hipEventCreate(&c->ev1));
+    hipDeviceProp_t prop;
+    ORQG_HIP(hipGetDeviceProperties(&prop, c->device));
+    c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    aes_make_tables(&c->tabs);
+    uint32_t rk[ORQG_NUM_KEYS * 60];
+    for (int k = 0; k < ORQG_NUM_KEYS; k++) aes256_expand(c->tabs, c->keys[k], rk + 60 * k);
+    c->d_rk = (uint32_t *)dev_alloc(c, sizeof(rk));
+    c->d_t0 = (uint32_t *)dev_alloc(c, sizeof(c->tabs.t0));
+    ORQG_HIP(hipMemcpy(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice));
+    ORQG_HIP(hipMemcpy(c->d_t0, c->tabs.t0, sizeof(c->tabs.t0), hipMemcpyHostToDevice));
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "orqg_ctx_create: %s\n", e.what());
+    delete c;
+    return ORQG_E_DEVICE;
+  }
+  *out = c;
+  return ORQG_OK;
+}
+
+int orqg_ctx_destroy(orqg_ctx *c) {
+  if (!c) return ORQG_E_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &kv : c->scratch) dev_free(kv.second.p);
+  for (auto &p : c->queue) free_perm(p);
+  dev_free(c->d_rk);
+  dev_free(c->d_t0);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ORQG_OK;
+}
+
+const char *orqg_last_error(orqg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int orqg_sync(orqg_ctx *c) {
+  API_BEGIN(c)
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+void *orqg_stream(orqg_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int orqg_prg_nonce(orqg_ctx *c, int role, uint64_t *nonce) {
+  if (!c || role < 0 || role >= ORQG_NUM_KEYS || !nonce) return ORQG_E_ARG;
+  *nonce = c->nonce[role];
+  return ORQG_OK;
+}
+
+int orqg_dev_alloc(orqg_ctx *c, size_t bytes, void **dev_out) {
+  API_BEGIN(c)
+  if (!dev_out) throw Error(ORQG_E_ARG, "dev_alloc: null output");
+  *dev_out = dev_alloc(c, bytes);
+  API_END(c)
+}
+
+int orqg_dev_free(void *dev) {
+  dev_free(dev);
+  return ORQG_OK;
+}
+
+int orqg_dev_to_host(orqg_ctx *c, void *host, const void *dev, size_t bytes) {
+  API_BEGIN(c)
+  ORQG_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int orqg_vec_create(orqg_ctx *c, size_t n, int width, int encoding, orqg_vec **out) {
+  API_BEGIN(c)
+  if (!out || (width != 4 && width != 8)) throw Error(ORQG_E_ARG, "vec_create: width must be 4 or 8");
+  orqg_vec *v = new orqg_vec;
+  v->ctx = c;
+  v->n = n;
+  v->width = width;
+  v->enc = encoding ? ORQG_BSHARED : ORQG_ASHARED;
+  try {
+    v->d = dev_alloc(c, 3 * n * (size_t)width + 64);
+    ORQG_HIP(hipMemsetAsync(v->d, 0, 3 * n * (size_t)width, c->stream));
+  } catch (...) {
+    delete v;
+    throw;
+  }
+  *out = v;
+  API_END(c)
+}
+
+// hipFree waits for outstanding work on the device; the owning context may
+// already be gone, so it is not touched here.
+int orqg_vec_free(orqg_vec *v) {
+  if (!v) return ORQG_E_ARG;
+  dev_free(v->d);
+  delete v;
+  return ORQG_OK;
+}
+
+size_t orqg_vec_size(const orqg_vec *v) { return v ? v->n : 0; }
+int orqg_vec_width(const orqg_vec *v) { return v ? v->width : 0; }
+void *orqg_vec_data(orqg_vec *v) { return v ? v->d : nullptr; }
+
+// host [3][n] <-> device [n][3]
+int orqg_vec_write(orqg_ctx *c, orqg_vec *v, const void *host) {
+  API_BEGIN(c)
+  size_t n = v->n, w = v->width;
+  std::vector<uint8_t> aos(3 * n * w);
+  const uint8_t *h = (const uint8_t *)host;
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) std::memcpy(&aos[(3 * i + k) * w], h + (k * n + i) * w, w);
+  ORQG_HIP(hipMemcpyAsync(v->d, aos.data(), aos.size(), hipMemcpyHostToDevice, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int orqg_vec_read(orqg_ctx *c, const orqg_vec *v, void *host) {
+  API_BEGIN(c)
+  size_t n = v->n, w = v->width;
+  std::vector<uint8_t> aos(3 * n * w);
+  ORQG_HIP(hipMemcpyAsync(aos.data(), v->d, aos.size(), hipMemcpyDeviceToHost, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  uint8_t *h = (uint8_t *)host;
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) std::memcpy(h + (k * n + i) * w, &aos[(3 * i + k) * w], w);
+  API_END(c)
+}
+
+int orqg_vec_copy(orqg_ctx *c, const orqg_vec *src, orqg_vec *dst) {
+  API_BEGIN(c)
+  if (src->n != dst->n || src->width != dst->width) throw Error(ORQG_E_ARG, "vec_copy: shape");
+  ORQG_HIP(hipMemcpyAsync(dst->d, src->d, 3 * src->n * (size_t)src->width,
+                          hipMemcpyDeviceToDevice, c->stream));
+  dst->enc = src->enc;
+  API_END(c)
+}
+
+int orqg_prg_keystream(orqg_ctx *c, int role, size_t bytes, void *dev_out) {
+  API_BEGIN(c)
+  prg_keystream(c, role, bytes, dev_out);
+  API_END(c)
+}
+
+int orqg_secret_share(orqg_ctx *c, const void *host_plain, orqg_vec *v) {
+  API_BEGIN(c)
+  secret_share(c, host_plain, v);
+  API_END(c)
+}
+
+int orqg_open(orqg_ctx *c, const orqg_vec *v, void *host_out) {
+  API_BEGIN(c)
+  open_vec(c, v, host_out);
+  API_END(c)
+}
+
+int orqg_public_share(orqg_ctx *c, const void *host_plain, orqg_vec *v) {
+  API_BEGIN(c)
+  size_t n = v->n, w = v->width;
+  std::vector<uint8_t> sh(3 * n * w, 0);
+  std::memcpy(sh.data(), host_plain, n * w);  // s_0 = x, s_1 = s_2 = 0
+  int rc = orqg_vec_write(c, v, sh.data());
+  if (rc) return rc;
+  API_END(c)
+}
+
+static void check3(const orqg_vec *x, const orqg_vec *y, const orqg_vec *z) {
+  if (!x || !z || (y && (y->n != x->n || y->width != x->width)) || z->n != x->n ||
+      z->width != x->width)
+    throw Error(ORQG_E_ARG, "operand shapes differ");
+}
+
+int orqg_xor_b(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  elementwise_local(c, kXor, x, y, z);
+  API_END(c)
+}
+int orqg_add_a(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  elementwise_local(c, kAdd, x, y, z);
+  API_END(c)
+}
+int orqg_sub_a(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  elementwise_local(c, kSub, x, y, z);
+  API_END(c)
+}
+int orqg_not_b(orqg_ctx *c, const orqg_vec *x, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, nullptr, z);
+  elementwise_local(c, kNot, x, nullptr, z);
+  API_END(c)
+}
+int orqg_and_b(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  mult(c, false, x, y, z);
+  API_END(c)
+}
+int orqg_mul_a(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  mult(c, true, x, y, z);
+  API_END(c)
+}
+int orqg_b2a_bit(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
+  API_BEGIN(c)
+  check3(x, nullptr, y);
+  b2a_bit(c, x, y);
+  y->enc = ORQG_ASHARED;
+  API_END(c)
+}
+int orqg_reshare(orqg_ctx *c, orqg_vec *v, int group) {
+  API_BEGIN(c)
+  reshare(c, v, group);
+  API_END(c)
+}
+
+int orqg_reserve_perms(orqg_ctx *c, size_t n, size_t count) {
+  API_BEGIN(c)
+  reserve_perms(c, n, count);
+  API_END(c)
+}
+size_t orqg_perm_queue_size(orqg_ctx *c) { return c ? c->queue.size() : 0; }
+
+int orqg_peek_perm(orqg_ctx *c, size_t index, int group, int32_t *host_out) {
+  API_BEGIN(c)
+  if (index >= c->queue.size() || group < 0 || group > 2) throw Error(ORQG_E_ARG, "peek_perm");
+  const ShardedPerm &p = c->queue[index];
+  ORQG_HIP(hipMemcpyAsync(host_out, p.fwd[group], p.n * 4, hipMemcpyDeviceToHost, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int orqg_apply_sharded_perm(orqg_ctx *c, orqg_vec *x, int inverse) {
+  API_BEGIN(c)
+  ShardedPerm pi = next_perm(c, x->n);
+  try {
+    apply_sharded(c, x, pi, inverse != 0);
+  } catch (...) {
+    free_perm(pi);
+    throw;
+  }
+  free_perm(pi);
+  API_END(c)
+}
+
+int orqg_apply_elementwise_perm(orqg_ctx *c, orqg_vec *x, const orqg_vec *perm) {
+  API_BEGIN(c)
+  if (perm->n != x->n || perm->width != 4) throw Error(ORQG_E_ARG, "apply_elementwise: perm");
+  apply_elementwise(c, x, perm);
+  API_END(c)
+}
+
+int orqg_radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out) {
+  API_BEGIN(c)
+  if (!perm_out || perm_out->n != v->n || perm_out->width != 4)
+    throw Error(ORQG_E_ARG, "radix_sort: perm_out must be a width-4 vector of the same size");
+  if (bits < 1 || bits > 8 * v->width) throw Error(ORQG_E_ARG, "radix_sort: bits");
+  if (v->n >= (1ull << 31)) throw Error(ORQG_E_ARG, "radix_sort: at most 2^31-1 rows");
+  radix_sort(c, v, order, bits, perm_out);
+  perm_out->enc = ORQG_ASHARED;
+  API_END(c)
+}
+
+size_t orqg_msglog_count(orqg_ctx *c) { return c ? c->log.size() : 0; }
+int orqg_msglog_get(orqg_ctx *c, size_t i, orqg_msg *out) {
+  if (!c || !out || i >= c->log.size()) return ORQG_E_ARG;
+  *out = c->log[i];
+  return ORQG_OK;
+}
+int orqg_msglog_clear(orqg_ctx *c) {
+  if (!c) return ORQG_E_ARG;
+  c->log.clear();
+  return ORQG_OK;
+}
+
+int orqg_timer_enable(orqg_ctx *c, int enable) {
+  if (!c) return ORQG_E_ARG;
+  c->timing = enable != 0;
+  return ORQG_OK;
+}
+int orqg_timer_get(orqg_ctx *c, const char *family, double *ms, uint64_t *launches) {
+  if (!c || !family) return ORQG_E_ARG;
+  auto it = c->timers.find(family);
+  if (it == c->timers.end()) {
+    if (ms) *ms = 0;
+    if (launches) *launches = 0;
+    return ORQG_OK;
+  }
+  if (ms) *ms = it->second.ms;
+  if (launches) *launches = it->second.launches;
+  return ORQG_OK;
+}
+int orqg_timer_reset(orqg_ctx *c) {
+  if (!c) return ORQG_E_ARG;
+  c->timers.clear();
+  return ORQG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// Internal definitions shared by the translation units of liborqg.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <deque>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/orqg.h"
+#include "aes.h"
+
+namespace orqg {
+
+// Largest request the reference's AESPRGAlgorithm hands to one AES-GCM call
+// (prg_algorithm.h:14-15): each such chunk consumes one nonce.
+constexpr uint64_t kChunkBytes = 1ull << 20;
+constexpr uint32_t kBlocksPerChunk = (uint32_t)(kChunkBytes / 16);
+
+// Key roles, see orqg.h.
+constexpr int kLocal0 = 0, kPair0 = 3, kGroup0 = 6;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define ORQG_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::orqg::Error(ORQG_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- runtime batch schedule -------------------------------------------------
+// RunTime::getThreadBatchBoundaries + Task::execute with one worker thread
+// (include/backend/common/runtime.h:198-247, task.h:34-44,89-94).
+inline uint64_t batch_elems(uint64_t n, long batch_size) {
+  if (n == 0) return 1;
+  if (batch_size < 0) {
+    uint64_t nb = (uint64_t)(-batch_size);
+    uint64_t chunks = (n + 255) / 256;
+    return (chunks + nb - 1) / nb * 256;
+  }
+  return (uint64_t)batch_size;
+}
+
+// Keystream positions of one runtime call: n elements of `tsz` bytes, batches
+// of `bs` elements, and `calls` getNext calls per batch on each PRG involved.
+// Inside batch b the j-th call starts at nonce offset b*calls*cfull + j*c(b),
+// where c(b) = ceil(|b| * tsz / 1 MiB).
+struct CallSched {
+  uint64_t n, bs, cfull;
+  uint32_t tsz, calls;
+};
+
+inline CallSched make_sched(uint64_t n, long batch_size, uint32_t tsz, uint32_t calls) {
+  CallSched s;
+  s.n = n;
+  s.bs = batch_elems(n, batch_size);
+  s.tsz = tsz;
+  s.calls = calls;
+  s.cfull = (s.bs * tsz + kChunkBytes - 1) / kChunkBytes;
+  return s;
+}
+
+// nonces consumed on each PRG by the whole call
+inline uint64_t sched_nonces(const CallSched &s) {
+  if (s.n == 0) return 0;
+  uint64_t nb = (s.n + s.bs - 1) / s.bs;
+  uint64_t last = s.n - (nb - 1) * s.bs;
+  return (nb - 1) * s.calls * s.cfull + (uint64_t)s.calls * ((last * s.tsz + kChunkBytes - 1) / kChunkBytes);
+}
+
+// Locate element e's bytes for call j: nonce offset, CTR counter and the byte
+// offset inside the 16-byte block.
+__device__ __host__ inline void sched_locate(const CallSched &s, uint64_t e, uint32_t j,
+                                             uint64_t &nonce_off, uint32_t &ctr,
+                                             uint32_t &byte_in_block) {
+  uint64_t b = e / s.bs;
+  uint64_t l = e - b * s.bs;
+  uint64_t size_b = s.n - b * s.bs;
+  if (size_b > s.bs) size_b = s.bs;
+  uint64_t cb = (size_b * s.tsz + kChunkBytes - 1) / kChunkBytes;
+  uint64_t byte = l * s.tsz;
+  nonce_off = b * s.calls * s.cfull + j * cb + (byte >> 20);
+  uint32_t in_chunk = (uint32_t)(byte & (kChunkBytes - 1));
+  ctr = 2 + (in_chunk >> 4);
+  byte_in_block = in_chunk & 15;
+}
+
+// ---- device memory pool --------------------------------------------------------
+struct Scratch {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct ShardedPerm {
+  size_t n = 0;
+  int32_t *fwd[3] = {nullptr, nullptr, nullptr};  // pi_g      (scatter form)
+  int32_t *inv[3] = {nullptr, nullptr, nullptr};  // pi_g^{-1} (gather form)
+};
+
+struct Timer {
+  double ms = 0;
+  uint64_t launches = 0;
+};
+
+}  // namespace orqg
+
+struct orqg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  long batch_size = -12;
+  size_t budget = 0;
+  uint64_t nonce[ORQG_NUM_KEYS] = {0};
+  uint8_t keys[ORQG_NUM_KEYS][32];
+  orqg::AesTables tabs;
+  uint32_t *d_rk = nullptr;  // ORQG_NUM_KEYS x 60 round-key words
+  uint32_t *d_t0 = nullptr;  // 256 words
+  std::string err;
+  std::vector<orqg_msg> log;
+  std::map<std::string, orqg::Scratch> scratch;
+  std::deque<orqg::ShardedPerm> queue;
+  size_t stored_size = 0;
+  int num_cu = 256;
+  bool timing = false;
+  std::map<std::string, orqg::Timer> timers;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  void *scratch_get(const std::string &name, size_t bytes);
+  void log_msg(const char *op, int from, int to, uint64_t bytes);
+  const uint32_t *rk(int role) const { return d_rk + 60 * role; }
+};
+
+struct orqg_vec {
+  orqg_ctx *ctx = nullptr;
+  size_t n = 0;
+  int width = 4;
+  int enc = ORQG_BSHARED;
+  void *d = nullptr;  // [n][3] elements of `width` bytes
+};
+
+namespace orqg {
+
+// Allocation that counts against the HBM budget.
+void *dev_alloc(orqg_ctx *c, size_t bytes);
+void dev_free(void *p);
+
+// Scoped timing of one kernel family on the context stream.
+struct TimedRegion {
+  orqg_ctx *c;
+  const char *fam;
+  TimedRegion(orqg_ctx *c_, const char *f) : c(c_), fam(f) {
+    if (c->timing) ORQG_HIP(hipEventRecord(c->ev0, c->stream));
+  }
+  ~TimedRegion() noexcept(false) {
+    if (c->timing) {
+      ORQG_HIP(hipEventRecord(c->ev1, c->stream));
+      ORQG_HIP(hipEventSynchronize(c->ev1));
+      float ms = 0;
+      ORQG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      auto &t = c->timers[fam];
+      t.ms += ms;
+      t.launches += 1;
+    }
+  }
+};
+
+inline unsigned grid_for(uint64_t items, unsigned block, unsigned cap) {
+  uint64_t g = (items + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// ---- entry points implemented in the .hip files -------------------------------
+void prg_keystream(orqg_ctx *c, int role, size_t bytes, void *out);
+void secret_share(orqg_ctx *c, const void *host, orqg_vec *v);
+void open_vec(orqg_ctx *c, const orqg_vec *v, void *host_out);
+void elementwise_local(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+void mult(orqg_ctx *c, bool arith, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+void b2a_bit(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
+void reshare(orqg_ctx *c, orqg_vec *v, int group);
+void reserve_perms(orqg_ctx *c, size_t n, size_t count);
+ShardedPerm next_perm(orqg_ctx *c, size_t n);
+void free_perm(ShardedPerm &p);
+void apply_sharded(orqg_ctx *c, orqg_vec *x, const ShardedPerm &pi, bool inverse);
+void apply_elementwise(orqg_ctx *c, orqg_vec *x, const orqg_vec *perm);
+void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out);
+
+enum LocalOp { kXor = 0, kAdd = 1, kSub = 2, kNot = 3 };
+
+}  // namespace orqg

@@ -1,0 +1,483 @@
+// Oblivious radix sort (include/core/operators/radixsort.h:116-229) for all
+// three parties on one GPU.
+//
+// The padded vector of pad_input (include/core/operators/sorting.h:431-462)
+// is kept as L 32-bit words per share, element-major [n][3][L]:
+//   int32 keys -> PadWidth int64  -> L = 2 (index word, key word)
+//   int64 keys -> PadWidth int128 -> L = 3 (index word, 2 key words); bits
+//   96..127 of the reference's int128 shares never reach an output (the sort
+//   reads bits 32..95, remove_padding keeps bits 0..95), so they are not
+//   stored, but their keystream is still generated: resharing masks are drawn
+//   as whole 16-byte elements exactly as the reference draws them.
+//
+// One bit iteration (radixsort.h:141-192):
+//   k_bit_b2a   bit 32+i of each share -> b2a_bit (3 zero sharings per batch)
+//               -> f1, plus per-block sums of f1 for the prefix scan
+//   k_scan3     exclusive scan of the block sums (3 shares)
+//   k_bit_perm  s = prefix_sum([1-f1 ; f1]) - 1 in closed form, then
+//               perm = s0 + f1*(s1-s0) with multiply_a (1 zero sharing)
+//   k_round x3  sharded permutation rounds: gather both v and perm through
+//               pi_g^{-1} and add the group-g resharing masks
+//   k_open_scatter  open pi(perm) and scatter pi(v) by it
+#include <algorithm>
+#include <numeric>
+
+#include <cstring>
+#include <cmath>
+
+#include "internal.h"
+#include "kern_common.h"
+
+namespace orqg {
+
+void exclusive_scan_u32(orqg_ctx *c, const uint32_t *in, uint32_t *out, uint64_t n_plus_1);
+template <typename T>
+void launch_round(orqg_ctx *c, bool arith, int g, const int32_t *idx, const T *in, T *out,
+                  uint64_t n);
+
+namespace {
+
+constexpr int kBT = 256;            // threads per block
+constexpr int kEPB = kBT * 4;       // elements per scan block
+
+__device__ inline uint32_t bexcl(uint32_t x, uint32_t *s_w, uint32_t &total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t inc = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  __syncthreads();
+  if (lane == 63) s_w[wv] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  total = 0;
+  for (int w = 0; w < nw; w++) {
+    if (w < wv) base += s_w[w];
+    total += s_w[w];
+  }
+  return base + inc - x;
+}
+
+// ---- pad_input ----------------------------------------------------------------
+// share k of element i: (sign-extended key << 32) | idx_k, idx_0 = i (ASC) or
+// (uint32)(-1-i) (DESC), idx_1 = idx_2 = 0 (public_share, replicated_3pc.h:456-470);
+// DESC then reverses the element order.
+template <int L, typename K>
+__global__ void k_pad(uint64_t n, bool desc, const K *__restrict__ v, uint32_t *__restrict__ P) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t dst = desc ? n - 1 - i : i;
+    uint32_t idx = desc ? (uint32_t)(-1 - (int64_t)i) : (uint32_t)i;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      uint32_t *o = P + (dst * 3 + k) * L;
+      K key = v[3 * i + k];
+      o[0] = k == 0 ? idx : 0u;
+      o[1] = (uint32_t)key;
+      if (L == 3) o[2] = (uint32_t)((uint64_t)key >> 32);
+    }
+  }
+}
+
+// ---- one bit: b2a_bit + block sums -------------------------------------------------
+template <int L>
+__global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0,
+                                                 const uint32_t *__restrict__ rk_pair,
+                                                 NonceTriple nb, CallSched s,
+                                                 const uint32_t *__restrict__ P, int word,
+                                                 int bit, uint32_t flip, uint32_t *__restrict__ F,
+                                                 uint32_t *__restrict__ bsum) {
+  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  __shared__ uint32_t s_w[kBT / 64];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const uint64_t e0 = (uint64_t)blockIdx.x * kEPB + threadIdx.x * 4;
+  uint32_t bits[4][3], f[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      bits[i][k] = (e0 + i < s.n) ? (((P[((e0 + i) * 3 + k) * L + word] >> bit) & 1u) ^ flip) : 0u;
+  b2a_bit_group<uint32_t>(A, rk_pair, nb, s, e0, bits, f);
+  uint32_t loc[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    if (e0 + i < s.n) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        F[(e0 + i) * 3 + k] = f[i][k];
+        loc[k] += f[i][k];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    uint32_t tot;
+    bexcl(loc[k], s_w, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x * 3 + k] = tot;
+  }
+}
+
+// exclusive scan of [nb][3] block sums in place; totals to tot[3]
+__global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restrict__ bsum,
+                                                uint32_t *__restrict__ tot) {
+  __shared__ uint32_t s_w[1024 / 64];
+  __shared__ uint32_t s_carry[3];
+  if (threadIdx.x < 3) s_carry[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+    uint32_t j = b0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      uint32_t x = j < nb ? bsum[j * 3 + k] : 0;
+      uint32_t t;
+      uint32_t e = bexcl(x, s_w, t);
+      uint32_t carry = s_carry[k];
+      if (j < nb) bsum[j * 3 + k] = carry + e;
+      __syncthreads();
+      if (threadIdx.x == 0) s_carry[k] = carry + t;
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x < 3) tot[threadIdx.x] = s_carry[threadIdx.x];
+}
+
+// s0_k(e) = one_k*e - F_k(e), s1_k(e) = one_k*(n-1) - Ftot_k + F_k(e) with F the
+// inclusive prefix of f1 and one = (1,0,0); perm = s0 + f1*(s1 - s0).
+__global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t0,
+                                                  const uint32_t *__restrict__ rk_pair,
+                                                  NonceTriple nb, CallSched s,
+                                                  const uint32_t *__restrict__ F,
+                                                  const uint32_t *__restrict__ boff,
+                                                  const uint32_t *__restrict__ tot,
+                                                  uint32_t *__restrict__ perm) {
+  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  __shared__ uint32_t s_w[kBT / 64];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const uint64_t e0 = (uint64_t)blockIdx.x * kEPB + threadIdx.x * 4;
+  uint32_t f[4][3];
+  uint32_t loc[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      f[i][k] = (e0 + i < s.n) ? F[(e0 + i) * 3 + k] : 0u;
+      loc[k] += f[i][k];
+    }
+  uint32_t pre[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    uint32_t t;
+    pre[k] = bexcl(loc[k], s_w, t) + boff[blockIdx.x * 3 + k];
+  }
+  uint32_t S[3][4];
+  {
+    uint64_t off;
+    uint32_t ctr, bib;
+    sched_locate(s, e0, 0, off, ctr, bib);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      uint4 blk = aes_ctr_block<kCopies>(A, rk_pair + 60 * k, nb.v[k] + off, ctr);
+      unpack_block<uint32_t>(blk, S[k]);
+    }
+  }
+  const uint32_t n1 = (uint32_t)(s.n - 1);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    uint32_t s0[3], y[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      pre[k] += f[i][k];  // inclusive
+      uint32_t one = k == 0 ? 1u : 0u;
+      s0[k] = one * (uint32_t)(e0 + i) - pre[k];
+      uint32_t s1 = one * n1 - tot[k] + pre[k];
+      y[k] = s1 - s0[k];
+    }
+    if (e0 + i < s.n) {
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        int q = (p + 1) % 3, pm = (p + 2) % 3;
+        uint32_t z = (S[pm][i] - S[p][i]) + f[i][p] * y[p] + f[i][p] * y[q] + f[i][q] * y[p];
+        perm[(e0 + i) * 3 + p] = s0[p] + z;
+      }
+    }
+  }
+}
+
+// ---- sharded round: gather v (binary, tszv-byte masks) and perm (arithmetic,
+// 4-byte masks) through idx, masks drawn from group PRG g ---------------------------
+template <int L>
+__global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
+                                                const uint32_t *__restrict__ rk, uint64_t nv,
+                                                CallSched sv, uint64_t np, CallSched sp,
+                                                const int32_t *__restrict__ idx,
+                                                const uint32_t *__restrict__ Pin,
+                                                uint32_t *__restrict__ Pout,
+                                                const uint32_t *__restrict__ qin,
+                                                uint32_t *__restrict__ qout) {
+  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const uint64_t n = sv.n;
+  const uint64_t ngrp = (n + 3) / 4;
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < ngrp;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e0 = g * 4;
+    // perm masks: one block per call covers the 4 elements
+    uint32_t pm[3][4];
+    group_masks<uint32_t, true, kCopies>(A, rk, np, sp, e0, pm);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint64_t e = e0 + i;
+      if (e >= n) break;
+      const uint64_t src = (uint64_t)(uint32_t)idx[e];
+      // v masks: element e's tszv bytes of calls 0 and 1
+      uint32_t mv[3][L];
+      {
+        uint32_t r[2][L];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          uint64_t off;
+          uint32_t ctr, bib;
+          sched_locate(sv, e, j, off, ctr, bib);
+          uint4 blk = aes_ctr_block<kCopies>(A, rk, nv + off, ctr);
+          uint32_t w[4] = {blk.x, blk.y, blk.z, blk.w};
+#pragma unroll
+          for (int q = 0; q < L; q++) r[j][q] = w[bib / 4 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < L; q++) {
+          mv[0][q] = r[0][q];
+          mv[1][q] = r[1][q];
+          mv[2][q] = r[0][q] ^ r[1][q];
+        }
+      }
+      const uint32_t *ps = Pin + src * 3 * L;
+      uint32_t *pd = Pout + e * 3 * L;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int q = 0; q < L; q++) pd[k * L + q] = ps[k * L + q] ^ mv[k][q];
+      const uint32_t *qs = qin + src * 3;
+      uint32_t *qd = qout + e * 3;
+#pragma unroll
+      for (int k = 0; k < 3; k++) qd[k] = qs[k] + pm[k][i];
+    }
+  }
+}
+
+template <int L>
+__global__ void k_open_scatter_pad(uint64_t n, const uint32_t *__restrict__ q,
+                                   const uint32_t *__restrict__ Pin, uint32_t *__restrict__ Pout,
+                                   uint32_t *__restrict__ err) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t sigma = q[3 * i] + q[3 * i + 1] + q[3 * i + 2];
+    if (sigma >= n) {
+      atomicOr(err, 1u);
+      continue;
+    }
+#pragma unroll
+    for (int w = 0; w < 3 * L; w++) Pout[(uint64_t)sigma * 3 * L + w] = Pin[i * 3 * L + w];
+  }
+}
+
+// remove_padding (sorting.h:475-495): perm_b = low 32 bits (B-shared int32),
+// v = bits 32.. of each share.  DESC reverses the element order first.
+template <int L, typename K>
+__global__ void k_unpad(uint64_t n, bool desc, const uint32_t *__restrict__ P, K *__restrict__ v,
+                        uint32_t *__restrict__ pb) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t src = desc ? n - 1 - i : i;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const uint32_t *p = P + (src * 3 + k) * L;
+      pb[3 * i + k] = p[0];
+      uint64_t key = p[1];
+      if (L == 3) key |= (uint64_t)p[2] << 32;
+      v[3 * i + k] = (K)key;
+    }
+  }
+}
+
+// open a [n][3] uint32 vector (XOR or sum) and write it as the public sharing
+// (o, 0, 0) into out (open_shares_* then public_share)
+__global__ void k_open_public(uint64_t n, bool arith_in, const uint32_t *__restrict__ in,
+                              uint32_t *__restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t *s = in + 3 * i;
+    uint32_t o = arith_in ? s[0] + s[1] + s[2] : s[0] ^ s[1] ^ s[2];
+    out[3 * i] = o;
+    out[3 * i + 1] = 0;
+    out[3 * i + 2] = 0;
+  }
+}
+
+__global__ void k_iota_public(uint64_t n, uint32_t *__restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    out[3 * i] = (uint32_t)i;
+    out[3 * i + 1] = 0;
+    out[3 * i + 2] = 0;
+  }
+}
+
+// x + (-1, 0, 0) share-wise (adding the public sharing of -1)
+__global__ void k_add_public_m1(uint64_t n, uint32_t *__restrict__ x) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    x[3 * i] += 0xffffffffu;
+}
+
+template <int L>
+void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) {
+  const uint32_t tszv = L == 3 ? 16 : 8;  // PadWidth element size
+  uint32_t *P2 = (uint32_t *)c->scratch_get("rs_P2", n * 3 * L * 4 + 64);
+  uint32_t *F = (uint32_t *)c->scratch_get("rs_F", n * 12 + 64);
+  uint32_t *q0 = (uint32_t *)c->scratch_get("rs_q0", n * 12 + 64);
+  uint32_t *q1 = (uint32_t *)c->scratch_get("rs_q1", n * 12 + 64);
+  const uint32_t nblk = (uint32_t)((n + kEPB - 1) / kEPB);
+  uint32_t *bsum = (uint32_t *)c->scratch_get("rs_bsum", (size_t)nblk * 12 + 64);
+  uint32_t *tot = (uint32_t *)c->scratch_get("rs_tot", 64);
+  uint32_t *err = (uint32_t *)c->scratch_get("rs_err", 16);
+  ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
+  const unsigned grid_rounds = grid_for((n + 3) / 4, kBT, c->num_cu * 8);
+  const unsigned grid_flat = grid_for(n, 256, c->num_cu * 16);
+
+  for (int i = 0; i < bits; i++) {
+    const int bitpos = 32 + i;
+    const uint32_t flip = (full_width && i == bits - 1) ? 1u : 0u;  // inplace_invert on the MSB
+    NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+    {
+      TimedRegion tr(c, "sort_b2a");
+      CallSched s = make_sched(n, c->batch_size, 4, 3);
+      hipLaunchKernelGGL(k_bit_b2a<L>, dim3(nblk), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
+                         nb, s, P, bitpos / 32, bitpos % 32, flip, F, bsum);
+      uint64_t used = sched_nonces(s);
+      for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+      for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * n * 4);
+      hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c->stream, nblk, bsum, tot);
+    }
+    {
+      TimedRegion tr(c, "sort_perm");
+      NonceTriple nb2 = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+      CallSched s = make_sched(n, c->batch_size, 4, 1);
+      hipLaunchKernelGGL(k_bit_perm, dim3(nblk), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
+                         nb2, s, F, bsum, tot, q0);
+      uint64_t used = sched_nonces(s);
+      for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+      for (int p = 0; p < 3; p++) c->log_msg("mul_a", p, (p + 2) % 3, n * 4);
+    }
+    // oblivious_apply_elementwise_perm(v, perm): pi from the queue
+    ShardedPerm pi = next_perm(c, n);
+    {
+      TimedRegion tr(c, "sort_round");
+      // key G_g is used first by v's round g, then by perm's round g
+      uint32_t *pin = P, *pout = P2, *qin = q0, *qout = q1;
+      CallSched sv = make_sched(n, c->batch_size, tszv, 2);
+      CallSched sp = make_sched(n, c->batch_size, 4, 2);
+      const uint64_t uv = sched_nonces(sv), up = sched_nonces(sp);
+      for (int g = 0; g < 3; g++) {
+        const int role = kGroup0 + g;
+        const uint64_t nv = c->nonce[role], np = nv + uv;
+        hipLaunchKernelGGL(k_round2<L>, dim3(grid_rounds), dim3(kBT), 0, c->stream, c->d_t0,
+                           c->rk(role), nv, sv, np, sp, pi.inv[g], pin, pout, qin, qout);
+        c->nonce[role] += uv + up;
+        std::swap(pin, pout);
+        std::swap(qin, qout);
+      }
+      // log in the reference's order: three rounds on v, then three on perm
+      for (int g = 0; g < 3; g++) log_reshare(c, g, n * tszv);
+      for (int g = 0; g < 3; g++) log_reshare(c, g, n * 4);
+      for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, n * 4);
+      // after three rounds: v in pin (= P2), perm in qin (= q1); scatter back into P
+      hipLaunchKernelGGL(k_open_scatter_pad<L>, dim3(grid_flat), dim3(256), 0, c->stream, n, qin,
+                         pin, P, err);
+    }
+    free_perm(pi);
+    ORQG_HIP(hipGetLastError());
+  }
+  uint32_t herr = 0;
+  ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  if (herr) throw Error(ORQG_E_PARITY, "radix_sort: opened bit permutation out of range");
+}
+
+}  // namespace
+
+void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out) {
+  const uint64_t n = v->n;
+  const bool desc = order == ORQG_DESC;
+  const bool full_width = bits == 8 * v->width;
+  if (n == 0) return;
+  // 1 permutation + (bits + 1) pairs; pairs are single permutations in 3PC
+  // (radixsort.h:121-128, permutation_manager.h:108-111)
+  reserve_perms(c, n, (size_t)bits + 2);
+  const int L = v->width == 8 ? 3 : 2;
+  uint32_t *P = (uint32_t *)c->scratch_get("rs_P", n * 3 * L * 4 + 64);
+  const unsigned grid_flat = grid_for(n, 256, c->num_cu * 16);
+  if (L == 3) {
+    hipLaunchKernelGGL((k_pad<3, uint64_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc,
+                       (const uint64_t *)v->d, P);
+    sort_body<3>(c, P, n, bits, full_width);
+  } else {
+    hipLaunchKernelGGL((k_pad<2, uint32_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc,
+                       (const uint32_t *)v->d, P);
+    sort_body<2>(c, P, n, bits, full_width);
+  }
+  // remove_padding
+  uint32_t *pb = (uint32_t *)perm_out->d;
+  if (L == 3)
+    hipLaunchKernelGGL((k_unpad<3, uint64_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc, P,
+                       (uint64_t *)v->d, pb);
+  else
+    hipLaunchKernelGGL((k_unpad<2, uint32_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc, P,
+                       (uint32_t *)v->d, pb);
+  ORQG_HIP(hipGetLastError());
+  TimedRegion tr(c, "sort_tail");
+  // ElementwisePermutation::b2a (permutation.h:222-258)
+  {
+    ShardedPerm pi = next_perm(c, n);
+    orqg_vec cp = *perm_out;
+    cp.enc = ORQG_BSHARED;
+    cp.d = c->scratch_get("rs_b2a_copy", n * 12 + 64);
+    ORQG_HIP(hipMemcpyAsync(cp.d, perm_out->d, n * 12, hipMemcpyDeviceToDevice, c->stream));
+    apply_sharded(c, &cp, pi, false);
+    for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, n * 4);
+    hipLaunchKernelGGL(k_open_public, dim3(grid_flat), dim3(256), 0, c->stream, n, false,
+                       (const uint32_t *)cp.d, (uint32_t *)perm_out->d);
+    perm_out->enc = ORQG_ASHARED;
+    apply_sharded(c, perm_out, pi, true);
+    free_perm(pi);
+  }
+  if (desc) {
+    // ElementwisePermutation::negate (permutation.h:268-280): p = p * (-1) + (-1)
+    orqg_vec m1 = *perm_out;
+    m1.d = c->scratch_get("rs_m1", n * 12 + 64);
+    std::vector<uint32_t> host(3 * n, 0);
+    for (uint64_t i = 0; i < n; i++) host[3 * i] = 0xffffffffu;
+    ORQG_HIP(hipMemcpyAsync(m1.d, host.data(), n * 12, hipMemcpyHostToDevice, c->stream));
+    mult(c, true, perm_out, &m1, perm_out);
+    hipLaunchKernelGGL(k_add_public_m1, dim3(grid_flat), dim3(256), 0, c->stream, n,
+                       (uint32_t *)perm_out->d);
+    ORQG_HIP(hipStreamSynchronize(c->stream));
+  }
+  // ElementwisePermutation::invert (permutation.h:205-212)
+  {
+    orqg_vec inv = *perm_out;
+    inv.enc = ORQG_ASHARED;
+    inv.d = c->scratch_get("rs_inv", n * 12 + 64);
+    hipLaunchKernelGGL(k_iota_public, dim3(grid_flat), dim3(256), 0, c->stream, n,
+                       (uint32_t *)inv.d);
+    apply_elementwise(c, &inv, perm_out);
+    ORQG_HIP(hipMemcpyAsync(perm_out->d, inv.d, n * 12, hipMemcpyDeviceToDevice, c->stream));
+  }
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace orqg

@@ -1,0 +1,248 @@
+"""GPU parity: liborqg (HIP, gfx950) against the CPU oracle on the same keys.
+
+Every test drives the same call sequence through the oracle (per-party
+simulation) and through the C ABI of the GPU library, then compares
+  * the three distinct shares bit for bit,
+  * the PRG nonce of every key afterwards,
+  * the message log (op, from, to, bytes),
+and the reconstructed values against plaintext closed forms.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle
+
+    return oracle
+
+
+@pytest.fixture(scope="module")
+def G():
+    import orq_amd
+
+    orq_amd.lib()  # fails loudly when the HIP library is missing
+    return orq_amd
+
+
+def _pair(O, G, keys, batch_size=-12):
+    return O.Ctx(keys, batch_size), G.Ctx(keys, batch_size)
+
+
+def _same_shares(ov, gv):
+    os_, gs = ov.shares(), gv.shares()
+    for k in range(3):
+        np.testing.assert_array_equal(os_[k], gs[k], err_msg=f"share {k} differs")
+    # party p's second share in the oracle is s_{p+1}
+    for p in range(3):
+        np.testing.assert_array_equal(ov.get(p, 1), gs[(p + 1) % 3])
+
+
+def _same_nonces(oc, gc):
+    # oracle roles per party: 0 local, 1 prev, 2 next, 3+g group g
+    for p in range(3):
+        assert oc.nonce(p, 0) == gc.nonce(p), f"local key {p}"
+        assert oc.nonce(p, 2) == gc.nonce(3 + p), f"pair key {p}"
+        assert oc.nonce(p, 1) == gc.nonce(3 + (p + 2) % 3), f"prev key of party {p}"
+    for g in range(3):
+        members = [p for p in range(3) if p != 2 - g]
+        for p in members:
+            assert oc.nonce(p, 3 + g) == gc.nonce(6 + g), f"group {g} seen by party {p}"
+
+
+def _same_log(oc, gc):
+    assert oc.msglog() == gc.msglog()
+
+
+def test_keystream_kat(G):
+    """AES-GCM spec test case 14 (K = 0^32, IV = 0^12): first block of the
+    keystream; second vector K = 00..1f, nonce 5 (libsodium == OpenSSL)."""
+    keys = bytes(32) + bytes(range(32)) + bytes(32 * 8)
+    c = G.Ctx(keys)
+    ks = c.keystream(0, 32)
+    assert ks.tobytes().hex() == ("cea7403d4d606b6e074ec5d3baf39d18"
+                                  "726003ca37a62a74d1a2f58e7506358e")
+    assert c.nonce(0) == 1
+    for _ in range(5):
+        c.keystream(1, 16)
+    assert c.nonce(1) == 5
+    ks = c.keystream(1, 32)
+    assert ks.tobytes().hex() == ("e66badc133a753d9b48a5d04097e24b2"
+                                  "25f5cbbe1d147d782ea91513b5c8400c")
+
+
+@pytest.mark.parametrize("nbytes", [1, 17, (1 << 20) + 5, 3 * (1 << 20)])
+def test_keystream_matches_oracle(O, G, keys, nbytes):
+    c = G.Ctx(keys)
+    got = c.keystream(4, nbytes)
+    want, nxt = O.keystream(keys[4 * 32:5 * 32], 0, nbytes)
+    np.testing.assert_array_equal(got, want)
+    assert c.nonce(4) == nxt
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("n", [1, 257, 5000, 70001])
+def test_share_and_mult(O, G, keys, width, n):
+    rng = np.random.default_rng(n + width)
+    dt = np.uint32 if width == 4 else np.uint64
+    hi = 2**32 if width == 4 else 2**63
+    x = rng.integers(0, hi, n, dtype=np.uint64).astype(dt)
+    y = rng.integers(0, hi, n, dtype=np.uint64).astype(dt)
+    oc, gc = _pair(O, G, keys)
+    ox, gx = oc.secret_share(x, width), gc.secret_share(x, width)
+    oy, gy = oc.secret_share(y, width), gc.secret_share(y, width)
+    _same_shares(ox, gx)
+    oz, gz = oc.and_b(ox, oy), gc.and_b(gx, gy)
+    _same_shares(oz, gz)
+    np.testing.assert_array_equal(gc.open(gz), x & y)
+    np.testing.assert_array_equal(oc.open(oz), x & y)
+    oa, ga = oc.secret_share(x, width, arith=True), gc.secret_share(x, width, arith=True)
+    ob, gb = oc.secret_share(y, width, arith=True), gc.secret_share(y, width, arith=True)
+    om, gm = oc.mul_a(oa, ob), gc.mul_a(ga, gb)
+    _same_shares(om, gm)
+    with np.errstate(over="ignore"):
+        np.testing.assert_array_equal(gc.open(gm), (x * y).astype(dt))
+        np.testing.assert_array_equal(oc.open(om, arith=True), (x * y).astype(dt))
+    obb, gbb = oc.b2a_bit(ox), gc.b2a_bit(gx)
+    _same_shares(obb, gbb)
+    np.testing.assert_array_equal(gc.open(gbb), (x & 1).astype(dt))
+    np.testing.assert_array_equal(oc.open(obb, arith=True), (x & 1).astype(dt))
+    _same_nonces(oc, gc)
+    _same_log(oc, gc)
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_reshare(O, G, keys, width):
+    n = 3001
+    rng = np.random.default_rng(7)
+    dt = np.uint32 if width == 4 else np.uint64
+    x = rng.integers(0, 2**31, n).astype(dt)
+    oc, gc = _pair(O, G, keys)
+    ov, gv = oc.secret_share(x, width), gc.secret_share(x, width)
+    before = gv.shares()
+    for g in range(3):
+        oc.reshare(ov, g)
+        gc.reshare(gv, g)
+        _same_shares(ov, gv)
+    after = gv.shares()
+    assert not np.array_equal(before[0], after[0])
+    np.testing.assert_array_equal(gc.open(gv), x)
+    _same_nonces(oc, gc)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 1000, 4099])
+def test_gen_perm_matches_oracle(O, G, keys, n):
+    oc, gc = _pair(O, G, keys)
+    count = 3
+    oc.reserve(n, count)
+    gc.reserve(n, count)
+    for k in range(count):
+        for g in range(3):
+            members = [p for p in range(3) if p != 2 - g]
+            want = oc.peek_perm(k, members[0], g, n)
+            np.testing.assert_array_equal(oc.peek_perm(k, members[1], g, n), want)
+            got = gc.peek_perm(k, g, n)
+            np.testing.assert_array_equal(got, want, err_msg=f"perm {k} group {g}")
+            assert np.array_equal(np.sort(got), np.arange(n))
+    _same_nonces(oc, gc)
+
+
+def test_gen_perm_large(O, G, keys):
+    """2^17 rows: many windows per permutation, the tail with rare acceptances."""
+    n = 1 << 17
+    oc, gc = _pair(O, G, keys)
+    oc.reserve(n, 1)
+    gc.reserve(n, 1)
+    for g in range(3):
+        members = [p for p in range(3) if p != 2 - g]
+        np.testing.assert_array_equal(gc.peek_perm(0, g, n), oc.peek_perm(0, members[0], g, n))
+    _same_nonces(oc, gc)
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_apply_sharded(O, G, keys, width, inverse):
+    n = 2049
+    rng = np.random.default_rng(11)
+    dt = np.uint32 if width == 4 else np.uint64
+    x = rng.integers(0, 2**31, n).astype(dt)
+    oc, gc = _pair(O, G, keys)
+    ov, gv = oc.secret_share(x, width), gc.secret_share(x, width)
+    oc.apply_sharded(ov, arith=False, inverse=inverse)
+    gc.apply_sharded(gv, inverse=inverse)
+    _same_shares(ov, gv)
+    _same_nonces(oc, gc)
+    _same_log(oc, gc)
+
+
+def test_apply_elementwise(O, G, keys):
+    n = 1500
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 2**63, n, dtype=np.uint64)
+    sigma = rng.permutation(n).astype(np.uint32)
+    oc, gc = _pair(O, G, keys)
+    ov, gv = oc.secret_share(x, 8), gc.secret_share(x, 8)
+    op, gp = oc.secret_share(sigma, 4, arith=True), gc.secret_share(sigma, 4, arith=True)
+    oc.apply_elementwise(ov, op, x_arith=False, perm_arith=True)
+    gc.apply_elementwise(gv, gp)
+    _same_shares(ov, gv)
+    want = np.empty_like(x)
+    want[sigma] = x
+    _same_nonces(oc, gc)
+    _same_log(oc, gc)
+    np.testing.assert_array_equal(gc.open(gv), want)
+
+
+def _sort_case(O, G, keys, keys_plain, width, desc):
+    n = keys_plain.size
+    oc, gc = _pair(O, G, keys)
+    ov, gv = oc.secret_share(keys_plain, width), gc.secret_share(keys_plain, width)
+    operm = oc.radix_sort(ov, desc=desc)
+    gperm = gc.radix_sort(gv, desc=desc)
+    _same_shares(ov, gv)
+    _same_shares(operm, gperm)
+    _same_nonces(oc, gc)
+    _same_log(oc, gc)
+    assert gc.queue_size() == 0
+    sdt = np.int32 if width == 4 else np.int64
+    signed = keys_plain.view(sdt)
+    # ASC and DESC are both stable w.r.t. the original index (index padding:
+    # sorting.h:431-462); ~x reverses signed order without overflow
+    order = np.argsort(~signed if desc else signed, kind="stable")
+    got = gc.open(gv).view(sdt)
+    np.testing.assert_array_equal(got, signed[order])
+    perm = gc.open(gperm).view(np.int32)
+    ranks = np.empty(n, dtype=np.int64)
+    ranks[order] = np.arange(n)
+    np.testing.assert_array_equal(perm, ranks)
+
+
+@pytest.mark.parametrize("desc", [False, True])
+def test_radix_sort_int32(O, G, keys, desc):
+    rng = np.random.default_rng(1)
+    k = rng.integers(0, 2**32, 1000, dtype=np.uint64).astype(np.uint32)
+    k[500] = 0xFFFFFFFF  # -1
+    k[10] = 0x80000000   # INT_MIN
+    k[11] = 0x7FFFFFFF   # INT_MAX
+    k[20:30] = k[40]     # duplicates
+    _sort_case(O, G, keys, k, 4, desc)
+
+
+@pytest.mark.parametrize("desc", [False, True])
+def test_radix_sort_int64(O, G, keys, desc):
+    rng = np.random.default_rng(2)
+    k = rng.integers(-2**63, 2**63 - 1, 777, dtype=np.int64).view(np.uint64)
+    k[3] = np.uint64(1 << 63)
+    k[4] = np.uint64((1 << 63) - 1)
+    k[100:140] = k[7]
+    _sort_case(O, G, keys, k, 8, desc)
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 4097])
+def test_radix_sort_sizes(O, G, keys, n):
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 50, n).astype(np.uint32)
+    _sort_case(O, G, keys, k, 4, False)
