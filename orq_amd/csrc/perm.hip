@@ -40,9 +40,10 @@ namespace orqg {
 
 namespace {
 
-constexpr int kTileThreads = 256;
-constexpr int kDrawsPerThread = 8;                          // 2 AES blocks
-constexpr uint32_t kTile = kTileThreads * kDrawsPerThread;  // 2048 draws
+constexpr int kTileThreads = 512;
+constexpr int kDrawsPerThread = 16;                         // 4 AES blocks
+constexpr uint32_t kTile = kTileThreads * kDrawsPerThread;  // 8192 draws
+constexpr int kGpCopies = 16;                               // AES table replication
 constexpr uint32_t kWinMax = 1u << 24;                      // draws per window
 constexpr uint32_t kMaxTiles = kWinMax / kTile;
 constexpr uint32_t kAmbCap = 8192;
@@ -66,6 +67,8 @@ struct ChainState {
   uint32_t overflow;
   uint32_t done;
   uint32_t err;
+  // what k_gp_commit writes for the window just resolved
+  uint32_t c_valid, c_i0, c_k, c_C, c_R;
 };
 
 struct ChainBufs {
@@ -79,12 +82,14 @@ struct ChainBufs {
   uint64_t *nonce_at;  // [3][count+1]   nonce at the start of each permutation
 };
 
-__device__ inline uint32_t choose_window(uint32_t k, uint32_t maskN) {
-  // expected ambiguous items ~ W^2 k / (2 maskN^2); aim for ~2K of kAmbCap.
-  double w = 64.0 * (double)maskN / sqrt((double)(k ? k : 1));
-  uint32_t W = kTile;
-  while ((double)W * 2 <= w && W < kWinMax) W *= 2;
-  return W;
+// Window size for k open positions: the expected number of ambiguous
+// candidates is about W^2 k / (2 maskN^2); 90 maskN / sqrt(k) keeps it near
+// half of kAmbCap.  An overflowing window is retried at a quarter size.
+__host__ __device__ inline uint32_t choose_window(uint32_t k, uint32_t maskN) {
+  double w = 90.0 * (double)maskN / sqrt((double)(k ? k : 1));
+  if (w >= (double)kWinMax) return kWinMax;
+  uint32_t W = (uint32_t)(w / kTile) * kTile;
+  return W < kTile ? kTile : W;
 }
 
 // status word: [63:62] state (see the look-back in k_gp_window), [61:31]
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gp_window(const uint32_t *__re
     const uint64_t dbase = d0 + (uint64_t)tile * kTile + (uint64_t)threadIdx.x * kDrawsPerThread;
     uint32_t v[kDrawsPerThread];
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < kDrawsPerThread / 4; b++) {
       uint64_t d = dbase + 4 * b;
       uint4 blk = aes_ctr_block<COPIES>(A, rk, nonce + d / kWordsPerNonce,
                                         2 + (uint32_t)((d % kWordsPerNonce) / 4));
@@ -304,7 +309,8 @@ __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t *s_w, uint32_t &
   return base + inc - x;
 }
 
-__global__ void __launch_bounds__(kResolveThreads) k_gp_resolve(ChainBufs B, uint32_t n) {
+__global__ void __launch_bounds__(kResolveThreads) k_gp_resolve(ChainBufs B, uint32_t n,
+                                                                uint32_t count, uint32_t maskN) {
   constexpr int PER = kAmbCap / kResolveThreads;  // items per thread (8)
   __shared__ uint16_t s_idx[kAmbCap];
   __shared__ int32_t s_thr[kAmbCap];
@@ -313,7 +319,10 @@ __global__ void __launch_bounds__(kResolveThreads) k_gp_resolve(ChainBufs B, uin
   __shared__ uint32_t s_cnt;
   const int chain = blockIdx.x;
   ChainState *st = B.st + chain;
-  if (st->done) return;
+  if (st->done) {
+    if (threadIdx.x == 0) st->c_valid = 0;
+    return;
+  }
   const uint32_t m = st->m;
   const uint32_t *dummy = nullptr;
   (void)dummy;
@@ -321,8 +330,18 @@ __global__ void __launch_bounds__(kResolveThreads) k_gp_resolve(ChainBufs B, uin
   uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
   uint64_t *cand = B.cand + (size_t)chain * kWinMax;
   const uint32_t kstart = n - st->i;
+  uint64_t *status = B.status + (size_t)chain * kMaxTiles;
+  const uint32_t used_tiles = st->W / kTile;
   if (m > kAmbCap) {
-    if (threadIdx.x == 0) st->overflow = 1;
+    // too many ambiguous candidates: retry the same window at a quarter size
+    for (uint32_t t = threadIdx.x; t < used_tiles; t += blockDim.x) status[t] = 0;
+    if (threadIdx.x == 0) {
+      st->c_valid = 0;
+      st->W = st->W / 4 >= kTile ? (st->W / 4) / kTile * kTile : kTile;
+      st->ticket = 0;
+      st->C = 0;
+      st->m = 0;
+    }
     return;
   }
   // initial list: all ambiguous items, type B thresholds U = a - (k - v)
@@ -413,23 +432,54 @@ __global__ void __launch_bounds__(kResolveThreads) k_gp_resolve(ChainBufs B, uin
     uint32_t j = beg + q;
     if (j < m && s_dec[j] == 2) rej[pos++] = (uint32_t)(amb[j] >> 32);
   }
+  for (uint32_t t = threadIdx.x; t < used_tiles; t += blockDim.x) status[t] = 0;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const uint32_t C = st->C;
-    st->R = tot;
-    st->acc = C - tot;
-    st->ended = (st->acc >= kstart) ? 1u : 0u;
-    if (st->acc > kstart) atomicOr(&st->err, 4u);
-    s_cnt = tot;
+    const uint32_t C = st->C, R = tot, acc = C - tot;
+    if (acc > kstart) atomicOr(&st->err, 4u);
+    st->c_valid = 1;
+    st->c_i0 = st->i;
+    st->c_k = st->k;
+    st->c_C = C;
+    st->c_R = R;
+    if (acc >= kstart) {
+      // the permutation ends inside this window: find the candidate rank of
+      // the kstart-th acceptance, a* = kstart-1 shifted past every rejected
+      // rank at or before it
+      uint32_t a = kstart - 1;
+      for (uint32_t q = 0; q < R; q++) {
+        if (rej[q] <= a) a++;
+        else break;
+      }
+      const uint64_t D_end = st->d0 + (cand[a] >> 32) + 1;
+      const uint32_t k = st->k;
+      B.D[(size_t)chain * count + k] = D_end;
+      // gen_perm refills 4 nonces per 2^20 draws, eagerly
+      // (hm_sharded_permutation_generator.h:93-102)
+      st->nonce += 4 * (1 + D_end / kRefillWords);
+      st->k = k + 1;
+      st->i = 0;
+      st->d0 = 0;
+      B.nonce_at[(size_t)chain * (count + 1) + k + 1] = st->nonce;
+      if (st->k >= count) st->done = 1;
+      st->W = choose_window(n, maskN);
+    } else {
+      st->i += acc;
+      st->d0 += st->W;
+      st->W = choose_window(n - st->i, maskN);
+    }
+    st->ticket = 0;
+    st->C = 0;
+    st->m = 0;
   }
-  (void)cand;
 }
 
 // Write r_i for accepted candidates; locate the draw that finished the permutation.
 __global__ void __launch_bounds__(256) k_gp_commit(ChainBufs B, uint32_t n, uint32_t count) {
   const int chain = blockIdx.y;
-  ChainState *st = B.st + chain;
-  if (st->done || st->overflow) return;
-  const uint32_t C = st->C, R = st->R, i0 = st->i, k = st->k;
+  const ChainState *st = B.st + chain;
+  if (!st->c_valid) return;
+  const uint32_t C = st->c_C, R = st->c_R, i0 = st->c_i0, k = st->c_k;
   const uint32_t kstart = n - i0;
   const uint64_t *cand = B.cand + (size_t)chain * kWinMax;
   const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
@@ -445,51 +495,8 @@ __global__ void __launch_bounds__(256) k_gp_commit(ChainBufs B, uint32_t n, uint
     if (lo < R && rej[lo] == a) continue;
     uint32_t accr = a - lo;
     if (accr >= kstart) continue;  // cannot happen for an exact resolution
-    uint64_t e = cand[a];
-    r[i0 + accr] = (uint32_t)e;
-    if (accr == kstart - 1) st->D_end = st->d0 + (e >> 32) + 1;
+    r[i0 + accr] = (uint32_t)cand[a];
   }
-}
-
-__global__ void __launch_bounds__(256) k_gp_advance(ChainBufs B, uint32_t n, uint32_t count,
-                                                    uint32_t maskN) {
-  for (int chain = 0; chain < 3; chain++) {
-    ChainState *st = B.st + chain;
-    if (st->done) continue;
-    uint64_t *status = B.status + (size_t)chain * kMaxTiles;
-    uint32_t used_tiles = st->W / kTile;
-    for (uint32_t t = threadIdx.x; t < used_tiles; t += blockDim.x) status[t] = 0;
-  }
-  __syncthreads();
-  const int chain = threadIdx.x;
-  if (chain >= 3) return;
-  ChainState *st = B.st + chain;
-  if (st->done) return;
-  if (st->overflow) {
-    st->W = st->W / 4 >= kTile ? st->W / 4 : kTile;
-    st->overflow = 0;
-  } else if (st->ended) {
-    const uint32_t k = st->k;
-    B.D[(size_t)chain * count + k] = st->D_end;
-    // gen_perm refills 4 nonces per 2^20 draws, eagerly (hm_sharded_permutation_generator.h:93-102)
-    st->nonce += 4 * (1 + st->D_end / kRefillWords);
-    st->k = k + 1;
-    st->i = 0;
-    st->d0 = 0;
-    B.nonce_at[(size_t)chain * (count + 1) + k + 1] = st->nonce;
-    if (st->k >= count) st->done = 1;
-    st->W = choose_window(n, maskN);
-  } else {
-    st->i += st->acc;
-    st->d0 += st->W;
-    st->W = choose_window(n - st->i, maskN);
-  }
-  st->ticket = 0;
-  st->C = 0;
-  st->m = 0;
-  st->R = 0;
-  st->acc = 0;
-  st->ended = 0;
 }
 
 __global__ void k_gp_done(ChainBufs B, uint32_t *flag) {
@@ -501,59 +508,62 @@ __global__ void k_gp_done(ChainBufs B, uint32_t *flag) {
 }
 
 // ---- swap resolution ----------------------------------------------------------
-__global__ void k_fy_count(uint32_t n, const uint32_t *__restrict__ r, uint32_t *__restrict__ cnt) {
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    uint32_t t = j + r[j];
-    if (t < n) atomicAdd(&cnt[t], 1u);
-  }
-}
-
-__global__ void k_fy_fill(uint32_t n, const uint32_t *__restrict__ r, const uint32_t *__restrict__ off,
-                          uint32_t *__restrict__ cnt, uint32_t *__restrict__ bucket) {
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    uint32_t t = j + r[j];
-    if (t < n) {
-      uint32_t slot = atomicSub(&cnt[t], 1u) - 1u;
-      bucket[off[t] + slot] = j;
-    }
-  }
-}
-
-// largest step index < lim in bucket of position pos, or -1
-__device__ inline int64_t fy_last_before(const uint32_t *__restrict__ off,
-                                         const uint32_t *__restrict__ bucket, uint32_t pos,
-                                         uint32_t lim) {
-  int64_t best = -1;
-  for (uint32_t q = off[pos]; q < off[pos + 1]; q++) {
-    uint32_t s = bucket[q];
-    if (s < lim && (int64_t)s > best) best = s;
-  }
-  return best;
-}
-
-__global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r, const uint32_t *__restrict__ off,
-                          const uint32_t *__restrict__ bucket, int32_t *__restrict__ fwd,
-                          int32_t *__restrict__ inv, uint32_t *__restrict__ err) {
+// Step j of Fisher-Yates swaps positions j and t_j = j + r_j.  Steps are
+// chained per target position (head[t] -> most recently linked step ->
+// next[...]); list order is arbitrary, the walks below take maxima.
+__global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r_all, size_t chain_stride,
+                          int32_t *__restrict__ head, int32_t *__restrict__ next,
+                          uint32_t *__restrict__ err) {
+  const uint32_t *r = r_all + blockIdx.y * chain_stride;
+  int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     uint32_t t = j + r[j];
     if (t >= n) {
       atomicOr(err, 8u);
+      nx[j] = -1;
       continue;
     }
-    // value at position t just before step j
-    int64_t q = fy_last_before(off, bucket, t, j);
-    uint32_t val;
-    if (q < 0) {
-      val = t;
-    } else {
-      // V(m): value at position m just before step m
-      uint32_t mpos = (uint32_t)q;
+    nx[j] = atomicExch(&hd[t], (int32_t)j);
+  }
+}
+
+// largest step index < lim whose target is pos, or -1
+__device__ inline int32_t fy_last_before(const int32_t *__restrict__ hd, const int32_t *__restrict__ nx,
+                                         uint32_t pos, uint32_t lim, uint32_t n) {
+  int32_t best = -1;
+  int32_t q = hd[pos];
+  for (uint32_t guard = 0; q >= 0 && guard < n; guard++) {
+    if ((uint32_t)q < lim && q > best) best = q;
+    q = nx[q];
+  }
+  return best;
+}
+
+// Position j is final after step j; it receives what position t_j held just
+// before step j: V(q) for the last earlier step q that targeted t_j, or t_j
+// itself.  V(m), the value at position m just before step m, is V(p) for the
+// last step p < m that targeted m, or m.
+__global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r_all, size_t chain_stride,
+                          const int32_t *__restrict__ head, const int32_t *__restrict__ next,
+                          int32_t *fwd0, int32_t *fwd1, int32_t *fwd2, int32_t *inv0, int32_t *inv1,
+                          int32_t *inv2) {
+  const uint32_t *r = r_all + blockIdx.y * chain_stride;
+  const int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
+  int32_t *fwd = blockIdx.y == 0 ? fwd0 : blockIdx.y == 1 ? fwd1 : fwd2;
+  int32_t *inv = blockIdx.y == 0 ? inv0 : blockIdx.y == 1 ? inv1 : inv2;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t t = j + r[j];
+    if (t >= n) continue;  // flagged by k_fy_link
+    int32_t q = fy_last_before(hd, nx, t, j, n);
+    uint32_t val = t;
+    if (q >= 0) {
+      uint32_t m = (uint32_t)q;
       for (uint32_t guard = 0; guard < n; guard++) {
-        int64_t p = fy_last_before(off, bucket, mpos, mpos);
+        int32_t p = fy_last_before(hd, nx, m, m, n);
         if (p < 0) break;
-        mpos = (uint32_t)p;
+        m = (uint32_t)p;
       }
-      val = mpos;
+      val = m;
     }
     fwd[j] = (int32_t)val;
     inv[val] = (int32_t)j;
@@ -686,27 +696,24 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   std::vector<uint64_t> nat(3ull * (count + 1), 0);
   for (int g = 0; g < 3; g++) {
     init[g].nonce = c->nonce[kGroup0 + g];
-    double w = 64.0 * (double)maskN / std::sqrt((double)n);
-    uint32_t W = kTile;
-    while ((double)W * 2 <= w && W < kWinMax) W *= 2;
-    init[g].W = W;
+    init[g].W = choose_window(n, maskN);
     nat[(size_t)g * (count + 1)] = init[g].nonce;
   }
   ORQG_HIP(hipMemcpyAsync(B.st, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   ORQG_HIP(hipMemcpyAsync(B.nonce_at, nat.data(), nat.size() * 8, hipMemcpyHostToDevice, c->stream));
   ORQG_HIP(hipMemsetAsync(B.status, 0, 3ull * kMaxTiles * 8, c->stream));
 
-  const unsigned grid_win = std::max(1, c->num_cu);
+  const unsigned grid_win = std::max(1, (2 * c->num_cu) / 3);  // 2 blocks/CU over 3 chains
   const unsigned grid_commit = std::max(1, c->num_cu);
   uint32_t host_flag[2] = {0, 0};
   const int kBatch = 16;
   for (uint64_t it = 0;; it++) {
     for (int w = 0; w < kBatch; w++) {
-      hipLaunchKernelGGL(k_gp_window<kCopies>, dim3(grid_win, 3), dim3(kTileThreads), 0, c->stream,
-                         c->d_t0, c->rk(kGroup0), B, n, mask);
-      hipLaunchKernelGGL(k_gp_resolve, dim3(3), dim3(kResolveThreads), 0, c->stream, B, n);
+      hipLaunchKernelGGL(k_gp_window<kGpCopies>, dim3(grid_win, 3), dim3(kTileThreads), 0,
+                         c->stream, c->d_t0, c->rk(kGroup0), B, n, mask);
+      hipLaunchKernelGGL(k_gp_resolve, dim3(3), dim3(kResolveThreads), 0, c->stream, B, n, count,
+                         maskN);
       hipLaunchKernelGGL(k_gp_commit, dim3(grid_commit, 3), dim3(256), 0, c->stream, B, n, count);
-      hipLaunchKernelGGL(k_gp_advance, dim3(1), dim3(256), 0, c->stream, B, n, count, maskN);
     }
     hipLaunchKernelGGL(k_gp_done, dim3(1), dim3(64), 0, c->stream, B, flag);
     ORQG_HIP(hipGetLastError());
@@ -721,23 +728,22 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   ORQG_HIP(hipStreamSynchronize(c->stream));
   for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] = fin[g].nonce;
 
-  // swap resolution, one local permutation at a time
-  uint32_t *cnt = (uint32_t *)c->scratch_get("fy_cnt", ((size_t)n + 2) * 4);
-  uint32_t *off = (uint32_t *)c->scratch_get("fy_off", ((size_t)n + 2) * 4);
-  uint32_t *bucket = (uint32_t *)c->scratch_get("fy_bucket", ((size_t)n + 1) * 4);
+  // swap resolution: the three local permutations of one sharded permutation per launch
+  int32_t *head = (int32_t *)c->scratch_get("fy_head", 3ull * n * 4 + 64);
+  int32_t *next = (int32_t *)c->scratch_get("fy_next", 3ull * n * 4 + 64);
   uint32_t *err = (uint32_t *)c->scratch_get("fy_err", 16);
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
-  unsigned grid = grid_for(n, 256, c->num_cu * 16);
-  for (uint32_t k = 0; k < count; k++)
-    for (int g = 0; g < 3; g++) {
-      const uint32_t *r = B.r_out + ((size_t)g * count + k) * (size_t)n;
-      ORQG_HIP(hipMemsetAsync(cnt, 0, ((size_t)n + 1) * 4, c->stream));
-      hipLaunchKernelGGL(k_fy_count, dim3(grid), dim3(256), 0, c->stream, n, r, cnt);
-      exclusive_scan_u32(c, cnt, off, (uint64_t)n + 1);
-      hipLaunchKernelGGL(k_fy_fill, dim3(grid), dim3(256), 0, c->stream, n, r, off, cnt, bucket);
-      hipLaunchKernelGGL(k_fy_walk, dim3(grid), dim3(256), 0, c->stream, n, r, off, bucket,
-                         fresh[k].fwd[g], fresh[k].inv[g], err);
-    }
+  unsigned grid = grid_for(n, 256, c->num_cu * 8);
+  const size_t chain_stride = (size_t)count * n;
+  for (uint32_t k = 0; k < count; k++) {
+    const uint32_t *r = B.r_out + (size_t)k * n;
+    ORQG_HIP(hipMemsetAsync(head, 0xff, 3ull * n * 4, c->stream));
+    hipLaunchKernelGGL(k_fy_link, dim3(grid, 3), dim3(256), 0, c->stream, n, r, chain_stride, head,
+                       next, err);
+    hipLaunchKernelGGL(k_fy_walk, dim3(grid, 3), dim3(256), 0, c->stream, n, r, chain_stride, head,
+                       next, fresh[k].fwd[0], fresh[k].fwd[1], fresh[k].fwd[2], fresh[k].inv[0],
+                       fresh[k].inv[1], fresh[k].inv[2]);
+  }
   ORQG_HIP(hipGetLastError());
   uint32_t herr = 0;
   ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
