@@ -11,7 +11,10 @@
 // with T0[x] = (2S, S, S, 3S) and T1..T3 its byte rotations (FIPS-197 5.1,
 // table form).  The tables live in LDS, replicated COPIES times so that lane l
 // reads copy (l % COPIES): with 32 copies every lane of a half-wave hits its
-// own bank and table lookups are conflict-free.
+// own bank and table lookups are conflict-free.  The 128 KiB that takes leaves
+// room for one workgroup per CU, so AES kernels run 1024-thread persistent
+// workgroups (measured: ~1 TB/s of keystream on MI355X, against ~0.7 TB/s
+// with 16 copies and ~0.5 TB/s with 8).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -78,17 +81,28 @@ struct AesLds {
   static constexpr int kBytes = kWords * 4;
 };
 
+// Host-built LDS image of the replicated tables (word layout above).
+inline void aes_make_lds_image(const AesTables &t, int copies, uint32_t *img) {
+  for (int j = 0; j < 4; j++)
+    for (int x = 0; x < 256; x++) {
+      uint32_t v = t.t0[x];
+      uint32_t r = j == 0 ? v : ((v << (8 * j)) | (v >> (32 - 8 * j)));
+      for (int c = 0; c < copies; c++) img[(j * 256 + x) * copies + c] = r;
+    }
+}
+
+// Copy the table image into LDS with 16-byte loads (lds must be 16-byte aligned).
 template <int COPIES>
-__device__ inline void aes_load_tables(uint32_t *lds, const uint32_t *__restrict__ g_t0) {
-  for (int w = threadIdx.x; w < 4 * 256 * COPIES; w += blockDim.x) {
-    int c = w % COPIES;
-    int x = (w / COPIES) & 255;
-    int j = w / (256 * COPIES);
-    uint32_t v = g_t0[x];
-    (void)c;
-    lds[w] = j == 0 ? v : ((v << (8 * j)) | (v >> (32 - 8 * j)));
-  }
+__device__ inline void aes_load_tables(uint32_t *lds, const uint32_t *__restrict__ g_img) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(g_img);
+  uint4 *dst = reinterpret_cast<uint4 *>(lds);
+  for (int w = threadIdx.x; w < AesLds<COPIES>::kWords / 4; w += blockDim.x) dst[w] = src[w];
   __syncthreads();
+}
+
+// three-input XOR in one VALU op (gfx950 v_bitop3_b32, truth table 0x96)
+__device__ inline uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
 template <int COPIES>
@@ -105,14 +119,14 @@ __device__ inline void aes256_encrypt(const AesCtx<COPIES> &A, const uint32_t *_
   uint32_t s0 = w0 ^ rk[0], s1 = w1 ^ rk[1], s2 = w2 ^ rk[2], s3 = w3 ^ rk[3];
 #pragma unroll
   for (int r = 1; r < 14; r++) {
-    uint32_t t0 = A.T(0, s0 & 255) ^ A.T(1, (s1 >> 8) & 255) ^ A.T(2, (s2 >> 16) & 255) ^
-                  A.T(3, s3 >> 24) ^ rk[4 * r + 0];
-    uint32_t t1 = A.T(0, s1 & 255) ^ A.T(1, (s2 >> 8) & 255) ^ A.T(2, (s3 >> 16) & 255) ^
-                  A.T(3, s0 >> 24) ^ rk[4 * r + 1];
-    uint32_t t2 = A.T(0, s2 & 255) ^ A.T(1, (s3 >> 8) & 255) ^ A.T(2, (s0 >> 16) & 255) ^
-                  A.T(3, s1 >> 24) ^ rk[4 * r + 2];
-    uint32_t t3 = A.T(0, s3 & 255) ^ A.T(1, (s0 >> 8) & 255) ^ A.T(2, (s1 >> 16) & 255) ^
-                  A.T(3, s2 >> 24) ^ rk[4 * r + 3];
+    uint32_t t0 = xor3(xor3(A.T(0, s0 & 255), A.T(1, (s1 >> 8) & 255), A.T(2, (s2 >> 16) & 255)),
+                       A.T(3, s3 >> 24), rk[4 * r + 0]);
+    uint32_t t1 = xor3(xor3(A.T(0, s1 & 255), A.T(1, (s2 >> 8) & 255), A.T(2, (s3 >> 16) & 255)),
+                       A.T(3, s0 >> 24), rk[4 * r + 1]);
+    uint32_t t2 = xor3(xor3(A.T(0, s2 & 255), A.T(1, (s3 >> 8) & 255), A.T(2, (s0 >> 16) & 255)),
+                       A.T(3, s1 >> 24), rk[4 * r + 2]);
+    uint32_t t3 = xor3(xor3(A.T(0, s3 & 255), A.T(1, (s0 >> 8) & 255), A.T(2, (s1 >> 16) & 255)),
+                       A.T(3, s2 >> 24), rk[4 * r + 3]);
     s0 = t0;
     s1 = t1;
     s2 = t2;

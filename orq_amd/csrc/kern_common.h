@@ -6,10 +6,9 @@
 namespace orqg {
 
 // LDS replication factor of the AES tables for ordinary kernels.
-#ifndef ORQG_AES_COPIES
-#define ORQG_AES_COPIES 8
-#endif
-constexpr int kCopies = ORQG_AES_COPIES;
+constexpr int kCopies = 32;       // AES table replication (conflict-free LDS lookups)
+constexpr int kAesThreads = 1024; // threads per AES workgroup (one workgroup per CU)
+#define AES_LDS __shared__ __align__(16) uint32_t lds[AesLds<kCopies>::kWords]
 
 struct NonceTriple {
   uint64_t v[3];
@@ -104,34 +103,43 @@ __device__ inline void b2a_bit_group(const AesCtx<kCopies> &A, const uint32_t *_
                                      const T (&bits)[16 / sizeof(T)][3],
                                      T (&y)[16 / sizeof(T)][3]) {
   constexpr int E = 16 / sizeof(T);
-  T R[3][3][E];  // [call][key][elem]
-#pragma unroll
+  T d[E][3];
+  // the three calls are consumed one after the other to bound register use
+#pragma unroll 1
   for (int j = 0; j < 3; j++) {
     uint64_t off;
     uint32_t ctr, bib;
     sched_locate(s, e0, j, off, ctr, bib);
+    T R[3][E];  // [key][elem]
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       uint4 blk = aes_ctr_block<kCopies>(A, rk_pair + 60 * k, nb.v[k] + off, ctr);
-      unpack_block<T>(blk, R[j][k]);
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < E; i++) {
-    T d[3];
-#pragma unroll
-    for (int p = 0; p < 3; p++) {
-      int pm = (p + 2) % 3;
-      T a = (T)(R[0][pm][i] - R[0][p][i]);
-      T b = (T)(R[1][pm][i] - R[1][p][i]);
-      if (p == 0) a += bits[i][0] ^ bits[i][1];
-      if (p == 1) b += bits[i][2];
-      d[p] = a - b;
+      unpack_block<T>(blk, R[k]);
     }
 #pragma unroll
-    for (int p = 0; p < 3; p++) {
-      int q = (p + 1) % 3, pm = (p + 2) % 3;
-      y[i][p] = (T)(R[2][pm][i] - R[2][p][i]) + d[p] * d[p] + d[p] * d[q] + d[q] * d[p];
+    for (int i = 0; i < E; i++) {
+      if (j == 0) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+          T a = (T)(R[(p + 2) % 3][i] - R[p][i]);
+          if (p == 0) a += bits[i][0] ^ bits[i][1];
+          d[i][p] = a;
+        }
+      } else if (j == 1) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+          T b = (T)(R[(p + 2) % 3][i] - R[p][i]);
+          if (p == 1) b += bits[i][2];
+          d[i][p] -= b;
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+          int q = (p + 1) % 3, pm = (p + 2) % 3;
+          y[i][p] = (T)(R[pm][i] - R[p][i]) + d[i][p] * d[i][p] + d[i][p] * d[i][q] +
+                    d[i][q] * d[i][p];
+        }
+      }
     }
   }
 }

@@ -40,11 +40,12 @@ namespace orqg {
 
 namespace {
 
-constexpr int kTileThreads = 512;
+constexpr int kTileThreads = 1024;
 constexpr int kDrawsPerThread = 16;                         // 4 AES blocks
-constexpr uint32_t kTile = kTileThreads * kDrawsPerThread;  // 8192 draws
-constexpr int kGpCopies = 16;                               // AES table replication
+constexpr uint32_t kTile = kTileThreads * kDrawsPerThread;  // 16384 draws
+constexpr int kGpCopies = kCopies;                          // AES table replication
 constexpr uint32_t kWinMax = 1u << 24;                      // draws per window
+constexpr uint32_t kWinMin = 1024;  // <= kAmbCap candidates, so a window always resolves
 constexpr uint32_t kMaxTiles = kWinMax / kTile;
 constexpr uint32_t kAmbCap = 8192;
 constexpr int kResolveThreads = 1024;
@@ -88,8 +89,8 @@ struct ChainBufs {
 __host__ __device__ inline uint32_t choose_window(uint32_t k, uint32_t maskN) {
   double w = 90.0 * (double)maskN / sqrt((double)(k ? k : 1));
   if (w >= (double)kWinMax) return kWinMax;
-  uint32_t W = (uint32_t)(w / kTile) * kTile;
-  return W < kTile ? kTile : W;
+  uint32_t W = w >= kTile ? (uint32_t)(w / kTile) * kTile : (uint32_t)(w / kWinMin) * kWinMin;
+  return W < kWinMin ? kWinMin : W;
 }
 
 // status word: [63:62] state (see the look-back in k_gp_window), [61:31]
@@ -141,12 +142,42 @@ __device__ inline uint32_t lookback(const uint64_t *status, uint32_t tile, uint3
   return acc;
 }
 
+// Draws of one thread of tile `tile`: d = d0 + tile*kTile + threadIdx*kDrawsPerThread + j.
+// Draws at window offset >= W are returned as 0xffffffff, which is never a
+// candidate (masked values are < 2^31).
+template <int COPIES>
+__device__ inline void gp_draws(const AesCtx<COPIES> &A, const uint32_t *__restrict__ rk,
+                                uint64_t nonce, uint64_t d0, uint32_t tile, uint32_t mask,
+                                uint32_t W, uint32_t (&v)[kDrawsPerThread]) {
+  const uint32_t off0 = tile * kTile + threadIdx.x * kDrawsPerThread;
+  if (off0 >= W) {
+#pragma unroll
+    for (int j = 0; j < kDrawsPerThread; j++) v[j] = 0xffffffffu;
+    return;
+  }
+  const uint64_t dbase = d0 + off0;
+#pragma unroll
+  for (int b = 0; b < kDrawsPerThread / 4; b++) {
+    uint64_t d = dbase + 4 * b;
+    uint4 blk = aes_ctr_block<COPIES>(A, rk, nonce + d / kWordsPerNonce,
+                                      2 + (uint32_t)((d % kWordsPerNonce) / 4));
+    v[4 * b + 0] = blk.x & mask;
+    v[4 * b + 1] = blk.y & mask;
+    v[4 * b + 2] = blk.z & mask;
+    v[4 * b + 3] = blk.w & mask;
+  }
+}
+
+// One window of one chain (blockIdx.y): keystream -> candidates -> ranks ->
+// sure accepts / ambiguous list.  Tiles are taken from a ticket counter; the
+// keystream of the next tile is computed before the look-back of the current
+// one so that the look-back latency overlaps AES work.
 template <int COPIES>
 __global__ void __launch_bounds__(kTileThreads) k_gp_window(const uint32_t *__restrict__ t0,
                                                             const uint32_t *__restrict__ rk_groups,
                                                             ChainBufs B, uint32_t n,
                                                             uint32_t mask) {
-  __shared__ uint32_t lds[AesLds<COPIES>::kWords];
+  __shared__ __align__(16) uint32_t lds[AesLds<COPIES>::kWords];
   __shared__ uint32_t s_ticket;
   __shared__ uint32_t s_wc[kTileThreads / 64], s_wa[kTileThreads / 64];
   __shared__ uint32_t s_prefix_c, s_prefix_a;
@@ -158,36 +189,23 @@ __global__ void __launch_bounds__(kTileThreads) k_gp_window(const uint32_t *__re
   const uint32_t *rk = rk_groups + 60 * chain;
   const uint64_t nonce = st->nonce, d0 = st->d0;
   const uint32_t kstart = n - st->i, W = st->W;
-  const uint32_t ntiles = W / kTile;
+  const uint32_t ntiles = (W + kTile - 1) / kTile;
   uint64_t *status = B.status + (size_t)chain * kMaxTiles;
   uint64_t *cand = B.cand + (size_t)chain * kWinMax;
   uint64_t *amb = B.amb + (size_t)chain * kAmbCap;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
-  for (;;) {
-    if (threadIdx.x == 0) s_ticket = atomicAdd(&st->ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_ticket;
-    __syncthreads();
-    if (tile >= ntiles) break;
+  if (threadIdx.x == 0) s_ticket = atomicAdd(&st->ticket, 1u);
+  __syncthreads();
+  uint32_t tile = s_ticket;
+  __syncthreads();
+  uint32_t v[kDrawsPerThread];
+  if (tile < ntiles) gp_draws<COPIES>(A, rk, nonce, d0, tile, mask, W, v);
 
-    // draws of this thread: d = d0 + tile*kTile + threadIdx*8 + j
-    const uint64_t dbase = d0 + (uint64_t)tile * kTile + (uint64_t)threadIdx.x * kDrawsPerThread;
-    uint32_t v[kDrawsPerThread];
-#pragma unroll
-    for (int b = 0; b < kDrawsPerThread / 4; b++) {
-      uint64_t d = dbase + 4 * b;
-      uint4 blk = aes_ctr_block<COPIES>(A, rk, nonce + d / kWordsPerNonce,
-                                        2 + (uint32_t)((d % kWordsPerNonce) / 4));
-      v[4 * b + 0] = blk.x & mask;
-      v[4 * b + 1] = blk.y & mask;
-      v[4 * b + 2] = blk.z & mask;
-      v[4 * b + 3] = blk.w & mask;
-    }
+  while (tile < ntiles) {
     uint32_t nc = 0;
 #pragma unroll
     for (int j = 0; j < kDrawsPerThread; j++) nc += v[j] < kstart;
-    // block-wide exclusive scan of candidate counts
     uint32_t incl = nc;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -201,19 +219,28 @@ __global__ void __launch_bounds__(kTileThreads) k_gp_window(const uint32_t *__re
       if (w < wv) wbase += s_wc[w];
       tile_c += s_wc[w];
     }
-    uint32_t excl = wbase + incl - nc;
+    const uint32_t excl = wbase + incl - nc;
     // Decoupled look-back with three states per tile:
     //   1: candidate aggregate        2: candidate prefix + ambiguous aggregate
     //   3: candidate prefix + ambiguous prefix (both inclusive)
-    // Wave 0 inspects 64 predecessors per step.  A tile only waits for lower
-    // tiles, which hold earlier tickets and are therefore resident.
-    if (wv == 0) {
-      if (tile > 0 && lane == 0)
+    // A tile only waits for lower tiles, which hold earlier tickets and are
+    // therefore being worked on by running blocks.
+    if (threadIdx.x == 0) {
+      if (tile > 0)
         __hip_atomic_store(&status[tile], pack_status(1, tile_c, 0), __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_AGENT);
+      s_ticket = atomicAdd(&st->ticket, 1u);
+    }
+    __syncthreads();
+    const uint32_t next_tile = s_ticket;
+    // wave 0 runs the look-back while the other waves already compute the
+    // next tile's keystream; wave 0 computes its own share afterwards
+    if (wv == 0) {
       uint32_t pc = lookback(status, tile, 2, 1, lane, &st->err);
       if (lane == 0) s_prefix_c = pc;
     }
+    uint32_t vn[kDrawsPerThread];
+    if (next_tile < ntiles) gp_draws<COPIES>(A, rk, nonce, d0, next_tile, mask, W, vn);
     __syncthreads();
     const uint32_t pre_c = s_prefix_c;
     // ambiguous flags need the global rank a = pre_c + excl + (index among own)
@@ -241,7 +268,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gp_window(const uint32_t *__re
       if (w < wv) wabase += s_wa[w];
       tile_a += s_wa[w];
     }
-    uint32_t excla = wabase + inca - na;
+    const uint32_t excla = wabase + inca - na;
     if (wv == 0) {
       if (tile > 0 && lane == 0)
         __hip_atomic_store(&status[tile], pack_status(2, pre_c + tile_c, tile_a), __ATOMIC_RELEASE,
@@ -261,7 +288,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gp_window(const uint32_t *__re
     const uint32_t pre_a = s_prefix_a;
     {
       uint32_t a = pre_c + excl, q = pre_a + excla;
-      uint32_t doff = tile * kTile + threadIdx.x * kDrawsPerThread;
+      const uint32_t doff = tile * kTile + threadIdx.x * kDrawsPerThread;
 #pragma unroll
       for (int j = 0; j < kDrawsPerThread; j++) {
         if (v[j] < kstart) {
@@ -274,6 +301,9 @@ __global__ void __launch_bounds__(kTileThreads) k_gp_window(const uint32_t *__re
         }
       }
     }
+#pragma unroll
+    for (int j = 0; j < kDrawsPerThread; j++) v[j] = vn[j];
+    tile = next_tile;
     __syncthreads();
   }
 }
@@ -331,13 +361,14 @@ __global__ void __launch_bounds__(kResolveThreads) k_gp_resolve(ChainBufs B, uin
   uint64_t *cand = B.cand + (size_t)chain * kWinMax;
   const uint32_t kstart = n - st->i;
   uint64_t *status = B.status + (size_t)chain * kMaxTiles;
-  const uint32_t used_tiles = st->W / kTile;
+  const uint32_t used_tiles = (st->W + kTile - 1) / kTile;
   if (m > kAmbCap) {
     // too many ambiguous candidates: retry the same window at a quarter size
     for (uint32_t t = threadIdx.x; t < used_tiles; t += blockDim.x) status[t] = 0;
     if (threadIdx.x == 0) {
       st->c_valid = 0;
-      st->W = st->W / 4 >= kTile ? (st->W / 4) / kTile * kTile : kTile;
+      uint32_t w = st->W / 4;
+      st->W = w >= kTile ? w / kTile * kTile : (w >= kWinMin ? w / kWinMin * kWinMin : kWinMin);
       st->ticket = 0;
       st->C = 0;
       st->m = 0;
@@ -703,11 +734,15 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   ORQG_HIP(hipMemcpyAsync(B.nonce_at, nat.data(), nat.size() * 8, hipMemcpyHostToDevice, c->stream));
   ORQG_HIP(hipMemsetAsync(B.status, 0, 3ull * kMaxTiles * 8, c->stream));
 
-  const unsigned grid_win = std::max(1, (2 * c->num_cu) / 3);  // 2 blocks/CU over 3 chains
+  const unsigned grid_win = std::max(1, c->num_cu / 3);  // one workgroup per CU over 3 chains
   const unsigned grid_commit = std::max(1, c->num_cu);
   uint32_t host_flag[2] = {0, 0};
   const int kBatch = 16;
+  // every window either resolves or retries at a quarter size down to kWinMin,
+  // which always resolves; the cap below only guards against logic errors
+  const uint64_t max_windows = 64ull * count * (1 + (uint64_t)n / 1024) + (1u << 20);
   for (uint64_t it = 0;; it++) {
+    if (it * kBatch > max_windows) throw Error(ORQG_E_PARITY, "gen_perm: window limit exceeded");
     for (int w = 0; w < kBatch; w++) {
       hipLaunchKernelGGL(k_gp_window<kGpCopies>, dim3(grid_win, 3), dim3(kTileThreads), 0,
                          c->stream, c->d_t0, c->rk(kGroup0), B, n, mask);
@@ -765,12 +800,12 @@ ShardedPerm next_perm(orqg_ctx *c, size_t n) {
 // inverse application new[i] = x[pi[i]]) and then reshare: share k gets the
 // group PRG mask rand[k] at its new position.
 template <typename T, bool ARITH>
-__global__ void __launch_bounds__(256) k_round(const uint32_t *__restrict__ t0,
+__global__ void __launch_bounds__(kAesThreads) k_round(const uint32_t *__restrict__ t0,
                                                const uint32_t *__restrict__ rk, uint64_t nonce0,
                                                CallSched s, const int32_t *__restrict__ src_idx,
                                                const T *__restrict__ in, T *__restrict__ out) {
   constexpr int E = 16 / sizeof(T);
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   uint64_t ngrp = (s.n + E - 1) / E;
@@ -797,8 +832,8 @@ void launch_round(orqg_ctx *c, bool arith, int g, const int32_t *idx, const T *i
                   uint64_t n) {
   int role = kGroup0 + g;
   CallSched s = make_sched(n, c->batch_size, sizeof(T), 2);
-  unsigned grid = grid_for((n * sizeof(T) + 15) / 16, 256, c->num_cu * 8);
-  hipLaunchKernelGGL((arith ? k_round<T, true> : k_round<T, false>), dim3(grid), dim3(256), 0,
+  unsigned grid = grid_for((n * sizeof(T) + 15) / 16, kAesThreads, c->num_cu);
+  hipLaunchKernelGGL((arith ? k_round<T, true> : k_round<T, false>), dim3(grid), dim3(kAesThreads), 0,
                      c->stream, c->d_t0, c->rk(role), c->nonce[role], s, idx, in, out);
   ORQG_HIP(hipGetLastError());
   c->nonce[role] += sched_nonces(s);

@@ -14,11 +14,11 @@ namespace orqg {
 
 // ---- raw keystream (AESPRGAlgorithm::fillBytes) --------------------------------
 template <int COPIES>
-__global__ void __launch_bounds__(256) k_keystream(const uint32_t *__restrict__ t0,
+__global__ void __launch_bounds__(kAesThreads) k_keystream(const uint32_t *__restrict__ t0,
                                                    const uint32_t *__restrict__ rk,
                                                    uint64_t nonce0, uint64_t nbytes,
                                                    uint8_t *__restrict__ out) {
-  __shared__ uint32_t lds[AesLds<COPIES>::kWords];
+  __shared__ __align__(16) uint32_t lds[AesLds<COPIES>::kWords];
   aes_load_tables<COPIES>(lds, t0);
   AesCtx<COPIES> A(lds);
   uint64_t nblk = (nbytes + 15) / 16;
@@ -40,7 +40,8 @@ void prg_keystream(orqg_ctx *c, int role, size_t bytes, void *out) {
   if (bytes == 0) return;
   uint64_t nblk = (bytes + 15) / 16;
   TimedRegion tr(c, "keystream");
-  hipLaunchKernelGGL(k_keystream<kCopies>, dim3(grid_for(nblk, 256, c->num_cu * 8)), dim3(256), 0,
+  hipLaunchKernelGGL(k_keystream<kCopies>, dim3(grid_for(nblk, kAesThreads, c->num_cu)),
+                     dim3(kAesThreads), 0,
                      c->stream, c->d_t0, c->rk(role), c->nonce[role], (uint64_t)bytes,
                      (uint8_t *)out);
   ORQG_HIP(hipGetLastError());
@@ -51,13 +52,13 @@ void prg_keystream(orqg_ctx *c, int role, size_t bytes, void *out) {
 // get_shares_b/_a (replicated_3pc.h:338-392): share_1, share_2 from party 0's
 // localPRG (two getNext calls per batch), share_3 = x ^ s1 ^ s2 (x - s1 - s2).
 template <typename T, bool ARITH>
-__global__ void __launch_bounds__(256) k_secret_share(const uint32_t *__restrict__ t0,
+__global__ void __launch_bounds__(kAesThreads) k_secret_share(const uint32_t *__restrict__ t0,
                                                       const uint32_t *__restrict__ rk,
                                                       uint64_t nonce0, CallSched s,
                                                       const T *__restrict__ plain,
                                                       T *__restrict__ out) {
   constexpr int E = 16 / sizeof(T);
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   uint64_t ngrp = (s.n + E - 1) / E;
@@ -89,16 +90,16 @@ void secret_share(orqg_ctx *c, const void *host, orqg_vec *v) {
   void *tmp = c->scratch_get("plain_in", bytes);
   ORQG_HIP(hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, c->stream));
   CallSched s = make_sched(v->n, c->batch_size, v->width, 2);
-  unsigned grid = grid_for((v->n + 3) / 4, 256, c->num_cu * 8);
+  unsigned grid = grid_for((v->n + 3) / 4, kAesThreads, c->num_cu);
   int role = kLocal0;  // data owner = party 0
   bool arith = v->enc == ORQG_ASHARED;
   if (v->width == 4) {
     auto k = arith ? k_secret_share<uint32_t, true> : k_secret_share<uint32_t, false>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, c->stream, c->d_t0, c->rk(role),
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
                        c->nonce[role], s, (const uint32_t *)tmp, (uint32_t *)v->d);
   } else {
     auto k = arith ? k_secret_share<uint64_t, true> : k_secret_share<uint64_t, false>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, c->stream, c->d_t0, c->rk(role),
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
                        c->nonce[role], s, (const uint64_t *)tmp, (uint64_t *)v->d);
   }
   ORQG_HIP(hipGetLastError());
@@ -183,13 +184,13 @@ void elementwise_local(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y
 // S_k is K_k's keystream for this batch.  z_p is sent to p-1, whose second share
 // it becomes: the stored s_p.
 template <typename T, bool ARITH>
-__global__ void __launch_bounds__(256) k_mult(const uint32_t *__restrict__ t0,
+__global__ void __launch_bounds__(kAesThreads) k_mult(const uint32_t *__restrict__ t0,
                                               const uint32_t *__restrict__ rk_pair,
                                               NonceTriple nb, CallSched s,
                                               const T *__restrict__ x, const T *__restrict__ y,
                                               T *__restrict__ z) {
   constexpr int E = 16 / sizeof(T);
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   uint64_t ngrp = (s.n + E - 1) / E;
@@ -228,15 +229,15 @@ void mult(orqg_ctx *c, bool arith, const orqg_vec *x, const orqg_vec *y, orqg_ve
   if (x->n == 0) return;
   CallSched s = make_sched(x->n, c->batch_size, x->width, 1);
   NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
-  unsigned grid = grid_for((x->n * x->width + 15) / 16, 256, c->num_cu * 8);
+  unsigned grid = grid_for((x->n * x->width + 15) / 16, kAesThreads, c->num_cu);
   TimedRegion tr(c, "mult");
   if (x->width == 4)
     hipLaunchKernelGGL((arith ? k_mult<uint32_t, true> : k_mult<uint32_t, false>), dim3(grid),
-                       dim3(256), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
+                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
                        (const uint32_t *)x->d, (const uint32_t *)y->d, (uint32_t *)z->d);
   else
     hipLaunchKernelGGL((arith ? k_mult<uint64_t, true> : k_mult<uint64_t, false>), dim3(grid),
-                       dim3(256), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
+                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
                        (const uint64_t *)x->d, (const uint64_t *)y->d, (uint64_t *)z->d);
   ORQG_HIP(hipGetLastError());
   uint64_t used = sched_nonces(s);
@@ -247,12 +248,12 @@ void mult(orqg_ctx *c, bool arith, const orqg_vec *x, const orqg_vec *y, orqg_ve
 
 // ---- b2a_bit (group helper b2a_bit_group lives in kern_common.h) ----
 template <typename T>
-__global__ void __launch_bounds__(256) k_b2a_bit(const uint32_t *__restrict__ t0,
+__global__ void __launch_bounds__(kAesThreads) k_b2a_bit(const uint32_t *__restrict__ t0,
                                                  const uint32_t *__restrict__ rk_pair,
                                                  NonceTriple nb, CallSched s,
                                                  const T *__restrict__ x, T *__restrict__ y) {
   constexpr int E = 16 / sizeof(T);
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   uint64_t ngrp = (s.n + E - 1) / E;
@@ -275,12 +276,12 @@ void b2a_bit(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
   if (x->n == 0) return;
   CallSched s = make_sched(x->n, c->batch_size, x->width, 3);
   NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
-  unsigned grid = grid_for((x->n * x->width + 15) / 16, 256, c->num_cu * 8);
+  unsigned grid = grid_for((x->n * x->width + 15) / 16, kAesThreads, c->num_cu);
   if (x->width == 4)
-    hipLaunchKernelGGL(k_b2a_bit<uint32_t>, dim3(grid), dim3(256), 0, c->stream, c->d_t0,
+    hipLaunchKernelGGL(k_b2a_bit<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
                        c->rk(kPair0), nb, s, (const uint32_t *)x->d, (uint32_t *)y->d);
   else
-    hipLaunchKernelGGL(k_b2a_bit<uint64_t>, dim3(grid), dim3(256), 0, c->stream, c->d_t0,
+    hipLaunchKernelGGL(k_b2a_bit<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
                        c->rk(kPair0), nb, s, (const uint64_t *)x->d, (uint64_t *)y->d);
   ORQG_HIP(hipGetLastError());
   uint64_t used = sched_nonces(s);
@@ -294,11 +295,11 @@ void b2a_bit(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
 // dependent vector, and share k is masked with rand[k]; the outsider receives
 // its two shares.
 template <typename T, bool ARITH>
-__global__ void __launch_bounds__(256) k_reshare(const uint32_t *__restrict__ t0,
+__global__ void __launch_bounds__(kAesThreads) k_reshare(const uint32_t *__restrict__ t0,
                                                  const uint32_t *__restrict__ rk, uint64_t nonce0,
                                                  CallSched s, T *__restrict__ v) {
   constexpr int E = 16 / sizeof(T);
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   uint64_t ngrp = (s.n + E - 1) / E;
@@ -322,15 +323,15 @@ void reshare(orqg_ctx *c, orqg_vec *v, int group) {
   if (v->n == 0) return;
   int role = kGroup0 + group;
   CallSched s = make_sched(v->n, c->batch_size, v->width, 2);
-  unsigned grid = grid_for((v->n * v->width + 15) / 16, 256, c->num_cu * 8);
+  unsigned grid = grid_for((v->n * v->width + 15) / 16, kAesThreads, c->num_cu);
   bool arith = v->enc == ORQG_ASHARED;
   if (v->width == 4)
     hipLaunchKernelGGL((arith ? k_reshare<uint32_t, true> : k_reshare<uint32_t, false>), dim3(grid),
-                       dim3(256), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
+                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
                        (uint32_t *)v->d);
   else
     hipLaunchKernelGGL((arith ? k_reshare<uint64_t, true> : k_reshare<uint64_t, false>), dim3(grid),
-                       dim3(256), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
+                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
                        (uint64_t *)v->d);
   ORQG_HIP(hipGetLastError());
   c->nonce[role] += sched_nonces(s);

@@ -37,7 +37,7 @@ void launch_round(orqg_ctx *c, bool arith, int g, const int32_t *idx, const T *i
 
 namespace {
 
-constexpr int kBT = 256;            // threads per block
+constexpr int kBT = kAesThreads;    // threads per block
 constexpr int kEPB = kBT * 4;       // elements per scan block
 
 __device__ inline uint32_t bexcl(uint32_t x, uint32_t *s_w, uint32_t &total) {
@@ -89,11 +89,13 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
                                                  const uint32_t *__restrict__ P, int word,
                                                  int bit, uint32_t flip, uint32_t *__restrict__ F,
                                                  uint32_t *__restrict__ bsum) {
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   __shared__ uint32_t s_w[kBT / 64];
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
-  const uint64_t e0 = (uint64_t)blockIdx.x * kEPB + threadIdx.x * 4;
+  const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
+  for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
+  const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
   uint32_t bits[4][3], f[4][3];
 #pragma unroll
   for (int i = 0; i < 4; i++)
@@ -116,7 +118,8 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
   for (int k = 0; k < 3; k++) {
     uint32_t tot;
     bexcl(loc[k], s_w, tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x * 3 + k] = tot;
+    if (threadIdx.x == 0) bsum[sb * 3 + k] = tot;
+  }
   }
 }
 
@@ -153,11 +156,13 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
                                                   const uint32_t *__restrict__ boff,
                                                   const uint32_t *__restrict__ tot,
                                                   uint32_t *__restrict__ perm) {
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   __shared__ uint32_t s_w[kBT / 64];
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
-  const uint64_t e0 = (uint64_t)blockIdx.x * kEPB + threadIdx.x * 4;
+  const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
+  for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
+  const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
   uint32_t f[4][3];
   uint32_t loc[3] = {0, 0, 0};
 #pragma unroll
@@ -171,7 +176,7 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     uint32_t t;
-    pre[k] = bexcl(loc[k], s_w, t) + boff[blockIdx.x * 3 + k];
+    pre[k] = bexcl(loc[k], s_w, t) + boff[sb * 3 + k];
   }
   uint32_t S[3][4];
   {
@@ -205,6 +210,7 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
       }
     }
   }
+  }
 }
 
 // ---- sharded round: gather v (binary, tszv-byte masks) and perm (arithmetic,
@@ -218,7 +224,7 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
                                                 uint32_t *__restrict__ Pout,
                                                 const uint32_t *__restrict__ qin,
                                                 uint32_t *__restrict__ qout) {
-  __shared__ uint32_t lds[AesLds<kCopies>::kWords];
+  AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   const uint64_t n = sv.n;
@@ -226,45 +232,50 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
   for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < ngrp;
        g += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e0 = g * 4;
+    const int ne = (int)(n - e0 < 4 ? n - e0 : 4);
+    // issue every gather first so that the loads are in flight during the AES work
+    uint32_t pv[4][3 * L], pq[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (i < ne) {
+        const uint64_t src = (uint64_t)(uint32_t)idx[e0 + i];
+        const uint32_t *ps = Pin + src * 3 * L;
+        const uint32_t *qs = qin + src * 3;
+#pragma unroll
+        for (int w = 0; w < 3 * L; w++) pv[i][w] = ps[w];
+#pragma unroll
+        for (int k = 0; k < 3; k++) pq[i][k] = qs[k];
+      }
+    }
     // perm masks: one block per call covers the 4 elements
     uint32_t pm[3][4];
     group_masks<uint32_t, true, kCopies>(A, rk, np, sp, e0, pm);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
+      if (i >= ne) break;
       const uint64_t e = e0 + i;
-      if (e >= n) break;
-      const uint64_t src = (uint64_t)(uint32_t)idx[e];
-      // v masks: element e's tszv bytes of calls 0 and 1
-      uint32_t mv[3][L];
-      {
-        uint32_t r[2][L];
+      // v masks: element e's bytes of calls 0 and 1
+      uint32_t r[2][L];
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-          uint64_t off;
-          uint32_t ctr, bib;
-          sched_locate(sv, e, j, off, ctr, bib);
-          uint4 blk = aes_ctr_block<kCopies>(A, rk, nv + off, ctr);
-          uint32_t w[4] = {blk.x, blk.y, blk.z, blk.w};
+      for (int j = 0; j < 2; j++) {
+        uint64_t off;
+        uint32_t ctr, bib;
+        sched_locate(sv, e, j, off, ctr, bib);
+        uint4 blk = aes_ctr_block<kCopies>(A, rk, nv + off, ctr);
+        uint32_t w4[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
-          for (int q = 0; q < L; q++) r[j][q] = w[bib / 4 + q];
-        }
-#pragma unroll
-        for (int q = 0; q < L; q++) {
-          mv[0][q] = r[0][q];
-          mv[1][q] = r[1][q];
-          mv[2][q] = r[0][q] ^ r[1][q];
-        }
+        for (int q = 0; q < L; q++) r[j][q] = w4[bib / 4 + q];
       }
-      const uint32_t *ps = Pin + src * 3 * L;
       uint32_t *pd = Pout + e * 3 * L;
 #pragma unroll
-      for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int q = 0; q < L; q++) pd[k * L + q] = ps[k * L + q] ^ mv[k][q];
-      const uint32_t *qs = qin + src * 3;
+      for (int q = 0; q < L; q++) {
+        pd[0 * L + q] = pv[i][0 * L + q] ^ r[0][q];
+        pd[1 * L + q] = pv[i][1 * L + q] ^ r[1][q];
+        pd[2 * L + q] = pv[i][2 * L + q] ^ r[0][q] ^ r[1][q];
+      }
       uint32_t *qd = qout + e * 3;
 #pragma unroll
-      for (int k = 0; k < 3; k++) qd[k] = qs[k] + pm[k][i];
+      for (int k = 0; k < 3; k++) qd[k] = pq[i][k] + pm[k][i];
     }
   }
 }
@@ -346,7 +357,8 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
   uint32_t *tot = (uint32_t *)c->scratch_get("rs_tot", 64);
   uint32_t *err = (uint32_t *)c->scratch_get("rs_err", 16);
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
-  const unsigned grid_rounds = grid_for((n + 3) / 4, kBT, c->num_cu * 8);
+  const unsigned grid_rounds = grid_for((n + 3) / 4, kBT, c->num_cu);
+  const unsigned grid_pers = std::min<unsigned>(nblk, (unsigned)c->num_cu);
   const unsigned grid_flat = grid_for(n, 256, c->num_cu * 16);
 
   for (int i = 0; i < bits; i++) {
@@ -356,7 +368,7 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
     {
       TimedRegion tr(c, "sort_b2a");
       CallSched s = make_sched(n, c->batch_size, 4, 3);
-      hipLaunchKernelGGL(k_bit_b2a<L>, dim3(nblk), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
+      hipLaunchKernelGGL(k_bit_b2a<L>, dim3(grid_pers), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
                          nb, s, P, bitpos / 32, bitpos % 32, flip, F, bsum);
       uint64_t used = sched_nonces(s);
       for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
@@ -367,7 +379,7 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
       TimedRegion tr(c, "sort_perm");
       NonceTriple nb2 = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
       CallSched s = make_sched(n, c->batch_size, 4, 1);
-      hipLaunchKernelGGL(k_bit_perm, dim3(nblk), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
+      hipLaunchKernelGGL(k_bit_perm, dim3(grid_pers), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
                          nb2, s, F, bsum, tot, q0);
       uint64_t used = sched_nonces(s);
       for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
