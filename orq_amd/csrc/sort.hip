@@ -30,7 +30,6 @@
 
 namespace orqg {
 
-void exclusive_scan_u32(orqg_ctx *c, const uint32_t *in, uint32_t *out, uint64_t n_plus_1);
 template <typename T>
 void launch_round(orqg_ctx *c, bool arith, int g, const int32_t *idx, const T *in, T *out,
                   uint64_t n);
