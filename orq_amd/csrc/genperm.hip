@@ -73,7 +73,9 @@ struct ChainBufs {
   uint32_t *toff;      // [3][kMaxTiles] rank of the tile's first candidate
   uint64_t *ta;        // [3][kWinMax]   per tile: (rank << 32) | v of its ambiguous candidates
   uint32_t *tacnt;     // [3][kMaxTiles]
-  uint8_t *rflag;      // [3][kAmbCap]   1 = ambiguous candidate rejected
+  uint32_t *amb_a;     // [3][kAmbCap]   ambiguous candidates in rank order: rank
+  int32_t *amb_u;      // [3][kAmbCap]   ... and U = rank - (k - v)
+  uint32_t *m_tot;     // [3]            ambiguous candidates in the window
   uint32_t *rej;       // [3][kAmbCap]   ranks of rejected candidates, ascending
   uint32_t *r_out;     // [3][count][n]  accepted offsets r_i
   uint64_t *D;         // [3][count]     draws consumed per permutation
@@ -234,6 +236,37 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_classify(ChainBufs B, uint32_
   }
 }
 
+// ---- (b') ambiguous candidates into one rank-ordered array ------------------------
+__global__ void __launch_bounds__(kGpThreads) k_gp_gather(ChainBufs B, uint32_t n) {
+  __shared__ uint32_t s_w[kGpThreads / 64];
+  uint32_t nt[3];
+  for (int c = 0; c < 3; c++) nt[c] = ntiles_of(B.st[c]);
+  const uint32_t total = nt[0] + nt[1] + nt[2];
+  for (uint32_t task = blockIdx.x; task < total; task += gridDim.x) {
+    int chain;
+    uint32_t tile;
+    task_tile(nt, task, chain, tile);
+    const uint32_t kstart = n - B.st[chain].i;
+    const uint32_t *tacnt = B.tacnt + (size_t)chain * kMaxTiles;
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < tile; t += blockDim.x) part += tacnt[t];
+    uint32_t off;
+    block_scan(part, s_w, off);
+    const uint32_t cnt = tacnt[tile];
+    if (tile == nt[chain] - 1 && threadIdx.x == 0) B.m_tot[chain] = off + cnt;
+    if (off >= kAmbCap) continue;
+    const uint64_t *in = B.ta + (size_t)chain * kWinMax + (size_t)tile * kTile;
+    uint32_t *oa = B.amb_a + (size_t)chain * kAmbCap;
+    int32_t *ou = B.amb_u + (size_t)chain * kAmbCap;
+    for (uint32_t j = threadIdx.x; j < cnt && off + j < kAmbCap; j += blockDim.x) {
+      const uint64_t e = in[j];
+      const uint32_t a = (uint32_t)(e >> 32);
+      oa[off + j] = a;
+      ou[off + j] = (int32_t)(a - (kstart - (uint32_t)e));
+    }
+  }
+}
+
 // ---- (c) exact resolution of the ambiguous candidates, chain advance -------------
 // Ambiguous candidate j (list index t_j, rank a_j, T_j = k - v_j <= a_j) is
 // accepted iff fewer than T_j acceptances precede it, i.e. iff more than
@@ -249,13 +282,13 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_classify(ChainBufs B, uint32_
 // decides at least its first item, so the loop ends.
 __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t n,
                                                            uint32_t count, uint32_t maskN) {
-  __shared__ uint32_t s_aoff[kMaxTiles + 1];
+  constexpr int PMAX = kAmbCap / kGpThreads;  // 64 items per thread at most
   __shared__ uint16_t s_t[kL2Cap];   // level-1 index of each undecided item
   __shared__ uint16_t s_idx[kL2Cap]; // current list -> undecided item
   __shared__ int32_t s_thr[kL2Cap];
   __shared__ uint8_t s_dec[kL2Cap];  // 1 accepted, 2 rejected
   __shared__ uint32_t s_w[kGpThreads / 64];
-  __shared__ uint32_t s_l2, s_over;
+  __shared__ uint32_t s_l2;
   const int chain = blockIdx.x;
   ChainState *st = B.st + chain;
   if (st->done) {
@@ -264,51 +297,30 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
   }
   const uint32_t kstart = n - st->i;
   const uint32_t ntiles = (st->W + kTile - 1) / kTile;
-  const uint32_t *tacnt = B.tacnt + (size_t)chain * kMaxTiles;
   const uint32_t *tcnt = B.tcnt + (size_t)chain * kMaxTiles;
-  const uint64_t *ta = B.ta + (size_t)chain * kWinMax;
-  uint8_t *rflag = B.rflag + (size_t)chain * kAmbCap;
+  const uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
+  const int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
   uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
-
-  // prefix of the per-tile ambiguous counts (ntiles <= kMaxTiles = blockDim)
-  uint32_t m, C;
-  {
-    uint32_t x = threadIdx.x < ntiles ? tacnt[threadIdx.x] : 0;
-    uint32_t y = threadIdx.x < ntiles ? tcnt[threadIdx.x] : 0;
-    uint32_t e = block_scan(x, s_w, m);
-    s_aoff[threadIdx.x] = e;
-    if (threadIdx.x == 0) s_aoff[kMaxTiles] = 0;
-    __syncthreads();
-    block_scan(y, s_w, C);
-  }
-  if (threadIdx.x == 0) s_over = m > kAmbCap;
-  __syncthreads();
-  // level 1 (type B over all m): contiguous runs of P items per thread
-  const uint32_t P = (m + blockDim.x - 1) / blockDim.x;
-  uint32_t nund = 0;
-  if (!s_over) {
-    for (uint32_t q = 0; q < P; q++) {
-      const uint32_t t = threadIdx.x * P + q;
-      if (t >= m) break;
-      // tile holding level-1 item t
-      uint32_t lo = 0, hi = ntiles;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (s_aoff[mid] <= t) lo = mid;
-        else hi = mid;
-      }
-      const uint64_t e = ta[(size_t)lo * kTile + (t - s_aoff[lo])];
-      const int32_t U = (int32_t)((uint32_t)(e >> 32) - (kstart - (uint32_t)e));
-      const bool rejected = (int32_t)t <= U;  // U >= 0 here: no sure accepts
-      rflag[t] = rejected ? 1 : 0;
-      nund += !rejected;
-    }
+  const uint32_t m = B.m_tot[chain];
+  uint32_t C;
+  block_scan(threadIdx.x < ntiles ? tcnt[threadIdx.x] : 0u, s_w, C);
+  const bool over = m > kAmbCap;
+  // level 1 (type B over all m; U >= 0, so no sure accepts): each thread owns
+  // the contiguous run [t0, t0 + P); bit q of `rejm` = item t0+q rejected
+  const uint32_t P = over ? 0 : (m + kGpThreads - 1) / kGpThreads;
+  const uint32_t t0 = threadIdx.x * P;
+  uint64_t rejm = 0, undm = 0;
+  for (uint32_t q = 0; q < P && t0 + q < m; q++) {
+    const int32_t U = amb_u[t0 + q];
+    if ((int32_t)(t0 + q) <= U) rejm |= 1ull << q;
+    else undm |= 1ull << q;
   }
   uint32_t l2;
-  uint32_t upos = block_scan(nund, s_w, l2);
+  uint32_t upos = block_scan((uint32_t)__builtin_popcountll(undm), s_w, l2);
+  const uint32_t ubase = upos;
   if (threadIdx.x == 0) s_l2 = l2;
   __syncthreads();
-  if (s_over || s_l2 > kL2Cap) {
+  if (over || s_l2 > kL2Cap) {
     // too many ambiguous candidates: retry this window at a quarter size
     if (threadIdx.x == 0) {
       uint32_t w = st->W / 4;
@@ -317,31 +329,19 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
     }
     return;
   }
-  // undecided items, in order, with type A thresholds T' = t - U
-  if (!s_over) {
-    for (uint32_t q = 0; q < P; q++) {
-      const uint32_t t = threadIdx.x * P + q;
-      if (t >= m) break;
-      if (rflag[t]) continue;
-      uint32_t lo = 0, hi = ntiles;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (s_aoff[mid] <= t) lo = mid;
-        else hi = mid;
-      }
-      const uint64_t e = ta[(size_t)lo * kTile + (t - s_aoff[lo])];
-      const int32_t U = (int32_t)((uint32_t)(e >> 32) - (kstart - (uint32_t)e));
-      s_t[upos] = (uint16_t)t;
-      s_idx[upos] = (uint16_t)upos;
-      s_thr[upos] = (int32_t)t - U;
-      s_dec[upos] = 0;
-      upos++;
-    }
+  // undecided items continue as a type A list with T' = t - U
+  for (uint64_t mm = undm; mm; mm &= mm - 1) {
+    const uint32_t q = (uint32_t)__builtin_ctzll(mm), t = t0 + q;
+    s_t[upos] = (uint16_t)t;
+    s_idx[upos] = (uint16_t)upos;
+    s_thr[upos] = (int32_t)t - amb_u[t];
+    s_dec[upos] = 0;
+    upos++;
   }
   __syncthreads();
   constexpr int PER = kL2Cap / kGpThreads;  // 16
   uint32_t len = s_l2;
-  int type_b = 0;  // the undecided list continues as type A
+  int type_b = 0;
   while (len > 0) {
     const uint32_t beg = threadIdx.x * PER;
     uint8_t dec[PER];
@@ -395,30 +395,19 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
     len = new_len;
     type_b = !type_b;
   }
-  for (uint32_t u = threadIdx.x; u < s_l2; u += blockDim.x)
-    if (s_dec[u] == 2) rflag[s_t[u]] = 1;
-  __syncthreads();
-  // rejected ranks in ascending order (the ambiguous list is in rank order)
-  uint32_t nr = 0;
-  for (uint32_t q = 0; q < P; q++) {
-    const uint32_t t = threadIdx.x * P + q;
-    if (t >= m) break;
-    nr += rflag[t];
-  }
-  uint32_t R;
-  uint32_t rpos = block_scan(nr, s_w, R);
-  for (uint32_t q = 0; q < P; q++) {
-    const uint32_t t = threadIdx.x * P + q;
-    if (t >= m) break;
-    if (!rflag[t]) continue;
-    uint32_t lo = 0, hi = ntiles;
-    while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (s_aoff[mid] <= t) lo = mid;
-      else hi = mid;
+  // this thread's undecided items sit at l2 positions ubase, ubase+1, ...
+  {
+    uint32_t u = ubase;
+    for (uint64_t mm = undm; mm; mm &= mm - 1) {
+      if (s_dec[u] == 2) rejm |= 1ull << __builtin_ctzll(mm);
+      u++;
     }
-    rej[rpos++] = (uint32_t)(ta[(size_t)lo * kTile + (t - s_aoff[lo])] >> 32);
   }
+  (void)PMAX;
+  // rejected ranks in ascending order (the ambiguous list is in rank order)
+  uint32_t R;
+  uint32_t rpos = block_scan((uint32_t)__builtin_popcountll(rejm), s_w, R);
+  for (uint64_t mm = rejm; mm; mm &= mm - 1) rej[rpos++] = amb_a[t0 + __builtin_ctzll(mm)];
   __syncthreads();
   if (threadIdx.x == 0) {
     const uint32_t acc = C - R;
@@ -622,7 +611,9 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   B.toff = (uint32_t *)c->scratch_get("gp_toff", 3ull * kMaxTiles * 4);
   B.ta = (uint64_t *)c->scratch_get("gp_ta", 3ull * kWinMax * 8);
   B.tacnt = (uint32_t *)c->scratch_get("gp_tacnt", 3ull * kMaxTiles * 4);
-  B.rflag = (uint8_t *)c->scratch_get("gp_rflag", 3ull * kAmbCap);
+  B.amb_a = (uint32_t *)c->scratch_get("gp_amb_a", 3ull * kAmbCap * 4);
+  B.amb_u = (int32_t *)c->scratch_get("gp_amb_u", 3ull * kAmbCap * 4);
+  B.m_tot = (uint32_t *)c->scratch_get("gp_m_tot", 64);
   B.rej = (uint32_t *)c->scratch_get("gp_rej", 3ull * kAmbCap * 4);
   B.r_out = (uint32_t *)c->scratch_get("gp_r", 3ull * count * n * 4);
   B.D = (uint64_t *)c->scratch_get("gp_D", 3ull * count * 8);
@@ -652,6 +643,7 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
       hipLaunchKernelGGL(k_gp_draws<kCopies>, dim3(grid), dim3(kGpThreads), 0, c->stream, c->d_t0,
                          c->rk(kGroup0), B, n, mask);
       hipLaunchKernelGGL(k_gp_classify, dim3(grid), dim3(kGpThreads), 0, c->stream, B, n);
+      hipLaunchKernelGGL(k_gp_gather, dim3(grid), dim3(kGpThreads), 0, c->stream, B, n);
       hipLaunchKernelGGL(k_gp_resolve, dim3(3), dim3(kGpThreads), 0, c->stream, B, n, count,
                          maskN);
       hipLaunchKernelGGL(k_gp_commit, dim3(grid), dim3(kGpThreads), 0, c->stream, B, n, count);
@@ -671,21 +663,24 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] = fin[g].nonce;
 
   // swap resolution: the three local permutations of one sharded permutation per launch
-  int32_t *head = (int32_t *)c->scratch_get("fy_head", 3ull * n * 4 + 64);
-  int32_t *next = (int32_t *)c->scratch_get("fy_next", 3ull * n * 4 + 64);
+  int32_t *head = (int32_t *)c->scratch_get("fy_head", (size_t)n * 4 + 64);
+  int32_t *next = (int32_t *)c->scratch_get("fy_next", (size_t)n * 4 + 64);
   uint32_t *err = (uint32_t *)c->scratch_get("fy_err", 16);
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
   const unsigned fgrid = grid_for(n, 256, c->num_cu * 8);
   const size_t chain_stride = (size_t)count * n;
-  for (uint32_t k = 0; k < count; k++) {
-    const uint32_t *r = B.r_out + (size_t)k * n;
-    ORQG_HIP(hipMemsetAsync(head, 0xff, 3ull * n * 4, c->stream));
-    hipLaunchKernelGGL(k_fy_link, dim3(fgrid, 3), dim3(256), 0, c->stream, n, r, chain_stride,
-                       head, next, err);
-    hipLaunchKernelGGL(k_fy_walk, dim3(fgrid, 3), dim3(256), 0, c->stream, n, r, chain_stride,
-                       head, next, fresh[k].fwd[0], fresh[k].fwd[1], fresh[k].fwd[2],
-                       fresh[k].inv[0], fresh[k].inv[1], fresh[k].inv[2]);
-  }
+  // one local permutation per launch keeps the randomly accessed arrays
+  // (~200 MB at n = 2^24) within the device's last-level cache
+  for (uint32_t k = 0; k < count; k++)
+    for (int g = 0; g < 3; g++) {
+      const uint32_t *r = B.r_out + (size_t)g * chain_stride + (size_t)k * n;
+      ORQG_HIP(hipMemsetAsync(head, 0xff, (size_t)n * 4, c->stream));
+      hipLaunchKernelGGL(k_fy_link, dim3(fgrid, 1), dim3(256), 0, c->stream, n, r, chain_stride,
+                         head, next, err);
+      hipLaunchKernelGGL(k_fy_walk, dim3(fgrid, 1), dim3(256), 0, c->stream, n, r, chain_stride,
+                         head, next, fresh[k].fwd[g], fresh[k].fwd[g], fresh[k].fwd[g],
+                         fresh[k].inv[g], fresh[k].inv[g], fresh[k].inv[g]);
+    }
   ORQG_HIP(hipGetLastError());
   uint32_t herr = 0;
   ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
