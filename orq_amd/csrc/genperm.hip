@@ -44,11 +44,11 @@ namespace orqg {
 namespace {
 
 constexpr int kGpThreads = kAesThreads;                   // 1024
-constexpr int kDrawsPerThread = 16;                       // 4 AES blocks per thread
-constexpr uint32_t kTile = kGpThreads * kDrawsPerThread;  // 16384 draws
-constexpr uint32_t kWinMax = 1u << 24;                    // draws per window
+constexpr int kDrawsPerThread = 32;                       // 8 AES blocks per thread
+constexpr uint32_t kTile = kGpThreads * kDrawsPerThread;  // 32768 draws
+constexpr uint32_t kWinMax = 1u << 26;                    // draws per window
 constexpr uint32_t kWinMin = 1024;
-constexpr uint32_t kMaxTiles = kWinMax / kTile;           // 1024
+constexpr uint32_t kMaxTiles = kWinMax / kTile;           // 2048
 constexpr uint32_t kAmbCap = 65536;  // ambiguous candidates per window (level 1)
 constexpr uint32_t kL2Cap = 16384;   // undecided after the first exact pass (in LDS)
 constexpr uint64_t kRefillWords = 1ull << 20;  // gen_perm buffer: 2^20 uint32
@@ -303,7 +303,11 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
   uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
   const uint32_t m = B.m_tot[chain];
   uint32_t C;
-  block_scan(threadIdx.x < ntiles ? tcnt[threadIdx.x] : 0u, s_w, C);
+  {
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) part += tcnt[t];
+    block_scan(part, s_w, C);
+  }
   const bool over = m > kAmbCap;
   // level 1 (type B over all m; U >= 0, so no sure accepts): each thread owns
   // the contiguous run [t0, t0 + P); bit q of `rejm` = item t0+q rejected
@@ -469,15 +473,24 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_commit(ChainBufs B, uint32_t 
     const uint64_t *in = B.tc + (size_t)chain * kWinMax + (size_t)tile * kTile;
     const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
     uint32_t *r = B.r_out + ((size_t)chain * count + k) * (size_t)n;
-    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+    // each thread commits a contiguous run of candidates: one binary search
+    // into the ascending rejected ranks, then a merge walk
+    const uint32_t per = (cnt + blockDim.x - 1) / blockDim.x;
+    const uint32_t j0 = threadIdx.x * per;
+    if (j0 >= cnt) continue;
+    const uint32_t j1 = j0 + per < cnt ? j0 + per : cnt;
+    uint32_t lo = 0, hi = R;  // lower_bound of off + j0 in rej
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (rej[mid] < off + j0) lo = mid + 1;
+      else hi = mid;
+    }
+    for (uint32_t j = j0; j < j1; j++) {
       const uint32_t a = off + j;
-      uint32_t lo = 0, hi = R;  // lower_bound of a in rej
-      while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (rej[mid] < a) lo = mid + 1;
-        else hi = mid;
+      if (lo < R && rej[lo] == a) {
+        lo++;
+        continue;
       }
-      if (lo < R && rej[lo] == a) continue;
       const uint32_t accr = a - lo;
       if (accr >= kstart) continue;  // cannot happen for an exact resolution
       r[i0 + accr] = (uint32_t)in[j];
@@ -634,11 +647,9 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   const unsigned grid = std::max(1, c->num_cu);
   uint32_t host_flag[2] = {0, 0};
   const int kBatch = 16;
-  // every window either resolves or is retried at a quarter size down to
-  // kWinMin draws, which always resolves; the cap only guards logic errors
-  const uint64_t max_windows = 64ull * count * (1 + (uint64_t)n / 1024) + (1u << 20);
-  for (uint64_t it = 0;; it++) {
-    if (it * kBatch > max_windows) throw Error(ORQG_E_PARITY, "gen_perm: window limit exceeded");
+  // The window kernels take all their state from device memory, so a batch of
+  // windows is one static launch sequence: it is captured once as a graph.
+  auto enqueue_batch = [&]() {
     for (int w = 0; w < kBatch; w++) {
       hipLaunchKernelGGL(k_gp_draws<kCopies>, dim3(grid), dim3(kGpThreads), 0, c->stream, c->d_t0,
                          c->rk(kGroup0), B, n, mask);
@@ -648,6 +659,27 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
                          maskN);
       hipLaunchKernelGGL(k_gp_commit, dim3(grid), dim3(kGpThreads), 0, c->stream, B, n, count);
     }
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  ORQG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_batch();
+  ORQG_HIP(hipStreamEndCapture(c->stream, &graph));
+  ORQG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } guard{graph, gexec};
+  // every window either resolves or is retried at a quarter size down to
+  // kWinMin draws, which always resolves; the cap only guards logic errors
+  const uint64_t max_windows = 64ull * count * (1 + (uint64_t)n / 1024) + (1u << 20);
+  for (uint64_t it = 0;; it++) {
+    if (it * kBatch > max_windows) throw Error(ORQG_E_PARITY, "gen_perm: window limit exceeded");
+    ORQG_HIP(hipGraphLaunch(gexec, c->stream));
     hipLaunchKernelGGL(k_gp_done, dim3(1), dim3(64), 0, c->stream, B, flag);
     ORQG_HIP(hipGetLastError());
     ORQG_HIP(hipMemcpyAsync(host_flag, flag, 8, hipMemcpyDeviceToHost, c->stream));
