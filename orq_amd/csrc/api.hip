@@ -84,6 +84,8 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     std::memcpy(c->keys, cfg->keys, sizeof(c->keys));
     ORQG_HIP(hipSetDevice(c->device));
     ORQG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    ORQG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    ORQG_HIP(hipEventCreateWithFlags(&c->ev_aux, hipEventDisableTiming));
     ORQG_HIP(hipEventCreate(&c->ev0));
     ORQG_HIP(hipEventCreate(&c->ev1));
     hipDeviceProp_t prop;
@@ -110,12 +112,18 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
 int orqg_ctx_destroy(orqg_ctx *c) {
   if (!c) return ORQG_E_ARG;
   (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto &kv : c->scratch) dev_free(kv.second.p);
   for (auto &p : c->queue) free_perm(p);
   dev_free(c->d_rk);
   dev_free(c->d_t0);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_aux) (void)hipEventDestroy(c->ev_aux);
+  if (c->stream2) {
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamDestroy(c->stream2);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ORQG_OK;

@@ -502,6 +502,7 @@ __global__ void k_gp_done(ChainBufs B, uint32_t *flag) {
   if (threadIdx.x == 0) {
     flag[0] = B.st[0].done & B.st[1].done & B.st[2].done;
     flag[1] = B.st[0].err | B.st[1].err | B.st[2].err;
+    for (int g = 0; g < 3; g++) flag[2 + g] = B.st[g].k;  // permutations completed
   }
 }
 
@@ -631,7 +632,7 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   B.r_out = (uint32_t *)c->scratch_get("gp_r", 3ull * count * n * 4);
   B.D = (uint64_t *)c->scratch_get("gp_D", 3ull * count * 8);
   B.nonce_at = (uint64_t *)c->scratch_get("gp_nonce_at", 3ull * (count + 1) * 8);
-  uint32_t *flag = (uint32_t *)c->scratch_get("gp_flag", 16);
+  uint32_t *flag = (uint32_t *)c->scratch_get("gp_flag", 32);
 
   ChainState init[3];
   std::memset(init, 0, sizeof(init));
@@ -645,7 +646,7 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   ORQG_HIP(hipMemcpyAsync(B.nonce_at, nat.data(), nat.size() * 8, hipMemcpyHostToDevice, c->stream));
 
   const unsigned grid = std::max(1, c->num_cu);
-  uint32_t host_flag[2] = {0, 0};
+  uint32_t host_flag[5] = {0, 0, 0, 0, 0};
   const int kBatch = 16;
   // The window kernels take all their state from device memory, so a batch of
   // windows is one static launch sequence: it is captured once as a graph.
@@ -674,6 +675,30 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
       if (g) (void)hipGraphDestroy(g);
     }
   } guard{graph, gexec};
+  // Swap resolution of a finished local permutation runs on a second stream
+  // while the windows of the next ones proceed: it is memory-latency bound,
+  // the windows are AES (VALU/LDS) bound.
+  int32_t *head = (int32_t *)c->scratch_get("fy_head", (size_t)n * 4 + 64);
+  int32_t *next = (int32_t *)c->scratch_get("fy_next", (size_t)n * 4 + 64);
+  uint32_t *err = (uint32_t *)c->scratch_get("fy_err", 16);
+  ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
+  ORQG_HIP(hipEventRecord(c->ev_aux, c->stream));
+  ORQG_HIP(hipStreamWaitEvent(c->stream2, c->ev_aux, 0));
+  const unsigned fgrid = grid_for(n, 256, c->num_cu * 8);
+  const size_t chain_stride = (size_t)count * n;
+  uint32_t launched[3] = {0, 0, 0};
+  auto resolve_swaps = [&](int g, uint32_t upto) {
+    for (uint32_t k = launched[g]; k < upto; k++) {
+      const uint32_t *r = B.r_out + (size_t)g * chain_stride + (size_t)k * n;
+      ORQG_HIP(hipMemsetAsync(head, 0xff, (size_t)n * 4, c->stream2));
+      hipLaunchKernelGGL(k_fy_link, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
+                         head, next, err);
+      hipLaunchKernelGGL(k_fy_walk, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
+                         head, next, fresh[k].fwd[g], fresh[k].fwd[g], fresh[k].fwd[g],
+                         fresh[k].inv[g], fresh[k].inv[g], fresh[k].inv[g]);
+    }
+    if (upto > launched[g]) launched[g] = upto;
+  };
   // every window either resolves or is retried at a quarter size down to
   // kWinMin draws, which always resolves; the cap only guards logic errors
   const uint64_t max_windows = 64ull * count * (1 + (uint64_t)n / 1024) + (1u << 20);
@@ -682,41 +707,25 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
     ORQG_HIP(hipGraphLaunch(gexec, c->stream));
     hipLaunchKernelGGL(k_gp_done, dim3(1), dim3(64), 0, c->stream, B, flag);
     ORQG_HIP(hipGetLastError());
-    ORQG_HIP(hipMemcpyAsync(host_flag, flag, 8, hipMemcpyDeviceToHost, c->stream));
+    ORQG_HIP(hipMemcpyAsync(host_flag, flag, sizeof(host_flag), hipMemcpyDeviceToHost, c->stream));
     ORQG_HIP(hipStreamSynchronize(c->stream));
     if (host_flag[1])
       throw Error(ORQG_E_PARITY,
                   "gen_perm: device-side check failed (flags " + std::to_string(host_flag[1]) + ")");
+    // r of permutations below host_flag[2+g] is complete (the commits ran
+    // before the synchronisation above)
+    for (int g = 0; g < 3; g++) resolve_swaps(g, std::min(host_flag[2 + g], count));
     if (host_flag[0]) break;
   }
   ChainState fin[3];
   ORQG_HIP(hipMemcpyAsync(fin, B.st, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
   for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] = fin[g].nonce;
-
-  // swap resolution: the three local permutations of one sharded permutation per launch
-  int32_t *head = (int32_t *)c->scratch_get("fy_head", (size_t)n * 4 + 64);
-  int32_t *next = (int32_t *)c->scratch_get("fy_next", (size_t)n * 4 + 64);
-  uint32_t *err = (uint32_t *)c->scratch_get("fy_err", 16);
-  ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
-  const unsigned fgrid = grid_for(n, 256, c->num_cu * 8);
-  const size_t chain_stride = (size_t)count * n;
-  // one local permutation per launch keeps the randomly accessed arrays
-  // (~200 MB at n = 2^24) within the device's last-level cache
-  for (uint32_t k = 0; k < count; k++)
-    for (int g = 0; g < 3; g++) {
-      const uint32_t *r = B.r_out + (size_t)g * chain_stride + (size_t)k * n;
-      ORQG_HIP(hipMemsetAsync(head, 0xff, (size_t)n * 4, c->stream));
-      hipLaunchKernelGGL(k_fy_link, dim3(fgrid, 1), dim3(256), 0, c->stream, n, r, chain_stride,
-                         head, next, err);
-      hipLaunchKernelGGL(k_fy_walk, dim3(fgrid, 1), dim3(256), 0, c->stream, n, r, chain_stride,
-                         head, next, fresh[k].fwd[g], fresh[k].fwd[g], fresh[k].fwd[g],
-                         fresh[k].inv[g], fresh[k].inv[g], fresh[k].inv[g]);
-    }
+  for (int g = 0; g < 3; g++) resolve_swaps(g, count);
   ORQG_HIP(hipGetLastError());
   uint32_t herr = 0;
-  ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
-  ORQG_HIP(hipStreamSynchronize(c->stream));
+  ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream2));
+  ORQG_HIP(hipStreamSynchronize(c->stream2));
   if (herr) throw Error(ORQG_E_PARITY, "gen_perm: swap resolution check failed");
   for (auto &p : fresh) c->queue.push_back(p);
 }

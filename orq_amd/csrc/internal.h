@@ -113,6 +113,8 @@ struct Timer {
 struct orqg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // auxiliary stream (same default priority)
+  hipEvent_t ev_aux = nullptr;
   long batch_size = -12;
   size_t budget = 0;
   uint64_t nonce[ORQG_NUM_KEYS] = {0};
