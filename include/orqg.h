@@ -39,6 +39,8 @@ enum {
 
 enum { ORQG_BSHARED = 1, ORQG_ASHARED = 0 }; /* include/core/containers/encoding.h:8 */
 enum { ORQG_ASC = 0, ORQG_DESC = 1 };
+/* BSharedVector comparison operators (include/core/containers/b_shared_vector.h:493-580) */
+enum { ORQG_EQ = 0, ORQG_NE = 1, ORQG_LT = 2, ORQG_GT = 3, ORQG_LE = 4, ORQG_GE = 5 };
 
 /* PRG roles of the 10 AES-256 keys (include/backend/common/rand_setup.h:21-146):
  *   0..2  localPRG of party p (secret sharing)
@@ -113,6 +115,20 @@ int orqg_and_b(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z)
 int orqg_mul_a(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
 /* b2a_bit (replicated_3pc.h:220-251): y = A-sharing of the LSB of x          */
 int orqg_b2a_bit(orqg_ctx *ctx, const orqg_vec *x, orqg_vec *y);
+/* BSharedVector::_compare (include/core/operators/circuits.h:231-295):
+ * eq = (x == y) and gt = (x > y) as boolean shares of the LSB, elements taken
+ * as signed two's complement.  Consumes the pair-PRG randomness of its
+ * log2(bits) + 1 and_b calls over n elements and one over n/bits packed
+ * words.  eq or gt may be NULL when not wanted.                              */
+int orqg_compare(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq,
+                 orqg_vec *gt);
+/* z = (x op y) for op = ORQG_EQ..ORQG_GE (b_shared_vector.h:493-580):
+ * == is bit_same & 1 (b_shared_vector.h:164-178), < is y > x, != / <= / >=
+ * apply not_b_1 (replicated_3pc.h:207) to ==, > and <.                        */
+int orqg_cmp(orqg_ctx *ctx, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+/* BSharedVector::b2a (b_shared_vector.h:296-329): y = A-sharing of the whole
+ * boolean-shared value, one b2a_bit and one multiply_a by 2^i per bit.        */
+int orqg_b2a(orqg_ctx *ctx, const orqg_vec *x, orqg_vec *y);
 /* Protocol::reshare for group 0..2 = {0,1},{0,2},{1,2} (protocol.h:500-541) */
 int orqg_reshare(orqg_ctx *ctx, orqg_vec *v, int group);
 

@@ -54,6 +54,9 @@ def lib():
             ("oc_apply_sharded", [vp, vp, i32, i32]),
             ("oc_apply_elementwise", [vp, vp, i32, vp, i32]),
             ("oc_radix_sort", [vp, vp, i32, i32, vp]),
+            ("oc_compare", [vp, vp, vp, vp, vp]),
+            ("oc_b2a", [vp, vp, vp]),
+            ("oc_cmp", [vp, i32, vp, vp, vp]),
         ]:
             f = getattr(L, name)
             f.restype = i32
@@ -198,6 +201,25 @@ class Ctx:
         perm = Vec(v.n, 4)
         _check(lib().oc_radix_sort(self.h, v.h, bits, int(desc), perm.h), "radix_sort")
         return perm
+
+    def compare(self, x, y):
+        """x._compare(y): (eq, gt) boolean shares, signed order."""
+        eq, gt = Vec(x.n, x.width), Vec(x.n, x.width)
+        _check(lib().oc_compare(self.h, x.h, y.h, eq.h, gt.h), "compare")
+        return eq, gt
+
+    CMP_OPS = {"==": 0, "!=": 1, "<": 2, ">": 3, "<=": 4, ">=": 5}
+
+    def cmp(self, op, x, y):
+        """BSharedVector comparison operator `op` ('==', '!=', '<', '>', '<=', '>=')."""
+        z = Vec(x.n, x.width)
+        _check(lib().oc_cmp(self.h, self.CMP_OPS[op], x.h, y.h, z.h), "cmp")
+        return z
+
+    def b2a(self, x):
+        y = Vec(x.n, x.width)
+        _check(lib().oc_b2a(self.h, x.h, y.h), "b2a")
+        return y
 
     def msglog(self):
         L = lib()

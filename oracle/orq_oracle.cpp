@@ -586,6 +586,174 @@ void reverse_vec(oc_vec *v) {
     }
 }
 
+// ----------------------------------------------------- comparison circuits
+template <typename T>
+void not_vec(oc_vec *v) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *a = S<T>(v, p, k);
+      for (size_t i = 0; i < v->n; i++) a[i] = ~a[i];
+    }
+}
+
+// BSharedVector::bit_same (include/core/containers/b_shared_vector.h:164-178):
+// e = ~(x ^ y); for l < log2(bits): e &= e >> 2^l (arithmetic shift of each share).
+template <typename T>
+oc_vec *bit_same(oc_ctx *c, oc_vec *x, oc_vec *y) {
+  typedef typename std::make_signed<T>::type ST;
+  const size_t n = x->n;
+  oc_vec *e = new_vec(n, sizeof(T)), *t = new_vec(n, sizeof(T));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(x, p, k), *b = S<T>(y, p, k);
+      T *d = S<T>(e, p, k);
+      for (size_t i = 0; i < n; i++) d[i] = ~(a[i] ^ b[i]);
+    }
+  int logb = 0;
+  while ((1 << logb) < (int)(8 * sizeof(T))) logb++;  // bit_width(bits - 1)
+  for (int l = 0; l < logb; l++) {
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++) {
+        const T *a = S<T>(e, p, k);
+        T *d = S<T>(t, p, k);
+        for (size_t i = 0; i < n; i++) d[i] = (T)((ST)a[i] >> (1 << l));
+      }
+    mult<T, false>(c, e, t, e, "and_b");
+  }
+  delete t;
+  return e;
+}
+
+// BSharedVector::_compare (include/core/operators/circuits.h:231-295).
+template <typename T>
+void compare(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *eq, oc_vec *gt) {
+  typedef typename std::make_signed<T>::type ST;
+  const size_t n = x->n;
+  const int B = 8 * sizeof(T);
+  const size_t nc = n / B + (n % B > 0);
+  oc_vec *e = bit_same<T>(c, x, y);
+  // gt = parity(((e >> 1) ^ e) & x)
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(e, p, k);
+      T *d = S<T>(gt, p, k);
+      for (size_t i = 0; i < n; i++) d[i] = (T)((ST)a[i] >> 1) ^ a[i];
+    }
+  mult<T, false>(c, gt, x, gt, "and_b");
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *d = S<T>(gt, p, k);
+      for (size_t i = 0; i < n; i++) d[i] = (T)(__builtin_popcountll((unsigned long long)d[i]) & 1);
+    }
+  // signed inputs: fix the sign bit on MSB-packed words
+  oc_vec *s1 = new_vec(nc, sizeof(T)), *s2 = new_vec(nc, sizeof(T)), *r = new_vec(nc, sizeof(T));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *xa = S<T>(x, p, k), *ya = S<T>(y, p, k), *g = S<T>(gt, p, k);
+      T *a1 = S<T>(s1, p, k), *a2 = S<T>(s2, p, k), *ar = S<T>(r, p, k);
+      for (size_t w = 0; w < nc; w++) {  // pack_from (mapping_access_vector.h:974-991)
+        T v1 = 0, v2 = 0, vr = 0;
+        for (int q = 0; q < B && w * B + q < n; q++) {
+          v1 |= (T)((xa[w * B + q] >> (B - 1)) & 1) << q;
+          v2 |= (T)((ya[w * B + q] >> (B - 1)) & 1) << q;
+          vr |= (T)(g[w * B + q] & 1) << q;
+        }
+        a1[w] = v1;
+        a2[w] = v2;
+        ar[w] = vr;
+      }
+      // r ^= s2 ; s2 ^= s1
+      for (size_t w = 0; w < nc; w++) {
+        ar[w] ^= a2[w];
+        a2[w] ^= a1[w];
+      }
+    }
+  // s2 |= r  ->  s2 = ~(~s2 & ~r)  (BSharedVector::operator|=, b_shared_vector.h:452-456)
+  not_vec<T>(s2);
+  not_vec<T>(r);
+  mult<T, false>(c, s2, r, s2, "and_b");
+  not_vec<T>(s2);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *a1 = S<T>(s1, p, k);
+      const T *a2 = S<T>(s2, p, k);
+      T *g = S<T>(gt, p, k);
+      for (size_t w = 0; w < nc; w++) a1[w] ^= a2[w];  // s1 ^= s2
+      for (size_t i = 0; i < n; i++)  // unpack_from(s1, 0), then mask(1)
+        g[i] = (T)((a1[i / B] >> (i % B)) & 1);
+    }
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(e, p, k);
+      T *d = S<T>(eq, p, k);
+      for (size_t i = 0; i < n; i++) d[i] = a[i] & 1;  // eq_bits.mask(1)
+    }
+  delete e;
+  delete s1;
+  delete s2;
+  delete r;
+}
+
+// BSharedVector::b2a (b_shared_vector.h:296-329): sum over bits of
+// b2a_bit(bit i) * 2^i, the scaling being a secure multiply_a by a public
+// sharing of 2^i.
+template <typename T>
+void b2a_full(oc_ctx *c, oc_vec *x, oc_vec *y) {
+  const size_t n = x->n;
+  const int B = 8 * sizeof(T);
+  oc_vec *bit = new_vec(n, sizeof(T)), *ba = new_vec(n, sizeof(T)), *pc = new_vec(n, sizeof(T));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) std::fill(y->sh[p][k].begin(), y->sh[p][k].end(), 0);
+  for (int i = 0; i < B; i++) {
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++) {
+        const T *a = S<T>(x, p, k);
+        T *d = S<T>(bit, p, k);
+        for (size_t j = 0; j < n; j++) d[j] = (a[j] >> i) & 1;  // logical shift, mask(1)
+      }
+    b2a_bit<T>(c, bit, ba);
+    if (i > 0) {
+      std::vector<T> cst(n, (T)((T)1 << i));
+      public_share<T>(pc, cst.data());
+      mult<T, true>(c, ba, pc, ba, "mul_a");
+    }
+    for (int p = 0; p < 3; p++)
+      for (int k = 0; k < 2; k++) {
+        T *d = S<T>(y, p, k);
+        const T *a = S<T>(ba, p, k);
+        for (size_t j = 0; j < n; j++) d[j] += a[j];
+      }
+  }
+  delete bit;
+  delete ba;
+  delete pc;
+}
+
+// BSharedVector comparison operators (b_shared_vector.h:493-580):
+// op 0 ==, 1 !=, 2 <, 3 >, 4 <=, 5 >=.  `!` is not_b_1 (replicated_3pc.h:207):
+// every share becomes !(s & 1).
+template <typename T>
+void cmp_op(oc_ctx *c, int op, oc_vec *x, oc_vec *y, oc_vec *z) {
+  oc_vec *r;
+  if (op == 0 || op == 1) {
+    r = bit_same<T>(c, x, y);
+  } else {
+    oc_vec *eq = new_vec(x->n, sizeof(T));
+    r = new_vec(x->n, sizeof(T));
+    bool swap = (op == 2 || op == 5);  // x < y and x >= y are built on y > x
+    compare<T>(c, swap ? y : x, swap ? x : y, eq, r);
+    delete eq;
+  }
+  bool neg = (op == 1 || op == 4 || op == 5);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(r, p, k);
+      T *d = S<T>(z, p, k);
+      for (size_t i = 0; i < x->n; i++) d[i] = neg ? (T)((a[i] & 1) == 0) : (T)(a[i] & 1);
+    }
+  delete r;
+}
+
 // ---------------------------------------------------------- radix sort ---
 // radix_sort (include/core/operators/radixsort.h:205-229) with pad_input /
 // remove_padding (include/core/operators/sorting.h:431-495).  K is the key
@@ -885,6 +1053,40 @@ size_t oc_batches(size_t n, long batch_size, uint64_t *sizes, size_t cap) {
   auto b = batches_for(n, batch_size);
   for (size_t i = 0; i < b.size() && i < cap; i++) sizes[i] = b[i].e - b[i].s;
   return b.size();
+}
+
+// x._compare(y, eq, gt): boolean shares of x == y and x > y (signed)
+int oc_compare(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *eq, oc_vec *gt) {
+  OC_TRY({
+    if (x->width == 4)
+      compare<uint32_t>(c, x, y, eq, gt);
+    else if (x->width == 8)
+      compare<uint64_t>(c, x, y, eq, gt);
+    else
+      throw std::runtime_error("compare: width");
+  })
+}
+
+int oc_cmp(oc_ctx *c, int op, oc_vec *x, oc_vec *y, oc_vec *z) {
+  OC_TRY({
+    if (op < 0 || op > 5) throw std::runtime_error("cmp: op");
+    if (x->width == 4)
+      cmp_op<uint32_t>(c, op, x, y, z);
+    else
+      cmp_op<uint64_t>(c, op, x, y, z);
+  })
+}
+
+// full-width b2a of a boolean-shared vector
+int oc_b2a(oc_ctx *c, oc_vec *x, oc_vec *y) {
+  OC_TRY({
+    if (x->width == 4)
+      b2a_full<uint32_t>(c, x, y);
+    else if (x->width == 8)
+      b2a_full<uint64_t>(c, x, y);
+    else
+      throw std::runtime_error("b2a: width");
+  })
 }
 
 size_t oc_msglog_count(oc_ctx *c) { return c->log.size(); }

@@ -47,6 +47,7 @@ EXPORTS = [
     "orqg_vec_data", "orqg_vec_write", "orqg_vec_read", "orqg_vec_copy", "orqg_prg_keystream",
     "orqg_secret_share", "orqg_open", "orqg_public_share", "orqg_xor_b", "orqg_add_a",
     "orqg_sub_a", "orqg_not_b", "orqg_and_b", "orqg_mul_a", "orqg_b2a_bit", "orqg_reshare",
+    "orqg_compare", "orqg_cmp", "orqg_b2a",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -90,6 +91,9 @@ def lib():
             "orqg_and_b": (i32, [vp, vp, vp, vp]),
             "orqg_mul_a": (i32, [vp, vp, vp, vp]),
             "orqg_b2a_bit": (i32, [vp, vp, vp]),
+            "orqg_compare": (i32, [vp, vp, vp, vp, vp]),
+            "orqg_cmp": (i32, [vp, i32, vp, vp, vp]),
+            "orqg_b2a": (i32, [vp, vp, vp]),
             "orqg_reshare": (i32, [vp, vp, i32]),
             "orqg_reserve_perms": (i32, [vp, sz, sz]),
             "orqg_perm_queue_size": (sz, [vp]),
@@ -256,6 +260,27 @@ class Ctx:
     def b2a_bit(self, x):
         y = Vec(self, x.n, x.width, ORQG_ASHARED)
         self._check(lib().orqg_b2a_bit(self.h, x.h, y.h), "b2a_bit")
+        return y
+
+    CMP_OPS = {"==": 0, "!=": 1, "<": 2, ">": 3, "<=": 4, ">=": 5}
+
+    def compare(self, x, y):
+        """x._compare(y): (eq, gt) boolean shares of x == y and x > y (signed)."""
+        eq = Vec(self, x.n, x.width, ORQG_BSHARED)
+        gt = Vec(self, x.n, x.width, ORQG_BSHARED)
+        self._check(lib().orqg_compare(self.h, x.h, y.h, eq.h, gt.h), "compare")
+        return eq, gt
+
+    def cmp(self, op, x, y):
+        """Comparison operator op ('==', '!=', '<', '>', '<=', '>=') on boolean shares."""
+        z = Vec(self, x.n, x.width, ORQG_BSHARED)
+        self._check(lib().orqg_cmp(self.h, self.CMP_OPS[op], x.h, y.h, z.h), "cmp")
+        return z
+
+    def b2a(self, x):
+        """Full-width boolean-to-arithmetic conversion (BSharedVector::b2a)."""
+        y = Vec(self, x.n, x.width, ORQG_ASHARED)
+        self._check(lib().orqg_b2a(self.h, x.h, y.h), "b2a")
         return y
 
     def reshare(self, v, group):

@@ -307,6 +307,44 @@ int orqg_b2a_bit(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
   y->enc = ORQG_ASHARED;
   API_END(c)
 }
+int orqg_compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, orqg_vec *gt) {
+  API_BEGIN(c)
+  check3(x, y, eq ? eq : gt ? gt : x);
+  if (gt) check3(x, y, gt);
+  if (x->enc != ORQG_BSHARED || y->enc != ORQG_BSHARED)
+    throw Error(ORQG_E_ARG, "compare: operands must be boolean-shared");
+  compare(c, x, y, eq, gt, true, 0, 0);
+  if (eq) eq->enc = ORQG_BSHARED;
+  if (gt) gt->enc = ORQG_BSHARED;
+  API_END(c)
+}
+int orqg_cmp(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  if (!y) throw Error(ORQG_E_ARG, "cmp: missing operand");
+  if (x->enc != ORQG_BSHARED || y->enc != ORQG_BSHARED)
+    throw Error(ORQG_E_ARG, "cmp: operands must be boolean-shared");
+  switch (op) {
+    case ORQG_EQ: compare(c, x, y, z, nullptr, false, 0, 0); break;
+    case ORQG_NE: compare(c, x, y, z, nullptr, false, 1, 0); break;
+    case ORQG_GT: compare(c, x, y, nullptr, z, true, 0, 0); break;
+    case ORQG_LT: compare(c, y, x, nullptr, z, true, 0, 0); break;
+    case ORQG_LE: compare(c, x, y, nullptr, z, true, 0, 1); break;
+    case ORQG_GE: compare(c, y, x, nullptr, z, true, 0, 1); break;
+    default: throw Error(ORQG_E_ARG, "cmp: op");
+  }
+  z->enc = ORQG_BSHARED;
+  API_END(c)
+}
+int orqg_b2a(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
+  API_BEGIN(c)
+  check3(x, nullptr, y);
+  if (x == y) throw Error(ORQG_E_ARG, "b2a: output must not alias the input");
+  if (x->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "b2a: input must be boolean-shared");
+  b2a_full(c, x, y);
+  y->enc = ORQG_ASHARED;
+  API_END(c)
+}
 int orqg_reshare(orqg_ctx *c, orqg_vec *v, int group) {
   API_BEGIN(c)
   reshare(c, v, group);

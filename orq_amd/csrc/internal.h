@@ -192,6 +192,9 @@ void free_perm(ShardedPerm &p);
 void apply_sharded(orqg_ctx *c, orqg_vec *x, const ShardedPerm &pi, bool inverse);
 void apply_elementwise(orqg_ctx *c, orqg_vec *x, const orqg_vec *perm);
 void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out);
+void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, orqg_vec *gt,
+             bool full, int neg_eq, int neg_gt);
+void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
 
 enum LocalOp { kXor = 0, kAdd = 1, kSub = 2, kNot = 3 };
 
