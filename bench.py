@@ -56,10 +56,19 @@ def cpu_baseline(n_sample):
     }
 
 
-# Algorithmic HBM bytes per row for the sort's round kernel (k_round2, L=3 words
-# per share): per row it reads idx (4 B), gathers v (36 B) and perm (12 B),
-# writes v (36 B) and perm (12 B).
+# Algorithmic bytes per row of one k_round2 launch (one sharded-permutation
+# round applied to the padded key v, 3 x 32-bit words per share, and to the
+# bit permutation, 1 word per share, all three shares): read the gather index
+# (4 B), gather v (36 B) and perm (12 B), write v (36 B) and perm (12 B).
 ROUND_BYTES_PER_ROW = 4 + 36 + 12 + 36 + 12
+# Keystream the same launch generates per row: two reshare masks of the
+# reference's 16-byte padded element for v, two 4-byte masks for perm.
+ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
+# HBM bytes one k_round2 launch at 2^24 rows actually moves, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the factors a plain
+# 384 MiB device copy gives (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE):
+# (2478423.4 * 2.0 + 1961569.8 * 1.0) KiB; profiles/radix_sort_2p24_int64_kernels.txt.
+ROUND_TRAFFIC_2P24 = int((2478423.4 * 2.0 + 1961569.8 * 1.0) * 1024)
 
 
 def main():
@@ -68,7 +77,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 24)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 14)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -95,33 +104,29 @@ def main():
 
     for _ in range(args.warmup):
         one_step()
+    # the timed steps record HIP events around every kernel family on the
+    # library's stream (no waits inside the region; read after the last sync)
+    ctx.timer_reset()
+    ctx.timer_enable(True)
     times = [one_step() for _ in range(args.steps)]
+    ctx.timer_enable(False)
     t_step = sum(times) / len(times)
+    fams = {}
+    for f in ["gen_perm", "sort_b2a", "sort_perm", "sort_round", "k_round2", "sort_tail"]:
+        ms, cnt = ctx.timer(f)
+        fams[f] = {"ms_per_step": round(ms / args.steps, 3), "launches": cnt}
+    online_ms = sum(fams[f]["ms_per_step"] for f in ["sort_b2a", "sort_perm", "sort_round",
+                                                      "sort_tail"])
 
     # correctness of the last step (outside the timed region)
     got = ctx.open(work).view(np.int64)
     ok = bool(np.array_equal(got, np.sort(keys_plain.view(np.int64), kind="stable")))
 
-    # per-kernel-family device time (events on the library stream)
-    ctx.timer_enable(True)
-    ctx.timer_reset()
-    orq_amd.lib().orqg_vec_copy(ctx.h, base.h, work.h)
-    t0 = time.perf_counter()
-    ctx.radix_sort(work).free()
-    ctx.sync()
-    t_timed = time.perf_counter() - t0
-    fams = {}
-    for f in ["gen_perm", "sort_b2a", "sort_perm", "sort_round", "sort_tail"]:
-        ms, cnt = ctx.timer(f)
-        fams[f] = {"ms": round(ms, 3), "launch_groups": cnt}
-    ctx.timer_enable(False)
-    online_ms = sum(fams[f]["ms"] for f in ["sort_b2a", "sort_perm", "sort_round", "sort_tail"])
-
-    # roofline of the round kernel family: 64 bits x 3 rounds, n rows each
-    r_ms, r_cnt = fams["sort_round"]["ms"], fams["sort_round"]["launch_groups"]
+    # roofline of the dominant online kernel, k_round2 (64 bits x 3 launches per step)
+    r_ms, r_cnt = ctx.timer("k_round2")
     per_launch_ms = r_ms / max(r_cnt, 1)
-    round_bytes = 3 * n * ROUND_BYTES_PER_ROW + n * (12 + 36 + 36)  # 3 gathers + open/scatter
-    achieved = round_bytes / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
+    achieved = n * ROUND_BYTES_PER_ROW / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
+    ks_gbs = n * ROUND_KEYSTREAM_PER_ROW / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
 
     line = {
         "metric": "Oblivious radix sort Mrows/s @2^24x64b 3PC",
@@ -141,12 +146,15 @@ def main():
                    "rows": n, "key_bits": 64, "parties": 3, "batch_size": -12},
         "correct": ok,
         "online_ms": round(online_ms, 2),
-        "gen_perm_ms": fams["gen_perm"]["ms"],
-        "families_ms": fams,
-        "timed_pass_ms": round(t_timed * 1e3, 2),
-        "roofline": {"kernel": "sort_round (k_round2 x3 + k_open_scatter_pad) per bit",
-                     "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "gen_perm_ms": fams["gen_perm"]["ms_per_step"],
+        "families": fams,
+        "roofline": {"kernel": "k_round2", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": ROUND_TRAFFIC_2P24 if n == 1 << 24 else None,
+                     "avg_launch_us": round(per_launch_ms * 1e3, 1),
+                     "bytes_per_launch": n * ROUND_BYTES_PER_ROW,
+                     "keystream_GBps": round(ks_gbs, 1)},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample)

@@ -6,6 +6,34 @@
 
 using namespace orqg;
 
+size_t orqg_ctx::timer_event() {
+  // the pool only grows inside one API call; it is recycled at the next
+  // sync / timer read, when no region is open
+  if (ev_used == ev_pool.size()) {
+    hipEvent_t e;
+    ORQG_HIP(hipEventCreate(&e));
+    ev_pool.push_back(e);
+  }
+  return ev_used++;
+}
+
+void orqg_ctx::timer_flush() {
+  if (pending.empty()) {
+    ev_used = 0;
+    return;
+  }
+  ORQG_HIP(hipEventSynchronize(ev_pool[pending.back().b]));
+  for (const auto &t : pending) {
+    float ms = 0;
+    ORQG_HIP(hipEventElapsedTime(&ms, ev_pool[t.a], ev_pool[t.b]));
+    auto &tm = timers[t.fam];
+    tm.ms += ms;
+    tm.launches += 1;
+  }
+  pending.clear();
+  ev_used = 0;
+}
+
 void *orqg_ctx::scratch_get(const std::string &name, size_t bytes) {
   Scratch &s = scratch[name];
   if (s.bytes < bytes) {
@@ -86,8 +114,6 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     ORQG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     ORQG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     ORQG_HIP(hipEventCreateWithFlags(&c->ev_aux, hipEventDisableTiming));
-    ORQG_HIP(hipEventCreate(&c->ev0));
-    ORQG_HIP(hipEventCreate(&c->ev1));
     hipDeviceProp_t prop;
     ORQG_HIP(hipGetDeviceProperties(&prop, c->device));
     c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -117,8 +143,7 @@ int orqg_ctx_destroy(orqg_ctx *c) {
   for (auto &p : c->queue) free_perm(p);
   dev_free(c->d_rk);
   dev_free(c->d_t0);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev_aux) (void)hipEventDestroy(c->ev_aux);
   if (c->stream2) {
     (void)hipStreamSynchronize(c->stream2);
@@ -134,6 +159,7 @@ const char *orqg_last_error(orqg_ctx *c) { return c ? c->err.c_str() : "null con
 int orqg_sync(orqg_ctx *c) {
   API_BEGIN(c)
   ORQG_HIP(hipStreamSynchronize(c->stream));
+  c->timer_flush();
   API_END(c)
 }
 
@@ -417,6 +443,12 @@ int orqg_timer_enable(orqg_ctx *c, int enable) {
 }
 int orqg_timer_get(orqg_ctx *c, const char *family, double *ms, uint64_t *launches) {
   if (!c || !family) return ORQG_E_ARG;
+  try {
+    c->timer_flush();
+  } catch (const Error &e) {
+    c->err = e.what();
+    return e.code;
+  }
   auto it = c->timers.find(family);
   if (it == c->timers.end()) {
     if (ms) *ms = 0;
@@ -429,6 +461,12 @@ int orqg_timer_get(orqg_ctx *c, const char *family, double *ms, uint64_t *launch
 }
 int orqg_timer_reset(orqg_ctx *c) {
   if (!c) return ORQG_E_ARG;
+  try {
+    c->timer_flush();
+  } catch (const Error &e) {
+    c->err = e.what();
+    return e.code;
+  }
   c->timers.clear();
   return ORQG_OK;
 }

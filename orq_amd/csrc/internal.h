@@ -130,7 +130,17 @@ struct orqg_ctx {
   int num_cu = 256;
   bool timing = false;
   std::map<std::string, orqg::Timer> timers;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // timed regions record event pairs from a pool without waiting; the
+  // elapsed times are folded into `timers` at the next sync / timer read
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct PendingTimer {
+    std::string fam;
+    size_t a, b;
+  };
+  std::vector<PendingTimer> pending;
+  size_t timer_event();
+  void timer_flush();
 
   void *scratch_get(const std::string &name, size_t bytes);
   void log_msg(const char *op, int from, int to, uint64_t bytes);
@@ -151,22 +161,23 @@ namespace orqg {
 void *dev_alloc(orqg_ctx *c, size_t bytes);
 void dev_free(void *p);
 
-// Scoped timing of one kernel family on the context stream.
+// Scoped timing of one kernel family on the context stream: two events
+// bracket the region's launches; nothing waits for them here.
 struct TimedRegion {
   orqg_ctx *c;
   const char *fam;
+  size_t a = 0;
   TimedRegion(orqg_ctx *c_, const char *f) : c(c_), fam(f) {
-    if (c->timing) ORQG_HIP(hipEventRecord(c->ev0, c->stream));
+    if (c->timing) {
+      a = c->timer_event();
+      ORQG_HIP(hipEventRecord(c->ev_pool[a], c->stream));
+    }
   }
   ~TimedRegion() noexcept(false) {
     if (c->timing) {
-      ORQG_HIP(hipEventRecord(c->ev1, c->stream));
-      ORQG_HIP(hipEventSynchronize(c->ev1));
-      float ms = 0;
-      ORQG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      auto &t = c->timers[fam];
-      t.ms += ms;
-      t.launches += 1;
+      size_t b = c->timer_event();
+      ORQG_HIP(hipEventRecord(c->ev_pool[b], c->stream));
+      c->pending.push_back({fam, a, b});
     }
   }
 };

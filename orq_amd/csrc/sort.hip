@@ -396,8 +396,11 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
       for (int g = 0; g < 3; g++) {
         const int role = kGroup0 + g;
         const uint64_t nv = c->nonce[role], np = nv + uv;
-        hipLaunchKernelGGL(k_round2<L>, dim3(grid_rounds), dim3(kBT), 0, c->stream, c->d_t0,
-                           c->rk(role), nv, sv, np, sp, pi.inv[g], pin, pout, qin, qout);
+        {
+          TimedRegion tk(c, "k_round2");
+          hipLaunchKernelGGL(k_round2<L>, dim3(grid_rounds), dim3(kBT), 0, c->stream, c->d_t0,
+                             c->rk(role), nv, sv, np, sp, pi.inv[g], pin, pout, qin, qout);
+        }
         c->nonce[role] += uv + up;
         std::swap(pin, pout);
         std::swap(qin, qout);
