@@ -147,6 +147,37 @@ int orqg_apply_sharded_perm(orqg_ctx *ctx, orqg_vec *x, int inverse);
 /* oblivious_apply_elementwise_perm (shuffle.h:687-706); perm: width-4 vector */
 int orqg_apply_elementwise_perm(orqg_ctx *ctx, orqg_vec *x, const orqg_vec *perm);
 
+
+/* ---- ElementwisePermutation, shuffle, table sort ------------------------- */
+/* ElementwisePermutation(size, encoding) (include/core/containers/permutation.h:62-69):
+ * perm (width 4) = public sharing of the identity, encoding enc              */
+int orqg_perm_identity(orqg_ctx *ctx, orqg_vec *perm, int enc);
+/* ElementwisePermutation::invert (permutation.h:205-212)                     */
+int orqg_perm_invert(orqg_ctx *ctx, orqg_vec *perm);
+/* ElementwisePermutation::b2a for 3PC (permutation.h:222-258)                */
+int orqg_perm_b2a(orqg_ctx *ctx, orqg_vec *perm);
+/* ElementwisePermutation::negate (permutation.h:268-280): p = -p - 1         */
+int orqg_perm_negate(orqg_ctx *ctx, orqg_vec *perm);
+/* ElementwisePermutation::reverse (permutation.h:177-198): p = n - 1 - p     */
+int orqg_perm_reverse(orqg_ctx *ctx, orqg_vec *perm);
+/* compose_permutations(sigma, rho) (include/core/operators/shuffle.h:716-743):
+ * rho <- the composition; sigma is left holding pi(sigma) as in the reference */
+int orqg_perm_compose(orqg_ctx *ctx, orqg_vec *sigma, orqg_vec *rho);
+/* operators::shuffle of one vector (shuffle.h:752-757)                       */
+int orqg_shuffle(orqg_ctx *ctx, orqg_vec *x);
+/* operators::shuffle of a table (shuffle.h:768-786), called by
+ * EncodedTable::shuffle (encoded_table.h:1003-1018) with the A-shared and the
+ * B-shared columns in schema order                                            */
+int orqg_table_shuffle(orqg_ctx *ctx, orqg_vec *const *a_cols, size_t na,
+                       orqg_vec *const *b_cols, size_t nb);
+/* operators::table_sort with SortingProtocol::RADIXSORT (operators/sorting.h:509-612):
+ * keys[0] is the most significant sort column; orders[j] ORQG_ASC/ORQG_DESC;
+ * single_bit[j] != 0 sorts column j on its LSB only (NULL: none).  Every
+ * key, A-shared and B-shared column ends up in sorted row order.            */
+int orqg_table_sort(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, const int *orders,
+                    const int *single_bit, orqg_vec *const *a_cols, size_t na,
+                    orqg_vec *const *b_cols, size_t nb);
+
 /* ---- operators ------------------------------------------------------- */
 /* radix_sort (include/core/operators/radixsort.h:205-229): sorts the
  * B-shared vector v in place on its low `bits` bits (bits == 8*width sorts

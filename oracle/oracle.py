@@ -57,6 +57,14 @@ def lib():
             ("oc_compare", [vp, vp, vp, vp, vp]),
             ("oc_b2a", [vp, vp, vp]),
             ("oc_cmp", [vp, i32, vp, vp, vp]),
+            ("oc_perm_invert", [vp, vp, i32]),
+            ("oc_perm_b2a", [vp, vp]),
+            ("oc_perm_negate", [vp, vp]),
+            ("oc_perm_reverse", [vp, vp]),
+            ("oc_perm_compose", [vp, vp, vp, i32]),
+            ("oc_shuffle", [vp, vp, i32]),
+            ("oc_table_shuffle", [vp, vp, sz, vp, sz]),
+            ("oc_table_sort", [vp, vp, sz, vp, vp, vp, sz, vp, sz]),
         ]:
             f = getattr(L, name)
             f.restype = i32
@@ -220,6 +228,49 @@ class Ctx:
         y = Vec(x.n, x.width)
         _check(lib().oc_b2a(self.h, x.h, y.h), "b2a")
         return y
+
+    def perm_identity(self, n):
+        p = Vec(n, 4)
+        ident = np.arange(n, dtype=np.uint32)
+        z = np.zeros(n, np.uint32)
+        p.set_shares([ident, z, z])  # public_share: s_0 = value
+        return p
+
+    def perm_invert(self, p):
+        _check(lib().oc_perm_invert(self.h, p.h, 1), "perm_invert")
+
+    def perm_b2a(self, p):
+        _check(lib().oc_perm_b2a(self.h, p.h), "perm_b2a")
+
+    def perm_negate(self, p):
+        _check(lib().oc_perm_negate(self.h, p.h), "perm_negate")
+
+    def perm_reverse(self, p):
+        _check(lib().oc_perm_reverse(self.h, p.h), "perm_reverse")
+
+    def perm_compose(self, sigma, rho, arith=True):
+        _check(lib().oc_perm_compose(self.h, sigma.h, rho.h, int(arith)), "perm_compose")
+
+    def shuffle(self, x, arith=False):
+        _check(lib().oc_shuffle(self.h, x.h, int(arith)), "shuffle")
+
+    @staticmethod
+    def _vec_array(cols):
+        arr = (ctypes.c_void_p * max(len(cols), 1))(*[c.h for c in cols])
+        return arr, len(cols)
+
+    def table_shuffle(self, a_cols, b_cols):
+        a, na = self._vec_array(a_cols)
+        b, nb = self._vec_array(b_cols)
+        _check(lib().oc_table_shuffle(self.h, a, na, b, nb), "table_shuffle")
+
+    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=()):
+        k, nk = self._vec_array(keys)
+        o = (ctypes.c_int * nk)(*[1 if x in (1, "DESC") else 0 for x in (orders or [0] * nk)])
+        sb = (ctypes.c_int * nk)(*[int(bool(x)) for x in (single_bit or [0] * nk)])
+        a, na = self._vec_array(list(a_cols))
+        b, nb = self._vec_array(list(b_cols))
+        _check(lib().oc_table_sort(self.h, k, nk, o, sb, a, na, b, nb), "table_sort")
 
     def msglog(self):
         L = lib()

@@ -853,6 +853,118 @@ void radix_sort(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
   delete pad;
 }
 
+// ElementwisePermutation::reverse (permutation.h:177-198): p * (-1) + (n - 1)
+void perm_reverse(oc_ctx *c, oc_vec *perm) {
+  const size_t n = perm->n;
+  oc_vec *m1 = new_vec(n, 4), *ad = new_vec(n, 4);
+  std::vector<uint32_t> neg(n, 0xffffffffu), nm1(n, (uint32_t)(n - 1));
+  public_share<uint32_t>(m1, neg.data());
+  public_share<uint32_t>(ad, nm1.data());
+  mult<uint32_t, true>(c, perm, m1, perm, "mul_a");
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      uint32_t *a = S<uint32_t>(perm, p, k), *b = S<uint32_t>(ad, p, k);
+      for (size_t i = 0; i < n; i++) a[i] += b[i];
+    }
+  delete m1;
+  delete ad;
+}
+
+// compose_permutations (shuffle.h:716-743): rho <- rho o sigma
+void compose(oc_ctx *c, oc_vec *sigma, oc_vec *rho, bool arith) {
+  oc_ctx::Sharded pi = next_perm(c, sigma->n);
+  std::vector<int32_t> opened;
+  if (arith) {
+    apply_sharded<uint32_t, true>(c, sigma, pi, false);
+    auto o = open_vec<uint32_t, true>(c, sigma);
+    opened.assign(o.begin(), o.end());
+  } else {
+    apply_sharded<uint32_t, false>(c, sigma, pi, false);
+    auto o = open_vec<uint32_t, false>(c, sigma);
+    opened.assign(o.begin(), o.end());
+  }
+  for (size_t i = 0; i < rho->n; i++)
+    if (opened[i] < 0 || (size_t)opened[i] >= rho->n)
+      throw std::runtime_error("compose: opened permutation out of range");
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) gather<uint32_t>(S<uint32_t>(rho, p, k), opened.data(), rho->n);
+  if (arith)
+    apply_sharded<uint32_t, true>(c, rho, pi, true);
+  else
+    apply_sharded<uint32_t, false>(c, rho, pi, true);
+}
+
+void apply_ew_any(oc_ctx *c, oc_vec *x, bool x_arith, oc_vec *perm, bool perm_arith) {
+  if (x->width == 4) {
+    if (x_arith) apply_elementwise<uint32_t, true>(c, x, perm, perm_arith);
+    else apply_elementwise<uint32_t, false>(c, x, perm, perm_arith);
+  } else {
+    if (x_arith) apply_elementwise<uint64_t, true>(c, x, perm, perm_arith);
+    else apply_elementwise<uint64_t, false>(c, x, perm, perm_arith);
+  }
+}
+
+// operators::shuffle of a table (shuffle.h:768-786)
+void table_shuffle(oc_ctx *c, oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb) {
+  oc_vec *first = na ? a_cols[0] : nb ? b_cols[0] : nullptr;
+  if (!first) return;
+  const size_t n = first->n;
+  oc_vec *perm = new_vec(n, 4);
+  std::vector<uint32_t> id(n);
+  std::iota(id.begin(), id.end(), 0u);
+  public_share<uint32_t>(perm, id.data());
+  oc_ctx::Sharded pi = next_perm(c, n);
+  apply_sharded<uint32_t, false>(c, perm, pi, false);
+  for (size_t i = 0; i < na; i++) apply_ew_any(c, a_cols[i], true, perm, false);
+  for (size_t i = 0; i < nb; i++) apply_ew_any(c, b_cols[i], false, perm, false);
+  delete perm;
+}
+
+void radix_any(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
+  if (v->width == 4)
+    radix_sort<uint32_t, uint64_t, int64_t>(c, v, bits, desc, perm_out);
+  else
+    radix_sort<uint64_t, u128, i128>(c, v, bits, desc, perm_out);
+}
+
+// operators::table_sort with SortingProtocol::RADIXSORT (sorting.h:509-612)
+void table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, const int *single_bit,
+                oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb) {
+  const size_t n = keys[0]->n;
+  int ns = 0;
+  for (size_t j = 0; j < nkeys; j++) ns += single_bit && single_bit[j] ? 1 : 0;
+  const int nk = (int)nkeys - ns, nc = (int)(na + nb), L = 8 * keys[0]->width;
+  long perms_required = nk + ns - 1 + nk + ns;
+  long pairs_required = 4L * ns + 3L * nk + nc - 1 + (long)nk * L;
+  if (n == 0) return;
+  reserve(c, n, (size_t)(perms_required + pairs_required));
+  auto sort_sub = [&](int col, oc_vec *perm_out) {
+    int bits = single_bit && single_bit[col] ? 1 : 8 * keys[col]->width;
+    radix_any(c, keys[col], bits, orders && orders[col] == 1, perm_out);
+  };
+  int C = (int)nkeys - 1;
+  oc_vec *sort_perm = new_vec(n, 4);
+  oc_vec *orig = clone_vec(keys[C]);
+  sort_sub(C, sort_perm);
+  *keys[C] = *orig;
+  delete orig;
+  for (C--; C >= 0; C--) {
+    orig = clone_vec(keys[C]);
+    apply_ew_any(c, keys[C], false, sort_perm, true);
+    oc_vec *next = new_vec(n, 4);
+    sort_sub(C, next);
+    *keys[C] = *orig;
+    delete orig;
+    compose(c, sort_perm, next, true);
+    delete sort_perm;
+    sort_perm = next;
+  }
+  for (size_t j = 0; j < nkeys; j++) apply_ew_any(c, keys[j], false, sort_perm, true);
+  for (size_t i = 0; i < na; i++) apply_ew_any(c, a_cols[i], true, sort_perm, true);
+  for (size_t i = 0; i < nb; i++) apply_ew_any(c, b_cols[i], false, sort_perm, true);
+  delete sort_perm;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1087,6 +1199,38 @@ int oc_b2a(oc_ctx *c, oc_vec *x, oc_vec *y) {
     else
       throw std::runtime_error("b2a: width");
   })
+}
+
+int oc_perm_invert(oc_ctx *c, oc_vec *perm, int arith) {
+  OC_TRY({
+    if (!arith) throw std::runtime_error("perm_invert: only A-shared permutations");
+    perm_invert(c, perm);
+  })
+}
+int oc_perm_b2a(oc_ctx *c, oc_vec *perm) { OC_TRY(perm_b2a(c, perm)) }
+int oc_perm_negate(oc_ctx *c, oc_vec *perm) { OC_TRY(perm_negate(c, perm)) }
+int oc_perm_reverse(oc_ctx *c, oc_vec *perm) { OC_TRY(perm_reverse(c, perm)) }
+int oc_perm_compose(oc_ctx *c, oc_vec *sigma, oc_vec *rho, int arith) {
+  OC_TRY(compose(c, sigma, rho, arith != 0))
+}
+int oc_shuffle(oc_ctx *c, oc_vec *x, int arith) {
+  OC_TRY({
+    oc_ctx::Sharded pi = next_perm(c, x->n);
+    if (x->width == 4) {
+      if (arith) apply_sharded<uint32_t, true>(c, x, pi, false);
+      else apply_sharded<uint32_t, false>(c, x, pi, false);
+    } else {
+      if (arith) apply_sharded<uint64_t, true>(c, x, pi, false);
+      else apply_sharded<uint64_t, false>(c, x, pi, false);
+    }
+  })
+}
+int oc_table_shuffle(oc_ctx *c, oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb) {
+  OC_TRY(table_shuffle(c, a_cols, na, b_cols, nb))
+}
+int oc_table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, const int *single_bit,
+                  oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb) {
+  OC_TRY(table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb))
 }
 
 size_t oc_msglog_count(oc_ctx *c) { return c->log.size(); }

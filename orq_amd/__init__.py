@@ -48,6 +48,9 @@ EXPORTS = [
     "orqg_secret_share", "orqg_open", "orqg_public_share", "orqg_xor_b", "orqg_add_a",
     "orqg_sub_a", "orqg_not_b", "orqg_and_b", "orqg_mul_a", "orqg_b2a_bit", "orqg_reshare",
     "orqg_compare", "orqg_cmp", "orqg_b2a",
+    "orqg_perm_identity", "orqg_perm_invert", "orqg_perm_b2a", "orqg_perm_negate",
+    "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
+    "orqg_table_sort",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -94,6 +97,15 @@ def lib():
             "orqg_compare": (i32, [vp, vp, vp, vp, vp]),
             "orqg_cmp": (i32, [vp, i32, vp, vp, vp]),
             "orqg_b2a": (i32, [vp, vp, vp]),
+            "orqg_perm_identity": (i32, [vp, vp, i32]),
+            "orqg_perm_invert": (i32, [vp, vp]),
+            "orqg_perm_b2a": (i32, [vp, vp]),
+            "orqg_perm_negate": (i32, [vp, vp]),
+            "orqg_perm_reverse": (i32, [vp, vp]),
+            "orqg_perm_compose": (i32, [vp, vp, vp]),
+            "orqg_shuffle": (i32, [vp, vp]),
+            "orqg_table_shuffle": (i32, [vp, vp, sz, vp, sz]),
+            "orqg_table_sort": (i32, [vp, vp, sz, vp, vp, vp, sz, vp, sz]),
             "orqg_reshare": (i32, [vp, vp, i32]),
             "orqg_reserve_perms": (i32, [vp, sz, sz]),
             "orqg_perm_queue_size": (sz, [vp]),
@@ -314,6 +326,51 @@ class Ctx:
 
     def apply_elementwise(self, x, perm):
         self._check(lib().orqg_apply_elementwise_perm(self.h, x.h, perm.h), "apply_elementwise")
+
+    # ---- ElementwisePermutation, shuffle, table sort ----
+    def perm_identity(self, n, encoding=ORQG_ASHARED):
+        p = Vec(self, n, 4, encoding)
+        self._check(lib().orqg_perm_identity(self.h, p.h, encoding), "perm_identity")
+        return p
+
+    def perm_invert(self, p):
+        self._check(lib().orqg_perm_invert(self.h, p.h), "perm_invert")
+
+    def perm_b2a(self, p):
+        self._check(lib().orqg_perm_b2a(self.h, p.h), "perm_b2a")
+        p.encoding = ORQG_ASHARED
+
+    def perm_negate(self, p):
+        self._check(lib().orqg_perm_negate(self.h, p.h), "perm_negate")
+
+    def perm_reverse(self, p):
+        self._check(lib().orqg_perm_reverse(self.h, p.h), "perm_reverse")
+
+    def perm_compose(self, sigma, rho):
+        """rho <- compose_permutations(sigma, rho); sigma is consumed."""
+        self._check(lib().orqg_perm_compose(self.h, sigma.h, rho.h), "perm_compose")
+
+    def shuffle(self, x):
+        self._check(lib().orqg_shuffle(self.h, x.h), "shuffle")
+
+    @staticmethod
+    def _vec_array(cols):
+        arr = (ctypes.c_void_p * max(len(cols), 1))(*[c.h for c in cols])
+        return arr, len(cols)
+
+    def table_shuffle(self, a_cols, b_cols):
+        a, na = self._vec_array(a_cols)
+        b, nb = self._vec_array(b_cols)
+        self._check(lib().orqg_table_shuffle(self.h, a, na, b, nb), "table_shuffle")
+
+    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=()):
+        k, nk = self._vec_array(keys)
+        o = (ctypes.c_int * nk)(*[(DESC if x in (DESC, "DESC") else ASC) for x in
+                                  (orders or [ASC] * nk)])
+        sb = (ctypes.c_int * nk)(*[int(bool(x)) for x in (single_bit or [0] * nk)])
+        a, na = self._vec_array(list(a_cols))
+        b, nb = self._vec_array(list(b_cols))
+        self._check(lib().orqg_table_sort(self.h, k, nk, o, sb, a, na, b, nb), "table_sort")
 
     def radix_sort(self, v, bits=None, desc=False):
         if bits is None:

@@ -1,6 +1,8 @@
 // extern "C" entry points of liborqg (declared in include/orqg.h) and the
 // context / vector / message-log plumbing behind them.
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 
 #include "internal.h"
 
@@ -66,10 +68,20 @@ void orqg_ctx::log_msg(const char *op, int from, int to, uint64_t bytes) {
 }
 
 namespace orqg {
+// Device bytes held by the library in this process (all contexts), checked
+// against a context's HBM budget when it allocates.
 static size_t g_live_bytes = 0;
+static std::unordered_map<void *, size_t> g_sizes;
+static std::mutex g_alloc_mu;
+
+size_t live_bytes() {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  return g_live_bytes;
+}
 
 void *dev_alloc(orqg_ctx *c, size_t bytes) {
   if (bytes == 0) bytes = 16;
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
   if (c->budget && g_live_bytes + bytes > c->budget)
     throw Error(ORQG_E_BUDGET, "allocation exceeds the configured HBM budget");
   void *p = nullptr;
@@ -78,10 +90,18 @@ void *dev_alloc(orqg_ctx *c, size_t bytes) {
     throw Error(ORQG_E_OOM, "hipMalloc failed");
   }
   g_live_bytes += bytes;
+  g_sizes[p] = bytes;
   return p;
 }
 void dev_free(void *p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  (void)hipFree(p);
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  auto it = g_sizes.find(p);
+  if (it != g_sizes.end()) {
+    g_live_bytes -= it->second;
+    g_sizes.erase(it);
+  }
 }
 }  // namespace orqg
 
@@ -374,6 +394,95 @@ int orqg_b2a(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
 int orqg_reshare(orqg_ctx *c, orqg_vec *v, int group) {
   API_BEGIN(c)
   reshare(c, v, group);
+  API_END(c)
+}
+
+static void check_perm(const orqg_vec *p) {
+  if (!p || p->width != 4) throw Error(ORQG_E_ARG, "permutation must be a width-4 vector");
+}
+static void check_cols(orqg_vec *const *cols, size_t k, size_t n) {
+  if (k && !cols) throw Error(ORQG_E_ARG, "column array is NULL");
+  for (size_t i = 0; i < k; i++)
+    if (!cols[i] || cols[i]->n != n) throw Error(ORQG_E_ARG, "columns differ in size");
+}
+
+int orqg_perm_identity(orqg_ctx *c, orqg_vec *perm, int enc) {
+  API_BEGIN(c)
+  check_perm(perm);
+  if (enc != ORQG_ASHARED && enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "encoding");
+  perm_identity(c, perm, enc);
+  API_END(c)
+}
+int orqg_perm_invert(orqg_ctx *c, orqg_vec *perm) {
+  API_BEGIN(c)
+  check_perm(perm);
+  perm_invert(c, perm);
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+int orqg_perm_b2a(orqg_ctx *c, orqg_vec *perm) {
+  API_BEGIN(c)
+  check_perm(perm);
+  perm_b2a(c, perm);
+  API_END(c)
+}
+int orqg_perm_negate(orqg_ctx *c, orqg_vec *perm) {
+  API_BEGIN(c)
+  check_perm(perm);
+  perm_negate(c, perm);
+  API_END(c)
+}
+int orqg_perm_reverse(orqg_ctx *c, orqg_vec *perm) {
+  API_BEGIN(c)
+  check_perm(perm);
+  perm_reverse(c, perm);
+  API_END(c)
+}
+int orqg_perm_compose(orqg_ctx *c, orqg_vec *sigma, orqg_vec *rho) {
+  API_BEGIN(c)
+  check_perm(sigma);
+  check_perm(rho);
+  perm_compose(c, sigma, rho);
+  API_END(c)
+}
+int orqg_shuffle(orqg_ctx *c, orqg_vec *x) {
+  API_BEGIN(c)
+  if (!x) throw Error(ORQG_E_ARG, "shuffle: NULL vector");
+  if (x->n > 0) {
+    ShardedPerm pi = next_perm(c, x->n);
+    try {
+      apply_sharded(c, x, pi, false);
+    } catch (...) {
+      (void)hipStreamSynchronize(c->stream);
+      free_perm(pi);
+      throw;
+    }
+    ORQG_HIP(hipStreamSynchronize(c->stream));
+    free_perm(pi);
+  }
+  API_END(c)
+}
+int orqg_table_shuffle(orqg_ctx *c, orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols,
+                       size_t nb) {
+  API_BEGIN(c)
+  size_t n = na ? a_cols[0]->n : nb ? b_cols[0]->n : 0;
+  check_cols(a_cols, na, n);
+  check_cols(b_cols, nb, n);
+  table_shuffle(c, a_cols, na, b_cols, nb);
+  API_END(c)
+}
+int orqg_table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
+                    const int *single_bit, orqg_vec *const *a_cols, size_t na,
+                    orqg_vec *const *b_cols, size_t nb) {
+  API_BEGIN(c)
+  if (nkeys == 0 || !keys || !keys[0]) throw Error(ORQG_E_ARG, "table_sort: no sort keys");
+  const size_t n = keys[0]->n;
+  check_cols(keys, nkeys, n);
+  check_cols(a_cols, na, n);
+  check_cols(b_cols, nb, n);
+  for (size_t j = 0; j < nkeys; j++)
+    if (keys[j]->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "table_sort: keys must be B-shared");
+  table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb);
   API_END(c)
 }
 

@@ -206,6 +206,17 @@ void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_ou
 void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, orqg_vec *gt,
              bool full, int neg_eq, int neg_gt);
 void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
+void perm_identity(orqg_ctx *c, orqg_vec *perm, int enc);
+void perm_b2a(orqg_ctx *c, orqg_vec *perm);
+void perm_negate(orqg_ctx *c, orqg_vec *perm);
+void perm_reverse(orqg_ctx *c, orqg_vec *perm);
+void perm_invert(orqg_ctx *c, orqg_vec *perm);
+void perm_compose(orqg_ctx *c, orqg_vec *sigma, orqg_vec *rho);
+void table_shuffle(orqg_ctx *c, orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols,
+                   size_t nb);
+void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
+                const int *single_bit, orqg_vec *const *a_cols, size_t na,
+                orqg_vec *const *b_cols, size_t nb);
 
 enum LocalOp { kXor = 0, kAdd = 1, kSub = 2, kNot = 3 };
 

@@ -316,34 +316,6 @@ __global__ void k_unpad(uint64_t n, bool desc, const uint32_t *__restrict__ P, K
 
 // open a [n][3] uint32 vector (XOR or sum) and write it as the public sharing
 // (o, 0, 0) into out (open_shares_* then public_share)
-__global__ void k_open_public(uint64_t n, bool arith_in, const uint32_t *__restrict__ in,
-                              uint32_t *__restrict__ out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t *s = in + 3 * i;
-    uint32_t o = arith_in ? s[0] + s[1] + s[2] : s[0] ^ s[1] ^ s[2];
-    out[3 * i] = o;
-    out[3 * i + 1] = 0;
-    out[3 * i + 2] = 0;
-  }
-}
-
-__global__ void k_iota_public(uint64_t n, uint32_t *__restrict__ out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    out[3 * i] = (uint32_t)i;
-    out[3 * i + 1] = 0;
-    out[3 * i + 2] = 0;
-  }
-}
-
-// x + (-1, 0, 0) share-wise (adding the public sharing of -1)
-__global__ void k_add_public_m1(uint64_t n, uint32_t *__restrict__ x) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    x[3 * i] += 0xffffffffu;
-}
-
 template <int L>
 void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) {
   const uint32_t tszv = L == 3 ? 16 : 8;  // PadWidth element size
@@ -454,43 +426,13 @@ void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_ou
                        (uint32_t *)v->d, pb);
   ORQG_HIP(hipGetLastError());
   TimedRegion tr(c, "sort_tail");
-  // ElementwisePermutation::b2a (permutation.h:222-258)
-  {
-    ShardedPerm pi = next_perm(c, n);
-    orqg_vec cp = *perm_out;
-    cp.enc = ORQG_BSHARED;
-    cp.d = c->scratch_get("rs_b2a_copy", n * 12 + 64);
-    ORQG_HIP(hipMemcpyAsync(cp.d, perm_out->d, n * 12, hipMemcpyDeviceToDevice, c->stream));
-    apply_sharded(c, &cp, pi, false);
-    for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, n * 4);
-    hipLaunchKernelGGL(k_open_public, dim3(grid_flat), dim3(256), 0, c->stream, n, false,
-                       (const uint32_t *)cp.d, (uint32_t *)perm_out->d);
-    perm_out->enc = ORQG_ASHARED;
-    apply_sharded(c, perm_out, pi, true);
-    free_perm(pi);
-  }
-  if (desc) {
-    // ElementwisePermutation::negate (permutation.h:268-280): p = p * (-1) + (-1)
-    orqg_vec m1 = *perm_out;
-    m1.d = c->scratch_get("rs_m1", n * 12 + 64);
-    std::vector<uint32_t> host(3 * n, 0);
-    for (uint64_t i = 0; i < n; i++) host[3 * i] = 0xffffffffu;
-    ORQG_HIP(hipMemcpyAsync(m1.d, host.data(), n * 12, hipMemcpyHostToDevice, c->stream));
-    mult(c, true, perm_out, &m1, perm_out);
-    hipLaunchKernelGGL(k_add_public_m1, dim3(grid_flat), dim3(256), 0, c->stream, n,
-                       (uint32_t *)perm_out->d);
-    ORQG_HIP(hipStreamSynchronize(c->stream));
-  }
-  // ElementwisePermutation::invert (permutation.h:205-212)
-  {
-    orqg_vec inv = *perm_out;
-    inv.enc = ORQG_ASHARED;
-    inv.d = c->scratch_get("rs_inv", n * 12 + 64);
-    hipLaunchKernelGGL(k_iota_public, dim3(grid_flat), dim3(256), 0, c->stream, n,
-                       (uint32_t *)inv.d);
-    apply_elementwise(c, &inv, perm_out);
-    ORQG_HIP(hipMemcpyAsync(perm_out->d, inv.d, n * 12, hipMemcpyDeviceToDevice, c->stream));
-  }
+  // ElementwisePermutation::b2a, negate (DESC) and invert
+  // (permutation.h:205-280), as radix_sort does after remove_padding
+  // (radixsort.h:222-227, sorting.h:475-495)
+  perm_out->enc = ORQG_BSHARED;
+  perm_b2a(c, perm_out);
+  if (desc) perm_negate(c, perm_out);
+  perm_invert(c, perm_out);
   ORQG_HIP(hipStreamSynchronize(c->stream));
 }
 
