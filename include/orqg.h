@@ -90,6 +90,11 @@ void *orqg_vec_data(orqg_vec *v);
 int orqg_vec_write(orqg_ctx *ctx, orqg_vec *v, const void *host_shares);
 int orqg_vec_read(orqg_ctx *ctx, const orqg_vec *v, void *host_shares);
 int orqg_vec_copy(orqg_ctx *ctx, const orqg_vec *src, orqg_vec *dst);
+/* EVector resize as EncodedTable::resize uses it (encoded_table.h:1426-1436):
+ * keep the first min(n, size) rows, new rows are zero shares                */
+int orqg_vec_resize(orqg_ctx *ctx, orqg_vec *v, size_t n);
+/* keep the last n rows (EncodedTable::tail, encoded_table.h:1407-1418)       */
+int orqg_vec_tail(orqg_ctx *ctx, orqg_vec *v, size_t n);
 
 /* ---- randomness ------------------------------------------------------ */
 /* AESPRGAlgorithm::fillBytes of PRG `role` into device memory, advancing its
@@ -170,6 +175,11 @@ int orqg_shuffle(orqg_ctx *ctx, orqg_vec *x);
  * B-shared columns in schema order                                            */
 int orqg_table_shuffle(orqg_ctx *ctx, orqg_vec *const *a_cols, size_t na,
                        orqg_vec *const *b_cols, size_t nb);
+/* operators::bitonic_sort (operators/sorting.h:266-353): keys[0] most
+ * significant, orders[j] ORQG_ASC/ORQG_DESC; all columns of one width, the
+ * row count a power of two; at most 8 columns of each kind.  Not stable.     */
+int orqg_bitonic_sort(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, const int *orders,
+                      orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols, size_t nb);
 /* operators::table_sort with SortingProtocol::RADIXSORT (operators/sorting.h:509-612):
  * keys[0] is the most significant sort column; orders[j] ORQG_ASC/ORQG_DESC;
  * single_bit[j] != 0 sorts column j on its LSB only (NULL: none).  Every

@@ -65,6 +65,7 @@ def lib():
             ("oc_shuffle", [vp, vp, i32]),
             ("oc_table_shuffle", [vp, vp, sz, vp, sz]),
             ("oc_table_sort", [vp, vp, sz, vp, vp, vp, sz, vp, sz]),
+            ("oc_bitonic_sort", [vp, vp, sz, vp, vp, sz, vp, sz]),
         ]:
             f = getattr(L, name)
             f.restype = i32
@@ -271,6 +272,13 @@ class Ctx:
         a, na = self._vec_array(list(a_cols))
         b, nb = self._vec_array(list(b_cols))
         _check(lib().oc_table_sort(self.h, k, nk, o, sb, a, na, b, nb), "table_sort")
+
+    def bitonic_sort(self, keys, orders=None, a_cols=(), b_cols=()):
+        k, nk = self._vec_array(keys)
+        o = (ctypes.c_int * nk)(*[1 if x in (1, "DESC") else 0 for x in (orders or [0] * nk)])
+        a, na = self._vec_array(list(a_cols))
+        b, nb = self._vec_array(list(b_cols))
+        _check(lib().oc_bitonic_sort(self.h, k, nk, o, a, na, b, nb), "bitonic_sort")
 
     def msglog(self):
         L = lib()

@@ -965,6 +965,155 @@ void table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, const
   delete sort_perm;
 }
 
+// ---- bitonic sort (operators/sorting.h:80-353, operators/common.h:20-48) ----
+template <typename T>
+oc_vec *gather_rows(const oc_vec *v, const std::vector<size_t> &idx) {
+  oc_vec *o = new_vec(idx.size(), sizeof(T));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(const_cast<oc_vec *>(v), p, k);
+      T *d = S<T>(o, p, k);
+      for (size_t i = 0; i < idx.size(); i++) d[i] = a[idx[i]];
+    }
+  return o;
+}
+template <typename T>
+void scatter_rows(oc_vec *v, const oc_vec *src, const std::vector<size_t> &idx) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(const_cast<oc_vec *>(src), p, k);
+      T *d = S<T>(v, p, k);
+      for (size_t i = 0; i < idx.size(); i++) d[idx[i]] = a[i];
+    }
+}
+// z = x op y share-wise (local)
+template <typename T>
+oc_vec *local_op(const oc_vec *x, const oc_vec *y, int op) {
+  oc_vec *z = new_vec(x->n, sizeof(T));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(const_cast<oc_vec *>(x), p, k), *b = S<T>(const_cast<oc_vec *>(y), p, k);
+      T *d = S<T>(z, p, k);
+      for (size_t i = 0; i < x->n; i++)
+        d[i] = op == 0 ? (T)(a[i] ^ b[i]) : op == 1 ? (T)(a[i] + b[i]) : (T)(a[i] - b[i]);
+    }
+  return z;
+}
+// multiplex (common.h:20-48): boolean a ^ (extend_lsb(sel) & (b ^ a)),
+// arithmetic a + sel * (b - a)
+template <typename T>
+oc_vec *multiplex_b(oc_ctx *c, const oc_vec *sel, const oc_vec *a, const oc_vec *b) {
+  oc_vec *se = new_vec(sel->n, sizeof(T));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *s = S<T>(const_cast<oc_vec *>(sel), p, k);
+      T *d = S<T>(se, p, k);
+      for (size_t i = 0; i < sel->n; i++) d[i] = (T)(0 - (s[i] & 1));  // extend_lsb
+    }
+  oc_vec *d = local_op<T>(b, a, 0);
+  mult<T, false>(c, se, d, d, "and_b");
+  oc_vec *r = local_op<T>(a, d, 0);
+  delete se;
+  delete d;
+  return r;
+}
+template <typename T>
+oc_vec *multiplex_a(oc_ctx *c, const oc_vec *sel, const oc_vec *a, const oc_vec *b) {
+  oc_vec *d = local_op<T>(b, a, 2);
+  mult<T, true>(c, const_cast<oc_vec *>(sel), d, d, "mul_a");
+  oc_vec *r = local_op<T>(a, d, 1);
+  delete d;
+  return r;
+}
+
+template <typename T>
+void bitonic_sort(oc_ctx *c, oc_vec **keys, size_t nk, const int *orders, oc_vec **a_cols,
+                  size_t na, oc_vec **b_cols, size_t nb) {
+  const size_t n = keys[0]->n;
+  if (n < 2) return;
+  if (n & (n - 1)) throw std::runtime_error("bitonic_sort: size must be a power of two");
+  int rounds = 0;
+  while (((size_t)1 << rounds) < n) rounds++;
+  const size_t m = n / 2;
+  std::vector<size_t> xi(m), yi(m);
+  for (int i = 0; i < rounds; i++)
+    for (int j = 0; j <= i; j++) {
+      const size_t h = (size_t)1 << (i - j);
+      const bool mirror = j == 0;  // box_direction_2 == -1
+      for (size_t q = 0; q < m; q++) {
+        size_t box = q / h, t = q % h;
+        xi[q] = box * 2 * h + t;
+        yi[q] = mirror ? box * 2 * h + 2 * h - 1 - t : box * 2 * h + h + t;
+      }
+      // compare_rows (sorting.h:80-108)
+      std::vector<oc_vec *> X(nk), Y(nk);
+      for (size_t k = 0; k < nk; k++) {
+        X[k] = gather_rows<T>(keys[k], xi);
+        Y[k] = gather_rows<T>(keys[k], yi);
+      }
+      oc_vec *eq = new_vec(m, sizeof(T)), *gt = new_vec(m, sizeof(T));
+      {
+        bool inv = orders && orders[0] == 1;
+        compare<T>(c, inv ? Y[0] : X[0], inv ? X[0] : Y[0], eq, gt);
+      }
+      for (size_t k = 1; k < nk; k++) {
+        bool inv = orders && orders[k] == 1;
+        oc_vec *neq = new_vec(m, sizeof(T)), *ngt = new_vec(m, sizeof(T));
+        compare<T>(c, inv ? Y[k] : X[k], inv ? X[k] : Y[k], neq, ngt);
+        oc_vec *t = new_vec(m, sizeof(T));
+        mult<T, false>(c, ngt, eq, t, "and_b");  // new_gt & eq
+        oc_vec *g2 = local_op<T>(gt, t, 0);
+        delete gt;
+        gt = g2;
+        mult<T, false>(c, eq, neq, eq, "and_b");  // eq & new_eq
+        delete neq;
+        delete ngt;
+        delete t;
+      }
+      for (int p = 0; p < 3; p++)
+        for (int k = 0; k < 2; k++) {
+          T *g = S<T>(gt, p, k);
+          for (size_t q = 0; q < m; q++) g[q] &= 1;  // mask(1)
+        }
+      // swap (sorting.h:146-162): tmp = mux(bits, x, y); y = mux(bits, y, x); x = tmp
+      auto swap_b = [&](oc_vec *col) {
+        oc_vec *x = gather_rows<T>(col, xi), *y = gather_rows<T>(col, yi);
+        oc_vec *tmp = multiplex_b<T>(c, gt, x, y);
+        oc_vec *ny = multiplex_b<T>(c, gt, y, x);
+        scatter_rows<T>(col, tmp, xi);
+        scatter_rows<T>(col, ny, yi);
+        delete x;
+        delete y;
+        delete tmp;
+        delete ny;
+      };
+      for (size_t k = 0; k < nk; k++) swap_b(keys[k]);
+      for (size_t k = 0; k < nb; k++) swap_b(b_cols[k]);
+      if (na > 0) {
+        oc_vec *ga = new_vec(m, sizeof(T));
+        b2a_bit<T>(c, gt, ga);
+        for (size_t k = 0; k < na; k++) {
+          oc_vec *x = gather_rows<T>(a_cols[k], xi), *y = gather_rows<T>(a_cols[k], yi);
+          oc_vec *tmp = multiplex_a<T>(c, ga, x, y);
+          oc_vec *ny = multiplex_a<T>(c, ga, y, x);
+          scatter_rows<T>(a_cols[k], tmp, xi);
+          scatter_rows<T>(a_cols[k], ny, yi);
+          delete x;
+          delete y;
+          delete tmp;
+          delete ny;
+        }
+        delete ga;
+      }
+      for (size_t k = 0; k < nk; k++) {
+        delete X[k];
+        delete Y[k];
+      }
+      delete eq;
+      delete gt;
+    }
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1231,6 +1380,16 @@ int oc_table_shuffle(oc_ctx *c, oc_vec **a_cols, size_t na, oc_vec **b_cols, siz
 int oc_table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, const int *single_bit,
                   oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb) {
   OC_TRY(table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb))
+}
+
+int oc_bitonic_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, oc_vec **a_cols,
+                    size_t na, oc_vec **b_cols, size_t nb) {
+  OC_TRY({
+    if (keys[0]->width == 4)
+      bitonic_sort<uint32_t>(c, keys, nkeys, orders, a_cols, na, b_cols, nb);
+    else
+      bitonic_sort<uint64_t>(c, keys, nkeys, orders, a_cols, na, b_cols, nb);
+  })
 }
 
 size_t oc_msglog_count(oc_ctx *c) { return c->log.size(); }

@@ -50,7 +50,7 @@ EXPORTS = [
     "orqg_compare", "orqg_cmp", "orqg_b2a",
     "orqg_perm_identity", "orqg_perm_invert", "orqg_perm_b2a", "orqg_perm_negate",
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
-    "orqg_table_sort",
+    "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -106,6 +106,9 @@ def lib():
             "orqg_shuffle": (i32, [vp, vp]),
             "orqg_table_shuffle": (i32, [vp, vp, sz, vp, sz]),
             "orqg_table_sort": (i32, [vp, vp, sz, vp, vp, vp, sz, vp, sz]),
+            "orqg_bitonic_sort": (i32, [vp, vp, sz, vp, vp, sz, vp, sz]),
+            "orqg_vec_resize": (i32, [vp, vp, sz]),
+            "orqg_vec_tail": (i32, [vp, vp, sz]),
             "orqg_reshare": (i32, [vp, vp, i32]),
             "orqg_reserve_perms": (i32, [vp, sz, sz]),
             "orqg_perm_queue_size": (sz, [vp]),
@@ -371,6 +374,22 @@ class Ctx:
         a, na = self._vec_array(list(a_cols))
         b, nb = self._vec_array(list(b_cols))
         self._check(lib().orqg_table_sort(self.h, k, nk, o, sb, a, na, b, nb), "table_sort")
+
+    def bitonic_sort(self, keys, orders=None, a_cols=(), b_cols=()):
+        k, nk = self._vec_array(keys)
+        o = (ctypes.c_int * nk)(*[(DESC if x in (DESC, "DESC") else ASC) for x in
+                                  (orders or [ASC] * nk)])
+        a, na = self._vec_array(list(a_cols))
+        b, nb = self._vec_array(list(b_cols))
+        self._check(lib().orqg_bitonic_sort(self.h, k, nk, o, a, na, b, nb), "bitonic_sort")
+
+    def resize(self, v, n):
+        self._check(lib().orqg_vec_resize(self.h, v.h, n), "vec_resize")
+        v.n = n
+
+    def tail(self, v, n):
+        self._check(lib().orqg_vec_tail(self.h, v.h, n), "vec_tail")
+        v.n = n
 
     def radix_sort(self, v, bits=None, desc=False):
         if bits is None:

@@ -229,6 +229,39 @@ int orqg_vec_create(orqg_ctx *c, size_t n, int width, int encoding, orqg_vec **o
   API_END(c)
 }
 
+// EVector::resize (e_vector.h): keep the first min(n, old) rows; new rows
+// hold zero shares.
+int orqg_vec_resize(orqg_ctx *c, orqg_vec *v, size_t n) {
+  API_BEGIN(c)
+  if (!v) throw Error(ORQG_E_ARG, "vec_resize: NULL vector");
+  if (n == v->n) return ORQG_OK;
+  const size_t w = v->width;
+  void *d = dev_alloc(c, 3 * n * w + 64);
+  const size_t keep = n < v->n ? n : v->n;
+  ORQG_HIP(hipMemsetAsync(d, 0, 3 * n * w, c->stream));
+  ORQG_HIP(hipMemcpyAsync(d, v->d, 3 * keep * w, hipMemcpyDeviceToDevice, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  dev_free(v->d);
+  v->d = d;
+  v->n = n;
+  API_END(c)
+}
+// EncodedTable::tail (encoded_table.h:1407-1418): keep the last n rows.
+int orqg_vec_tail(orqg_ctx *c, orqg_vec *v, size_t n) {
+  API_BEGIN(c)
+  if (!v || n > v->n) throw Error(ORQG_E_ARG, "vec_tail: n exceeds the vector size");
+  if (n == v->n) return ORQG_OK;
+  const size_t w = v->width;
+  void *d = dev_alloc(c, 3 * n * w + 64);
+  ORQG_HIP(hipMemcpyAsync(d, (char *)v->d + 3 * (v->n - n) * w, 3 * n * w,
+                          hipMemcpyDeviceToDevice, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  dev_free(v->d);
+  v->d = d;
+  v->n = n;
+  API_END(c)
+}
+
 // hipFree waits for outstanding work on the device; the owning context may
 // already be gone, so it is not touched here.
 int orqg_vec_free(orqg_vec *v) {
@@ -483,6 +516,20 @@ int orqg_table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int 
   for (size_t j = 0; j < nkeys; j++)
     if (keys[j]->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "table_sort: keys must be B-shared");
   table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb);
+  API_END(c)
+}
+
+int orqg_bitonic_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
+                      orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols, size_t nb) {
+  API_BEGIN(c)
+  if (nkeys == 0 || !keys || !keys[0]) throw Error(ORQG_E_ARG, "bitonic_sort: no sort keys");
+  const size_t n = keys[0]->n;
+  check_cols(keys, nkeys, n);
+  check_cols(a_cols, na, n);
+  check_cols(b_cols, nb, n);
+  for (size_t j = 0; j < nkeys; j++)
+    if (keys[j]->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "bitonic_sort: keys must be B-shared");
+  bitonic_sort(c, keys, nkeys, orders, a_cols, na, b_cols, nb);
   API_END(c)
 }
 

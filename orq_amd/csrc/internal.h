@@ -214,6 +214,8 @@ void perm_invert(orqg_ctx *c, orqg_vec *perm);
 void perm_compose(orqg_ctx *c, orqg_vec *sigma, orqg_vec *rho);
 void table_shuffle(orqg_ctx *c, orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols,
                    size_t nb);
+void bitonic_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const int *orders,
+                  orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols, size_t nb);
 void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
                 const int *single_bit, orqg_vec *const *a_cols, size_t na,
                 orqg_vec *const *b_cols, size_t nb);

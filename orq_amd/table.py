@@ -18,6 +18,7 @@ from . import ASC, DESC, ORQG_ASHARED, ORQG_BSHARED
 VALID = "[##V]"
 JOIN_ID = "[##TID]"
 RADIXSORT = "RADIXSORT"
+BITONICSORT = "BITONICSORT"
 
 
 def is_bshared(name):
@@ -62,10 +63,26 @@ class EncodedTable:
         self.ctx.table_shuffle(a, b)
         return self
 
+    def resize(self, n):
+        """EncodedTable::resize (encoded_table.h:1426-1436): new rows are zero shares."""
+        for v in self.schema.values():
+            self.ctx.resize(v, n)
+        self.rows = n
+
+    def tail(self, n):
+        """EncodedTable::tail (encoded_table.h:1407-1418): keep the last n rows."""
+        for v in self.schema.values():
+            self.ctx.tail(v, n)
+        self.rows = n
+
+    def pad_power_of_two(self):
+        """encoded_table.h:1446-1466 with pad value 0."""
+        self.resize(1 << max(self.rows - 1, 0).bit_length())
+
     def sort(self, spec, to_be_sorted=None, protocol=RADIXSORT):
         """EncodedTable::sort (encoded_table.h:900-998).  `spec` is a list of
         column names (all ascending) or of (name, order) pairs."""
-        if protocol != RADIXSORT:
+        if protocol not in (RADIXSORT, BITONICSORT):
             raise ValueError(f"sorting protocol {protocol!r} is not provided")
         spec = [(s, ASC) if isinstance(s, str) else (s[0], s[1]) for s in spec]
         keys = [k for k, _ in spec]
@@ -78,6 +95,16 @@ class EncodedTable:
         data = [c for c in self.column_names() if c not in keys and c in tbs]
         a = [self.schema[c] for c in data if not is_bshared(c)]
         b = [self.schema[c] for c in data if is_bshared(c)]
+        if protocol == BITONICSORT:
+            original = self.rows
+            self.pad_power_of_two()
+            self.ctx.bitonic_sort([self.schema[k] for k in keys], orders, a, b)
+            if VALID in keys:  # padded rows are invalid: drop them again
+                if orders[keys.index(VALID)] == ASC:
+                    self.tail(original)
+                else:
+                    self.resize(original)
+            return self
         self.ctx.table_sort([self.schema[k] for k in keys], orders, single, a, b)
         return self
 

@@ -156,3 +156,80 @@ def test_table_sort_valid_bit_and_int64_payload(O, G, keys):
     np.testing.assert_array_equal(got["[K]"], key[order])
     np.testing.assert_array_equal(got["P"], pay[order])
     np.testing.assert_array_equal(got["[##V]"], valid[order])
+
+
+# ---- bitonic sort ----------------------------------------------------------------
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("n", [2, 256, 4096])
+@pytest.mark.parametrize("shape", ["one_key", "two_keys_data"])
+def test_bitonic_sort(O, G, keys, width, n, shape):
+    rng = np.random.default_rng(n + width)
+    ut = np.uint32 if width == 4 else np.uint64
+    k1 = rng.integers(-4, 4, n).astype(np.int64)
+    k2 = rng.integers(-(2**30), 2**30, n).astype(np.int64)
+    a = rng.integers(0, 2**31, n).astype(np.int64)
+    b = rng.integers(0, 2**31, n).astype(np.int64)
+    oc, gc = O.Ctx(keys), G.Ctx(keys)
+
+    def share(c, v, arith=False):
+        return c.secret_share(v.astype(ut), width, arith=arith)
+
+    if shape == "one_key":
+        ov, gv = [share(oc, k2)], [share(gc, k2)]
+        oc.bitonic_sort(ov, ["DESC"])
+        gc.bitonic_sort(gv, ["DESC"])
+        oa = ga = ob = gb = []
+    else:
+        ov, gv = [share(oc, k1), share(oc, k2)], [share(gc, k1), share(gc, k2)]
+        oa, ga = [share(oc, a, True)], [share(gc, a, True)]
+        ob, gb = [share(oc, b)], [share(gc, b)]
+        oc.bitonic_sort(ov, ["ASC", "DESC"], oa, ob)
+        gc.bitonic_sort(gv, ["ASC", "DESC"], ga, gb)
+    for x, y in zip(ov + oa + ob, gv + ga + gb):
+        _same_shares(x, y)
+    _same_nonces(oc, gc)
+    _same_log(oc, gc)
+    sdt = np.int32 if width == 4 else np.int64
+    if shape == "one_key":
+        np.testing.assert_array_equal(gc.open(gv[0]).view(sdt), np.sort(k2.astype(sdt))[::-1])
+    else:
+        order = np.lexsort((-k2, k1))
+        np.testing.assert_array_equal(gc.open(gv[0]).view(sdt), k1[order].astype(sdt))
+        np.testing.assert_array_equal(gc.open(gv[1]).view(sdt), k2[order].astype(sdt))
+
+
+def test_reference_bitonic_tables_gpu(G, keys):
+    # tests/test_bitonic_sort.cpp:141-196 through the EncodedTable mirror
+    from orq_amd.table import BITONICSORT, EncodedTable
+    from test_oracle_table import BITONIC_COLUMNS, INT_MAX, INT_MIN
+
+    gc = G.Ctx(keys)
+    t = EncodedTable.secret_share(gc, BITONIC_COLUMNS, ["[col_1]", "[col_2]", "[col_3]"])
+    t.sort([("[col_1]", "DESC"), ("[col_2]", "ASC"), ("[col_3]", "DESC")], protocol=BITONICSORT)
+    got = t.open()
+    np.testing.assert_array_equal(got["[col_1]"], [INT_MAX, 111, 111, 111, 5, 5, 5, 0])
+    np.testing.assert_array_equal(got["[col_2]"], [INT_MIN, -14, -14, -4, 13, 13, 13, 0])
+    np.testing.assert_array_equal(got["[col_3]"], [INT_MIN, -11233, INT_MIN, 0, 7, 7, 7, 6423])
+    t = EncodedTable.secret_share(gc, BITONIC_COLUMNS, ["[col_1]", "col_2", "[col_3]"])
+    t.sort([("[col_1]", "DESC"), ("[col_3]", "ASC")], protocol=BITONICSORT)
+    got = t.open()
+    np.testing.assert_array_equal(got["[col_3]"], [INT_MIN, INT_MIN, -11233, 0, 7, 7, 7, 6423])
+    np.testing.assert_array_equal(got["col_2"], [INT_MIN, -14, -14, -4, 13, 13, 13, 0])
+
+
+def test_bitonic_table_pads_and_unpads(G, keys):
+    """A non-power-of-two table is padded with invalid rows, sorted with the
+    valid column as the first key (DESC) and cut back to its size."""
+    from orq_amd.table import BITONICSORT, EncodedTable
+
+    n = 1000
+    rng = np.random.default_rng(3)
+    k = rng.integers(-100, 100, n)
+    gc = G.Ctx(keys)
+    t = EncodedTable.secret_share(gc, [k, np.arange(n)], ["[K]", "P"])
+    t.sort([("[##V]", "DESC"), ("[K]", "ASC")], protocol=BITONICSORT)
+    assert t.rows == n
+    got = t.open()
+    np.testing.assert_array_equal(got["[K]"], np.sort(k))
+    np.testing.assert_array_equal(got["[##V]"], np.ones(n))
+    np.testing.assert_array_equal(k[got["P"]], got["[K]"])

@@ -161,3 +161,82 @@ def test_table_sort_on_valid_bit(keys):
     np.testing.assert_array_equal(c.open(V), valid[order])
     np.testing.assert_array_equal(c.open(P), payload[order])
     assert c.queue_size() == 0
+
+
+# ---- bitonic sort (operators/sorting.h:266-353) ------------------------------------
+INT_MIN, INT_MAX = -(2**31), 2**31 - 1
+# tests/test_bitonic_sort.cpp:107-109
+BITONIC_COLUMNS = [[111, 111, 0, 111, INT_MAX, 5, 5, 5],
+                   [-14, -4, 0, -14, INT_MIN, 13, 13, 13],
+                   [INT_MIN, 0, 6423, -11233, INT_MIN, 7, 7, 7]]
+
+
+def _bshare(c, vals, width=4):
+    dt = np.int32 if width == 4 else np.int64
+    return c.secret_share(np.asarray(vals, dt).view(np.uint32 if width == 4 else np.uint64), width)
+
+
+@pytest.mark.parametrize("desc", [False, True])
+def test_reference_bitonic_vector(keys, desc):
+    # tests/test_bitonic_sort.cpp:48-92
+    c = _ctx(keys)
+    v = _bshare(c, [0, 5, 4, 7, -1, -2, 8, 1])
+    c.bitonic_sort([v], ["DESC" if desc else "ASC"])
+    want = [8, 7, 5, 4, 1, 0, -1, -2] if desc else [-2, -1, 0, 1, 4, 5, 7, 8]
+    np.testing.assert_array_equal(c.open(v).view(np.int32), want)
+    for x in ([0, INT_MAX // 2, -1, 42, 87, 100, -444, -13],
+              [0, INT_MAX // 2 + 1, -1, 42, 87, 100, -444, -13]):
+        v = _bshare(c, x)
+        c.bitonic_sort([v])
+        np.testing.assert_array_equal(c.open(v).view(np.int32), sorted(x))
+
+
+def _bitonic_table(c, columns, schema, spec):
+    """EncodedTable::sort with BITONICSORT on a table of n = 8 rows (no padding)."""
+    cols = {name: c.secret_share(np.asarray(v, np.int32).view(np.uint32), 4,
+                                 arith="[" not in name) for v, name in zip(columns, schema)}
+    ones = np.ones(len(columns[0]), np.uint32)
+    z = np.zeros_like(ones)
+    valid = O.Vec(len(ones), 4)
+    valid.set_shares([ones, z, z])
+    cols["[##V]"] = valid
+    ks = [k for k, _ in spec]
+    a, b = _split(cols, keys=ks)
+    c.bitonic_sort([cols[k] for k in ks], [o for _, o in spec], a, b)
+    return {k: c.open(v, arith="[" not in k).view(np.int32) for k, v in cols.items()}
+
+
+def test_reference_bitonic_tables(keys):
+    # tests/test_bitonic_sort.cpp:141-196
+    c = _ctx(keys)
+    got = _bitonic_table(c, BITONIC_COLUMNS, ["[col_1]", "[col_2]", "[col_3]"],
+                         [("[col_1]", "DESC"), ("[col_2]", "ASC"), ("[col_3]", "DESC")])
+    np.testing.assert_array_equal(got["[col_1]"], [INT_MAX, 111, 111, 111, 5, 5, 5, 0])
+    np.testing.assert_array_equal(got["[col_2]"], [INT_MIN, -14, -14, -4, 13, 13, 13, 0])
+    np.testing.assert_array_equal(got["[col_3]"], [INT_MIN, -11233, INT_MIN, 0, 7, 7, 7, 6423])
+    got = _bitonic_table(c, BITONIC_COLUMNS, ["[col_1]", "col_2", "[col_3]"],
+                         [("[col_1]", "DESC"), ("[col_3]", "ASC")])
+    np.testing.assert_array_equal(got["[col_1]"], [INT_MAX, 111, 111, 111, 5, 5, 5, 0])
+    np.testing.assert_array_equal(got["[col_3]"], [INT_MIN, INT_MIN, -11233, 0, 7, 7, 7, 6423])
+    np.testing.assert_array_equal(got["col_2"], [INT_MIN, -14, -14, -4, 13, 13, 13, 0])
+
+
+@pytest.mark.parametrize("n", [2, 64, 1024])
+def test_bitonic_random_rows(keys, n):
+    rng = np.random.default_rng(n)
+    k1 = rng.integers(-4, 4, n)
+    k2 = rng.integers(-(2**40), 2**40, n)
+    a = rng.integers(0, 2**50, n)
+    b = rng.integers(0, 2**50, n)
+    c = _ctx(keys)
+    K1, K2 = _bshare(c, k1, 8), _bshare(c, k2, 8)
+    A = c.secret_share(a.astype(np.uint64), 8, arith=True)
+    B = c.secret_share(b.astype(np.uint64), 8)
+    c.bitonic_sort([K1, K2], ["ASC", "DESC"], [A], [B])
+    g1, g2 = c.open(K1).view(np.int64), c.open(K2).view(np.int64)
+    order = np.lexsort((-k2, k1))
+    np.testing.assert_array_equal(g1, k1[order])
+    np.testing.assert_array_equal(g2, k2[order])
+    rows = sorted(zip(k1, k2, a, b))
+    got = sorted(zip(g1, g2, c.open(A, arith=True).astype(np.int64), c.open(B).astype(np.int64)))
+    assert rows == got
