@@ -71,6 +71,96 @@ ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
 ROUND_TRAFFIC_2P24 = int((2478423.4 * 2.0 + 1961569.8 * 1.0) * 1024)
 
 
+def _timed(ctx, fn, reps):
+    fn()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    ctx.sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def bench_primitives(args):
+    """3PC primitives on 2^24 boolean-shared uint64 (xor, and with resharing,
+    ==, <, full b2a) and the AES keystream of 2^24 64-bit words."""
+    import orq_amd
+    from orq_amd import splitmix64
+
+    n = args.n
+    ctx = orq_amd.Ctx(orq_amd.keys_from_seed(0x0A11CE02))
+    x = splitmix64(0x0A11CE02, n)
+    y = splitmix64(0x0A11CE12, n)
+    X, Y = ctx.secret_share(x, 8), ctx.secret_share(y, 8)
+    Z = ctx.vec(n, 8)
+    import ctypes
+
+    ks_buf = ctypes.c_void_p()
+    ctx._check(orq_amd.lib().orqg_dev_alloc(ctx.h, 8 * n, ctypes.byref(ks_buf)), "dev_alloc")
+    ops = {
+        "xor_b": lambda: orq_amd.lib().orqg_xor_b(ctx.h, X.h, Y.h, Z.h),
+        "and_b": lambda: orq_amd.lib().orqg_and_b(ctx.h, X.h, Y.h, Z.h),
+        "eq": lambda: orq_amd.lib().orqg_cmp(ctx.h, 0, X.h, Y.h, Z.h),
+        "lt": lambda: orq_amd.lib().orqg_cmp(ctx.h, 2, X.h, Y.h, Z.h),
+        "b2a": lambda: orq_amd.lib().orqg_b2a(ctx.h, X.h, Z.h),
+        "keystream_8B_words": lambda: orq_amd.lib().orqg_prg_keystream(ctx.h, 6, 8 * n, ks_buf),
+    }
+    res = {}
+    for name, fn in ops.items():
+        dt = _timed(ctx, fn, 1 if name == "b2a" else args.steps)
+        res[name] = {"ms": round(dt * 1e3, 3), "Melem_per_s": round(n / dt / 1e6, 1)}
+    res["keystream_8B_words"]["GB_per_s"] = round(8 * n / (res["keystream_8B_words"]["ms"] / 1e3) / 1e9, 1)
+    orq_amd.lib().orqg_dev_free(ks_buf)
+    ok = bool(np.array_equal(ctx.open(ctx.cmp("<", X, Y)),
+                             (x.view(np.int64) < y.view(np.int64)).astype(np.uint64)))
+    line = {"metric": "3PC primitive microbench @2^24 x uint64", "value": res["lt"]["Melem_per_s"],
+            "unit": "Melem/s (64-bit <)", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+            "dtype": "u64", "data": "synthetic: splitmix64 seeds 0x0A11CE02 / 0x0A11CE12",
+            "config": {"workload": f"xor/and/==/</b2a/keystream over {n} boolean-shared uint64"},
+            "ops": res, "correct": ok}
+    print(json.dumps(line), flush=True)
+
+
+def bench_table(args):
+    """Table of 2^24 rows: [K] int64 Zipf key, two uint32 payloads [P1], [P2]
+    and the valid column; shuffle, then sort on [K] by radix sort, and the
+    same by bitonic sort."""
+    import orq_amd
+    from orq_amd import splitmix64
+    from orq_amd.table import BITONICSORT, RADIXSORT, EncodedTable
+
+    n = args.n
+    key = zipf_keys(n).view(np.int64)
+    p1 = (splitmix64(0x0A11CE13, n) >> np.uint64(32)).astype(np.int64)
+    p2 = (splitmix64(0x0A11CE23, n) >> np.uint64(32)).astype(np.int64)
+    res = {}
+    for proto in (RADIXSORT, BITONICSORT):
+        ctx = orq_amd.Ctx(orq_amd.keys_from_seed(0x0A11CE03))
+        t = EncodedTable.secret_share(ctx, [key, p1, p2], ["[K]", "[P1]", "[P2]"], width=8)
+        ctx.sync()
+        t0 = time.perf_counter()
+        t.shuffle()
+        ctx.sync()
+        t1 = time.perf_counter()
+        t.sort([("[K]", "ASC")], protocol=proto)
+        ctx.sync()
+        t2 = time.perf_counter()
+        got = t.open()
+        ok = bool(np.array_equal(got["[K]"], np.sort(key)))
+        order_ok = bool(np.array_equal(np.sort(got["[P1]"]), np.sort(p1)))
+        res[proto] = {"shuffle_ms": round((t1 - t0) * 1e3, 1), "sort_ms": round((t2 - t1) * 1e3, 1),
+                      "Mrows_per_s": round(n / (t2 - t0) / 1e6, 3), "keys_sorted": ok,
+                      "payload_multiset_kept": order_ok}
+        del t, ctx
+    line = {"metric": "3PC table shuffle + sort Mrows/s @2^24 rows", "value": res[RADIXSORT]["Mrows_per_s"],
+            "unit": "Mrows/s", "n_gpus": 1, "steps": 1, "higher_is_better": True, "dtype": "int64",
+            "data": "synthetic: Zipf(s=1.1) int64 key over [0,2^20), two uint32 payloads (splitmix64)",
+            "config": {"workload": f"EncodedTable {{[K],[P1],[P2],[##V]}} x {n} rows: shuffle, "
+                                   "then sort on [K] (radix; bitonic beside it)"},
+            "protocols": res}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,11 +169,16 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["radix", "primitives", "table"], default="radix")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: all three parties share "
                                    "one device; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "primitives":
+        return bench_primitives(args)
+    if args.workload == "table":
+        return bench_table(args)
 
     import orq_amd
 

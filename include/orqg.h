@@ -175,6 +175,30 @@ int orqg_shuffle(orqg_ctx *ctx, orqg_vec *x);
  * B-shared columns in schema order                                            */
 int orqg_table_shuffle(orqg_ctx *ctx, orqg_vec *const *a_cols, size_t na,
                        orqg_vec *const *b_cols, size_t nb);
+/* ---- aggregation network and distinct -------------------------------- */
+/* aggregation functions of include/core/operators/aggregation.h:27-140:
+ * sum, count, copy (A-shared); copy, max, min, bit_or, valid (B-shared)      */
+enum { ORQG_AGG_SUM = 0, ORQG_AGG_COUNT = 1, ORQG_AGG_COPY = 2, ORQG_AGG_MAX = 3,
+       ORQG_AGG_MIN = 4, ORQG_AGG_OR = 5, ORQG_AGG_VALID = 6 };
+typedef struct {
+  orqg_vec *in;  /* input column */
+  orqg_vec *out; /* result column (may be `in`) */
+  int func;      /* ORQG_AGG_* */
+} orqg_agg_spec;
+/* aggregators::aggregate (operators/aggregation.h:167-346): the odd-even
+ * aggregation network over n = 2^k rows, grouped by equality of all `keys`.
+ * reverse != 0 runs Direction::Reverse; sel (NULL: none) is the selection
+ * column of joins (table id).  Forward leaves each group's aggregate in its
+ * first row, reverse in its last.                                            */
+int orqg_aggregate(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys,
+                   const orqg_agg_spec *b_specs, size_t nb, const orqg_agg_spec *a_specs,
+                   size_t na, int reverse, const orqg_vec *sel);
+/* operators::distinct (operators/distinct.h:19-44): res[0] = 1 and
+ * res[i] = (row i differs from row i-1 on some key), as boolean shares      */
+int orqg_distinct(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, orqg_vec *res);
+/* public_share of one value in every row (replicated_3pc.h:456-470)         */
+int orqg_public_fill(orqg_ctx *ctx, orqg_vec *v, uint64_t value, int enc);
+
 /* operators::bitonic_sort (operators/sorting.h:266-353): keys[0] most
  * significant, orders[j] ORQG_ASC/ORQG_DESC; all columns of one width, the
  * row count a power of two; at most 8 columns of each kind.  Not stable.     */

@@ -66,6 +66,8 @@ def lib():
             ("oc_table_shuffle", [vp, vp, sz, vp, sz]),
             ("oc_table_sort", [vp, vp, sz, vp, vp, vp, sz, vp, sz]),
             ("oc_bitonic_sort", [vp, vp, sz, vp, vp, sz, vp, sz]),
+            ("oc_aggregate", [vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, sz, i32, vp]),
+            ("oc_distinct", [vp, vp, sz, vp]),
         ]:
             f = getattr(L, name)
             f.restype = i32
@@ -279,6 +281,24 @@ class Ctx:
         a, na = self._vec_array(list(a_cols))
         b, nb = self._vec_array(list(b_cols))
         _check(lib().oc_bitonic_sort(self.h, k, nk, o, a, na, b, nb), "bitonic_sort")
+
+    AGG = {"sum": 0, "count": 1, "copy": 2, "max": 3, "min": 4, "or": 5, "valid": 6}
+
+    def aggregate(self, keys, b_specs=(), a_specs=(), reverse=False, sel=None):
+        """aggregators::aggregate; specs are (in, out, fn-name) tuples."""
+        k, nk = self._vec_array(list(keys))
+        bi, _ = self._vec_array([x[0] for x in b_specs])
+        bo, nb = self._vec_array([x[1] for x in b_specs])
+        bf = (ctypes.c_int * max(nb, 1))(*[self.AGG[x[2]] for x in b_specs])
+        ai, _ = self._vec_array([x[0] for x in a_specs])
+        ao, na = self._vec_array([x[1] for x in a_specs])
+        af = (ctypes.c_int * max(na, 1))(*[self.AGG[x[2]] for x in a_specs])
+        _check(lib().oc_aggregate(self.h, k, nk, bi, bo, bf, nb, ai, ao, af, na, int(reverse),
+                                  sel.h if sel is not None else None), "aggregate")
+
+    def distinct(self, keys, res):
+        k, nk = self._vec_array(list(keys))
+        _check(lib().oc_distinct(self.h, k, nk, res.h), "distinct")
 
     def msglog(self):
         L = lib()

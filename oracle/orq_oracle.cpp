@@ -1114,6 +1114,269 @@ void bitonic_sort(oc_ctx *c, oc_vec **keys, size_t nk, const int *orders, oc_vec
     }
 }
 
+// ---- aggregation network (operators/aggregation.h:27-346) and distinct ----
+enum AggFn { kAggSum = 0, kAggCount = 1, kAggCopy = 2, kAggMax = 3, kAggMin = 4, kAggOr = 5,
+             kAggValid = 6 };
+struct AggSpec {
+  oc_vec *in, *out;
+  int fn;
+};
+
+template <typename T>
+oc_vec *slice_of(const oc_vec *v, size_t start, size_t len) {
+  std::vector<size_t> idx(len);
+  for (size_t i = 0; i < len; i++) idx[i] = start + i;
+  return gather_rows<T>(v, idx);
+}
+template <typename T>
+void write_slice(oc_vec *v, size_t start, const oc_vec *src) {
+  std::vector<size_t> idx(src->n);
+  for (size_t i = 0; i < src->n; i++) idx[i] = start + i;
+  scatter_rows<T>(v, src, idx);
+}
+template <typename T>
+oc_vec *not_copy(const oc_vec *x) {
+  oc_vec *r = clone_vec(x);
+  not_vec<T>(r);
+  return r;
+}
+// x | y = ~(~x & ~y) (b_shared_vector.h:438-441)
+template <typename T>
+oc_vec *or_b(oc_ctx *c, const oc_vec *x, const oc_vec *y) {
+  oc_vec *nx = not_copy<T>(x), *ny = not_copy<T>(y);
+  mult<T, false>(c, nx, ny, nx, "and_b");
+  not_vec<T>(nx);
+  delete ny;
+  return nx;
+}
+template <typename T>
+void mask1(oc_vec *v) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *a = S<T>(v, p, k);
+      for (size_t i = 0; i < v->n; i++) a[i] &= 1;
+    }
+}
+
+// the boolean aggregation functions of aggregation.h:27-140 on (g, a, b);
+// returns the new accumulator a
+template <typename T>
+oc_vec *agg_b(oc_ctx *c, int fn, const oc_vec *g, const oc_vec *a, const oc_vec *b) {
+  switch (fn) {
+    case kAggCopy:
+      return multiplex_b<T>(c, g, a, b);
+    case kAggOr: {
+      oc_vec *ext = new_vec(g->n, sizeof(T));
+      for (int p = 0; p < 3; p++)
+        for (int k = 0; k < 2; k++) {
+          const T *s = S<T>(const_cast<oc_vec *>(g), p, k);
+          T *d = S<T>(ext, p, k);
+          for (size_t i = 0; i < g->n; i++) d[i] = (T)(0 - (s[i] & 1));
+        }
+      mult<T, false>(c, ext, const_cast<oc_vec *>(b), ext, "and_b");
+      oc_vec *r = or_b<T>(c, a, ext);
+      delete ext;
+      return r;
+    }
+    case kAggMax:
+    case kAggMin: {
+      oc_vec *eq = new_vec(a->n, sizeof(T)), *bg = new_vec(a->n, sizeof(T));
+      compare<T>(c, const_cast<oc_vec *>(b), const_cast<oc_vec *>(a), eq, bg);  // b > a
+      if (fn == kAggMin)
+        for (int p = 0; p < 3; p++)
+          for (int k = 0; k < 2; k++) {
+            T *d = S<T>(bg, p, k);
+            for (size_t i = 0; i < bg->n; i++) d[i] = (T)((d[i] & 1) == 0);  // not_b_1
+          }
+      oc_vec *inner = multiplex_b<T>(c, bg, a, b);
+      oc_vec *r = multiplex_b<T>(c, g, a, inner);
+      delete eq;
+      delete bg;
+      delete inner;
+      return r;
+    }
+    case kAggValid: {
+      oc_vec *agg = new_vec(a->n, sizeof(T));
+      mult<T, false>(c, const_cast<oc_vec *>(a), const_cast<oc_vec *>(b), agg, "and_b");
+      oc_vec *r = multiplex_b<T>(c, g, a, agg);
+      delete agg;
+      return r;
+    }
+    default:
+      throw std::runtime_error("aggregate: not a boolean aggregation");
+  }
+}
+template <typename T>
+oc_vec *agg_a(oc_ctx *c, int fn, const oc_vec *g, const oc_vec *a, const oc_vec *b) {
+  switch (fn) {
+    case kAggSum:
+    case kAggCount: {  // a + g * b
+      oc_vec *t = new_vec(a->n, sizeof(T));
+      mult<T, true>(c, const_cast<oc_vec *>(g), const_cast<oc_vec *>(b), t, "mul_a");
+      oc_vec *r = local_op<T>(a, t, 1);
+      delete t;
+      return r;
+    }
+    case kAggCopy:
+      return multiplex_a<T>(c, g, a, b);
+    default:
+      throw std::runtime_error("aggregate: not an arithmetic aggregation");
+  }
+}
+
+template <typename T>
+void aggregate(oc_ctx *c, oc_vec **keys, size_t nk, AggSpec *bs, size_t nb, AggSpec *as,
+               size_t na, bool reverse, oc_vec *sel) {
+  size_t n = nk ? keys[0]->n : nb ? bs[0].in->n : na ? as[0].in->n : 0;
+  if (!nk && !nb && !na) throw std::runtime_error("Empty aggregation!");
+  if (n & (n - 1)) throw std::runtime_error("aggregate: size must be a power of two");
+  bool a_copy = false, a_noncopy = false, b_copy = false, b_noncopy = false;
+  std::vector<T> onesv(n, 1);
+  for (size_t i = 0; i < na; i++) {
+    if (as[i].fn == kAggCount) {
+      if (sel) {
+        b2a_bit<T>(c, sel, as[i].out);
+      } else {
+        public_share<T>(as[i].out, onesv.data());
+      }
+    } else if (as[i].out != as[i].in) {
+      *as[i].out = *as[i].in;
+    }
+    (as[i].fn == kAggCopy ? a_copy : a_noncopy) = true;
+  }
+  for (size_t i = 0; i < nb; i++) {
+    if (bs[i].out != bs[i].in) *bs[i].out = *bs[i].in;
+    (bs[i].fn == kAggCopy ? b_copy : b_noncopy) = true;
+  }
+  int log_size = 0;
+  while (((size_t)1 << (log_size + 1)) <= n) log_size++;
+  for (int lv = 1; lv <= log_size; lv++) {
+    size_t d = n / ((size_t)1 << lv);
+    if (reverse) d = n / ((size_t)1 << (log_size - lv + 1));
+    const size_t m = n - d;
+    oc_vec *g;
+    if (nk == 0) {
+      g = new_vec(m, sizeof(T));
+      std::vector<T> o(m, 1);
+      public_share<T>(g, o.data());
+    } else {
+      oc_vec *x = slice_of<T>(keys[0], 0, m), *y = slice_of<T>(keys[0], d, m);
+      g = bit_same<T>(c, x, y);
+      mask1<T>(g);  // operator==
+      delete x;
+      delete y;
+      for (size_t k = 1; k < nk; k++) {
+        x = slice_of<T>(keys[k], 0, m);
+        y = slice_of<T>(keys[k], d, m);
+        oc_vec *e = bit_same<T>(c, x, y);
+        mask1<T>(e);
+        mult<T, false>(c, g, e, g, "and_b");  // group_bits_b &= ...
+        delete e;
+        delete x;
+        delete y;
+      }
+    }
+    oc_vec *jg = clone_vec(g);
+    if (sel && (a_noncopy || b_noncopy)) {
+      oc_vec *s1 = slice_of<T>(sel, d, m), *s0 = slice_of<T>(sel, 0, m);
+      oc_vec *sx = local_op<T>(s1, s0, 0);
+      not_vec<T>(sx);
+      mult<T, false>(c, g, sx, g, "and_b");
+      delete s1;
+      delete s0;
+      delete sx;
+    }
+    mask1<T>(g);
+    for (size_t i = 0; i < nb; i++) {
+      oc_vec *a = slice_of<T>(bs[i].out, 0, m), *b = slice_of<T>(bs[i].out, d, m);
+      const oc_vec *gg = bs[i].fn == kAggCopy ? jg : g;
+      if (reverse) {
+        oc_vec *r = agg_b<T>(c, bs[i].fn, gg, b, a);
+        write_slice<T>(bs[i].out, d, r);
+        delete r;
+      } else {
+        oc_vec *r = agg_b<T>(c, bs[i].fn, gg, a, b);
+        write_slice<T>(bs[i].out, 0, r);
+        delete r;
+      }
+      delete a;
+      delete b;
+    }
+    oc_vec *ga = nullptr, *jga = nullptr;
+    if (nk > 0 && na > 0) {
+      if (a_noncopy) {
+        ga = new_vec(m, sizeof(T));
+        b2a_bit<T>(c, g, ga);
+      }
+      if (a_copy) {
+        jga = new_vec(m, sizeof(T));
+        b2a_bit<T>(c, jg, jga);
+      }
+    } else if (na > 0) {  // no keys: public ones
+      std::vector<T> o(m, 1);
+      ga = new_vec(m, sizeof(T));
+      public_share<T>(ga, o.data());
+      jga = clone_vec(ga);
+    }
+    for (size_t i = 0; i < na; i++) {
+      oc_vec *a = slice_of<T>(as[i].out, 0, m), *b = slice_of<T>(as[i].out, d, m);
+      const oc_vec *gg = as[i].fn == kAggCopy ? jga : ga;
+      if (reverse) {
+        oc_vec *r = agg_a<T>(c, as[i].fn, gg, b, a);
+        write_slice<T>(as[i].out, d, r);
+        delete r;
+      } else {
+        oc_vec *r = agg_a<T>(c, as[i].fn, gg, a, b);
+        write_slice<T>(as[i].out, 0, r);
+        delete r;
+      }
+      delete a;
+      delete b;
+    }
+    delete g;
+    delete jg;
+    delete ga;
+    delete jga;
+  }
+}
+
+// operators::distinct (distinct.h:19-44): res[0] = public 1; for each key,
+// res[1..n) |= key[0..n-1) != key[1..n)
+template <typename T>
+void distinct(oc_ctx *c, oc_vec **keys, size_t nk, oc_vec *res) {
+  const size_t n = res->n;
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) std::fill(res->sh[p][k].begin(), res->sh[p][k].end(), 0);
+  if (n == 0) return;
+  oc_vec *one = new_vec(1, sizeof(T));
+  T o1 = 1;
+  public_share<T>(one, &o1);
+  write_slice<T>(res, 0, one);
+  delete one;
+  if (n == 1) return;
+  for (size_t k = 0; k < nk; k++) {
+    oc_vec *a = slice_of<T>(keys[k], 0, n - 1), *b = slice_of<T>(keys[k], 1, n - 1);
+    oc_vec *e = bit_same<T>(c, a, b);
+    for (int p = 0; p < 3; p++)  // (a == b) -> mask(1) -> not_b_1
+      for (int q = 0; q < 2; q++) {
+        T *d = S<T>(e, p, q);
+        for (size_t i = 0; i < e->n; i++) d[i] = (T)(((d[i] & 1) & 1) == 0);
+      }
+    oc_vec *rest = slice_of<T>(res, 1, n - 1);
+    // rest |= ne: invert rest, rest &= ~ne, invert rest (b_shared_vector.h:452-456)
+    not_vec<T>(rest);
+    oc_vec *nne = not_copy<T>(e);
+    mult<T, false>(c, rest, nne, rest, "and_b");
+    not_vec<T>(rest);
+    write_slice<T>(res, 1, rest);
+    delete a;
+    delete b;
+    delete e;
+    delete rest;
+    delete nne;
+  }
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1389,6 +1652,30 @@ int oc_bitonic_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, o
       bitonic_sort<uint32_t>(c, keys, nkeys, orders, a_cols, na, b_cols, nb);
     else
       bitonic_sort<uint64_t>(c, keys, nkeys, orders, a_cols, na, b_cols, nb);
+  })
+}
+
+// specs: arrays of (in, out) vector pointers and function ids
+int oc_aggregate(oc_ctx *c, oc_vec **keys, size_t nk, oc_vec **b_in, oc_vec **b_out,
+                 const int *b_fn, size_t nb, oc_vec **a_in, oc_vec **a_out, const int *a_fn,
+                 size_t na, int reverse, oc_vec *sel) {
+  OC_TRY({
+    std::vector<AggSpec> bs(nb), as(na);
+    for (size_t i = 0; i < nb; i++) bs[i] = {b_in[i], b_out[i], b_fn[i]};
+    for (size_t i = 0; i < na; i++) as[i] = {a_in[i], a_out[i], a_fn[i]};
+    int w = nk ? keys[0]->width : nb ? b_in[0]->width : a_in[0]->width;
+    if (w == 4)
+      aggregate<uint32_t>(c, keys, nk, bs.data(), nb, as.data(), na, reverse != 0, sel);
+    else
+      aggregate<uint64_t>(c, keys, nk, bs.data(), nb, as.data(), na, reverse != 0, sel);
+  })
+}
+int oc_distinct(oc_ctx *c, oc_vec **keys, size_t nk, oc_vec *res) {
+  OC_TRY({
+    if (res->width == 4)
+      distinct<uint32_t>(c, keys, nk, res);
+    else
+      distinct<uint64_t>(c, keys, nk, res);
   })
 }
 

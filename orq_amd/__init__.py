@@ -51,6 +51,7 @@ EXPORTS = [
     "orqg_perm_identity", "orqg_perm_invert", "orqg_perm_b2a", "orqg_perm_negate",
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
+    "orqg_aggregate", "orqg_distinct", "orqg_public_fill",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -109,6 +110,9 @@ def lib():
             "orqg_bitonic_sort": (i32, [vp, vp, sz, vp, vp, sz, vp, sz]),
             "orqg_vec_resize": (i32, [vp, vp, sz]),
             "orqg_vec_tail": (i32, [vp, vp, sz]),
+            "orqg_aggregate": (i32, [vp, vp, sz, vp, sz, vp, sz, i32, vp]),
+            "orqg_distinct": (i32, [vp, vp, sz, vp]),
+            "orqg_public_fill": (i32, [vp, vp, u64, i32]),
             "orqg_reshare": (i32, [vp, vp, i32]),
             "orqg_reserve_perms": (i32, [vp, sz, sz]),
             "orqg_perm_queue_size": (sz, [vp]),
@@ -390,6 +394,37 @@ class Ctx:
     def tail(self, v, n):
         self._check(lib().orqg_vec_tail(self.h, v.h, n), "vec_tail")
         v.n = n
+
+    AGG = {"sum": 0, "count": 1, "copy": 2, "max": 3, "min": 4, "or": 5, "valid": 6}
+
+    class _AggSpec(ctypes.Structure):
+        _fields_ = [("inp", ctypes.c_void_p), ("out", ctypes.c_void_p), ("func", ctypes.c_int)]
+
+    def _specs(self, specs):
+        arr = (self._AggSpec * max(len(specs), 1))()
+        for i, (inp, out, fn) in enumerate(specs):
+            arr[i].inp, arr[i].out, arr[i].func = inp.h, out.h, self.AGG[fn]
+        return arr, len(specs)
+
+    def aggregate(self, keys, b_specs=(), a_specs=(), reverse=False, sel=None):
+        """aggregators::aggregate; specs are (in, out, function-name) tuples."""
+        k, nk = self._vec_array(list(keys))
+        b, nb = self._specs(list(b_specs))
+        a, na = self._specs(list(a_specs))
+        self._check(lib().orqg_aggregate(self.h, k, nk, b, nb, a, na, int(reverse),
+                                         sel.h if sel is not None else None), "aggregate")
+        for _, out, fn in b_specs:
+            out.encoding = ORQG_BSHARED
+        for _, out, fn in a_specs:
+            out.encoding = ORQG_ASHARED
+
+    def distinct(self, keys, res):
+        k, nk = self._vec_array(list(keys))
+        self._check(lib().orqg_distinct(self.h, k, nk, res.h), "distinct")
+
+    def public_fill(self, v, value, encoding=ORQG_BSHARED):
+        self._check(lib().orqg_public_fill(self.h, v.h, value, encoding), "public_fill")
+        v.encoding = encoding
 
     def radix_sort(self, v, bits=None, desc=False):
         if bits is None:

@@ -533,6 +533,32 @@ int orqg_bitonic_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const in
   API_END(c)
 }
 
+int orqg_aggregate(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys,
+                   const orqg_agg_spec *b_specs, size_t nb, const orqg_agg_spec *a_specs,
+                   size_t na, int reverse, const orqg_vec *sel) {
+  API_BEGIN(c)
+  if ((nkeys && !keys) || (nb && !b_specs) || (na && !a_specs))
+    throw Error(ORQG_E_ARG, "aggregate: NULL array");
+  for (size_t k = 0; k < nkeys; k++)
+    if (!keys[k] || keys[k]->enc != ORQG_BSHARED)
+      throw Error(ORQG_E_ARG, "aggregate: keys must be B-shared");
+  aggregate(c, keys, nkeys, b_specs, nb, a_specs, na, reverse, sel);
+  API_END(c)
+}
+int orqg_distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, orqg_vec *res) {
+  API_BEGIN(c)
+  if (!res || (nkeys && !keys)) throw Error(ORQG_E_ARG, "distinct: NULL argument");
+  distinct(c, keys, nkeys, res);
+  API_END(c)
+}
+int orqg_public_fill(orqg_ctx *c, orqg_vec *v, uint64_t value, int enc) {
+  API_BEGIN(c)
+  if (!v) throw Error(ORQG_E_ARG, "public_fill: NULL vector");
+  public_fill(c, v, value);
+  v->enc = enc ? ORQG_BSHARED : ORQG_ASHARED;
+  API_END(c)
+}
+
 int orqg_reserve_perms(orqg_ctx *c, size_t n, size_t count) {
   API_BEGIN(c)
   reserve_perms(c, n, count);

@@ -207,6 +207,11 @@ void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, or
              bool full, int neg_eq, int neg_gt);
 void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
 void perm_identity(orqg_ctx *c, orqg_vec *perm, int enc);
+void public_fill_raw(orqg_ctx *c, void *d, uint64_t m, uint64_t value, int w);
+void public_fill(orqg_ctx *c, orqg_vec *v, uint64_t value);
+void aggregate(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const orqg_agg_spec *bs, size_t nbs,
+               const orqg_agg_spec *as, size_t nas, int reverse, const orqg_vec *sel);
+void distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res);
 void perm_b2a(orqg_ctx *c, orqg_vec *perm);
 void perm_negate(orqg_ctx *c, orqg_vec *perm);
 void perm_reverse(orqg_ctx *c, orqg_vec *perm);

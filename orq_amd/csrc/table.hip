@@ -65,7 +65,32 @@ __global__ void k_open_gather(uint64_t n, bool arith, const uint32_t *__restrict
   }
 }
 
+template <typename T>
+__global__ void k_public_fill_t(uint64_t n, T value, T *__restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    out[3 * i] = value;
+    out[3 * i + 1] = 0;
+    out[3 * i + 2] = 0;
+  }
+}
+
 static unsigned flat_grid(orqg_ctx *c, uint64_t n) { return grid_for(n, 256, c->num_cu * 16); }
+
+// m rows of a public sharing of `value` (replicated_3pc.h:456-470) at d
+void public_fill_raw(orqg_ctx *c, void *d, uint64_t m, uint64_t value, int w) {
+  if (m == 0) return;
+  if (w == 4)
+    hipLaunchKernelGGL(k_public_fill_t<uint32_t>, dim3(flat_grid(c, m)), dim3(256), 0, c->stream,
+                       m, (uint32_t)value, (uint32_t *)d);
+  else
+    hipLaunchKernelGGL(k_public_fill_t<uint64_t>, dim3(flat_grid(c, m)), dim3(256), 0, c->stream,
+                       m, (uint64_t)value, (uint64_t *)d);
+  ORQG_HIP(hipGetLastError());
+}
+void public_fill(orqg_ctx *c, orqg_vec *v, uint64_t value) {
+  public_fill_raw(c, v->d, v->n, value, v->width);
+}
 
 // ElementwisePermutation(size, encoding) (permutation.h:62-69): a public
 // sharing of the identity.
