@@ -95,6 +95,14 @@ int orqg_vec_copy(orqg_ctx *ctx, const orqg_vec *src, orqg_vec *dst);
 int orqg_vec_resize(orqg_ctx *ctx, orqg_vec *v, size_t n);
 /* keep the last n rows (EncodedTable::tail, encoded_table.h:1407-1418)       */
 int orqg_vec_tail(orqg_ctx *ctx, orqg_vec *v, size_t n);
+/* dst rows [dst_start, +count) = src rows [src_start, +count)
+ * (EncodedTable::copy_column_typed, encoded_table.h:1749-1754)              */
+int orqg_vec_copy_rows(orqg_ctx *ctx, const orqg_vec *src, size_t src_start, orqg_vec *dst,
+                       size_t dst_start, size_t count);
+/* dst rows [dst_start, +count) = row `row` of src, all three shares (the
+ * repeated_subset_reference assignment of mapping_access_vector.h:854-866)  */
+int orqg_vec_broadcast_row(orqg_ctx *ctx, const orqg_vec *src, size_t row, orqg_vec *dst,
+                           size_t dst_start, size_t count);
 
 /* ---- randomness ------------------------------------------------------ */
 /* AESPRGAlgorithm::fillBytes of PRG `role` into device memory, advancing its

@@ -68,6 +68,10 @@ def lib():
             ("oc_bitonic_sort", [vp, vp, sz, vp, vp, sz, vp, sz]),
             ("oc_aggregate", [vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, sz, i32, vp]),
             ("oc_distinct", [vp, vp, sz, vp]),
+            ("oc_vec_resize", [vp, sz]),
+            ("oc_vec_tail", [vp, sz]),
+            ("oc_vec_copy_rows", [vp, sz, vp, sz, sz]),
+            ("oc_vec_broadcast_row", [vp, sz, vp, sz, sz]),
         ]:
             f = getattr(L, name)
             f.restype = i32

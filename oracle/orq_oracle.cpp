@@ -1430,6 +1430,32 @@ void oc_gen_perm(const uint8_t *key, uint64_t *nonce, int32_t *out, size_t n, ui
 
 oc_vec *oc_vec_new(size_t n, int width) { return new_vec(n, width); }
 void oc_vec_free(oc_vec *v) { delete v; }
+// keep the first min(n, size) rows, new rows zero
+void oc_vec_resize(oc_vec *v, size_t n) {
+  for (int p = 0; p < 3; p++)
+    for (int i = 0; i < 2; i++) v->sh[p][i].resize(n * v->width, 0);
+  v->n = n;
+}
+// keep the last n rows
+void oc_vec_tail(oc_vec *v, size_t n) {
+  const size_t w = v->width, drop = v->n - n;
+  for (int p = 0; p < 3; p++)
+    for (int i = 0; i < 2; i++) v->sh[p][i].erase(v->sh[p][i].begin(), v->sh[p][i].begin() + drop * w);
+  v->n = n;
+}
+void oc_vec_copy_rows(const oc_vec *src, size_t s0, oc_vec *dst, size_t d0, size_t count) {
+  const size_t w = src->width;
+  for (int p = 0; p < 3; p++)
+    for (int i = 0; i < 2; i++)
+      std::memcpy(dst->sh[p][i].data() + d0 * w, src->sh[p][i].data() + s0 * w, count * w);
+}
+void oc_vec_broadcast_row(const oc_vec *src, size_t row, oc_vec *dst, size_t d0, size_t count) {
+  const size_t w = src->width;
+  for (int p = 0; p < 3; p++)
+    for (int i = 0; i < 2; i++)
+      for (size_t r = 0; r < count; r++)
+        std::memcpy(dst->sh[p][i].data() + (d0 + r) * w, src->sh[p][i].data() + row * w, w);
+}
 size_t oc_vec_size(oc_vec *v) { return v->n; }
 void oc_vec_set(oc_vec *v, int party, int idx, const void *src) {
   std::memcpy(v->sh[party][idx].data(), src, v->n * v->width);

@@ -51,7 +51,8 @@ EXPORTS = [
     "orqg_perm_identity", "orqg_perm_invert", "orqg_perm_b2a", "orqg_perm_negate",
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
-    "orqg_aggregate", "orqg_distinct", "orqg_public_fill",
+    "orqg_aggregate", "orqg_distinct", "orqg_public_fill", "orqg_vec_copy_rows",
+    "orqg_vec_broadcast_row",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -113,6 +114,8 @@ def lib():
             "orqg_aggregate": (i32, [vp, vp, sz, vp, sz, vp, sz, i32, vp]),
             "orqg_distinct": (i32, [vp, vp, sz, vp]),
             "orqg_public_fill": (i32, [vp, vp, u64, i32]),
+            "orqg_vec_copy_rows": (i32, [vp, vp, sz, vp, sz, sz]),
+            "orqg_vec_broadcast_row": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_reshare": (i32, [vp, vp, i32]),
             "orqg_reserve_perms": (i32, [vp, sz, sz]),
             "orqg_perm_queue_size": (sz, [vp]),
@@ -425,6 +428,14 @@ class Ctx:
     def public_fill(self, v, value, encoding=ORQG_BSHARED):
         self._check(lib().orqg_public_fill(self.h, v.h, value, encoding), "public_fill")
         v.encoding = encoding
+
+    def copy_rows(self, src, src_start, dst, dst_start, count):
+        self._check(lib().orqg_vec_copy_rows(self.h, src.h, src_start, dst.h, dst_start, count),
+                    "vec_copy_rows")
+
+    def broadcast_row(self, src, row, dst, dst_start, count):
+        self._check(lib().orqg_vec_broadcast_row(self.h, src.h, row, dst.h, dst_start, count),
+                    "vec_broadcast_row")
 
     def radix_sort(self, v, bits=None, desc=False):
         if bits is None:

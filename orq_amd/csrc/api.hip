@@ -262,6 +262,32 @@ int orqg_vec_tail(orqg_ctx *c, orqg_vec *v, size_t n) {
   API_END(c)
 }
 
+// dst rows [dst_start, +count) = src rows [src_start, +count)
+// (EncodedTable::copy_column_typed, encoded_table.h:1749-1754)
+int orqg_vec_copy_rows(orqg_ctx *c, const orqg_vec *src, size_t src_start, orqg_vec *dst,
+                       size_t dst_start, size_t count) {
+  API_BEGIN(c)
+  if (!src || !dst || src->width != dst->width || src_start + count > src->n ||
+      dst_start + count > dst->n)
+    throw Error(ORQG_E_ARG, "vec_copy_rows: range outside the vectors");
+  const size_t w = src->width;
+  if (count)
+    ORQG_HIP(hipMemcpyAsync((char *)dst->d + 3 * dst_start * w,
+                            (const char *)src->d + 3 * src_start * w, 3 * count * w,
+                            hipMemcpyDeviceToDevice, c->stream));
+  API_END(c)
+}
+// dst rows [dst_start, +count) = the three shares of src row `row`
+// (a repeated_subset_reference assignment, mapping_access_vector.h:854-866)
+int orqg_vec_broadcast_row(orqg_ctx *c, const orqg_vec *src, size_t row, orqg_vec *dst,
+                           size_t dst_start, size_t count) {
+  API_BEGIN(c)
+  if (!src || !dst || src->width != dst->width || row >= src->n || dst_start + count > dst->n)
+    throw Error(ORQG_E_ARG, "vec_broadcast_row: range outside the vectors");
+  broadcast_row(c, src, row, dst, dst_start, count);
+  API_END(c)
+}
+
 // hipFree waits for outstanding work on the device; the owning context may
 // already be gone, so it is not touched here.
 int orqg_vec_free(orqg_vec *v) {

@@ -209,6 +209,8 @@ void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
 void perm_identity(orqg_ctx *c, orqg_vec *perm, int enc);
 void public_fill_raw(orqg_ctx *c, void *d, uint64_t m, uint64_t value, int w);
 void public_fill(orqg_ctx *c, orqg_vec *v, uint64_t value);
+void broadcast_row(orqg_ctx *c, const orqg_vec *src, uint64_t row, orqg_vec *dst,
+                   uint64_t dst_start, uint64_t count);
 void aggregate(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const orqg_agg_spec *bs, size_t nbs,
                const orqg_agg_spec *as, size_t nas, int reverse, const orqg_vec *sel);
 void distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res);

@@ -92,6 +92,30 @@ void public_fill(orqg_ctx *c, orqg_vec *v, uint64_t value) {
   public_fill_raw(c, v->d, v->n, value, v->width);
 }
 
+template <typename T>
+__global__ void k_broadcast_row(uint64_t count, const T *__restrict__ src, T *__restrict__ dst) {
+  const T s0 = src[0], s1 = src[1], s2 = src[2];
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    dst[3 * i] = s0;
+    dst[3 * i + 1] = s1;
+    dst[3 * i + 2] = s2;
+  }
+}
+void broadcast_row(orqg_ctx *c, const orqg_vec *src, uint64_t row, orqg_vec *dst,
+                   uint64_t dst_start, uint64_t count) {
+  if (count == 0) return;
+  if (src->width == 4)
+    hipLaunchKernelGGL(k_broadcast_row<uint32_t>, dim3(flat_grid(c, count)), dim3(256), 0,
+                       c->stream, count, (const uint32_t *)src->d + 3 * row,
+                       (uint32_t *)dst->d + 3 * dst_start);
+  else
+    hipLaunchKernelGGL(k_broadcast_row<uint64_t>, dim3(flat_grid(c, count)), dim3(256), 0,
+                       c->stream, count, (const uint64_t *)src->d + 3 * row,
+                       (uint64_t *)dst->d + 3 * dst_start);
+  ORQG_HIP(hipGetLastError());
+}
+
 // ElementwisePermutation(size, encoding) (permutation.h:62-69): a public
 // sharing of the identity.
 void perm_identity(orqg_ctx *c, orqg_vec *perm, int enc) {

@@ -17,6 +17,10 @@ from . import ASC, DESC, ORQG_ASHARED, ORQG_BSHARED
 
 VALID = "[##V]"
 JOIN_ID = "[##TID]"
+UNIQ = "[##UNIQ]"
+VALID_TEMP = "[##VALID_TEMP]"
+RESERVED = (JOIN_ID, VALID, UNIQ)  # encoded_table.h:38
+NON_AGGREGATIONS = ("copy", "valid")  # AggregationSelector::isAggregation
 RADIXSORT = "RADIXSORT"
 BITONICSORT = "BITONICSORT"
 
@@ -107,6 +111,157 @@ class EncodedTable:
             return self
         self.ctx.table_sort([self.schema[k] for k in keys], orders, single, a, b)
         return self
+
+    # ---- columns (encoded_table.h:724-803) ----
+    def _zero_col(self, name, rows=None):
+        enc = ORQG_BSHARED if is_bshared(name) else ORQG_ASHARED
+        return self.ctx.vec(self.rows if rows is None else rows, self.width, enc)
+
+    def add_columns(self, names, rows=None):
+        """addColumns: zero-initialised columns, B-shared when the name has '['."""
+        for c in names:
+            self.schema[c] = self._zero_col(c, rows)
+
+    def delete_columns(self, names):
+        for c in names:
+            self.schema.pop(c, None)
+
+    def project(self, cols):
+        """Keep only `cols` and the reserved columns."""
+        keep = set(cols) | set(RESERVED)
+        self.delete_columns([c for c in self.column_names() if c not in keep])
+
+    def filter(self, col):
+        """filter (encoded_table.h:298-314): valid &= col."""
+        v = self.schema[col] if isinstance(col, str) else col
+        self.schema[VALID] = self.ctx.and_b(self.schema[VALID], v)
+
+    def head(self, n):
+        self.resize(n)
+
+    # ---- distinct (encoded_table.h:1199-1237) ----
+    def distinct_into(self, keys, res):
+        """Mark rows that differ from the previous row on some key."""
+        self.ctx.distinct([self.schema[k] for k in keys], self.schema[res])
+
+    def distinct(self, keys):
+        """Sort on valid + keys, keep the first row of each key group."""
+        self.sort([VALID] + list(keys))
+        self.add_columns([UNIQ])
+        self.distinct_into(keys, UNIQ)
+        self.filter(UNIQ)
+        self.delete_columns([UNIQ])
+        return self
+
+    # ---- concatenate (encoded_table.h:1548-1616) ----
+    def concatenate(self, other):
+        names = [JOIN_ID] + [c for c in self.column_names() if c != JOIN_ID]
+        names += [c for c in other.column_names() if c not in self.schema]
+        n = self.rows + other.rows
+        cols = {c: EncodedTable._zero_static(self.ctx, c, n, self.width) for c in names}
+        t = EncodedTable.__new__(EncodedTable)
+        t.ctx, t.name, t.width, t.schema, t.rows = self.ctx, self.name + "+" + other.name, \
+            self.width, cols, n
+        t.schema[VALID] = t._ones()  # configureValid, overwritten below
+        for c in self.column_names():
+            self.ctx.copy_rows(self.schema[c], 0, t.schema[c], 0, self.rows)
+        for c in other.column_names():
+            self.ctx.copy_rows(other.schema[c], 0, t.schema[c], self.rows, other.rows)
+        # table id: 0 above, a secret sharing of 1 repeated over the other table's rows
+        one = self.ctx.secret_share(np.ones(1, np.uint64 if self.width == 8 else np.uint32),
+                                    self.width)
+        self.ctx.broadcast_row(one, 0, t.schema[JOIN_ID], self.rows, other.rows)
+        return t
+
+    @staticmethod
+    def _zero_static(ctx, name, rows, width):
+        enc = ORQG_BSHARED if is_bshared(name) else ORQG_ASHARED
+        return ctx.vec(rows, width, enc)
+
+    # ---- aggregate (encoded_table.h:1059-1188) ----
+    def aggregate(self, keys, spec, reverse=False, do_sort=True, mark_valid=True, table_id=None):
+        """`spec`: (input column, result column, function) with function one
+        of sum, count, copy (A-shared) or copy, max, min, or, valid (B-shared)."""
+        keys = list(keys)
+        if do_sort:
+            keys = [VALID] + keys
+            self.sort(keys)
+        original = self.rows
+        self.pad_power_of_two()
+        key_vecs = [self.schema[k] for k in keys]
+        b_agg, a_agg, has_any = [], [], False
+        for data, result, fn in spec:
+            if is_bshared(data) != is_bshared(result):
+                raise ValueError(f"aggregation {data} -> {result}: encodings differ")
+            if fn not in NON_AGGREGATIONS:
+                has_any = True
+            (b_agg if is_bshared(data) else a_agg).append(
+                (self.schema[data], self.schema[result], fn))
+        sel = None
+        if mark_valid:
+            self.add_columns([UNIQ])
+            self.distinct_into(keys, UNIQ)
+            if table_id is not None:
+                sel = self.schema[table_id]
+        self.ctx.aggregate(key_vecs, b_agg, a_agg, reverse, sel)
+        if original < self.rows:
+            self.resize(original)
+        if mark_valid:
+            if has_any:
+                if reverse:
+                    # valid[0 .. n-1) &= uniq[1 .. n): the last row of each group stays
+                    n = self.rows
+                    if n > 1:
+                        v = self._zero_col(VALID, n - 1)
+                        u = self._zero_col(UNIQ, n - 1)
+                        self.ctx.copy_rows(self.schema[VALID], 0, v, 0, n - 1)
+                        self.ctx.copy_rows(self.schema[UNIQ], 1, u, 0, n - 1)
+                        z = self.ctx.and_b(v, u)
+                        self.ctx.copy_rows(z, 0, self.schema[VALID], 0, n - 1)
+                else:
+                    self.filter(UNIQ)
+            self.delete_columns([UNIQ])
+        return self
+
+    # ---- joins (operators/join.h:31-233) ----
+    def inner_join(self, right, keys, spec=(), trim_invalid=True):
+        """this (unique keys) JOIN right (foreign keys) ON keys, with the
+        aggregations in `spec` (e.g. copying this table's columns down)."""
+        concat = self.concatenate(right)
+        out = set(keys) | set(right.schema)
+        for data, result, _ in spec:
+            out |= {data, result}
+        concat.project(out)
+        keys = [VALID] + list(keys)
+        concat.sort(keys + [JOIN_ID])
+        concat.add_columns([UNIQ, VALID_TEMP])
+        spec = list(spec) + [(VALID_TEMP, VALID_TEMP, "valid")]
+        concat.distinct_into(keys, UNIQ)
+        # inner join: a row is valid if its key group is not a singleton
+        concat.schema[VALID_TEMP] = self.ctx.and_b(concat.schema[VALID],
+                                                  self.ctx.not_b(concat.schema[UNIQ]))
+        concat.delete_columns([UNIQ])
+        concat.aggregate(keys, spec, reverse=True, do_sort=False, table_id=JOIN_ID)
+        concat.filter(VALID_TEMP)
+        concat.delete_columns([VALID_TEMP, JOIN_ID])
+        if trim_invalid:
+            L, R = self.rows, right.rows
+            w = 8 * self.width
+            c = len(self.schema) - 1
+            if R < L * (w + c - 1) // (c + 2):
+                concat.sort([(VALID, "DESC")])
+                concat.head(R)
+        return concat
+
+    def open_with_schema(self, remove_invalid=True):
+        """encoded_table.h:682-717: every column but the valid one, and with
+        remove_invalid only the rows whose valid bit is set."""
+        cols = self.open()
+        valid = cols.pop(VALID)
+        if remove_invalid:
+            keep = valid != 0
+            cols = {k: v[keep] for k, v in cols.items()}
+        return cols
 
     def open(self):
         """{name: signed plaintext column}: XOR for boolean, sum for arithmetic."""
