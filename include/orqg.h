@@ -142,6 +142,24 @@ int orqg_cmp(orqg_ctx *ctx, int op, const orqg_vec *x, const orqg_vec *y, orqg_v
 /* BSharedVector::b2a (b_shared_vector.h:296-329): y = A-sharing of the whole
  * boolean-shared value, one b2a_bit and one multiply_a by 2^i per bit.        */
 int orqg_b2a(orqg_ctx *ctx, const orqg_vec *x, orqg_vec *y);
+/* ---- boolean adders and a2b (include/core/operators/circuits.h:35-216) -- */
+/* BSharedVector::operator+ / operator- with the default adder, the ripple
+ * carry adder (b_shared_vector.h:588-621, circuits.h:35-77): z = x + y,
+ * z = x + ~y + 1                                                             */
+int orqg_add_b(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+int orqg_sub_b(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+/* parallel_prefix_adder (circuits.h:138-216, Kogge-Stone): z = x + y, or
+ * x + ~y + 1 when subtract != 0                                              */
+int orqg_ppa(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, int subtract, orqg_vec *z);
+/* rca_compare (circuits.h:93-136): z = sign bit of x - y (mod 2^bits), as
+ * 0/1 boolean shares                                                         */
+int orqg_rca_compare(orqg_ctx *ctx, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+/* ASharedVector::a2b (a_shared_vector.h:138-144): redistribute_shares_b
+ * (replicated_3pc.h:259-262) then the ripple carry adder; y B-shared         */
+int orqg_a2b(orqg_ctx *ctx, const orqg_vec *x, orqg_vec *y);
+/* secret_share_b / _a with data owner `owner` (replicated_3pc.h:401-448)     */
+int orqg_secret_share_from(orqg_ctx *ctx, const void *host_data, orqg_vec *v, int owner);
+
 /* Protocol::reshare for group 0..2 = {0,1},{0,2},{1,2} (protocol.h:500-541) */
 int orqg_reshare(orqg_ctx *ctx, orqg_vec *v, int group);
 

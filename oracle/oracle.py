@@ -69,6 +69,8 @@ def lib():
             ("oc_aggregate", [vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, sz, i32, vp]),
             ("oc_distinct", [vp, vp, sz, vp]),
             ("oc_vec_resize", [vp, sz]),
+            ("oc_adder", [vp, i32, vp, vp, vp]),
+            ("oc_secret_share_from", [vp, vp, vp, i32, i32]),
             ("oc_vec_tail", [vp, sz]),
             ("oc_vec_copy_rows", [vp, sz, vp, sz, sz]),
             ("oc_vec_broadcast_row", [vp, sz, vp, sz, sz]),
@@ -303,6 +305,21 @@ class Ctx:
     def distinct(self, keys, res):
         k, nk = self._vec_array(list(keys))
         _check(lib().oc_distinct(self.h, k, nk, res.h), "distinct")
+
+    ADDER = {"add": 0, "sub": 1, "ppa_add": 2, "ppa_sub": 3, "rca_lt": 4, "a2b": 5}
+
+    def adder(self, op, x, y=None):
+        """ripple_carry_adder / parallel_prefix_adder / rca_compare / a2b."""
+        z = Vec(x.n, x.width)
+        _check(lib().oc_adder(self.h, self.ADDER[op], x.h, (y or x).h, z.h), op)
+        return z
+
+    def secret_share_from(self, data, width, owner, arith=False):
+        a = np.ascontiguousarray(data, dtype=_DT[width])
+        v = Vec(a.size, width)
+        _check(lib().oc_secret_share_from(self.h, _ptr(a), v.h, int(arith), owner),
+               "secret_share_from")
+        return v
 
     def msglog(self):
         L = lib()

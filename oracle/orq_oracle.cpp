@@ -362,20 +362,21 @@ void public_share(oc_vec *v, const T *x) {
 // secret_share_b / secret_share_a from party 0 (replicated_3pc.h:401-448),
 // batched by eval_protocol_1arg_alloc (runtime.h:584-603).
 template <typename T, bool kArith>
-void secret_share(oc_ctx *c, const T *data, oc_vec *v) {
+void secret_share(oc_ctx *c, const T *data, oc_vec *v, int owner = 0) {
   for (auto b : batches_for(v->n, c->batch_size)) {
     size_t nb = b.e - b.s;
     std::vector<T> s1(nb), s2(nb), s3(nb);
-    c->P[0].local.getNext(s1.data(), nb);
-    c->P[0].local.getNext(s2.data(), nb);
+    c->P[owner].local.getNext(s1.data(), nb);
+    c->P[owner].local.getNext(s2.data(), nb);
     for (size_t i = 0; i < nb; i++)
       s3[i] = kArith ? (T)(data[b.s + i] - s1[i] - s2[i]) : (T)(data[b.s + i] ^ s1[i] ^ s2[i]);
+    // the owner keeps (s1, s2), its successor gets (s2, s3), its predecessor (s3, s1)
     const std::vector<T> *parts[3][2] = {{&s1, &s2}, {&s2, &s3}, {&s3, &s1}};
-    for (int p = 0; p < 3; p++)
+    for (int j = 0; j < 3; j++)
       for (int k = 0; k < 2; k++)
-        std::memcpy(S<T>(v, p, k) + b.s, parts[p][k]->data(), nb * sizeof(T));
-    log_msg(c, "secret_share", 0, 1, 2 * nb * sizeof(T));
-    log_msg(c, "secret_share", 0, 2, 2 * nb * sizeof(T));
+        std::memcpy(S<T>(v, (owner + j) % 3, k) + b.s, parts[j][k]->data(), nb * sizeof(T));
+    log_msg(c, "secret_share", owner, (owner + 1) % 3, 2 * nb * sizeof(T));
+    log_msg(c, "secret_share", owner, (owner + 2) % 3, 2 * nb * sizeof(T));
   }
 }
 
@@ -1377,6 +1378,191 @@ void distinct(oc_ctx *c, oc_vec **keys, size_t nk, oc_vec *res) {
   }
 }
 
+// ---- adders (operators/circuits.h:35-216) and a2b (a_shared_vector.h:138-144) ----
+// pack_from (mapping_access_vector.h:974-991): word w, bit q = bit `pos` of
+// element w*B + q
+template <typename T>
+oc_vec *pack_bits(const oc_vec *v, int pos) {
+  const int B = 8 * sizeof(T);
+  const size_t n = v->n, nc = n / B + (n % B > 0);
+  oc_vec *o = new_vec(nc, sizeof(T));
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = S<T>(const_cast<oc_vec *>(v), p, k);
+      T *d = S<T>(o, p, k);
+      for (size_t e = 0; e < n; e++) d[e / B] |= (T)((a[e] >> pos) & 1) << (e % B);
+    }
+  return o;
+}
+// unpack_from (mapping_access_vector.h:993-1013): bit `pos` of element e =
+// bit e%B of word e/B; other bits kept
+template <typename T>
+void unpack_bits(oc_vec *v, const oc_vec *w, int pos) {
+  const int B = 8 * sizeof(T);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *a = S<T>(v, p, k);
+      const T *d = S<T>(const_cast<oc_vec *>(w), p, k);
+      for (size_t e = 0; e < v->n; e++) {
+        T bit = (d[e / B] >> (e % B)) & 1;
+        a[e] = (T)((a[e] & ~((T)1 << pos)) | (bit << pos));
+      }
+    }
+}
+template <typename T>
+void xor_into(oc_vec *x, const oc_vec *y) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *a = S<T>(x, p, k);
+      const T *b = S<T>(const_cast<oc_vec *>(y), p, k);
+      for (size_t i = 0; i < x->n; i++) a[i] ^= b[i];
+    }
+}
+
+// ripple_carry_adder (circuits.h:35-77): bit-serial on bit-packed words,
+// carry c_i = ((c_{i-1} ^ a_i) & axb_i) ^ a_i, B - 1 and_b calls on n/B words
+template <typename T>
+void rca(oc_ctx *c, const oc_vec *a, const oc_vec *b, bool carry_in, oc_vec *sum) {
+  const int B = 8 * sizeof(T);
+  const size_t n = a->n, nc = n / B + (n % B > 0);
+  oc_vec *axb = local_op<T>(a, b, 0);
+  oc_vec *carry = new_vec(nc, sizeof(T));
+  if (carry_in) not_vec<T>(carry);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) std::fill(sum->sh[p][k].begin(), sum->sh[p][k].end(), 0);
+  for (int i = 0; i < B; i++) {
+    oc_vec *axb_i = pack_bits<T>(axb, i), *a_i = pack_bits<T>(a, i);
+    oc_vec *t = local_op<T>(axb_i, carry, 0);
+    unpack_bits<T>(sum, t, i);
+    delete t;
+    if (i < B - 1) {
+      xor_into<T>(carry, a_i);
+      mult<T, false>(c, carry, axb_i, carry, "and_b");
+      xor_into<T>(carry, a_i);
+    }
+    delete axb_i;
+    delete a_i;
+  }
+  delete axb;
+  delete carry;
+}
+
+// rca_compare (circuits.h:93-136): a < b as the sign of a + ~b + 1
+template <typename T>
+void rca_compare(oc_ctx *c, const oc_vec *a, const oc_vec *b, oc_vec *out) {
+  const int B = 8 * sizeof(T);
+  const size_t n = a->n, nc = n / B + (n % B > 0);
+  oc_vec *nb = not_copy<T>(b);
+  oc_vec *axb = local_op<T>(a, nb, 0);
+  oc_vec *carry = new_vec(nc, sizeof(T));
+  not_vec<T>(carry);  // + 1
+  oc_vec *axb_i = nullptr;
+  for (int i = 0; i < B; i++) {
+    delete axb_i;
+    axb_i = pack_bits<T>(axb, i);
+    if (i == B - 1) break;
+    oc_vec *a_i = pack_bits<T>(a, i);
+    xor_into<T>(carry, a_i);
+    mult<T, false>(c, carry, axb_i, carry, "and_b");
+    xor_into<T>(carry, a_i);
+    delete a_i;
+  }
+  xor_into<T>(carry, axb_i);
+  oc_vec *sum = new_vec(n, sizeof(T));
+  unpack_bits<T>(sum, carry, B - 1);
+  typedef typename std::make_signed<T>::type ST;
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {  // ltz of each share
+      const T *s = S<T>(sum, p, k);
+      T *d = S<T>(out, p, k);
+      for (size_t i = 0; i < n; i++) d[i] = (T)((ST)s[i] < 0);
+    }
+  delete nb;
+  delete axb;
+  delete carry;
+  delete axb_i;
+  delete sum;
+}
+
+// parallel_prefix_adder (circuits.h:138-216), Kogge-Stone
+template <typename T>
+void ppa(oc_ctx *c, const oc_vec *x, const oc_vec *y, bool carry_in, oc_vec *out) {
+  const int B = 8 * sizeof(T);
+  const int LOG = B == 32 ? 5 : 6;
+  const size_t n = x->n;
+  oc_vec *p = local_op<T>(x, y, 0);
+  oc_vec *g = new_vec(n, sizeof(T));
+  mult<T, false>(c, const_cast<oc_vec *>(x), const_cast<oc_vec *>(y), g, "and_b");
+  if (carry_in) {
+    oc_vec *prop = clone_vec(p);
+    mask1<T>(prop);
+    for (int k = 0; k < 2; k++)  // p ^ public 1: s_0 only
+      for (int q = 0; q < 3; q++)
+        if ((q == 0 && k == 0) || (q == 2 && k == 1)) {
+          T *a = S<T>(p, q, k);
+          for (size_t i = 0; i < n; i++) a[i] ^= 1;
+        }
+    xor_into<T>(g, prop);
+    delete prop;
+  }
+  oc_vec *propagate = clone_vec(p);
+  for (int i = 0; i < LOG; i++) {
+    oc_vec *gs = new_vec(n, sizeof(T)), *ps = new_vec(n, sizeof(T));
+    for (int q = 0; q < 3; q++)
+      for (int k = 0; k < 2; k++) {
+        const T *ga = S<T>(g, q, k), *pa = S<T>(p, q, k);
+        T *gd = S<T>(gs, q, k), *pd = S<T>(ps, q, k);
+        for (size_t e = 0; e < n; e++) {
+          gd[e] = (T)(ga[e] << (1 << i));
+          pd[e] = (T)(pa[e] << (1 << i));
+        }
+      }
+    // prefix_sum((g, p), (gs, ps)) (circuits.h:138-164)
+    mult<T, false>(c, ps, p, ps, "and_b");  // p_y &= p_x
+    mult<T, false>(c, gs, p, gs, "and_b");  // g_y &= p_x
+    oc_vec *gpg = new_vec(n, sizeof(T));
+    mult<T, false>(c, g, gs, gpg, "and_b");  // g_x & g_y
+    oc_vec *t = local_op<T>(g, gs, 0);
+    xor_into<T>(t, gpg);
+    delete g;
+    delete p;
+    g = t;
+    p = ps;
+    delete gs;
+    delete gpg;
+  }
+  for (int q = 0; q < 3; q++)
+    for (int k = 0; k < 2; k++) {
+      const T *ga = S<T>(g, q, k), *pr = S<T>(propagate, q, k);
+      T *d = S<T>(out, q, k);
+      for (size_t e = 0; e < n; e++) d[e] = pr[e] ^ (T)(ga[e] << 1);
+    }
+  delete g;
+  delete p;
+  delete propagate;
+}
+
+// ASharedVector::a2b (a_shared_vector.h:138-144) with redistribute_shares_b
+// (replicated_3pc.h:259-262): party 0 shares s0 + s1, party 2 shares s2, and
+// the default adder (ripple carry) adds them.
+template <typename T>
+void a2b(oc_ctx *c, const oc_vec *x, oc_vec *y) {
+  const size_t n = x->n;
+  std::vector<T> v0(n), v2(n);
+  const T *x00 = S<T>(const_cast<oc_vec *>(x), 0, 0), *x01 = S<T>(const_cast<oc_vec *>(x), 0, 1);
+  const T *x20 = S<T>(const_cast<oc_vec *>(x), 2, 0);
+  for (size_t i = 0; i < n; i++) {
+    v0[i] = (T)(x00[i] + x01[i]);
+    v2[i] = x20[i];
+  }
+  oc_vec *f = new_vec(n, sizeof(T)), *sv = new_vec(n, sizeof(T));
+  secret_share<T, false>(c, v0.data(), f, 0);
+  secret_share<T, false>(c, v2.data(), sv, 2);
+  rca<T>(c, f, sv, false, y);
+  delete f;
+  delete sv;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1702,6 +1888,42 @@ int oc_distinct(oc_ctx *c, oc_vec **keys, size_t nk, oc_vec *res) {
       distinct<uint32_t>(c, keys, nk, res);
     else
       distinct<uint64_t>(c, keys, nk, res);
+  })
+}
+
+// op 0: RCA add, 1: RCA subtract (x + ~y + 1), 2: PPA add, 3: PPA subtract,
+// 4: rca_compare (x < y), 5: a2b of x
+int oc_adder(oc_ctx *c, int op, oc_vec *x, oc_vec *y, oc_vec *z) {
+  OC_TRY({
+    auto run = [&](auto tag) {
+      typedef decltype(tag) T;
+      oc_vec *ny = (op == 1 || op == 3) ? not_copy<T>(y) : nullptr;
+      switch (op) {
+        case 0: rca<T>(c, x, y, false, z); break;
+        case 1: rca<T>(c, x, ny, true, z); break;
+        case 2: ppa<T>(c, x, y, false, z); break;
+        case 3: ppa<T>(c, x, ny, true, z); break;
+        case 4: rca_compare<T>(c, x, y, z); break;
+        case 5: a2b<T>(c, x, z); break;
+        default: throw std::runtime_error("adder: op");
+      }
+      delete ny;
+    };
+    if (x->width == 4)
+      run(uint32_t{});
+    else
+      run(uint64_t{});
+  })
+}
+int oc_secret_share_from(oc_ctx *c, const void *data, oc_vec *v, int arith, int owner) {
+  OC_TRY({
+    if (v->width == 4) {
+      if (arith) secret_share<uint32_t, true>(c, (const uint32_t *)data, v, owner);
+      else secret_share<uint32_t, false>(c, (const uint32_t *)data, v, owner);
+    } else {
+      if (arith) secret_share<uint64_t, true>(c, (const uint64_t *)data, v, owner);
+      else secret_share<uint64_t, false>(c, (const uint64_t *)data, v, owner);
+    }
   })
 }
 

@@ -52,7 +52,8 @@ EXPORTS = [
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
     "orqg_aggregate", "orqg_distinct", "orqg_public_fill", "orqg_vec_copy_rows",
-    "orqg_vec_broadcast_row",
+    "orqg_vec_broadcast_row", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
+    "orqg_a2b", "orqg_secret_share_from",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -116,6 +117,12 @@ def lib():
             "orqg_public_fill": (i32, [vp, vp, u64, i32]),
             "orqg_vec_copy_rows": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_vec_broadcast_row": (i32, [vp, vp, sz, vp, sz, sz]),
+            "orqg_add_b": (i32, [vp, vp, vp, vp]),
+            "orqg_sub_b": (i32, [vp, vp, vp, vp]),
+            "orqg_ppa": (i32, [vp, vp, vp, i32, vp]),
+            "orqg_rca_compare": (i32, [vp, vp, vp, vp]),
+            "orqg_a2b": (i32, [vp, vp, vp]),
+            "orqg_secret_share_from": (i32, [vp, vp, vp, i32]),
             "orqg_reshare": (i32, [vp, vp, i32]),
             "orqg_reserve_perms": (i32, [vp, sz, sz]),
             "orqg_perm_queue_size": (sz, [vp]),
@@ -428,6 +435,28 @@ class Ctx:
     def public_fill(self, v, value, encoding=ORQG_BSHARED):
         self._check(lib().orqg_public_fill(self.h, v.h, value, encoding), "public_fill")
         v.encoding = encoding
+
+    def adder(self, op, x, y=None):
+        """op: add / sub (ripple carry, the default adder), ppa_add / ppa_sub,
+        rca_lt (rca_compare), a2b."""
+        L = lib()
+        if op == "a2b":
+            z = Vec(self, x.n, x.width, ORQG_BSHARED)
+            self._check(L.orqg_a2b(self.h, x.h, z.h), "a2b")
+            return z
+        z = Vec(self, x.n, x.width, ORQG_BSHARED)
+        if op in ("add", "sub", "rca_lt"):
+            fn = {"add": L.orqg_add_b, "sub": L.orqg_sub_b, "rca_lt": L.orqg_rca_compare}[op]
+            self._check(fn(self.h, x.h, y.h, z.h), op)
+        else:
+            self._check(L.orqg_ppa(self.h, x.h, y.h, int(op == "ppa_sub"), z.h), op)
+        return z
+
+    def secret_share_from(self, data, width, owner, arith=False):
+        a = np.ascontiguousarray(data, dtype=_DT[width])
+        v = Vec(self, a.size, width, ORQG_ASHARED if arith else ORQG_BSHARED)
+        self._check(lib().orqg_secret_share_from(self.h, _ptr(a), v.h, owner), "secret_share_from")
+        return v
 
     def copy_rows(self, src, src_start, dst, dst_start, count):
         self._check(lib().orqg_vec_copy_rows(self.h, src.h, src_start, dst.h, dst_start, count),

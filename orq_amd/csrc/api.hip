@@ -450,6 +450,59 @@ int orqg_b2a(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
   y->enc = ORQG_ASHARED;
   API_END(c)
 }
+static void check_b(const orqg_vec *x) {
+  if (x->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "operand must be boolean-shared");
+}
+int orqg_add_b(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  check_b(x);
+  check_b(y);
+  adder(c, 0, x, y, z);
+  z->enc = ORQG_BSHARED;
+  API_END(c)
+}
+int orqg_sub_b(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  check_b(x);
+  check_b(y);
+  adder(c, 1, x, y, z);
+  z->enc = ORQG_BSHARED;
+  API_END(c)
+}
+int orqg_ppa(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, int subtract, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  check_b(x);
+  check_b(y);
+  adder(c, subtract ? 3 : 2, x, y, z);
+  z->enc = ORQG_BSHARED;
+  API_END(c)
+}
+int orqg_rca_compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  API_BEGIN(c)
+  check3(x, y, z);
+  check_b(x);
+  check_b(y);
+  adder(c, 4, x, y, z);
+  z->enc = ORQG_BSHARED;
+  API_END(c)
+}
+int orqg_a2b(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
+  API_BEGIN(c)
+  check3(x, nullptr, y);
+  if (x->enc != ORQG_ASHARED) throw Error(ORQG_E_ARG, "a2b: input must be arithmetic-shared");
+  if (x == y) throw Error(ORQG_E_ARG, "a2b: output must not alias the input");
+  a2b(c, x, y);
+  API_END(c)
+}
+int orqg_secret_share_from(orqg_ctx *c, const void *host, orqg_vec *v, int owner) {
+  API_BEGIN(c)
+  if (!v || !host || owner < 0 || owner > 2) throw Error(ORQG_E_ARG, "secret_share_from");
+  secret_share(c, host, v, owner);
+  API_END(c)
+}
 int orqg_reshare(orqg_ctx *c, orqg_vec *v, int group) {
   API_BEGIN(c)
   reshare(c, v, group);

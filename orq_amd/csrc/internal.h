@@ -191,7 +191,7 @@ inline unsigned grid_for(uint64_t items, unsigned block, unsigned cap) {
 
 // ---- entry points implemented in the .hip files -------------------------------
 void prg_keystream(orqg_ctx *c, int role, size_t bytes, void *out);
-void secret_share(orqg_ctx *c, const void *host, orqg_vec *v);
+void secret_share(orqg_ctx *c, const void *host, orqg_vec *v, int owner = 0);
 void open_vec(orqg_ctx *c, const orqg_vec *v, void *host_out);
 void elementwise_local(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
 void mult(orqg_ctx *c, bool arith, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
@@ -214,6 +214,8 @@ void broadcast_row(orqg_ctx *c, const orqg_vec *src, uint64_t row, orqg_vec *dst
 void aggregate(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const orqg_agg_spec *bs, size_t nbs,
                const orqg_agg_spec *as, size_t nas, int reverse, const orqg_vec *sel);
 void distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res);
+void adder(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
+void a2b(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
 void perm_b2a(orqg_ctx *c, orqg_vec *perm);
 void perm_negate(orqg_ctx *c, orqg_vec *perm);
 void perm_reverse(orqg_ctx *c, orqg_vec *perm);

@@ -48,15 +48,16 @@ void prg_keystream(orqg_ctx *c, int role, size_t bytes, void *out) {
   c->nonce[role] += (bytes + kChunkBytes - 1) / kChunkBytes;
 }
 
-// ---- secret sharing from party 0 --------------------------------------------------
-// get_shares_b/_a (replicated_3pc.h:338-392): share_1, share_2 from party 0's
-// localPRG (two getNext calls per batch), share_3 = x ^ s1 ^ s2 (x - s1 - s2).
+// ---- secret sharing from a data owner ---------------------------------------------
+// get_shares_b/_a (replicated_3pc.h:338-392) and secret_share_b/_a (:401-448):
+// share_1, share_2 from the owner's localPRG (two getNext calls per batch),
+// share_3 = x ^ s1 ^ s2 (x - s1 - s2).
 template <typename T, bool ARITH>
 __global__ void __launch_bounds__(kAesThreads) k_secret_share(const uint32_t *__restrict__ t0,
                                                       const uint32_t *__restrict__ rk,
                                                       uint64_t nonce0, CallSched s,
                                                       const T *__restrict__ plain,
-                                                      T *__restrict__ out) {
+                                                      T *__restrict__ out, int owner) {
   constexpr int E = 16 / sizeof(T);
   AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
@@ -76,36 +77,37 @@ __global__ void __launch_bounds__(kAesThreads) k_secret_share(const uint32_t *__
     for (int i = 0; i < E && e0 + i < s.n; i++) {
       T x = plain[e0 + i];
       T s2 = ARITH ? (T)(x - r[0][i] - r[1][i]) : (T)(x ^ r[0][i] ^ r[1][i]);
+      // the owner keeps (r1, r2), its successor gets (r2, r3), its predecessor (r3, r1)
       T *o = out + 3 * (e0 + i);
-      o[0] = r[0][i];
-      o[1] = r[1][i];
-      o[2] = s2;
+      o[owner] = r[0][i];
+      o[(owner + 1) % 3] = r[1][i];
+      o[(owner + 2) % 3] = s2;
     }
   }
 }
 
-void secret_share(orqg_ctx *c, const void *host, orqg_vec *v) {
+void secret_share(orqg_ctx *c, const void *host, orqg_vec *v, int owner) {
   size_t bytes = v->n * v->width;
   if (v->n == 0) return;
   void *tmp = c->scratch_get("plain_in", bytes);
   ORQG_HIP(hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, c->stream));
   CallSched s = make_sched(v->n, c->batch_size, v->width, 2);
   unsigned grid = grid_for((v->n + 3) / 4, kAesThreads, c->num_cu);
-  int role = kLocal0;  // data owner = party 0
+  int role = kLocal0 + owner;  // the data owner's local PRG
   bool arith = v->enc == ORQG_ASHARED;
   if (v->width == 4) {
     auto k = arith ? k_secret_share<uint32_t, true> : k_secret_share<uint32_t, false>;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
-                       c->nonce[role], s, (const uint32_t *)tmp, (uint32_t *)v->d);
+                       c->nonce[role], s, (const uint32_t *)tmp, (uint32_t *)v->d, owner);
   } else {
     auto k = arith ? k_secret_share<uint64_t, true> : k_secret_share<uint64_t, false>;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
-                       c->nonce[role], s, (const uint64_t *)tmp, (uint64_t *)v->d);
+                       c->nonce[role], s, (const uint64_t *)tmp, (uint64_t *)v->d, owner);
   }
   ORQG_HIP(hipGetLastError());
   c->nonce[role] += sched_nonces(s);
-  c->log_msg("secret_share", 0, 1, 2 * bytes);
-  c->log_msg("secret_share", 0, 2, 2 * bytes);
+  c->log_msg("secret_share", owner, (owner + 1) % 3, 2 * bytes);
+  c->log_msg("secret_share", owner, (owner + 2) % 3, 2 * bytes);
 }
 
 // ---- open --------------------------------------------------------------------------
