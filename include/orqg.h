@@ -63,6 +63,10 @@ typedef struct orqg_vec orqg_vec;
  * (include/backend/common/setup.h:7-87) with deterministic keys           */
 int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out);
 int orqg_ctx_destroy(orqg_ctx *ctx);
+/* Give the context fresh keys: every PRG nonce back to 0, the permutation
+ * queue emptied, the message log and timers cleared (a new session of the
+ * three parties, as RandomnessManager setup does in rand_setup.h:21-146).   */
+int orqg_ctx_rekey(orqg_ctx *ctx, const uint8_t keys[ORQG_NUM_KEYS][32]);
 const char *orqg_last_error(orqg_ctx *ctx);
 int orqg_sync(orqg_ctx *ctx);
 /* the context's HIP stream (hipStream_t), for callers that order their own work */
@@ -260,6 +264,52 @@ int orqg_msglog_clear(orqg_ctx *ctx);
 int orqg_timer_enable(orqg_ctx *ctx, int enable);
 int orqg_timer_get(orqg_ctx *ctx, const char *family, double *ms, uint64_t *launches);
 int orqg_timer_reset(orqg_ctx *ctx);
+
+
+/* ---- serving loop (no counterpart in the reference: its batched/served form) --
+ * Secret-shared query requests on one GPU with continuous batching: `slots`
+ * worker contexts run admitted requests concurrently; a request is admitted
+ * in arrival order when a slot is free and its estimated device footprint fits
+ * hbm_budget beside the running ones.  Each request re-keys its slot with its
+ * own keys, so its result shares equal those of the same request run alone.
+ * Host buffers named by a request must stay valid until its completion.      */
+enum { ORQG_REQ_SORT = 0,        /* EncodedTable::sort on [K], data V and [##V]  */
+       ORQG_REQ_GROUPBY_SUM = 1  /* EncodedTable::aggregate([K], {V, V, sum})     */ };
+typedef struct {
+  uint64_t id;
+  int op;                            /* ORQG_REQ_* */
+  int width;                         /* 4 or 8 bytes per element */
+  size_t n;                          /* rows */
+  const void *key_shares;            /* host [3][n]: s_0, s_1, s_2 of the B-shared key */
+  const void *val_shares;            /* host [3][n]: the A-shared value column */
+  uint8_t keys[ORQG_NUM_KEYS][32];   /* the request's PRG keys (roles as above) */
+  void *out_key, *out_val, *out_valid; /* host [3][n] result shares (NULL: skip) */
+} orqg_request;
+typedef struct {
+  uint64_t id;
+  int status;                        /* ORQG_OK or an ORQG_E_* code */
+  int slot;                          /* worker that ran it, -1 if rejected */
+  double t_submit_ms, t_start_ms, t_done_ms; /* server clock */
+} orqg_completion;
+typedef struct {
+  int device;
+  int slots;                         /* concurrent requests, 1..64 */
+  size_t hbm_budget;                 /* admission budget in bytes (0: 200 GiB) */
+  long batch_size;                   /* runtime batch size as orqg_config */
+} orqg_server_config;
+typedef struct {
+  uint64_t submitted, completed, rejected, max_running, peak_reserved, peak_live_bytes;
+} orqg_server_stats;
+typedef struct orqg_server orqg_server;
+int orqg_server_create(const orqg_server_config *cfg, orqg_server **out);
+int orqg_server_submit(orqg_server *srv, const orqg_request *req);
+/* up to `max` completions; waits up to wait_ms for the first one           */
+int orqg_server_poll(orqg_server *srv, orqg_completion *out, int max, int *count, int wait_ms);
+int orqg_server_pending(orqg_server *srv, int *queued, int *running);
+int orqg_server_get_stats(orqg_server *srv, orqg_server_stats *stats);
+double orqg_server_now_ms(orqg_server *srv);
+/* runs every queued request to completion, then stops the workers           */
+int orqg_server_destroy(orqg_server *srv);
 
 #ifdef __cplusplus
 }

@@ -52,8 +52,10 @@ EXPORTS = [
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
     "orqg_aggregate", "orqg_distinct", "orqg_public_fill", "orqg_vec_copy_rows",
-    "orqg_vec_broadcast_row", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
-    "orqg_a2b", "orqg_secret_share_from",
+    "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
+    "orqg_a2b", "orqg_secret_share_from", "orqg_server_create", "orqg_server_submit",
+    "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
+    "orqg_server_destroy",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -118,6 +120,14 @@ def lib():
             "orqg_vec_copy_rows": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_vec_broadcast_row": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_add_b": (i32, [vp, vp, vp, vp]),
+            "orqg_ctx_rekey": (i32, [vp, ctypes.c_char_p]),
+            "orqg_server_create": (i32, [vp, vp]),
+            "orqg_server_submit": (i32, [vp, vp]),
+            "orqg_server_poll": (i32, [vp, vp, i32, vp, i32]),
+            "orqg_server_pending": (i32, [vp, vp, vp]),
+            "orqg_server_get_stats": (i32, [vp, vp]),
+            "orqg_server_now_ms": (ctypes.c_double, [vp]),
+            "orqg_server_destroy": (i32, [vp]),
             "orqg_sub_b": (i32, [vp, vp, vp, vp]),
             "orqg_ppa": (i32, [vp, vp, vp, i32, vp]),
             "orqg_rca_compare": (i32, [vp, vp, vp, vp]),
@@ -457,6 +467,11 @@ class Ctx:
         v = Vec(self, a.size, width, ORQG_ASHARED if arith else ORQG_BSHARED)
         self._check(lib().orqg_secret_share_from(self.h, _ptr(a), v.h, owner), "secret_share_from")
         return v
+
+    def rekey(self, keys):
+        """Fresh keys: nonces back to 0, permutation queue and message log cleared."""
+        assert len(keys) == 32 * 10
+        self._check(lib().orqg_ctx_rekey(self.h, bytes(keys)), "ctx_rekey")
 
     def copy_rows(self, src, src_start, dst, dst_start, count):
         self._check(lib().orqg_vec_copy_rows(self.h, src.h, src_start, dst.h, dst_start, count),

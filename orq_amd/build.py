@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liborqg.so")
-SOURCES = ["api.hip", "prim.hip", "cmp.hip", "genperm.hip", "perm.hip", "sort.hip", "table.hip", "bitonic.hip", "agg.hip", "adder.hip"]
+SOURCES = ["api.hip", "prim.hip", "cmp.hip", "genperm.hip", "perm.hip", "sort.hip", "table.hip", "bitonic.hip", "agg.hip", "adder.hip", "serve.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -47,7 +47,7 @@ def build(verbose=False, extra=()):
     if failed:
         raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lpthread", "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

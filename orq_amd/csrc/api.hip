@@ -3,6 +3,7 @@
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "internal.h"
 
@@ -68,10 +69,21 @@ void orqg_ctx::log_msg(const char *op, int from, int to, uint64_t bytes) {
 }
 
 namespace orqg {
-// Device bytes held by the library in this process (all contexts), checked
-// against a context's HBM budget when it allocates.
+// Device memory: a caching allocator.  Blocks come in size classes (four per
+// power of two); a freed block goes back to the pool of the context that
+// allocated it and is reused by that context's later allocations.  All of a
+// context's device work is ordered on its stream, so a block reused by the
+// same context is never touched by earlier work still in flight, and no
+// hipFree (which waits for the whole device) sits on the steady-state path.
+// g_live_bytes counts everything held, pooled blocks included, and is what a
+// context's HBM budget is checked against.
+struct Block {
+  size_t bytes;
+  orqg_ctx *owner;
+};
 static size_t g_live_bytes = 0;
-static std::unordered_map<void *, size_t> g_sizes;
+static std::unordered_map<void *, Block> g_blocks;
+static std::unordered_set<orqg_ctx *> g_alive;
 static std::mutex g_alloc_mu;
 
 size_t live_bytes() {
@@ -79,29 +91,88 @@ size_t live_bytes() {
   return g_live_bytes;
 }
 
-void *dev_alloc(orqg_ctx *c, size_t bytes) {
-  if (bytes == 0) bytes = 16;
-  std::lock_guard<std::mutex> lk(g_alloc_mu);
-  if (c->budget && g_live_bytes + bytes > c->budget)
-    throw Error(ORQG_E_BUDGET, "allocation exceeds the configured HBM budget");
-  void *p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    throw Error(ORQG_E_OOM, "hipMalloc failed");
+static size_t size_class(size_t b) {
+  if (b <= 256) return 256;
+  const int k = 63 - __builtin_clzll((unsigned long long)(b - 1));  // 2^k < b <= 2^(k+1)
+  const size_t step = (size_t)1 << (k - 2);
+  return (b + step - 1) / step * step;
+}
+
+// hipFree every pooled block of every context (caller holds g_alloc_mu)
+static void release_pools_locked() {
+  for (orqg_ctx *c : g_alive) {
+    for (auto &kv : c->pool) {
+      (void)hipFree(kv.second);
+      g_live_bytes -= kv.first;
+      g_blocks.erase(kv.second);
+    }
+    c->pool.clear();
   }
-  g_live_bytes += bytes;
-  g_sizes[p] = bytes;
+}
+
+void register_ctx(orqg_ctx *c) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_alive.insert(c);
+}
+
+// The context's pooled blocks are released; blocks it still lends out (vectors
+// the caller has not freed) are released by hipFree when they are freed.
+void unregister_ctx(orqg_ctx *c) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  for (auto &kv : c->pool) {
+    (void)hipFree(kv.second);
+    g_live_bytes -= kv.first;
+    g_blocks.erase(kv.second);
+  }
+  c->pool.clear();
+  for (auto &kv : g_blocks)
+    if (kv.second.owner == c) kv.second.owner = nullptr;
+  g_alive.erase(c);
+}
+
+void *dev_alloc(orqg_ctx *c, size_t bytes) {
+  const size_t cls = size_class(bytes ? bytes : 16);
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  auto it = c->pool.find(cls);
+  if (it != c->pool.end()) {
+    void *p = it->second;
+    c->pool.erase(it);
+    return p;
+  }
+  if (c->budget && g_live_bytes + cls > c->budget) {
+    release_pools_locked();
+    if (g_live_bytes + cls > c->budget)
+      throw Error(ORQG_E_BUDGET, "allocation exceeds the configured HBM budget");
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, cls) != hipSuccess) {
+    (void)hipGetLastError();
+    release_pools_locked();
+    if (hipMalloc(&p, cls) != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error(ORQG_E_OOM, "hipMalloc failed");
+    }
+  }
+  g_live_bytes += cls;
+  g_blocks[p] = {cls, c};
   return p;
 }
 void dev_free(void *p) {
   if (!p) return;
-  (void)hipFree(p);
   std::lock_guard<std::mutex> lk(g_alloc_mu);
-  auto it = g_sizes.find(p);
-  if (it != g_sizes.end()) {
-    g_live_bytes -= it->second;
-    g_sizes.erase(it);
+  auto it = g_blocks.find(p);
+  if (it == g_blocks.end()) {
+    (void)hipFree(p);
+    return;
   }
+  orqg_ctx *owner = it->second.owner;
+  if (owner && g_alive.count(owner)) {
+    owner->pool.emplace(it->second.bytes, p);
+    return;
+  }
+  (void)hipFree(p);
+  g_live_bytes -= it->second.bytes;
+  g_blocks.erase(it);
 }
 }  // namespace orqg
 
@@ -131,6 +202,7 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     c->budget = cfg->hbm_budget;
     std::memcpy(c->keys, cfg->keys, sizeof(c->keys));
     ORQG_HIP(hipSetDevice(c->device));
+    register_ctx(c);
     ORQG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     ORQG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     ORQG_HIP(hipEventCreateWithFlags(&c->ev_aux, hipEventDisableTiming));
@@ -148,6 +220,7 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     ORQG_HIP(hipMemcpy(c->d_t0, img.data(), img.size() * 4, hipMemcpyHostToDevice));
   } catch (const std::exception &e) {
     std::fprintf(stderr, "orqg_ctx_create: %s\n", e.what());
+    unregister_ctx(c);
     delete c;
     return ORQG_E_DEVICE;
   }
@@ -163,6 +236,7 @@ int orqg_ctx_destroy(orqg_ctx *c) {
   for (auto &p : c->queue) free_perm(p);
   dev_free(c->d_rk);
   dev_free(c->d_t0);
+  unregister_ctx(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev_aux) (void)hipEventDestroy(c->ev_aux);
   if (c->stream2) {
@@ -172,6 +246,25 @@ int orqg_ctx_destroy(orqg_ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ORQG_OK;
+}
+
+int orqg_ctx_rekey(orqg_ctx *c, const uint8_t keys[ORQG_NUM_KEYS][32]) {
+  API_BEGIN(c)
+  if (!keys) throw Error(ORQG_E_ARG, "ctx_rekey: NULL keys");
+  ORQG_HIP(hipStreamSynchronize(c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream2));
+  std::memcpy(c->keys, keys, sizeof(c->keys));
+  uint32_t rk[ORQG_NUM_KEYS * 60];
+  for (int k = 0; k < ORQG_NUM_KEYS; k++) aes256_expand(c->tabs, c->keys[k], rk + 60 * k);
+  ORQG_HIP(hipMemcpy(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice));
+  for (int k = 0; k < ORQG_NUM_KEYS; k++) c->nonce[k] = 0;
+  for (auto &p : c->queue) free_perm(p);
+  c->queue.clear();
+  c->stored_size = 0;
+  c->log.clear();
+  c->timer_flush();
+  c->timers.clear();
+  API_END(c)
 }
 
 const char *orqg_last_error(orqg_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -288,8 +381,8 @@ int orqg_vec_broadcast_row(orqg_ctx *c, const orqg_vec *src, size_t row, orqg_ve
   API_END(c)
 }
 
-// hipFree waits for outstanding work on the device; the owning context may
-// already be gone, so it is not touched here.
+// The block goes back to its context's pool (or to hipFree when the context
+// is gone); the context itself is not touched here.
 int orqg_vec_free(orqg_vec *v) {
   if (!v) return ORQG_E_ARG;
   dev_free(v->d);

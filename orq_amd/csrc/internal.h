@@ -125,6 +125,7 @@ struct orqg_ctx {
   std::string err;
   std::vector<orqg_msg> log;
   std::map<std::string, orqg::Scratch> scratch;
+  std::multimap<size_t, void *> pool;  // free device blocks by size (allocator lock held)
   std::deque<orqg::ShardedPerm> queue;
   size_t stored_size = 0;
   int num_cu = 256;
@@ -157,9 +158,10 @@ struct orqg_vec {
 
 namespace orqg {
 
-// Allocation that counts against the HBM budget.
+// Allocation that counts against the HBM budget (a caching allocator per context).
 void *dev_alloc(orqg_ctx *c, size_t bytes);
 void dev_free(void *p);
+size_t live_bytes();
 
 // Scoped timing of one kernel family on the context stream: two events
 // bracket the region's launches; nothing waits for them here.

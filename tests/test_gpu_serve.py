@@ -1,0 +1,152 @@
+"""The serving loop (continuous batching of secret-shared requests): each
+result equals the same request run alone and the table-layer flow it stands
+for (bit-exact shares), the plaintext answer is right, and admission keeps to
+the HBM budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+UT = {4: np.uint32, 8: np.uint64}
+
+
+@pytest.fixture(scope="module")
+def G():
+    import orq_amd
+
+    orq_amd.lib()
+    return orq_amd
+
+
+def _shares(x, rng, width):
+    """A client's boolean and arithmetic sharings of x, [3][n] each."""
+    ut = UT[width]
+    x = np.asarray(x).astype(ut)
+    b = rng.integers(0, 2**64, (2, x.size), dtype=np.uint64).astype(ut)
+    bs = np.stack([b[0], b[1], x ^ b[0] ^ b[1]])
+    with np.errstate(over="ignore"):
+        a = rng.integers(0, 2**64, (2, x.size), dtype=np.uint64).astype(ut)
+        as_ = np.stack([a[0], a[1], x - a[0] - a[1]])
+    return bs, as_
+
+
+def _make_jobs(G, sizes, width=8, seed=0):
+    from orq_amd.serve import REQ_GROUPBY_SUM, REQ_SORT, Job
+
+    rng = np.random.default_rng(seed)
+    jobs, plain = [], []
+    for i, n in enumerate(sizes):
+        k = rng.integers(0, max(n // 4, 2), n)
+        v = rng.integers(0, 10**6, n)
+        kb, _ = _shares(k, rng, width)
+        _, va = _shares(v, rng, width)
+        op = REQ_SORT if i % 2 == 0 else REQ_GROUPBY_SUM
+        jobs.append(Job(i, op, kb, va, G.keys_from_seed(1000 + i)))
+        plain.append((op, k, v))
+    return jobs, plain
+
+
+def _check_plain(job, op, k, v):
+    from orq_amd.serve import REQ_SORT
+
+    sdt = np.int64 if job.out_key.dtype == np.uint64 else np.int32
+    key = (job.out_key[0] ^ job.out_key[1] ^ job.out_key[2]).view(sdt)
+    val = (job.out_val[0] + job.out_val[1] + job.out_val[2]).view(sdt)
+    valid = job.out_valid[0] ^ job.out_valid[1] ^ job.out_valid[2]
+    if op == REQ_SORT:
+        order = np.argsort(k, kind="stable")
+        np.testing.assert_array_equal(key, k[order])
+        np.testing.assert_array_equal(val, v[order])
+        assert valid.all()
+    else:
+        keys_u, inv = np.unique(k, return_inverse=True)
+        sums = np.bincount(inv, weights=v).astype(np.int64)
+        m = valid == 1
+        got = sorted(zip(key[m].tolist(), val[m].tolist()))
+        assert got == list(zip(keys_u.tolist(), sums.tolist()))
+
+
+def _table_flow(G, job):
+    """The same request through the host table layer on a fresh context."""
+    from orq_amd.serve import REQ_SORT
+    from orq_amd.table import EncodedTable
+
+    c = G.Ctx(bytes(job.req.keys))
+    w = job.key_in.dtype.itemsize
+    K = c.vec(job.n, w, G.ORQG_BSHARED)
+    K.set_shares(list(job.key_in))
+    V = c.vec(job.n, w, G.ORQG_ASHARED)
+    V.set_shares(list(job.val_in))
+    t = EncodedTable(c, "req", {"[K]": K, "V": V}, w)
+    if job.op == REQ_SORT:
+        t.sort(["[K]"])
+    else:
+        t.aggregate(["[K]"], [("V", "V", "sum")])
+    return [np.stack(x.shares()) for x in (t["[K]"], t["V"], t["[##V]"])]
+
+
+def test_server_matches_table_layer(G):
+    from orq_amd.serve import Server
+
+    jobs, plain = _make_jobs(G, [1024, 3000, 4096, 777], seed=1)
+    srv = Server(slots=2)
+    for j in jobs:
+        srv.submit(j)
+    done = srv.drain()
+    srv.close()
+    assert len(done) == len(jobs)
+    for j, (op, k, v) in zip(jobs, plain):
+        assert j.completion[0] == 0, j.completion
+        ref = _table_flow(G, j)
+        for got, want in zip((j.out_key, j.out_val, j.out_valid), ref):
+            np.testing.assert_array_equal(got, want)
+        _check_plain(j, op, k, v)
+
+
+def test_mix_equals_alone(G):
+    from orq_amd.serve import Server
+
+    sizes = [1 << 10, 1 << 14, 3000, 1 << 12, 1 << 16, 5000, 1 << 11, 20000]
+    mix, plain = _make_jobs(G, sizes, seed=2)
+    alone, _ = _make_jobs(G, sizes, seed=2)
+    srv = Server(slots=4)
+    for j in mix:
+        srv.submit(j)
+    srv.drain()
+    st = srv.stats()
+    srv.close()
+    assert st["max_running"] > 1  # requests did overlap
+    srv1 = Server(slots=1)
+    for j in alone:
+        srv1.submit(j)
+        srv1.drain()
+    srv1.close()
+    for a, b, (op, k, v) in zip(mix, alone, plain):
+        assert a.completion[0] == 0 and b.completion[0] == 0
+        for x, y in ((a.out_key, b.out_key), (a.out_val, b.out_val), (a.out_valid, b.out_valid)):
+            np.testing.assert_array_equal(x, y)
+        _check_plain(a, op, k, v)
+
+
+def test_admission_budget(G):
+    from orq_amd.serve import Server
+
+    sizes = [1 << 14] * 6
+    jobs, _ = _make_jobs(G, sizes, seed=3)
+    # the footprint estimate of one 2^14 request is ~ 100 MB: a 250 MB budget
+    # admits two at a time
+    srv = Server(slots=4, hbm_budget=250 << 20)
+    for j in jobs:
+        srv.submit(j)
+    big, _ = _make_jobs(G, [1 << 20], seed=4)
+    big[0].id = 99
+    big[0].req.id = 99
+    srv.submit(big[0])
+    done = srv.drain()
+    st = srv.stats()
+    srv.close()
+    status = {j.id: j.completion[0] for j in done}
+    assert status[99] == -3  # ORQG_E_BUDGET: larger than the whole budget
+    assert all(status[i] == 0 for i in range(6))
+    assert st["max_running"] <= 2
+    assert st["peak_reserved"] <= 250 << 20
