@@ -15,16 +15,18 @@
 // (1) walks the draw stream of each group PRG ({0,1}, {0,2}, {1,2}: three
 // independent chains, processed by the same launches) window by window.  In a
 // window that starts with k open positions:
-//   * k_gp_draws: keystream -> draws; draws with v >= k can never be accepted
-//     (k only decreases); the others ("candidates") are compacted per tile;
-//   * k_gp_classify: candidate a (rank in the window) is accepted for sure if
-//     v < k - a (at most a acceptances precede it); otherwise it is
-//     "ambiguous" and is listed, per tile, in rank order;
+//   * k_gp_window: keystream -> draws; draws with v >= k can never be accepted
+//     (k only decreases); the others ("candidates") get their rank a in the
+//     window through a decoupled look-back over the tiles and are written in
+//     rank order.  Candidate a is accepted for sure if v < k - a (at most a
+//     acceptances precede it); otherwise it is "ambiguous" and is written, in
+//     rank order too, to the resolver's list (a second look-back).  The same
+//     launch first commits the previously resolved window: accepted candidate
+//     a becomes r_{i0 + a - #rejected before a};
 //   * k_gp_resolve: one workgroup per chain decides the ambiguous candidates
 //     exactly with alternating optimistic passes, builds the sorted list of
 //     rejected ranks, and advances the chain (next window, or the next
-//     permutation at the refill boundary after the last acceptance);
-//   * k_gp_commit: accepted candidate a becomes r_{i0 + a - #rejected before a}.
+//     permutation at the refill boundary after the last acceptance).
 // The window size keeps the expected number of ambiguous candidates within the
 // resolver's capacity; an overflowing window is retried at a quarter of its
 // size, down to kWinMin draws, which always fits.
@@ -62,21 +64,25 @@ struct ChainState {
   uint32_t W;       // draws in the current window
   uint32_t done;
   uint32_t err;
-  // the window just resolved, for k_gp_commit
-  uint32_t c_valid, c_i0, c_k, c_ntiles, c_R;
+  uint32_t seq;     // window sequence number (tags the look-back status words)
+  uint32_t buf;     // candidate buffer of the current window
+  uint32_t C_tot;   // candidates in the current window (written by its last tile)
+  uint32_t m_tot;   // ambiguous candidates in it
+  // the window just resolved, committed by the next k_gp_window launch
+  uint32_t c_valid, c_i0, c_k, c_C, c_R, c_buf;
 };
 
 struct ChainBufs {
   ChainState *st;      // [3]
-  uint64_t *tc;        // [3][kWinMax]   per tile: (draw offset << 32) | v of its candidates
-  uint32_t *tcnt;      // [3][kMaxTiles] candidates per tile
-  uint32_t *toff;      // [3][kMaxTiles] rank of the tile's first candidate
-  uint64_t *ta;        // [3][kWinMax]   per tile: (rank << 32) | v of its ambiguous candidates
-  uint32_t *tacnt;     // [3][kMaxTiles]
+  uint64_t *cand;      // [2][3][cand_cap] candidates in rank order: (draw offset << 32) | v
+  uint32_t cand_cap;
+  uint64_t *stat_c;    // [3][kMaxTiles] look-back status of the candidate counts
+  uint64_t *stat_a;    // [3][kMaxTiles] ... of the ambiguous counts
+  uint32_t *ctr;       // [2] tile ticket, finished workgroups
   uint32_t *amb_a;     // [3][kAmbCap]   ambiguous candidates in rank order: rank
   int32_t *amb_u;      // [3][kAmbCap]   ... and U = rank - (k - v)
-  uint32_t *m_tot;     // [3]            ambiguous candidates in the window
   uint32_t *rej;       // [3][kAmbCap]   ranks of rejected candidates, ascending
+  uint32_t *rejblk;    // [3][cand_cap / 1024 + 2] rejected ranks below 1024 b
   uint32_t *r_out;     // [3][count][n]  accepted offsets r_i
   uint64_t *D;         // [3][count]     draws consumed per permutation
   uint64_t *nonce_at;  // [3][count+1]   nonce at the start of each permutation
@@ -96,24 +102,36 @@ __device__ inline uint32_t ntiles_of(const ChainState &s) {
 }
 
 // exclusive block scan of one value per thread (blockDim multiple of 64)
-__device__ inline uint32_t block_scan(uint32_t x, uint32_t *s_w, uint32_t &total) {
+template <typename T>
+__device__ inline T block_scan(T x, T *s_w, T &total) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  uint32_t inc = x;
+  T inc = x;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    uint32_t t = __shfl_up(inc, o, 64);
+    T t = __shfl_up(inc, o, 64);
     if (lane >= o) inc += t;
   }
   __syncthreads();
   if (lane == 63) s_w[wv] = inc;
   __syncthreads();
-  uint32_t base = 0;
+  T base = 0;
   total = 0;
   for (int w = 0; w < nw; w++) {
     if (w < wv) base += s_w[w];
     total += s_w[w];
   }
   return base + inc - x;
+}
+
+__device__ inline uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__device__ inline uint64_t lanes_below() {
+  const int lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
 // (chain, tile) of flat task `task` over the three chains' tiles; false if none
@@ -139,211 +157,379 @@ __device__ inline bool task_tile(const uint32_t (&nt)[3], uint32_t task, int &ch
   return false;
 }
 
-// ---- (a) keystream -> candidates, compacted per tile ------------------------------
-template <int COPIES>
-__global__ void __launch_bounds__(kGpThreads) k_gp_draws(const uint32_t *__restrict__ tab,
-                                                         const uint32_t *__restrict__ rk_groups,
-                                                         ChainBufs B, uint32_t n, uint32_t mask) {
-  __shared__ __align__(16) uint32_t lds[AesLds<COPIES>::kWords];
-  __shared__ uint32_t s_w[kGpThreads / 64];
-  uint32_t nt[3];
-  for (int c = 0; c < 3; c++) nt[c] = ntiles_of(B.st[c]);
-  const uint32_t total = nt[0] + nt[1] + nt[2];
-  if (blockIdx.x >= total) return;
-  aes_load_tables<COPIES>(lds, tab);
-  AesCtx<COPIES> A(lds);
-  for (uint32_t task = blockIdx.x; task < total; task += gridDim.x) {
-    int chain;
-    uint32_t tile;
-    task_tile(nt, task, chain, tile);
-    const ChainState &st = B.st[chain];
-    const uint32_t kstart = n - st.i, W = st.W;
-    const uint32_t off0 = tile * kTile + threadIdx.x * kDrawsPerThread;
-    uint32_t v[kDrawsPerThread];
-    if (off0 < W) {
-      const uint32_t *rk = rk_groups + 60 * chain;
-      const uint64_t dbase = st.d0 + off0;
-#pragma unroll
-      for (int b = 0; b < kDrawsPerThread / 4; b++) {
-        const uint64_t d = dbase + 4 * b;
-        uint4 blk = aes_ctr_block<COPIES>(A, rk, st.nonce + d / kWordsPerNonce,
-                                          2 + (uint32_t)((d % kWordsPerNonce) / 4));
-        v[4 * b + 0] = blk.x & mask;
-        v[4 * b + 1] = blk.y & mask;
-        v[4 * b + 2] = blk.z & mask;
-        v[4 * b + 3] = blk.w & mask;
+// ---- decoupled look-back over the tiles of one chain's window ----------------------
+// Status word of a tile: (seq << 2 | kind) << 32 | value, kind 1 = the tile's own
+// count, 2 = inclusive prefix; a word with another seq is not published yet.
+// Tiles are taken in increasing order from a ticket, so every tile waited on
+// belongs to a running (or finished) workgroup and publishes its count before
+// waiting itself.  Run by one whole wave; returns the exclusive prefix.
+constexpr uint32_t kSpinMax = 1u << 20;
+
+__device__ inline uint32_t look_back(uint64_t *stat, uint32_t tile, uint32_t seq, uint32_t cnt,
+                                     uint32_t *err) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t tag_own = (uint64_t)((seq << 2) | 1u) << 32;
+  const uint64_t tag_inc = (uint64_t)((seq << 2) | 2u) << 32;
+  if (tile == 0) {
+    if (lane == 0) __hip_atomic_store(stat, tag_inc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (lane == 0)
+    __hip_atomic_store(stat + tile, tag_own | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t excl = 0;
+  for (int32_t base = (int32_t)tile - 1;; base -= 64) {
+    const int32_t idx = base - lane;
+    uint32_t kind = 2, val = 0;  // before tile 0: an inclusive prefix of 0
+    if (idx >= 0) {
+      for (uint32_t spin = 0;; spin++) {
+        const uint64_t s =
+            __hip_atomic_load(stat + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t hi = (uint32_t)(s >> 32);
+        if ((hi >> 2) == seq && (hi & 3u)) {
+          kind = hi & 3u;
+          val = (uint32_t)s;
+          break;
+        }
+        if (spin >= kSpinMax) {  // never expected: give up loudly instead of hanging
+          atomicOr(err, 16u);
+          kind = 2;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
       }
-    } else {
-#pragma unroll
-      for (int j = 0; j < kDrawsPerThread; j++) v[j] = 0xffffffffu;  // never a candidate
     }
-    uint32_t nc = 0;
+    const uint64_t incm = __ballot(kind == 2);
+    const int first = __builtin_ctzll(incm | (1ull << 63));
+    excl += wave_sum(lane <= first ? val : 0u);
+    if (incm) break;
+  }
+  if (lane == 0)
+    __hip_atomic_store(stat + tile, tag_inc | (excl + cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+// ---- commit: r_i for the accepted candidates of a resolved window ------------------
+// Accepted candidate of rank a becomes r_{i0 + a - #rejected ranks below a}.  A
+// wave takes 1024 consecutive ranks: the rejected ones among them (a slice of
+// the ascending rejected list, located by rejblk) become a 1024-bit mask in
+// LDS, and lane l handles ranks base + 64 j + l, so loads and stores coalesce.
+__device__ inline void commit_window(const ChainBufs &B, int chain, uint32_t n, uint32_t count,
+                                     unsigned long long *s_mask) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lanes_below();
+  const ChainState &st = B.st[chain];
+  const uint32_t C = st.c_C, i0 = st.c_i0, kstart = n - i0;
+  const uint64_t *cand = B.cand + ((size_t)st.c_buf * 3 + chain) * B.cand_cap;
+  const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
+  const uint32_t *rejblk = B.rejblk + (size_t)chain * (B.cand_cap / 1024 + 2);
+  uint32_t *r = B.r_out + ((size_t)chain * count + st.c_k) * (size_t)n;
+  const uint32_t nb = (C + 1023) / 1024;
+  const uint32_t gwaves = gridDim.x * (blockDim.x / 64);
+  for (uint32_t b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); b < nb; b += gwaves) {
+    const uint32_t base = b * 1024;
+    uint32_t v[16];
 #pragma unroll
-    for (int j = 0; j < kDrawsPerThread; j++) nc += v[j] < kstart;
-    uint32_t tot;
-    uint32_t pos = block_scan(nc, s_w, tot);
-    uint64_t *out = B.tc + (size_t)chain * kWinMax + (size_t)tile * kTile;
+    for (int j = 0; j < 16; j++) {
+      const uint32_t f = base + 64 * j + lane;
+      v[j] = f < C ? (uint32_t)cand[f] : 0;
+    }
+    const uint32_t lo0 = rejblk[b], lo1 = rejblk[b + 1];
+    if (lane < 16) s_mask[lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t p = lo0 + lane; p < lo1; p += 64) {
+      const uint32_t e = rej[p] - base;  // < 1024
+      atomicOr(&s_mask[e >> 6], 1ull << (e & 63));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t rb = lo0;  // rejected ranks below the current row
 #pragma unroll
-    for (int j = 0; j < kDrawsPerThread; j++)
-      if (v[j] < kstart) out[pos++] = ((uint64_t)(off0 + j) << 32) | v[j];
-    if (threadIdx.x == 0) B.tcnt[(size_t)chain * kMaxTiles + tile] = tot;
-    __syncthreads();
+    for (int j = 0; j < 16; j++) {
+      const uint64_t M = s_mask[j];
+      const uint32_t f = base + 64 * j + lane;
+      if (f < C && !((M >> lane) & 1)) {
+        const uint32_t accr = f - rb - (uint32_t)__builtin_popcountll(M & below);
+        if (accr < kstart) r[i0 + accr] = v[j];  // always, for an exact resolution
+      }
+      rb += (uint32_t)__builtin_popcountll(M);
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
-// ---- (b) ranks, sure accepts, ambiguous lists -----------------------------------
-__global__ void __launch_bounds__(kGpThreads) k_gp_classify(ChainBufs B, uint32_t n) {
-  __shared__ uint32_t s_w[kGpThreads / 64];
+// ---- one window step: commit the last resolved windows, then draw and classify -----
+// Tile layout: wave w of a tile covers draws [w * 2048, (w + 1) * 2048); lane l
+// computes AES blocks j * 64 + l (j < 8) of that range, i.e. draws
+// j * 256 + 4 l + q, so rank order inside a wave is (j, l, q) and candidates
+// are compacted with ballots into consecutive addresses.
+// Candidate a (rank in the window) is accepted for sure if v < k - a (at most a
+// acceptances precede it); otherwise it is ambiguous and goes to the resolver.
+template <int COPIES>
+__global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__restrict__ tab,
+                                                          const uint32_t *__restrict__ rk_groups,
+                                                          ChainBufs B, uint32_t n, uint32_t mask,
+                                                          uint32_t count, uint32_t draws_on) {
+  __shared__ __align__(16) uint32_t lds[AesLds<COPIES>::kWords];
+  __shared__ uint32_t s_wc[kGpThreads / 64], s_wa[kGpThreads / 64];
+  __shared__ uint32_t s_task, s_off, s_aoff;
+  __shared__ unsigned long long s_mask[kGpThreads / 64][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t below = lanes_below();
+
+  for (int c = 0; c < 3; c++)
+    if (B.st[c].c_valid) commit_window(B, c, n, count, s_mask[wv]);
+
   uint32_t nt[3];
-  for (int c = 0; c < 3; c++) nt[c] = ntiles_of(B.st[c]);
+  for (int c = 0; c < 3; c++) nt[c] = draws_on ? ntiles_of(B.st[c]) : 0u;
   const uint32_t total = nt[0] + nt[1] + nt[2];
-  for (uint32_t task = blockIdx.x; task < total; task += gridDim.x) {
+  bool tables = false;
+  for (;;) {
+    if (total == 0) break;
+    __syncthreads();
+    if (threadIdx.x == 0) s_task = atomicAdd(&B.ctr[0], 1u);
+    __syncthreads();
+    const uint32_t task = s_task;
+    if (task >= total) break;
+    if (!tables) {
+      aes_load_tables<COPIES>(lds, tab);
+      tables = true;
+    }
+    AesCtx<COPIES> A(lds);
     int chain;
     uint32_t tile;
     task_tile(nt, task, chain, tile);
-    const uint32_t kstart = n - B.st[chain].i;
-    const uint32_t *tcnt = B.tcnt + (size_t)chain * kMaxTiles;
-    // rank of the tile's first candidate: sum over the earlier tiles
-    uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < tile; t += blockDim.x) part += tcnt[t];
-    uint32_t off;
-    block_scan(part, s_w, off);
-    const uint32_t cnt = tcnt[tile];
-    const uint64_t *in = B.tc + (size_t)chain * kWinMax + (size_t)tile * kTile;
-    uint64_t e[kDrawsPerThread];
-    uint32_t na = 0;
+    ChainState &st = B.st[chain];
+    const uint32_t kstart = n - st.i, W = st.W, seq = st.seq;
+    const uint32_t wbase = tile * kTile + (uint32_t)wv * 2048;
+    const uint32_t *rk = rk_groups + 60 * chain;
+    uint32_t v[kDrawsPerThread];
 #pragma unroll
-    for (int q = 0; q < kDrawsPerThread; q++) {
-      const uint32_t j = threadIdx.x * kDrawsPerThread + q;
-      e[q] = j < cnt ? in[j] : 0;
-      const uint32_t v = (uint32_t)e[q];
-      if (j < cnt && !(off + j < kstart - v)) na++;
-    }
-    uint32_t atot;
-    uint32_t pos = block_scan(na, s_w, atot);
-    uint64_t *out = B.ta + (size_t)chain * kWinMax + (size_t)tile * kTile;
+    for (int j = 0; j < 8; j++) {
+      const uint32_t d = wbase + j * 256 + lane * 4;
+      if (d < W) {
+        const uint64_t dabs = st.d0 + d;
+        const uint4 blk = aes_ctr_block<COPIES>(A, rk, st.nonce + dabs / kWordsPerNonce,
+                                                2 + (uint32_t)((dabs % kWordsPerNonce) / 4));
+        v[4 * j + 0] = blk.x & mask;
+        v[4 * j + 1] = blk.y & mask;
+        v[4 * j + 2] = blk.z & mask;
+        v[4 * j + 3] = blk.w & mask;
+      } else {
 #pragma unroll
-    for (int q = 0; q < kDrawsPerThread; q++) {
-      const uint32_t j = threadIdx.x * kDrawsPerThread + q;
-      const uint32_t v = (uint32_t)e[q];
-      if (j < cnt && !(off + j < kstart - v)) out[pos++] = ((uint64_t)(off + j) << 32) | v;
+        for (int q = 0; q < 4; q++) v[4 * j + q] = 0xffffffffu;  // never a candidate
+      }
     }
-    if (threadIdx.x == 0) {
-      B.toff[(size_t)chain * kMaxTiles + tile] = off;
-      B.tacnt[(size_t)chain * kMaxTiles + tile] = atot;
+    // candidates (v < k): counts per wave, tile offset by look-back
+    uint32_t wc = 0;
+#pragma unroll
+    for (int e = 0; e < kDrawsPerThread; e++) wc += __builtin_popcountll(__ballot(v[e] < kstart));
+    if (lane == 0) s_wc[wv] = wc;
+    __syncthreads();
+    if (wv == 0) {
+      uint32_t cnt = wave_sum(lane < kGpThreads / 64 ? s_wc[lane] : 0u);
+      uint32_t off = look_back(B.stat_c + (size_t)chain * kMaxTiles, tile, seq, cnt, &st.err);
+      if (lane == 0) {
+        s_off = off;
+        if (tile == nt[chain] - 1) st.C_tot = off + cnt;
+      }
     }
     __syncthreads();
+    uint32_t run = s_off;
+    for (int w = 0; w < wv; w++) run += s_wc[w];
+    // ranks, candidate records, ambiguous counts
+    uint64_t *cand = B.cand + ((size_t)st.buf * 3 + chain) * B.cand_cap;
+    const uint32_t cap = B.cand_cap;
+    uint32_t wa = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      uint64_t b[4];
+      uint32_t pre = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        b[q] = __ballot(v[4 * j + q] < kstart);
+        pre += __builtin_popcountll(b[q] & below);
+      }
+      uint32_t a = run + pre;
+      const uint32_t d = wbase + j * 256 + lane * 4;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t x = v[4 * j + q];
+        const bool is_c = x < kstart;
+        if (is_c && a < cap) cand[a] = ((uint64_t)(d + q) << 32) | x;
+        wa += __builtin_popcountll(__ballot(is_c && a + x >= kstart));
+        a += is_c;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) run += __builtin_popcountll(b[q]);
+    }
+    if (lane == 0) s_wa[wv] = wa;
+    __syncthreads();
+    if (wv == 0) {
+      uint32_t cnt = wave_sum(lane < kGpThreads / 64 ? s_wa[lane] : 0u);
+      uint32_t off = look_back(B.stat_a + (size_t)chain * kMaxTiles, tile, seq, cnt, &st.err);
+      if (lane == 0) {
+        s_aoff = off;
+        if (tile == nt[chain] - 1) st.m_tot = off + cnt;
+      }
+    }
+    __syncthreads();
+    // ambiguous records, in rank order
+    uint32_t arun = s_aoff;
+    for (int w = 0; w < wv; w++) arun += s_wa[w];
+    run = s_off;
+    for (int w = 0; w < wv; w++) run += s_wc[w];
+    uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
+    int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      uint64_t b[4];
+      uint32_t pre = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        b[q] = __ballot(v[4 * j + q] < kstart);
+        pre += __builtin_popcountll(b[q] & below);
+      }
+      uint32_t a = run + pre;
+      bool amb[4];
+      uint32_t apre = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t x = v[4 * j + q];
+        const bool is_c = x < kstart;
+        amb[q] = is_c && a + x >= kstart;
+        const uint64_t ab = __ballot(amb[q]);
+        apre += __builtin_popcountll(ab & below);
+        a += is_c;
+      }
+      // second walk: write this lane's ambiguous records after the lower lanes'
+      a = run + pre;
+      uint32_t r = arun + apre;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t x = v[4 * j + q];
+        if (amb[q]) {
+          if (r < kAmbCap) {
+            amb_a[r] = a;
+            amb_u[r] = (int32_t)(a + x - kstart);
+          }
+          r++;
+        }
+        a += x < kstart;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        run += __builtin_popcountll(b[q]);
+        arun += __builtin_popcountll(__ballot(amb[q]));
+      }
+    }
   }
-}
-
-// ---- (b') ambiguous candidates into one rank-ordered array ------------------------
-__global__ void __launch_bounds__(kGpThreads) k_gp_gather(ChainBufs B, uint32_t n) {
-  __shared__ uint32_t s_w[kGpThreads / 64];
-  uint32_t nt[3];
-  for (int c = 0; c < 3; c++) nt[c] = ntiles_of(B.st[c]);
-  const uint32_t total = nt[0] + nt[1] + nt[2];
-  for (uint32_t task = blockIdx.x; task < total; task += gridDim.x) {
-    int chain;
-    uint32_t tile;
-    task_tile(nt, task, chain, tile);
-    const uint32_t kstart = n - B.st[chain].i;
-    const uint32_t *tacnt = B.tacnt + (size_t)chain * kMaxTiles;
-    uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < tile; t += blockDim.x) part += tacnt[t];
-    uint32_t off;
-    block_scan(part, s_w, off);
-    const uint32_t cnt = tacnt[tile];
-    if (tile == nt[chain] - 1 && threadIdx.x == 0) B.m_tot[chain] = off + cnt;
-    if (off >= kAmbCap) continue;
-    const uint64_t *in = B.ta + (size_t)chain * kWinMax + (size_t)tile * kTile;
-    uint32_t *oa = B.amb_a + (size_t)chain * kAmbCap;
-    int32_t *ou = B.amb_u + (size_t)chain * kAmbCap;
-    for (uint32_t j = threadIdx.x; j < cnt && off + j < kAmbCap; j += blockDim.x) {
-      const uint64_t e = in[j];
-      const uint32_t a = (uint32_t)(e >> 32);
-      oa[off + j] = a;
-      ou[off + j] = (int32_t)(a - (kstart - (uint32_t)e));
+  // the last workgroup out resets the ticket and retires the committed windows
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t fin = atomicAdd(&B.ctr[1], 1u);
+    if (fin == gridDim.x - 1) {
+      B.ctr[0] = 0;
+      B.ctr[1] = 0;
+      for (int c = 0; c < 3; c++) B.st[c].c_valid = 0;
+      __threadfence();
     }
   }
 }
 
-// ---- (c) exact resolution of the ambiguous candidates, chain advance -------------
-// Ambiguous candidate j (list index t_j, rank a_j, T_j = k - v_j <= a_j) is
-// accepted iff fewer than T_j acceptances precede it, i.e. iff more than
-// U_j = a_j - T_j of the ambiguous candidates before it are rejected.
+// ---- exact resolution of the ambiguous candidates, chain advance -------------------
+// Ambiguous candidate t (list index t, rank a_t, T_t = k - v_t <= a_t) is
+// accepted iff fewer than T_t acceptances precede it, i.e. iff more than
+// U_t = a_t - T_t of the ambiguous candidates before it are rejected.
 //   type B list (accept iff rejected-before > U): index t <= U -> rejected;
 //     U < 0 -> accepted; otherwise accepted iff
 //     accepted-before-among-undecided < t - (sure accepts before) - U;
 //   type A list (accept iff accepted-before < T): t < T -> accepted;
 //     T <= 0 -> rejected; otherwise accepted iff
 //     rejected-before-among-undecided > t - (sure rejects before) - T.
-// The first pass (type B over all ambiguous candidates) runs over global
-// memory; the undecided rest, in order, is resolved in LDS.  Every pass
-// decides at least its first item, so the loop ends.
+// The first pass (type B over all ambiguous candidates) streams global memory
+// row by row (item t = row * 1024 + thread, compacted with ballots); the
+// undecided rest, in order, is resolved in LDS.  Every pass decides at least
+// its first item, so the loop ends.
 __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t n,
                                                            uint32_t count, uint32_t maskN) {
-  constexpr int PMAX = kAmbCap / kGpThreads;  // 64 items per thread at most
-  __shared__ uint16_t s_t[kL2Cap];   // level-1 index of each undecided item
-  __shared__ uint16_t s_idx[kL2Cap]; // current list -> undecided item
+  constexpr uint32_t kRows = kAmbCap / kGpThreads;  // 64
+  constexpr int PER = kL2Cap / kGpThreads;          // 16
+  __shared__ uint16_t s_idx[kL2Cap];  // current list -> undecided item
   __shared__ int32_t s_thr[kL2Cap];
-  __shared__ uint8_t s_dec[kL2Cap];  // 1 accepted, 2 rejected
+  __shared__ uint8_t s_dec[kL2Cap];   // 1 accepted, 2 rejected
+  __shared__ uint32_t s_cnt[kRows * (kGpThreads / 64)];
   __shared__ uint32_t s_w[kGpThreads / 64];
+  __shared__ uint64_t s_w64[kGpThreads / 64];
   __shared__ uint32_t s_l2;
   const int chain = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t below = lanes_below();
   ChainState *st = B.st + chain;
-  if (st->done) {
-    if (threadIdx.x == 0) st->c_valid = 0;
-    return;
-  }
+  if (st->done) return;
   const uint32_t kstart = n - st->i;
-  const uint32_t ntiles = (st->W + kTile - 1) / kTile;
-  const uint32_t *tcnt = B.tcnt + (size_t)chain * kMaxTiles;
   const uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
   const int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
   uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
-  const uint32_t m = B.m_tot[chain];
-  uint32_t C;
-  {
-    uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) part += tcnt[t];
-    block_scan(part, s_w, C);
+  const uint32_t m = st->m_tot, C = st->C_tot;
+  const bool over = m > kAmbCap || C > B.cand_cap;
+  const uint32_t rows = over ? 0 : (m + kGpThreads - 1) / kGpThreads;
+  // level 1: bit q of undm = item q * 1024 + threadIdx.x undecided, of rejm rejected
+  uint64_t undm = 0, rejm = 0;
+  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
+    int32_t u[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t t = (q0 + j) * kGpThreads + threadIdx.x;
+      u[j] = t < m ? amb_u[t] : 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t q = q0 + j, t = q * kGpThreads + threadIdx.x;
+      if (q >= rows) break;
+      const bool in = t < m, rj = in && (int32_t)t <= u[j], ud = in && !rj;
+      rejm |= (uint64_t)rj << q;
+      undm |= (uint64_t)ud << q;
+      const uint64_t b = __ballot(ud);
+      if (lane == 0) s_cnt[q * (kGpThreads / 64) + wv] = __builtin_popcountll(b);
+    }
   }
-  const bool over = m > kAmbCap;
-  // level 1 (type B over all m; U >= 0, so no sure accepts): each thread owns
-  // the contiguous run [t0, t0 + P); bit q of `rejm` = item t0+q rejected
-  const uint32_t P = over ? 0 : (m + kGpThreads - 1) / kGpThreads;
-  const uint32_t t0 = threadIdx.x * P;
-  uint64_t rejm = 0, undm = 0;
-  for (uint32_t q = 0; q < P && t0 + q < m; q++) {
-    const int32_t U = amb_u[t0 + q];
-    if ((int32_t)(t0 + q) <= U) rejm |= 1ull << q;
-    else undm |= 1ull << q;
-  }
-  uint32_t l2;
-  uint32_t upos = block_scan((uint32_t)__builtin_popcountll(undm), s_w, l2);
-  const uint32_t ubase = upos;
-  if (threadIdx.x == 0) s_l2 = l2;
+  // undecided positions: exclusive scan over (row, wave) counts
   __syncthreads();
+  const uint32_t nent = rows * (kGpThreads / 64);
+  {
+    const uint32_t x = threadIdx.x < nent ? s_cnt[threadIdx.x] : 0u;
+    uint32_t l2;
+    const uint32_t ex = block_scan(x, s_w, l2);
+    __syncthreads();
+    if (threadIdx.x < nent) s_cnt[threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_l2 = l2;
+    __syncthreads();
+  }
   if (over || s_l2 > kL2Cap) {
-    // too many ambiguous candidates: retry this window at a quarter size
+    // too many candidates or ambiguous ones: retry this window at a quarter size
     if (threadIdx.x == 0) {
       uint32_t w = st->W / 4;
       st->W = w >= kTile ? w / kTile * kTile : (w >= kWinMin ? w / kWinMin * kWinMin : kWinMin);
       st->c_valid = 0;
+      st->seq++;
     }
     return;
   }
   // undecided items continue as a type A list with T' = t - U
-  for (uint64_t mm = undm; mm; mm &= mm - 1) {
-    const uint32_t q = (uint32_t)__builtin_ctzll(mm), t = t0 + q;
-    s_t[upos] = (uint16_t)t;
-    s_idx[upos] = (uint16_t)upos;
-    s_thr[upos] = (int32_t)t - amb_u[t];
-    s_dec[upos] = 0;
-    upos++;
+  for (uint32_t q = 0; q < rows; q++) {
+    const bool ud = (undm >> q) & 1;
+    const uint64_t b = __ballot(ud);
+    if (!ud) continue;
+    const uint32_t t = q * kGpThreads + threadIdx.x;
+    const uint32_t p = s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below);
+    s_idx[p] = (uint16_t)p;
+    s_thr[p] = (int32_t)t - amb_u[t];
+    s_dec[p] = 0;
   }
   __syncthreads();
-  constexpr int PER = kL2Cap / kGpThreads;  // 16
   uint32_t len = s_l2;
   int type_b = 0;
   while (len > 0) {
@@ -363,11 +549,14 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
         n_un += dec[q] == 0;
       }
     }
-    uint32_t tot;
-    uint32_t sa = block_scan(n_sa, s_w, tot);
-    uint32_t sr = block_scan(n_sr, s_w, tot);
-    uint32_t un = block_scan(n_un, s_w, tot);
-    const uint32_t new_len = tot;
+    // one scan of the three counts packed 21 bits apart
+    uint64_t tot;
+    const uint64_t ex = block_scan<uint64_t>((uint64_t)n_sa | ((uint64_t)n_sr << 21) |
+                                                 ((uint64_t)n_un << 42),
+                                             s_w64, tot);
+    uint32_t sa = (uint32_t)(ex & 0x1fffff), sr = (uint32_t)((ex >> 21) & 0x1fffff),
+             un = (uint32_t)(ex >> 42);
+    const uint32_t new_len = (uint32_t)(tot >> 42);
     int32_t nthr[PER];
     uint16_t nidx[PER];
     uint32_t npos[PER];
@@ -399,44 +588,98 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
     len = new_len;
     type_b = !type_b;
   }
-  // this thread's undecided items sit at l2 positions ubase, ubase+1, ...
-  {
-    uint32_t u = ubase;
-    for (uint64_t mm = undm; mm; mm &= mm - 1) {
-      if (s_dec[u] == 2) rejm |= 1ull << __builtin_ctzll(mm);
-      u++;
+  // rejected items: level-1 rejects and the undecided ones the passes rejected
+  for (uint32_t q = 0; q < rows; q++) {
+    const bool ud = (undm >> q) & 1;
+    const uint64_t b = __ballot(ud);
+    if (ud) {
+      const uint32_t p = s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below);
+      if (s_dec[p] == 2) rejm |= 1ull << q;
     }
   }
-  (void)PMAX;
-  // rejected ranks in ascending order (the ambiguous list is in rank order)
+  __syncthreads();
+  for (uint32_t q = 0; q < rows; q++) {
+    const uint64_t b = __ballot((rejm >> q) & 1);
+    if (lane == 0) s_cnt[q * (kGpThreads / 64) + wv] = __builtin_popcountll(b);
+  }
+  __syncthreads();
   uint32_t R;
-  uint32_t rpos = block_scan((uint32_t)__builtin_popcountll(rejm), s_w, R);
-  for (uint64_t mm = rejm; mm; mm &= mm - 1) rej[rpos++] = amb_a[t0 + __builtin_ctzll(mm)];
+  {
+    const uint32_t x = threadIdx.x < nent ? s_cnt[threadIdx.x] : 0u;
+    const uint32_t ex = block_scan(x, s_w, R);
+    __syncthreads();
+    if (threadIdx.x < nent) s_cnt[threadIdx.x] = ex;
+    __syncthreads();
+  }
+  // rejected ranks in ascending order (the ambiguous list is in rank order);
+  // loads go out 8 rows at a time
+  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
+    uint32_t a[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t q = q0 + j;
+      a[j] = q < rows && ((rejm >> q) & 1) ? amb_a[q * kGpThreads + threadIdx.x] : 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t q = q0 + j;
+      if (q >= rows) break;
+      const bool rj = (rejm >> q) & 1;
+      const uint64_t b = __ballot(rj);
+      if (rj) rej[s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below)] = a[j];
+    }
+  }
+  __syncthreads();
+  // rejblk[b] = rejected ranks below 1024 b, for b <= ceil(C / 1024): entry p
+  // (rank a_p) sets the blocks b with a_{p-1} < 1024 b <= a_p
+  {
+    uint32_t *rejblk = B.rejblk + (size_t)chain * (B.cand_cap / 1024 + 2);
+    const uint32_t nb = (C + 1023) / 1024;
+    for (uint32_t p0 = 0; p0 <= R; p0 += 8 * kGpThreads) {
+      uint32_t ap[8], ac[8];
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++) {
+        const uint32_t p = p0 + j * kGpThreads + threadIdx.x;
+        ap[j] = p > 0 && p <= R ? rej[p - 1] : 0;
+        ac[j] = p < R ? rej[p] : 0;
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++) {
+        const uint32_t p = p0 + j * kGpThreads + threadIdx.x;
+        if (p > R) break;
+        const uint32_t b0 = p == 0 ? 0 : ap[j] / 1024 + 1;
+        const uint32_t b1 = p == R ? nb : ac[j] / 1024;
+        for (uint32_t b = b0; b <= b1 && b <= nb; b++) rejblk[b] = p;
+      }
+    }
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
+    __threadfence();
     const uint32_t acc = C - R;
     if (acc > kstart) atomicOr(&st->err, 4u);
+    const uint32_t buf = st->buf;
     st->c_valid = 1;
     st->c_i0 = st->i;
     st->c_k = st->k;
-    st->c_ntiles = ntiles;
+    st->c_C = C;
     st->c_R = R;
+    st->c_buf = buf;
+    st->buf = buf ^ 1u;
+    st->seq++;
     if (acc >= kstart) {
       // the permutation ends here: rank a* of the kstart-th acceptance is
-      // kstart-1 shifted past every rejected rank at or before it
-      uint32_t a = kstart - 1;
-      for (uint32_t q = 0; q < R; q++) {
-        if (rej[q] <= a) a++;
-        else break;
+      // kstart-1 shifted past every rejected rank at or before it (rej is
+      // strictly increasing, so rej[q] - q is non-decreasing: the rejected
+      // ranks passed are those with rej[q] - q <= kstart - 1)
+      uint32_t qlo = 0, qhi = R;
+      while (qlo < qhi) {
+        const uint32_t mid = (qlo + qhi) >> 1;
+        if (rej[mid] - mid <= kstart - 1) qlo = mid + 1;
+        else qhi = mid;
       }
-      const uint32_t *toff = B.toff + (size_t)chain * kMaxTiles;
-      uint32_t lo = 0, hi = ntiles;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (toff[mid] <= a) lo = mid;
-        else hi = mid;
-      }
-      const uint64_t e = B.tc[(size_t)chain * kWinMax + (size_t)lo * kTile + (a - toff[lo])];
+      const uint32_t a = kstart - 1 + qlo;
+      const uint64_t e = B.cand[((size_t)buf * 3 + chain) * B.cand_cap + a];
       const uint64_t D_end = st->d0 + (e >> 32) + 1;
       const uint32_t k = st->k;
       B.D[(size_t)chain * count + k] = D_end;
@@ -452,48 +695,6 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
       st->i += acc;
       st->d0 += st->W;
       st->W = choose_window(n - st->i, maskN);
-    }
-  }
-}
-
-// ---- (d) r_i for the accepted candidates --------------------------------------------
-__global__ void __launch_bounds__(kGpThreads) k_gp_commit(ChainBufs B, uint32_t n, uint32_t count) {
-  uint32_t nt[3];
-  for (int c = 0; c < 3; c++) nt[c] = B.st[c].c_valid ? B.st[c].c_ntiles : 0;
-  const uint32_t total = nt[0] + nt[1] + nt[2];
-  for (uint32_t task = blockIdx.x; task < total; task += gridDim.x) {
-    int chain;
-    uint32_t tile;
-    task_tile(nt, task, chain, tile);
-    const ChainState &st = B.st[chain];
-    const uint32_t R = st.c_R, i0 = st.c_i0, k = st.c_k;
-    const uint32_t kstart = n - i0;
-    const uint32_t cnt = B.tcnt[(size_t)chain * kMaxTiles + tile];
-    const uint32_t off = B.toff[(size_t)chain * kMaxTiles + tile];
-    const uint64_t *in = B.tc + (size_t)chain * kWinMax + (size_t)tile * kTile;
-    const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
-    uint32_t *r = B.r_out + ((size_t)chain * count + k) * (size_t)n;
-    // each thread commits a contiguous run of candidates: one binary search
-    // into the ascending rejected ranks, then a merge walk
-    const uint32_t per = (cnt + blockDim.x - 1) / blockDim.x;
-    const uint32_t j0 = threadIdx.x * per;
-    if (j0 >= cnt) continue;
-    const uint32_t j1 = j0 + per < cnt ? j0 + per : cnt;
-    uint32_t lo = 0, hi = R;  // lower_bound of off + j0 in rej
-    while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (rej[mid] < off + j0) lo = mid + 1;
-      else hi = mid;
-    }
-    for (uint32_t j = j0; j < j1; j++) {
-      const uint32_t a = off + j;
-      if (lo < R && rej[lo] == a) {
-        lo++;
-        continue;
-      }
-      const uint32_t accr = a - lo;
-      if (accr >= kstart) continue;  // cannot happen for an exact resolution
-      r[i0 + accr] = (uint32_t)in[j];
     }
   }
 }
@@ -526,40 +727,72 @@ __global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r_all, size_t
   }
 }
 
-// largest step index < lim whose target is pos, or -1
-__device__ inline int32_t fy_last_before(const int32_t *__restrict__ hd,
-                                         const int32_t *__restrict__ nx, uint32_t pos,
-                                         uint32_t lim, uint32_t n) {
-  int32_t best = -1;
-  int32_t q = hd[pos];
-  for (uint32_t guard = 0; q >= 0 && guard < n; guard++) {
-    if ((uint32_t)q < lim && q > best) best = q;
-    q = nx[q];
+// Each target's list sorted by step index, descending, in place: the lists
+// come out of k_fy_link nearly sorted (steps are linked in roughly increasing
+// order), so a list is only checked, and the rare unsorted one gets an
+// insertion sort.  Then parent(t) = the largest step < t that targeted t:
+// the head, or the one after it if the head is step t itself (r_t = 0).
+__global__ void k_fy_sort(uint32_t n, int32_t *__restrict__ head, int32_t *__restrict__ next,
+                          int32_t *__restrict__ parent, uint32_t *__restrict__ err) {
+  int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
+  int32_t *par = parent + (size_t)blockIdx.y * n;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    int32_t h = hd[t];
+    bool sorted = true;
+    uint32_t len = 0;
+    for (int32_t q = h; q >= 0; len++) {
+      const int32_t nq = nx[q];
+      if (nq > q || len >= n) {  // (len >= n: a cycle, never for valid links)
+        sorted = false;
+        break;
+      }
+      q = nq;
+    }
+    if (!sorted) {
+      int32_t sh = -1;  // sorted list built so far (descending)
+      uint32_t guard = 0;
+      for (int32_t q = h; q >= 0 && guard < n; guard++) {
+        const int32_t nq = nx[q];
+        if (sh < 0 || q > sh) {
+          nx[q] = sh;
+          sh = q;
+        } else {
+          int32_t p = sh;
+          for (int32_t pn = nx[p]; pn > q; pn = nx[p]) p = pn;
+          nx[q] = nx[p];
+          nx[p] = q;
+        }
+        q = nq;
+      }
+      if (guard >= n) atomicOr(err, 32u);
+      h = sh;
+      hd[t] = h;
+    }
+    par[t] = h == (int32_t)t ? nx[t] : h;
   }
-  return best;
 }
 
 // Position j is final after step j; it receives what position t_j held just
-// before step j: V(q) for the last earlier step q that targeted t_j, or t_j
-// itself.  V(m), the value at position m just before step m, is V(p) for the
-// last step p < m that targeted m, or m.
+// before step j: V(q) for q = the last earlier step that targeted t_j (the next
+// one in t_j's sorted list), or t_j itself.  V(m), the value at position m just
+// before step m, is V(parent(m)), or m when no earlier step targeted m.
 __global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r_all, size_t chain_stride,
-                          const int32_t *__restrict__ head, const int32_t *__restrict__ next,
+                          const int32_t *__restrict__ next, const int32_t *__restrict__ parent,
                           int32_t *fwd0, int32_t *fwd1, int32_t *fwd2, int32_t *inv0, int32_t *inv1,
                           int32_t *inv2) {
   const uint32_t *r = r_all + blockIdx.y * chain_stride;
-  const int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
+  const int32_t *nx = next + (size_t)blockIdx.y * n, *par = parent + (size_t)blockIdx.y * n;
   int32_t *fwd = blockIdx.y == 0 ? fwd0 : blockIdx.y == 1 ? fwd1 : fwd2;
   int32_t *inv = blockIdx.y == 0 ? inv0 : blockIdx.y == 1 ? inv1 : inv2;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    uint32_t t = j + r[j];
+    const uint32_t t = j + r[j];
     if (t >= n) continue;  // flagged by k_fy_link
-    int32_t q = fy_last_before(hd, nx, t, j, n);
+    const int32_t q = nx[j];
     uint32_t val = t;
     if (q >= 0) {
       uint32_t m = (uint32_t)q;
       for (uint32_t guard = 0; guard < n; guard++) {
-        int32_t p = fy_last_before(hd, nx, m, m, n);
+        const int32_t p = par[m];
         if (p < 0) break;
         m = (uint32_t)p;
       }
@@ -620,15 +853,19 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   TimedRegion tr(c, "gen_perm");
   ChainBufs B;
   B.st = (ChainState *)c->scratch_get("gp_state", 3 * sizeof(ChainState));
-  B.tc = (uint64_t *)c->scratch_get("gp_tc", 3ull * kWinMax * 8);
-  B.tcnt = (uint32_t *)c->scratch_get("gp_tcnt", 3ull * kMaxTiles * 4);
-  B.toff = (uint32_t *)c->scratch_get("gp_toff", 3ull * kMaxTiles * 4);
-  B.ta = (uint64_t *)c->scratch_get("gp_ta", 3ull * kWinMax * 8);
-  B.tacnt = (uint32_t *)c->scratch_get("gp_tacnt", 3ull * kMaxTiles * 4);
+  // candidates per window: about 256 sqrt(k) <= 256 sqrt(n); 4x headroom (a
+  // window over capacity is retried smaller)
+  const uint32_t cand_cap = (uint32_t)std::min<double>(
+      kWinMax, std::max<double>(1 << 16, 1024.0 * std::ceil(std::sqrt((double)n))));
+  B.cand_cap = cand_cap;
+  B.cand = (uint64_t *)c->scratch_get("gp_cand", 6ull * cand_cap * 8);
+  B.stat_c = (uint64_t *)c->scratch_get("gp_stat_c", 3ull * kMaxTiles * 8);
+  B.stat_a = (uint64_t *)c->scratch_get("gp_stat_a", 3ull * kMaxTiles * 8);
+  B.ctr = (uint32_t *)c->scratch_get("gp_ctr", 64);
   B.amb_a = (uint32_t *)c->scratch_get("gp_amb_a", 3ull * kAmbCap * 4);
   B.amb_u = (int32_t *)c->scratch_get("gp_amb_u", 3ull * kAmbCap * 4);
-  B.m_tot = (uint32_t *)c->scratch_get("gp_m_tot", 64);
   B.rej = (uint32_t *)c->scratch_get("gp_rej", 3ull * kAmbCap * 4);
+  B.rejblk = (uint32_t *)c->scratch_get("gp_rejblk", 3ull * (cand_cap / 1024 + 2) * 4);
   B.r_out = (uint32_t *)c->scratch_get("gp_r", 3ull * count * n * 4);
   B.D = (uint64_t *)c->scratch_get("gp_D", 3ull * count * 8);
   B.nonce_at = (uint64_t *)c->scratch_get("gp_nonce_at", 3ull * (count + 1) * 8);
@@ -640,8 +877,12 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   for (int g = 0; g < 3; g++) {
     init[g].nonce = c->nonce[kGroup0 + g];
     init[g].W = choose_window(n, maskN);
+    init[g].seq = 1;  // the status words are cleared to seq 0 below
     nat[(size_t)g * (count + 1)] = init[g].nonce;
   }
+  ORQG_HIP(hipMemsetAsync(B.stat_c, 0, 3ull * kMaxTiles * 8, c->stream));
+  ORQG_HIP(hipMemsetAsync(B.stat_a, 0, 3ull * kMaxTiles * 8, c->stream));
+  ORQG_HIP(hipMemsetAsync(B.ctr, 0, 64, c->stream));
   ORQG_HIP(hipMemcpyAsync(B.st, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   ORQG_HIP(hipMemcpyAsync(B.nonce_at, nat.data(), nat.size() * 8, hipMemcpyHostToDevice, c->stream));
 
@@ -650,15 +891,17 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   const int kBatch = 16;
   // The window kernels take all their state from device memory, so a batch of
   // windows is one static launch sequence: it is captured once as a graph.
+  // Each step commits the previously resolved windows, draws and classifies
+  // the current ones; a last commit-only step leaves every resolved window of
+  // the batch committed, so the completed-permutation count read after a
+  // batch covers committed offsets only.
   auto enqueue_batch = [&]() {
-    for (int w = 0; w < kBatch; w++) {
-      hipLaunchKernelGGL(k_gp_draws<kCopies>, dim3(grid), dim3(kGpThreads), 0, c->stream, c->d_t0,
-                         c->rk(kGroup0), B, n, mask);
-      hipLaunchKernelGGL(k_gp_classify, dim3(grid), dim3(kGpThreads), 0, c->stream, B, n);
-      hipLaunchKernelGGL(k_gp_gather, dim3(grid), dim3(kGpThreads), 0, c->stream, B, n);
-      hipLaunchKernelGGL(k_gp_resolve, dim3(3), dim3(kGpThreads), 0, c->stream, B, n, count,
-                         maskN);
-      hipLaunchKernelGGL(k_gp_commit, dim3(grid), dim3(kGpThreads), 0, c->stream, B, n, count);
+    for (int w = 0; w <= kBatch; w++) {
+      hipLaunchKernelGGL(k_gp_window<kCopies>, dim3(grid), dim3(kGpThreads), 0, c->stream,
+                         c->d_t0, c->rk(kGroup0), B, n, mask, count, (uint32_t)(w < kBatch));
+      if (w < kBatch)
+        hipLaunchKernelGGL(k_gp_resolve, dim3(3), dim3(kGpThreads), 0, c->stream, B, n, count,
+                           maskN);
     }
   };
   hipGraph_t graph = nullptr;
@@ -680,6 +923,7 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   // the windows are AES (VALU/LDS) bound.
   int32_t *head = (int32_t *)c->scratch_get("fy_head", (size_t)n * 4 + 64);
   int32_t *next = (int32_t *)c->scratch_get("fy_next", (size_t)n * 4 + 64);
+  int32_t *parent = (int32_t *)c->scratch_get("fy_parent", (size_t)n * 4 + 64);
   uint32_t *err = (uint32_t *)c->scratch_get("fy_err", 16);
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
   ORQG_HIP(hipEventRecord(c->ev_aux, c->stream));
@@ -693,8 +937,10 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
       ORQG_HIP(hipMemsetAsync(head, 0xff, (size_t)n * 4, c->stream2));
       hipLaunchKernelGGL(k_fy_link, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
                          head, next, err);
+      hipLaunchKernelGGL(k_fy_sort, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, head, next,
+                         parent, err);
       hipLaunchKernelGGL(k_fy_walk, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
-                         head, next, fresh[k].fwd[g], fresh[k].fwd[g], fresh[k].fwd[g],
+                         next, parent, fresh[k].fwd[g], fresh[k].fwd[g], fresh[k].fwd[g],
                          fresh[k].inv[g], fresh[k].inv[g], fresh[k].inv[g]);
     }
     if (upto > launched[g]) launched[g] = upto;
