@@ -214,49 +214,46 @@ __device__ inline uint32_t look_back(uint64_t *stat, uint32_t tile, uint32_t seq
 // wave takes 1024 consecutive ranks: the rejected ones among them (a slice of
 // the ascending rejected list, located by rejblk) become a 1024-bit mask in
 // LDS, and lane l handles ranks base + 64 j + l, so loads and stores coalesce.
-__device__ inline void commit_window(const ChainBufs &B, int chain, uint32_t n, uint32_t count,
-                                     unsigned long long *s_mask) {
+__device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, uint32_t n,
+                                    uint32_t count, unsigned long long *s_mask) {
   const int lane = threadIdx.x & 63;
   const uint64_t below = lanes_below();
   const ChainState &st = B.st[chain];
   const uint32_t C = st.c_C, i0 = st.c_i0, kstart = n - i0;
+  if (b * 1024 >= C) return;
   const uint64_t *cand = B.cand + ((size_t)st.c_buf * 3 + chain) * B.cand_cap;
   const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
   const uint32_t *rejblk = B.rejblk + (size_t)chain * (B.cand_cap / 1024 + 2);
   uint32_t *r = B.r_out + ((size_t)chain * count + st.c_k) * (size_t)n;
-  const uint32_t nb = (C + 1023) / 1024;
-  const uint32_t gwaves = gridDim.x * (blockDim.x / 64);
-  for (uint32_t b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); b < nb; b += gwaves) {
-    const uint32_t base = b * 1024;
-    uint32_t v[16];
+  const uint32_t base = b * 1024;
+  uint32_t v[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const uint32_t f = base + 64 * j + lane;
-      v[j] = f < C ? (uint32_t)cand[f] : 0;
-    }
-    const uint32_t lo0 = rejblk[b], lo1 = rejblk[b + 1];
-    if (lane < 16) s_mask[lane] = 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t p = lo0 + lane; p < lo1; p += 64) {
-      const uint32_t e = rej[p] - base;  // < 1024
-      atomicOr(&s_mask[e >> 6], 1ull << (e & 63));
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    uint32_t rb = lo0;  // rejected ranks below the current row
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const uint64_t M = s_mask[j];
-      const uint32_t f = base + 64 * j + lane;
-      if (f < C && !((M >> lane) & 1)) {
-        const uint32_t accr = f - rb - (uint32_t)__builtin_popcountll(M & below);
-        if (accr < kstart) r[i0 + accr] = v[j];  // always, for an exact resolution
-      }
-      rb += (uint32_t)__builtin_popcountll(M);
-    }
-    __builtin_amdgcn_wave_barrier();
+  for (int j = 0; j < 16; j++) {
+    const uint32_t f = base + 64 * j + lane;
+    v[j] = f < C ? (uint32_t)cand[f] : 0;
   }
+  const uint32_t lo0 = rejblk[b], lo1 = rejblk[b + 1];
+  if (lane < 16) s_mask[lane] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t p = lo0 + lane; p < lo1; p += 64) {
+    const uint32_t e = rej[p] - base;  // < 1024
+    atomicOr(&s_mask[e >> 6], 1ull << (e & 63));
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  uint32_t rb = lo0;  // rejected ranks below the current row
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const uint64_t M = s_mask[j];
+    const uint32_t f = base + 64 * j + lane;
+    if (f < C && !((M >> lane) & 1)) {
+      const uint32_t accr = f - rb - (uint32_t)__builtin_popcountll(M & below);
+      if (accr < kstart) r[i0 + accr] = v[j];  // always, for an exact resolution
+    }
+    rb += (uint32_t)__builtin_popcountll(M);
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
 // ---- one window step: commit the last resolved windows, then draw and classify -----
@@ -278,12 +275,15 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t below = lanes_below();
 
-  for (int c = 0; c < 3; c++)
-    if (B.st[c].c_valid) commit_window(B, c, n, count, s_mask[wv]);
-
-  uint32_t nt[3];
-  for (int c = 0; c < 3; c++) nt[c] = draws_on ? ntiles_of(B.st[c]) : 0u;
-  const uint32_t total = nt[0] + nt[1] + nt[2];
+  // tasks, in ticket order: commit chunks (16 blocks of 1024 ranks, one per
+  // wave) of the windows resolved last, then the tiles of the current windows
+  uint32_t nc[3], nt[3];
+  for (int c = 0; c < 3; c++) {
+    nc[c] = B.st[c].c_valid ? (B.st[c].c_C + 16383) / 16384 : 0u;
+    nt[c] = draws_on ? ntiles_of(B.st[c]) : 0u;
+  }
+  const uint32_t ncom = nc[0] + nc[1] + nc[2];
+  const uint32_t total = ncom + nt[0] + nt[1] + nt[2];
   bool tables = false;
   for (;;) {
     if (total == 0) break;
@@ -292,6 +292,13 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     __syncthreads();
     const uint32_t task = s_task;
     if (task >= total) break;
+    if (task < ncom) {
+      int chain;
+      uint32_t chunk;
+      task_tile(nc, task, chain, chunk);
+      commit_block(B, chain, chunk * (kGpThreads / 64) + wv, n, count, s_mask[wv]);
+      continue;
+    }
     if (!tables) {
       aes_load_tables<COPIES>(lds, tab);
       tables = true;
@@ -299,7 +306,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     AesCtx<COPIES> A(lds);
     int chain;
     uint32_t tile;
-    task_tile(nt, task, chain, tile);
+    task_tile(nt, task - ncom, chain, tile);
     ChainState &st = B.st[chain];
     const uint32_t kstart = n - st.i, W = st.W, seq = st.seq;
     const uint32_t wbase = tile * kTile + (uint32_t)wv * 2048;
@@ -351,6 +358,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
         b[q] = __ballot(v[4 * j + q] < kstart);
         pre += __builtin_popcountll(b[q] & below);
       }
+      if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
       uint32_t a = run + pre;
       const uint32_t d = wbase + j * 256 + lane * 4;
 #pragma unroll
@@ -383,7 +391,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
     int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < 8 && wa; j++) {
       uint64_t b[4];
       uint32_t pre = 0;
 #pragma unroll
@@ -391,6 +399,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
         b[q] = __ballot(v[4 * j + q] < kstart);
         pre += __builtin_popcountll(b[q] & below);
       }
+      if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
       uint32_t a = run + pre;
       bool amb[4];
       uint32_t apre = 0;
