@@ -67,8 +67,9 @@ ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
 # HBM bytes one k_round2 launch at 2^24 rows actually moves, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the factors a plain
 # 384 MiB device copy gives (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE):
-# (2478423.4 * 2.0 + 1961569.8 * 1.0) KiB; profiles/radix_sort_2p24_int64_kernels.txt.
-ROUND_TRAFFIC_2P24 = int((2478423.4 * 2.0 + 1961569.8 * 1.0) * 1024)
+# (1118420.2 * 2.0 + 1300981.3 * 1.0) KiB; profiles/radix_sort_2p24_int64_kernels.txt.
+# The gather reads one 128-byte line per 64-byte element record.
+ROUND_TRAFFIC_2P24 = int((1118420.2 * 2.0 + 1300981.3 * 1.0) * 1024)
 
 
 def _timed(ctx, fn, reps):
@@ -375,7 +376,8 @@ def main():
     ctx.timer_enable(False)
     t_step = sum(times) / len(times)
     fams = {}
-    for f in ["gen_perm", "sort_b2a", "sort_perm", "sort_round", "k_round2", "sort_tail"]:
+    for f in ["gen_perm", "sort_b2a", "sort_perm", "sort_round", "k_round2", "k_round2_scatter",
+              "sort_tail"]:
         ms, cnt = ctx.timer(f)
         fams[f] = {"ms_per_step": round(ms / args.steps, 3), "launches": cnt}
     online_ms = sum(fams[f]["ms_per_step"] for f in ["sort_b2a", "sort_perm", "sort_round",
@@ -385,7 +387,8 @@ def main():
     got = ctx.open(work).view(np.int64)
     ok = bool(np.array_equal(got, np.sort(keys_plain.view(np.int64), kind="stable")))
 
-    # roofline of the dominant online kernel, k_round2 (64 bits x 3 launches per step)
+    # roofline of the dominant online kernel, k_round2 (64 bits x 2 plain rounds per step;
+    # the third round of each bit, which also scatters, is timed as k_round2_scatter)
     r_ms, r_cnt = ctx.timer("k_round2")
     per_launch_ms = r_ms / max(r_cnt, 1)
     achieved = n * ROUND_BYTES_PER_ROW / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0

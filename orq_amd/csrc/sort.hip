@@ -2,7 +2,9 @@
 // three parties on one GPU.
 //
 // The padded vector of pad_input (include/core/operators/sorting.h:431-462)
-// is kept as L 32-bit words per share, element-major [n][3][L]:
+// is kept as L 32-bit words per share, in one 64-byte record per element
+// (words k*L .. k*L+L-1 = share k; the bit permutation's three shares ride in
+// words 12..14 during a bit iteration):
 //   int32 keys -> PadWidth int64  -> L = 2 (index word, key word)
 //   int64 keys -> PadWidth int128 -> L = 3 (index word, 2 key words); bits
 //   96..127 of the reference's int128 shares never reach an output (the sort
@@ -15,10 +17,14 @@
 //               -> f1, plus per-block sums of f1 for the prefix scan
 //   k_scan3     exclusive scan of the block sums (3 shares)
 //   k_bit_perm  s = prefix_sum([1-f1 ; f1]) - 1 in closed form, then
-//               perm = s0 + f1*(s1-s0) with multiply_a (1 zero sharing)
-//   k_round x3  sharded permutation rounds: gather both v and perm through
-//               pi_g^{-1} and add the group-g resharing masks
-//   k_open_scatter  open pi(perm) and scatter pi(v) by it
+//               perm = s0 + f1*(s1-s0) with multiply_a (1 zero sharing),
+//               written into the records' perm words
+//   k_round x3  sharded permutation rounds: gather whole records through
+//               pi_g^{-1} (one aligned 64-byte piece of a line per element)
+//               and add the group-g resharing masks to v and perm; the third
+//               round opens pi(perm) and scatters pi(v) by it (the record
+//               buffers alternate between bits, so it never writes a buffer
+//               it gathers from)
 #include <algorithm>
 #include <numeric>
 
@@ -38,6 +44,8 @@ namespace {
 
 constexpr int kBT = kAesThreads;    // threads per block
 constexpr int kEPB = kBT * 4;       // elements per scan block
+constexpr int kRec = 16;            // words per element record: v (3L words), perm at kRecPerm
+constexpr int kRecPerm = 12;
 
 __device__ inline uint32_t bexcl(uint32_t x, uint32_t *s_w, uint32_t &total) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -69,14 +77,20 @@ __global__ void k_pad(uint64_t n, bool desc, const K *__restrict__ v, uint32_t *
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t dst = desc ? n - 1 - i : i;
     uint32_t idx = desc ? (uint32_t)(-1 - (int64_t)i) : (uint32_t)i;
+    uint32_t w[kRec];
+#pragma unroll
+    for (int q = 0; q < kRec; q++) w[q] = 0u;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      uint32_t *o = P + (dst * 3 + k) * L;
       K key = v[3 * i + k];
-      o[0] = k == 0 ? idx : 0u;
-      o[1] = (uint32_t)key;
-      if (L == 3) o[2] = (uint32_t)((uint64_t)key >> 32);
+      w[k * L] = k == 0 ? idx : 0u;
+      w[k * L + 1] = (uint32_t)key;
+      if (L == 3) w[k * L + 2] = (uint32_t)((uint64_t)key >> 32);
     }
+    uint4 *o = reinterpret_cast<uint4 *>(P + dst * kRec);
+#pragma unroll
+    for (int q = 0; q < kRec / 4; q++)
+      o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
   }
 }
 
@@ -100,7 +114,7 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++)
-      bits[i][k] = (e0 + i < s.n) ? (((P[((e0 + i) * 3 + k) * L + word] >> bit) & 1u) ^ flip) : 0u;
+      bits[i][k] = (e0 + i < s.n) ? (((P[(e0 + i) * kRec + k * L + word] >> bit) & 1u) ^ flip) : 0u;
   b2a_bit_group<uint32_t>(A, rk_pair, nb, s, e0, bits, f);
   uint32_t loc[3] = {0, 0, 0};
 #pragma unroll
@@ -148,13 +162,14 @@ __global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restric
 
 // s0_k(e) = one_k*e - F_k(e), s1_k(e) = one_k*(n-1) - Ftot_k + F_k(e) with F the
 // inclusive prefix of f1 and one = (1,0,0); perm = s0 + f1*(s1 - s0).
+template <int L>
 __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t0,
                                                   const uint32_t *__restrict__ rk_pair,
                                                   NonceTriple nb, CallSched s,
                                                   const uint32_t *__restrict__ F,
                                                   const uint32_t *__restrict__ boff,
                                                   const uint32_t *__restrict__ tot,
-                                                  uint32_t *__restrict__ perm) {
+                                                  uint32_t *__restrict__ R) {
   AES_LDS;
   __shared__ uint32_t s_w[kBT / 64];
   aes_load_tables<kCopies>(lds, t0);
@@ -201,28 +216,32 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
       y[k] = s1 - s0[k];
     }
     if (e0 + i < s.n) {
+      uint32_t pr[3];
 #pragma unroll
       for (int p = 0; p < 3; p++) {
         int q = (p + 1) % 3, pm = (p + 2) % 3;
         uint32_t z = (S[pm][i] - S[p][i]) + f[i][p] * y[p] + f[i][p] * y[q] + f[i][q] * y[p];
-        perm[(e0 + i) * 3 + p] = s0[p] + z;
+        pr[p] = s0[p] + z;
       }
+      // into the element record's perm words
+      *reinterpret_cast<uint4 *>(R + (e0 + i) * kRec + kRecPerm) =
+          make_uint4(pr[0], pr[1], pr[2], 0u);
     }
   }
   }
 }
 
-// ---- sharded round: gather v (binary, tszv-byte masks) and perm (arithmetic,
-// 4-byte masks) through idx, masks drawn from group PRG g ---------------------------
-template <int L>
+// ---- sharded round on element records: gather through idx, then add the
+// resharing masks of v (binary, tszv-byte masks) and perm (arithmetic, 4-byte
+// masks), drawn from group PRG g ----------------------------------------------------
+template <int L, bool SCATTER>
 __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
                                                 const uint32_t *__restrict__ rk, uint64_t nv,
                                                 CallSched sv, uint64_t np, CallSched sp,
                                                 const int32_t *__restrict__ idx,
-                                                const uint32_t *__restrict__ Pin,
-                                                uint32_t *__restrict__ Pout,
-                                                const uint32_t *__restrict__ qin,
-                                                uint32_t *__restrict__ qout) {
+                                                const uint32_t *__restrict__ Rin,
+                                                uint32_t *__restrict__ Rout,
+                                                uint32_t *__restrict__ err) {
   AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
@@ -232,18 +251,18 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
        g += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t e0 = g * 4;
     const int ne = (int)(n - e0 < 4 ? n - e0 : 4);
-    // issue every gather first so that the loads are in flight during the AES work
-    uint32_t pv[4][3 * L], pq[4][3];
+    // issue every gather first so that the loads are in flight during the AES
+    // work: words 0..3L-1 (v) and kRecPerm.. (perm) of each source record
+    constexpr int kVq = (3 * L + 3) / 4;  // 16-byte pieces holding v
+    uint4 pv[4][kVq], pq[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       if (i < ne) {
         const uint64_t src = (uint64_t)(uint32_t)idx[e0 + i];
-        const uint32_t *ps = Pin + src * 3 * L;
-        const uint32_t *qs = qin + src * 3;
+        const uint4 *rs = reinterpret_cast<const uint4 *>(Rin + src * kRec);
 #pragma unroll
-        for (int w = 0; w < 3 * L; w++) pv[i][w] = ps[w];
-#pragma unroll
-        for (int k = 0; k < 3; k++) pq[i][k] = qs[k];
+        for (int q = 0; q < kVq; q++) pv[i][q] = rs[q];
+        pq[i] = rs[kRecPerm / 4];
       }
     }
     // perm masks: one block per call covers the 4 elements
@@ -265,33 +284,44 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
 #pragma unroll
         for (int q = 0; q < L; q++) r[j][q] = w4[bib / 4 + q];
       }
-      uint32_t *pd = Pout + e * 3 * L;
+      uint32_t w[kRec];
+#pragma unroll
+      for (int q = 0; q < kVq; q++) {
+        w[4 * q] = pv[i][q].x;
+        w[4 * q + 1] = pv[i][q].y;
+        w[4 * q + 2] = pv[i][q].z;
+        w[4 * q + 3] = pv[i][q].w;
+      }
+#pragma unroll
+      for (int q = 4 * kVq; q < kRec; q++) w[q] = 0u;
 #pragma unroll
       for (int q = 0; q < L; q++) {
-        pd[0 * L + q] = pv[i][0 * L + q] ^ r[0][q];
-        pd[1 * L + q] = pv[i][1 * L + q] ^ r[1][q];
-        pd[2 * L + q] = pv[i][2 * L + q] ^ r[0][q] ^ r[1][q];
+        w[0 * L + q] ^= r[0][q];
+        w[1 * L + q] ^= r[1][q];
+        w[2 * L + q] ^= r[0][q] ^ r[1][q];
       }
-      uint32_t *qd = qout + e * 3;
+      w[kRecPerm + 0] = pq[i].x + pm[0][i];
+      w[kRecPerm + 1] = pq[i].y + pm[1][i];
+      w[kRecPerm + 2] = pq[i].z + pm[2][i];
+      if (SCATTER) {
+        // open the permuted perm and put the permuted v at its position (the
+        // perm words there are rewritten by the next bit)
+        const uint32_t sigma = w[kRecPerm] + w[kRecPerm + 1] + w[kRecPerm + 2];
+        if (sigma >= n) {
+          atomicOr(err, 1u);
+          continue;
+        }
+        uint4 *o = reinterpret_cast<uint4 *>(Rout + (uint64_t)sigma * kRec);
 #pragma unroll
-      for (int k = 0; k < 3; k++) qd[k] = pq[i][k] + pm[k][i];
-    }
-  }
-}
-
-template <int L>
-__global__ void k_open_scatter_pad(uint64_t n, const uint32_t *__restrict__ q,
-                                   const uint32_t *__restrict__ Pin, uint32_t *__restrict__ Pout,
-                                   uint32_t *__restrict__ err) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t sigma = q[3 * i] + q[3 * i + 1] + q[3 * i + 2];
-    if (sigma >= n) {
-      atomicOr(err, 1u);
-      continue;
-    }
+        for (int q = 0; q < kVq; q++)
+          o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+      } else {
+        uint4 *o = reinterpret_cast<uint4 *>(Rout + e * kRec);
 #pragma unroll
-    for (int w = 0; w < 3 * L; w++) Pout[(uint64_t)sigma * 3 * L + w] = Pin[i * 3 * L + w];
+        for (int q = 0; q < kRec / 4; q++)
+          o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+      }
+    }
   }
 }
 
@@ -305,7 +335,7 @@ __global__ void k_unpad(uint64_t n, bool desc, const uint32_t *__restrict__ P, K
     uint64_t src = desc ? n - 1 - i : i;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const uint32_t *p = P + (src * 3 + k) * L;
+      const uint32_t *p = P + src * kRec + k * L;
       pb[3 * i + k] = p[0];
       uint64_t key = p[1];
       if (L == 3) key |= (uint64_t)p[2] << 32;
@@ -316,13 +346,12 @@ __global__ void k_unpad(uint64_t n, bool desc, const uint32_t *__restrict__ P, K
 
 // open a [n][3] uint32 vector (XOR or sum) and write it as the public sharing
 // (o, 0, 0) into out (open_shares_* then public_share)
+// Returns the record buffer that holds the sorted vector (P or the second one).
 template <int L>
-void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) {
+uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) {
   const uint32_t tszv = L == 3 ? 16 : 8;  // PadWidth element size
-  uint32_t *P2 = (uint32_t *)c->scratch_get("rs_P2", n * 3 * L * 4 + 64);
   uint32_t *F = (uint32_t *)c->scratch_get("rs_F", n * 12 + 64);
-  uint32_t *q0 = (uint32_t *)c->scratch_get("rs_q0", n * 12 + 64);
-  uint32_t *q1 = (uint32_t *)c->scratch_get("rs_q1", n * 12 + 64);
+  uint32_t *R1 = (uint32_t *)c->scratch_get("rs_R1", n * kRec * 4 + 64);
   const uint32_t nblk = (uint32_t)((n + kEPB - 1) / kEPB);
   uint32_t *bsum = (uint32_t *)c->scratch_get("rs_bsum", (size_t)nblk * 12 + 64);
   uint32_t *tot = (uint32_t *)c->scratch_get("rs_tot", 64);
@@ -330,7 +359,7 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
   const unsigned grid_rounds = grid_for((n + 3) / 4, kBT, c->num_cu);
   const unsigned grid_pers = std::min<unsigned>(nblk, (unsigned)c->num_cu);
-  const unsigned grid_flat = grid_for(n, 256, c->num_cu * 16);
+  uint32_t *cur = P, *other = R1;
 
   for (int i = 0; i < bits; i++) {
     const int bitpos = 32 + i;
@@ -340,7 +369,7 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
       TimedRegion tr(c, "sort_b2a");
       CallSched s = make_sched(n, c->batch_size, 4, 3);
       hipLaunchKernelGGL(k_bit_b2a<L>, dim3(grid_pers), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
-                         nb, s, P, bitpos / 32, bitpos % 32, flip, F, bsum);
+                         nb, s, cur, bitpos / 32, bitpos % 32, flip, F, bsum);
       uint64_t used = sched_nonces(s);
       for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
       for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * n * 4);
@@ -350,8 +379,8 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
       TimedRegion tr(c, "sort_perm");
       NonceTriple nb2 = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
       CallSched s = make_sched(n, c->batch_size, 4, 1);
-      hipLaunchKernelGGL(k_bit_perm, dim3(grid_pers), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
-                         nb2, s, F, bsum, tot, q0);
+      hipLaunchKernelGGL(k_bit_perm<L>, dim3(grid_pers), dim3(kBT), 0, c->stream, c->d_t0,
+                         c->rk(kPair0), nb2, s, F, bsum, tot, cur);
       uint64_t used = sched_nonces(s);
       for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
       for (int p = 0; p < 3; p++) c->log_msg("mul_a", p, (p + 2) % 3, n * 4);
@@ -361,7 +390,8 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
     {
       TimedRegion tr(c, "sort_round");
       // key G_g is used first by v's round g, then by perm's round g
-      uint32_t *pin = P, *pout = P2, *qin = q0, *qout = q1;
+      // cur -> other -> cur -> other (scattered)
+      uint32_t *rin = cur, *rout = other;
       CallSched sv = make_sched(n, c->batch_size, tszv, 2);
       CallSched sp = make_sched(n, c->batch_size, 4, 2);
       const uint64_t uv = sched_nonces(sv), up = sched_nonces(sp);
@@ -369,21 +399,22 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
         const int role = kGroup0 + g;
         const uint64_t nv = c->nonce[role], np = nv + uv;
         {
-          TimedRegion tk(c, "k_round2");
-          hipLaunchKernelGGL(k_round2<L>, dim3(grid_rounds), dim3(kBT), 0, c->stream, c->d_t0,
-                             c->rk(role), nv, sv, np, sp, pi.inv[g], pin, pout, qin, qout);
+          TimedRegion tk(c, g < 2 ? "k_round2" : "k_round2_scatter");
+          if (g < 2)
+            hipLaunchKernelGGL((k_round2<L, false>), dim3(grid_rounds), dim3(kBT), 0, c->stream,
+                               c->d_t0, c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err);
+          else
+            hipLaunchKernelGGL((k_round2<L, true>), dim3(grid_rounds), dim3(kBT), 0, c->stream,
+                               c->d_t0, c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err);
         }
         c->nonce[role] += uv + up;
-        std::swap(pin, pout);
-        std::swap(qin, qout);
+        std::swap(rin, rout);
       }
       // log in the reference's order: three rounds on v, then three on perm
       for (int g = 0; g < 3; g++) log_reshare(c, g, n * tszv);
       for (int g = 0; g < 3; g++) log_reshare(c, g, n * 4);
       for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, n * 4);
-      // after three rounds: v in pin (= P2), perm in qin (= q1); scatter back into P
-      hipLaunchKernelGGL(k_open_scatter_pad<L>, dim3(grid_flat), dim3(256), 0, c->stream, n, qin,
-                         pin, P, err);
+      std::swap(cur, other);
     }
     free_perm(pi);
     ORQG_HIP(hipGetLastError());
@@ -392,6 +423,7 @@ void sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) 
   ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
   if (herr) throw Error(ORQG_E_PARITY, "radix_sort: opened bit permutation out of range");
+  return cur;
 }
 
 }  // namespace
@@ -405,16 +437,16 @@ void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_ou
   // (radixsort.h:121-128, permutation_manager.h:108-111)
   reserve_perms(c, n, (size_t)bits + 2);
   const int L = v->width == 8 ? 3 : 2;
-  uint32_t *P = (uint32_t *)c->scratch_get("rs_P", n * 3 * L * 4 + 64);
+  uint32_t *P = (uint32_t *)c->scratch_get("rs_P", n * kRec * 4 + 64);
   const unsigned grid_flat = grid_for(n, 256, c->num_cu * 16);
   if (L == 3) {
     hipLaunchKernelGGL((k_pad<3, uint64_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc,
                        (const uint64_t *)v->d, P);
-    sort_body<3>(c, P, n, bits, full_width);
+    P = sort_body<3>(c, P, n, bits, full_width);
   } else {
     hipLaunchKernelGGL((k_pad<2, uint32_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc,
                        (const uint32_t *)v->d, P);
-    sort_body<2>(c, P, n, bits, full_width);
+    P = sort_body<2>(c, P, n, bits, full_width);
   }
   // remove_padding
   uint32_t *pb = (uint32_t *)perm_out->d;
