@@ -528,15 +528,26 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
     return;
   }
   // undecided items continue as a type A list with T' = t - U
-  for (uint32_t q = 0; q < rows; q++) {
-    const bool ud = (undm >> q) & 1;
-    const uint64_t b = __ballot(ud);
-    if (!ud) continue;
-    const uint32_t t = q * kGpThreads + threadIdx.x;
-    const uint32_t p = s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below);
-    s_idx[p] = (uint16_t)p;
-    s_thr[p] = (int32_t)t - amb_u[t];
-    s_dec[p] = 0;
+  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
+    int32_t u[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t q = q0 + j;
+      u[j] = q < rows && ((undm >> q) & 1) ? amb_u[q * kGpThreads + threadIdx.x] : 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t q = q0 + j;
+      if (q >= rows) break;
+      const bool ud = (undm >> q) & 1;
+      const uint64_t b = __ballot(ud);
+      if (!ud) continue;
+      const uint32_t t = q * kGpThreads + threadIdx.x;
+      const uint32_t p = s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below);
+      s_idx[p] = (uint16_t)p;
+      s_thr[p] = (int32_t)t - u[j];
+      s_dec[p] = 0;
+    }
   }
   __syncthreads();
   uint32_t len = s_l2;
@@ -620,14 +631,20 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
     if (threadIdx.x < nent) s_cnt[threadIdx.x] = ex;
     __syncthreads();
   }
-  // rejected ranks in ascending order (the ambiguous list is in rank order);
-  // loads go out 8 rows at a time
+  // rejected ranks in ascending order (the ambiguous list is in rank order),
+  // and rejblk[b] = rejected ranks below 1024 b for b <= nb = ceil(C / 1024):
+  // item t (rank a_t; a sentinel rank 1024 nb past the list) sets the blocks b
+  // with a_{t-1} < 1024 b <= a_t to the number of rejected items before it.
+  // Loads go out 8 rows at a time.
+  uint32_t *rejblk = B.rejblk + (size_t)chain * (B.cand_cap / 1024 + 2);
+  const uint32_t nb = (C + 1023) / 1024;
+  __shared__ uint32_t s_lastr[kRows * (kGpThreads / 64)];  // rank of each (row, wave)'s last item
   for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
     uint32_t a[8];
 #pragma unroll
     for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t q = q0 + j;
-      a[j] = q < rows && ((rejm >> q) & 1) ? amb_a[q * kGpThreads + threadIdx.x] : 0;
+      const uint32_t t = (q0 + j) * kGpThreads + threadIdx.x;
+      a[j] = q0 + j < rows && t < m ? amb_a[t] : nb * 1024;
     }
 #pragma unroll
     for (uint32_t j = 0; j < 8; j++) {
@@ -636,31 +653,35 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
       const bool rj = (rejm >> q) & 1;
       const uint64_t b = __ballot(rj);
       if (rj) rej[s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below)] = a[j];
+      if (lane == 63) s_lastr[q * (kGpThreads / 64) + wv] = a[j];
     }
   }
   __syncthreads();
-  // rejblk[b] = rejected ranks below 1024 b, for b <= ceil(C / 1024): entry p
-  // (rank a_p) sets the blocks b with a_{p-1} < 1024 b <= a_p
-  {
-    uint32_t *rejblk = B.rejblk + (size_t)chain * (B.cand_cap / 1024 + 2);
-    const uint32_t nb = (C + 1023) / 1024;
-    for (uint32_t p0 = 0; p0 <= R; p0 += 8 * kGpThreads) {
-      uint32_t ap[8], ac[8];
+  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
+    uint32_t a[8];
 #pragma unroll
-      for (uint32_t j = 0; j < 8; j++) {
-        const uint32_t p = p0 + j * kGpThreads + threadIdx.x;
-        ap[j] = p > 0 && p <= R ? rej[p - 1] : 0;
-        ac[j] = p < R ? rej[p] : 0;
-      }
-#pragma unroll
-      for (uint32_t j = 0; j < 8; j++) {
-        const uint32_t p = p0 + j * kGpThreads + threadIdx.x;
-        if (p > R) break;
-        const uint32_t b0 = p == 0 ? 0 : ap[j] / 1024 + 1;
-        const uint32_t b1 = p == R ? nb : ac[j] / 1024;
-        for (uint32_t b = b0; b <= b1 && b <= nb; b++) rejblk[b] = p;
-      }
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t t = (q0 + j) * kGpThreads + threadIdx.x;
+      a[j] = q0 + j < rows && t < m ? amb_a[t] : nb * 1024;
     }
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) {
+      const uint32_t q = q0 + j;
+      if (q >= rows) break;
+      const uint32_t t = q * kGpThreads + threadIdx.x;
+      const uint32_t g = q * (kGpThreads / 64) + wv;
+      const uint64_t b = __ballot((rejm >> q) & 1);
+      const uint32_t before = s_cnt[g] + __builtin_popcountll(b & below);
+      uint32_t ap = __shfl_up(a[j], 1, 64);
+      if (lane == 0) ap = g > 0 ? s_lastr[g - 1] : 0;
+      const uint32_t b0 = t == 0 ? 0 : ap / 1024 + 1, b1 = a[j] / 1024;
+      for (uint32_t bb = b0; bb <= b1 && bb <= nb; bb++) rejblk[bb] = before;
+    }
+  }
+  if (m == rows * kGpThreads) {
+    // no sentinel item in the rows: the blocks past the last rank get R
+    const uint32_t b0 = m ? s_lastr[rows * (kGpThreads / 64) - 1] / 1024 + 1 : 0;
+    for (uint32_t bb = b0 + threadIdx.x; bb <= nb; bb += kGpThreads) rejblk[bb] = R;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
