@@ -53,6 +53,11 @@ constexpr uint32_t kWinMin = 1024;
 constexpr uint32_t kMaxTiles = kWinMax / kTile;           // 2048
 constexpr uint32_t kAmbCap = 65536;  // ambiguous candidates per window (level 1)
 constexpr uint32_t kL2Cap = 16384;   // undecided after the first exact pass (in LDS)
+// sparse windows (candidate density <= 1/256) use super-tiles of 4 AES rounds
+// per workgroup, their few candidates stashed in LDS between the rounds
+constexpr uint32_t kSuperRounds = 4;
+constexpr uint32_t kSuperTile = kTile * kSuperRounds;  // 131072 draws
+constexpr int kStash = 192;                             // candidates per wave
 constexpr uint64_t kRefillWords = 1ull << 20;  // gen_perm buffer: 2^20 uint32
 constexpr uint64_t kWordsPerNonce = 1ull << 18;
 
@@ -68,6 +73,8 @@ struct ChainState {
   uint32_t buf;     // candidate buffer of the current window
   uint32_t C_tot;   // candidates in the current window (written by its last tile)
   uint32_t m_tot;   // ambiguous candidates in it
+  uint32_t ovf;     // a sparse tile's candidate stash overflowed
+  uint32_t force_dense;  // retry the current window with dense tiles
   // the window just resolved, committed by the next k_gp_window launch
   uint32_t c_valid, c_i0, c_k, c_C, c_R, c_buf;
 };
@@ -97,8 +104,13 @@ __host__ __device__ inline uint32_t choose_window(uint32_t k, uint32_t maskN) {
   return W < kWinMin ? kWinMin : W;
 }
 
-__device__ inline uint32_t ntiles_of(const ChainState &s) {
-  return s.done ? 0u : (s.W + kTile - 1) / kTile;
+__device__ inline bool sparse_window(const ChainState &s, uint32_t n, uint32_t maskN) {
+  return !s.force_dense && (uint64_t)(n - s.i) * 256 <= maskN;
+}
+
+__device__ inline uint32_t ntiles_of(const ChainState &s, uint32_t n, uint32_t maskN) {
+  const uint32_t td = sparse_window(s, n, maskN) ? kSuperTile : kTile;
+  return s.done ? 0u : (s.W + td - 1) / td;
 }
 
 // exclusive block scan of one value per thread (blockDim multiple of 64)
@@ -257,12 +269,38 @@ __device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, u
 }
 
 // ---- one window step: commit the last resolved windows, then draw and classify -----
-// Tile layout: wave w of a tile covers draws [w * 2048, (w + 1) * 2048); lane l
-// computes AES blocks j * 64 + l (j < 8) of that range, i.e. draws
+// Dense tile (32768 draws): wave w covers draws [w * 2048, (w + 1) * 2048); lane
+// l computes AES blocks j * 64 + l (j < 8) of that range, i.e. draws
 // j * 256 + 4 l + q, so rank order inside a wave is (j, l, q) and candidates
 // are compacted with ballots into consecutive addresses.
+// Sparse tile (131072 draws): wave w covers [w * 8192, (w + 1) * 8192) in four
+// rounds of 2048 laid out as above; its candidates go to an LDS stash in rank
+// order, so one ticket and one pair of look-backs serve four AES rounds.
 // Candidate a (rank in the window) is accepted for sure if v < k - a (at most a
 // acceptances precede it); otherwise it is ambiguous and goes to the resolver.
+template <int COPIES>
+__device__ inline void draw_round(const AesCtx<COPIES> &A, const uint32_t *__restrict__ rk,
+                                  const ChainState &st, uint32_t W, uint32_t mask, uint32_t base,
+                                  uint32_t (&v)[kDrawsPerThread]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const uint32_t d = base + j * 256 + lane * 4;
+    if (d < W) {
+      const uint64_t dabs = st.d0 + d;
+      const uint4 blk = aes_ctr_block<COPIES>(A, rk, st.nonce + dabs / kWordsPerNonce,
+                                              2 + (uint32_t)((dabs % kWordsPerNonce) / 4));
+      v[4 * j + 0] = blk.x & mask;
+      v[4 * j + 1] = blk.y & mask;
+      v[4 * j + 2] = blk.z & mask;
+      v[4 * j + 3] = blk.w & mask;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[4 * j + q] = 0xffffffffu;  // never a candidate
+    }
+  }
+}
+
 template <int COPIES>
 __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__restrict__ tab,
                                                           const uint32_t *__restrict__ rk_groups,
@@ -270,33 +308,38 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
                                                           uint32_t count, uint32_t draws_on) {
   __shared__ __align__(16) uint32_t lds[AesLds<COPIES>::kWords];
   __shared__ uint32_t s_wc[kGpThreads / 64], s_wa[kGpThreads / 64];
-  __shared__ uint32_t s_task, s_off, s_aoff;
+  __shared__ uint32_t s_next, s_off, s_aoff;
   __shared__ unsigned long long s_mask[kGpThreads / 64][16];
+  __shared__ uint32_t s_sd[kGpThreads / 64][kStash], s_sv[kGpThreads / 64][kStash];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t below = lanes_below();
+  const uint32_t maskN = mask + 1;
 
   // tasks, in ticket order: commit chunks (16 blocks of 1024 ranks, one per
   // wave) of the windows resolved last, then the tiles of the current windows
   uint32_t nc[3], nt[3];
   for (int c = 0; c < 3; c++) {
     nc[c] = B.st[c].c_valid ? (B.st[c].c_C + 16383) / 16384 : 0u;
-    nt[c] = draws_on ? ntiles_of(B.st[c]) : 0u;
+    nt[c] = draws_on ? ntiles_of(B.st[c], n, maskN) : 0u;
   }
   const uint32_t ncom = nc[0] + nc[1] + nc[2];
   const uint32_t total = ncom + nt[0] + nt[1] + nt[2];
+  // s_next holds the next task, taken one task ahead: a tile takes its
+  // successor's ticket while its first look-back waits anyway
+  if (threadIdx.x == 0 && total > 0) s_next = atomicAdd(&B.ctr[0], 1u);
   bool tables = false;
   for (;;) {
     if (total == 0) break;
     __syncthreads();
-    if (threadIdx.x == 0) s_task = atomicAdd(&B.ctr[0], 1u);
-    __syncthreads();
-    const uint32_t task = s_task;
+    const uint32_t task = s_next;
+    __syncthreads();  // read by all before it is rewritten
     if (task >= total) break;
     if (task < ncom) {
       int chain;
       uint32_t chunk;
       task_tile(nc, task, chain, chunk);
       commit_block(B, chain, chunk * (kGpThreads / 64) + wv, n, count, s_mask[wv]);
+      if (threadIdx.x == 0) s_next = atomicAdd(&B.ctr[0], 1u);
       continue;
     }
     if (!tables) {
@@ -309,35 +352,65 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     task_tile(nt, task - ncom, chain, tile);
     ChainState &st = B.st[chain];
     const uint32_t kstart = n - st.i, W = st.W, seq = st.seq;
-    const uint32_t wbase = tile * kTile + (uint32_t)wv * 2048;
+    const bool sparse = sparse_window(st, n, maskN);
     const uint32_t *rk = rk_groups + 60 * chain;
+    uint64_t *cand = B.cand + ((size_t)st.buf * 3 + chain) * B.cand_cap;
+    const uint32_t cap = B.cand_cap;
+    uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
+    int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
     uint32_t v[kDrawsPerThread];
+    uint32_t wc = 0;  // this wave's candidates
+    uint32_t wbase;
+    if (sparse) {
+      wbase = tile * kSuperTile + (uint32_t)wv * (kSuperTile / (kGpThreads / 64));
+      for (uint32_t r = 0; r < kSuperRounds; r++) {
+        const uint32_t rbase = wbase + r * 2048;
+        draw_round<COPIES>(A, rk, st, W, mask, rbase, v);
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const uint32_t d = wbase + j * 256 + lane * 4;
-      if (d < W) {
-        const uint64_t dabs = st.d0 + d;
-        const uint4 blk = aes_ctr_block<COPIES>(A, rk, st.nonce + dabs / kWordsPerNonce,
-                                                2 + (uint32_t)((dabs % kWordsPerNonce) / 4));
-        v[4 * j + 0] = blk.x & mask;
-        v[4 * j + 1] = blk.y & mask;
-        v[4 * j + 2] = blk.z & mask;
-        v[4 * j + 3] = blk.w & mask;
-      } else {
+        for (int j = 0; j < 8; j++) {
+          uint64_t b[4];
+          uint32_t pre = 0;
 #pragma unroll
-        for (int q = 0; q < 4; q++) v[4 * j + q] = 0xffffffffu;  // never a candidate
+          for (int q = 0; q < 4; q++) {
+            b[q] = __ballot(v[4 * j + q] < kstart);
+            pre += __builtin_popcountll(b[q] & below);
+          }
+          if (!(b[0] | b[1] | b[2] | b[3])) continue;
+          uint32_t i = wc + pre;
+          const uint32_t d = rbase + j * 256 + lane * 4;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const uint32_t x = v[4 * j + q];
+            if (x < kstart) {
+              if (i < (uint32_t)kStash) {
+                s_sd[wv][i] = d + q;
+                s_sv[wv][i] = x;
+              }
+              i++;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 4; q++) wc += __builtin_popcountll(b[q]);
+        }
       }
-    }
-    // candidates (v < k): counts per wave, tile offset by look-back
-    uint32_t wc = 0;
+      if (wc > (uint32_t)kStash && lane == 0) atomicOr(&st.ovf, 1u);  // window retried dense
+    } else {
+      wbase = tile * kTile + (uint32_t)wv * 2048;
+      draw_round<COPIES>(A, rk, st, W, mask, wbase, v);
 #pragma unroll
-    for (int e = 0; e < kDrawsPerThread; e++) wc += __builtin_popcountll(__ballot(v[e] < kstart));
+      for (int e = 0; e < kDrawsPerThread; e++) wc += __builtin_popcountll(__ballot(v[e] < kstart));
+    }
+    // candidate counts per wave, tile offset by look-back (wave 0 also takes
+    // the next ticket meanwhile)
     if (lane == 0) s_wc[wv] = wc;
     __syncthreads();
     if (wv == 0) {
+      uint32_t tk = 0;
+      if (lane == 0) tk = atomicAdd(&B.ctr[0], 1u);
       uint32_t cnt = wave_sum(lane < kGpThreads / 64 ? s_wc[lane] : 0u);
       uint32_t off = look_back(B.stat_c + (size_t)chain * kMaxTiles, tile, seq, cnt, &st.err);
       if (lane == 0) {
+        s_next = tk;
         s_off = off;
         if (tile == nt[chain] - 1) st.C_tot = off + cnt;
       }
@@ -345,32 +418,43 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     __syncthreads();
     uint32_t run = s_off;
     for (int w = 0; w < wv; w++) run += s_wc[w];
-    // ranks, candidate records, ambiguous counts
-    uint64_t *cand = B.cand + ((size_t)st.buf * 3 + chain) * B.cand_cap;
-    const uint32_t cap = B.cand_cap;
+    const uint32_t run0 = run;
+    // candidate records in rank order, ambiguous counts
     uint32_t wa = 0;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      uint64_t b[4];
-      uint32_t pre = 0;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        b[q] = __ballot(v[4 * j + q] < kstart);
-        pre += __builtin_popcountll(b[q] & below);
+    if (sparse) {
+      const uint32_t ns = wc < (uint32_t)kStash ? wc : (uint32_t)kStash;
+      for (uint32_t i0 = 0; i0 < ns; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool in = i < ns;
+        const uint32_t d = in ? s_sd[wv][i] : 0u, x = in ? s_sv[wv][i] : 0u;
+        const uint32_t a = run + i;
+        if (in && a < cap) cand[a] = ((uint64_t)d << 32) | x;
+        wa += __builtin_popcountll(__ballot(in && a + x >= kstart));
       }
-      if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
-      uint32_t a = run + pre;
-      const uint32_t d = wbase + j * 256 + lane * 4;
+    } else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint32_t x = v[4 * j + q];
-        const bool is_c = x < kstart;
-        if (is_c && a < cap) cand[a] = ((uint64_t)(d + q) << 32) | x;
-        wa += __builtin_popcountll(__ballot(is_c && a + x >= kstart));
-        a += is_c;
+      for (int j = 0; j < 8; j++) {
+        uint64_t b[4];
+        uint32_t pre = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          b[q] = __ballot(v[4 * j + q] < kstart);
+          pre += __builtin_popcountll(b[q] & below);
+        }
+        if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
+        uint32_t a = run + pre;
+        const uint32_t d = wbase + j * 256 + lane * 4;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t x = v[4 * j + q];
+          const bool is_c = x < kstart;
+          if (is_c && a < cap) cand[a] = ((uint64_t)(d + q) << 32) | x;
+          wa += __builtin_popcountll(__ballot(is_c && a + x >= kstart));
+          a += is_c;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) run += __builtin_popcountll(b[q]);
       }
-#pragma unroll
-      for (int q = 0; q < 4; q++) run += __builtin_popcountll(b[q]);
     }
     if (lane == 0) s_wa[wv] = wa;
     __syncthreads();
@@ -386,51 +470,66 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     // ambiguous records, in rank order
     uint32_t arun = s_aoff;
     for (int w = 0; w < wv; w++) arun += s_wa[w];
-    run = s_off;
-    for (int w = 0; w < wv; w++) run += s_wc[w];
-    uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
-    int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
-#pragma unroll
-    for (int j = 0; j < 8 && wa; j++) {
-      uint64_t b[4];
-      uint32_t pre = 0;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        b[q] = __ballot(v[4 * j + q] < kstart);
-        pre += __builtin_popcountll(b[q] & below);
-      }
-      if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
-      uint32_t a = run + pre;
-      bool amb[4];
-      uint32_t apre = 0;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint32_t x = v[4 * j + q];
-        const bool is_c = x < kstart;
-        amb[q] = is_c && a + x >= kstart;
-        const uint64_t ab = __ballot(amb[q]);
-        apre += __builtin_popcountll(ab & below);
-        a += is_c;
-      }
-      // second walk: write this lane's ambiguous records after the lower lanes'
-      a = run + pre;
-      uint32_t r = arun + apre;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint32_t x = v[4 * j + q];
-        if (amb[q]) {
-          if (r < kAmbCap) {
-            amb_a[r] = a;
-            amb_u[r] = (int32_t)(a + x - kstart);
-          }
-          r++;
+    run = run0;
+    if (sparse) {
+      const uint32_t ns = wc < (uint32_t)kStash ? wc : (uint32_t)kStash;
+      for (uint32_t i0 = 0; i0 < ns && wa; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool in = i < ns;
+        const uint32_t x = in ? s_sv[wv][i] : 0u;
+        const uint32_t a = run + i;
+        const bool amb = in && a + x >= kstart;
+        const uint64_t ab = __ballot(amb);
+        const uint32_t r = arun + __builtin_popcountll(ab & below);
+        if (amb && r < kAmbCap) {
+          amb_a[r] = a;
+          amb_u[r] = (int32_t)(a + x - kstart);
         }
-        a += x < kstart;
+        arun += __builtin_popcountll(ab);
       }
+    } else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        run += __builtin_popcountll(b[q]);
-        arun += __builtin_popcountll(__ballot(amb[q]));
+      for (int j = 0; j < 8 && wa; j++) {
+        uint64_t b[4];
+        uint32_t pre = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          b[q] = __ballot(v[4 * j + q] < kstart);
+          pre += __builtin_popcountll(b[q] & below);
+        }
+        if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
+        uint32_t a = run + pre;
+        bool amb[4];
+        uint32_t apre = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t x = v[4 * j + q];
+          const bool is_c = x < kstart;
+          amb[q] = is_c && a + x >= kstart;
+          const uint64_t ab = __ballot(amb[q]);
+          apre += __builtin_popcountll(ab & below);
+          a += is_c;
+        }
+        // second walk: write this lane's ambiguous records after the lower lanes'
+        a = run + pre;
+        uint32_t r = arun + apre;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t x = v[4 * j + q];
+          if (amb[q]) {
+            if (r < kAmbCap) {
+              amb_a[r] = a;
+              amb_u[r] = (int32_t)(a + x - kstart);
+            }
+            r++;
+          }
+          a += x < kstart;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          run += __builtin_popcountll(b[q]);
+          arun += __builtin_popcountll(__ballot(amb[q]));
+        }
       }
     }
   }
@@ -478,6 +577,18 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
   const uint64_t below = lanes_below();
   ChainState *st = B.st + chain;
   if (st->done) return;
+  const uint32_t ovf = st->ovf;
+  __syncthreads();
+  if (ovf) {
+    // a sparse tile's stash overflowed: the same window again, dense
+    if (threadIdx.x == 0) {
+      st->ovf = 0;
+      st->force_dense = 1;
+      st->c_valid = 0;
+      st->seq++;
+    }
+    return;
+  }
   const uint32_t kstart = n - st->i;
   const uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
   const int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
@@ -697,6 +808,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
     st->c_buf = buf;
     st->buf = buf ^ 1u;
     st->seq++;
+    st->force_dense = 0;
     if (acc >= kstart) {
       // the permutation ends here: rank a* of the kstart-th acceptance is
       // kstart-1 shifted past every rejected rank at or before it (rej is
