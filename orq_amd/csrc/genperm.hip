@@ -1072,7 +1072,7 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   ORQG_HIP(hipStreamWaitEvent(c->stream2, c->ev_aux, 0));
   // two 4-wave workgroups per CU: enough random loads in flight, and room
   // beside them for a resolve or window workgroup
-  const unsigned fgrid = grid_for(n, 256, c->num_cu * 1);
+  const unsigned fgrid = grid_for(n, 256, std::max(1, c->num_cu / 2));
   const size_t chain_stride = (size_t)count * n;
   uint32_t launched[3] = {0, 0, 0};
   auto resolve_swaps = [&](int g, uint32_t upto) {
