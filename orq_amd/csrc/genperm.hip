@@ -89,7 +89,6 @@ struct ChainBufs {
   uint32_t *amb_a;     // [3][kAmbCap]   ambiguous candidates in rank order: rank
   int32_t *amb_u;      // [3][kAmbCap]   ... and U = rank - (k - v)
   uint32_t *rej;       // [3][kAmbCap]   ranks of rejected candidates, ascending
-  uint32_t *rejblk;    // [3][cand_cap / 1024 + 2] rejected ranks below 1024 b
   uint32_t *r_out;     // [3][count][n]  accepted offsets r_i
   uint64_t *D;         // [3][count]     draws consumed per permutation
   uint64_t *nonce_at;  // [3][count+1]   nonce at the start of each permutation
@@ -221,21 +220,38 @@ __device__ inline uint32_t look_back(uint64_t *stat, uint32_t tile, uint32_t seq
   return excl;
 }
 
+// first index of the ascending a[0..len) with a[i] >= key, by one wave: 64-ary
+// search (a few dependent coalesced loads instead of log2(len) scalar ones)
+__device__ inline uint32_t wave_lower_bound(const uint32_t *__restrict__ a, uint32_t len,
+                                            uint32_t key) {
+  const int lane = threadIdx.x & 63;
+  uint32_t lo = 0, hi = len;  // answer in [lo, hi]
+  while (hi - lo > 64) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t p = lo + step * lane;
+    const uint32_t c = __builtin_popcountll(__ballot(p < hi && a[p] < key));
+    const uint32_t nlo = c == 0 ? lo : lo + step * (c - 1) + 1;
+    hi = lo + step * c < hi ? lo + step * c : hi;
+    lo = nlo;
+  }
+  const uint32_t p = lo + lane;
+  return lo + __builtin_popcountll(__ballot(p < hi && a[p] < key));
+}
+
 // ---- commit: r_i for the accepted candidates of a resolved window ------------------
 // Accepted candidate of rank a becomes r_{i0 + a - #rejected ranks below a}.  A
 // wave takes 1024 consecutive ranks: the rejected ones among them (a slice of
-// the ascending rejected list, located by rejblk) become a 1024-bit mask in
-// LDS, and lane l handles ranks base + 64 j + l, so loads and stores coalesce.
+// the ascending rejected list, found by a wave search) become a 1024-bit mask
+// in LDS, and lane l handles ranks base + 64 j + l, so loads and stores coalesce.
 __device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, uint32_t n,
                                     uint32_t count, unsigned long long *s_mask) {
   const int lane = threadIdx.x & 63;
   const uint64_t below = lanes_below();
   const ChainState &st = B.st[chain];
-  const uint32_t C = st.c_C, i0 = st.c_i0, kstart = n - i0;
+  const uint32_t C = st.c_C, R = st.c_R, i0 = st.c_i0, kstart = n - i0;
   if (b * 1024 >= C) return;
   const uint64_t *cand = B.cand + ((size_t)st.c_buf * 3 + chain) * B.cand_cap;
   const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
-  const uint32_t *rejblk = B.rejblk + (size_t)chain * (B.cand_cap / 1024 + 2);
   uint32_t *r = B.r_out + ((size_t)chain * count + st.c_k) * (size_t)n;
   const uint32_t base = b * 1024;
   uint32_t v[16];
@@ -244,13 +260,16 @@ __device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, u
     const uint32_t f = base + 64 * j + lane;
     v[j] = f < C ? (uint32_t)cand[f] : 0;
   }
-  const uint32_t lo0 = rejblk[b], lo1 = rejblk[b + 1];
+  const uint32_t lo0 = wave_lower_bound(rej, R, base);
   if (lane < 16) s_mask[lane] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  for (uint32_t p = lo0 + lane; p < lo1; p += 64) {
-    const uint32_t e = rej[p] - base;  // < 1024
-    atomicOr(&s_mask[e >> 6], 1ull << (e & 63));
+  for (uint32_t p0 = lo0;; p0 += 64) {
+    const uint32_t p = p0 + lane;
+    const uint32_t e = p < R ? rej[p] - base : 0xffffffffu;
+    const bool in = e < 1024;
+    if (in) atomicOr(&s_mask[e >> 6], 1ull << (e & 63));
+    if (~__ballot(in)) break;  // the slice ends in these 64
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -742,20 +761,14 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
     if (threadIdx.x < nent) s_cnt[threadIdx.x] = ex;
     __syncthreads();
   }
-  // rejected ranks in ascending order (the ambiguous list is in rank order),
-  // and rejblk[b] = rejected ranks below 1024 b for b <= nb = ceil(C / 1024):
-  // item t (rank a_t; a sentinel rank 1024 nb past the list) sets the blocks b
-  // with a_{t-1} < 1024 b <= a_t to the number of rejected items before it.
-  // Loads go out 8 rows at a time.
-  uint32_t *rejblk = B.rejblk + (size_t)chain * (B.cand_cap / 1024 + 2);
-  const uint32_t nb = (C + 1023) / 1024;
-  __shared__ uint32_t s_lastr[kRows * (kGpThreads / 64)];  // rank of each (row, wave)'s last item
+  // rejected ranks in ascending order (the ambiguous list is in rank order);
+  // loads go out 8 rows at a time
   for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
     uint32_t a[8];
 #pragma unroll
     for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t t = (q0 + j) * kGpThreads + threadIdx.x;
-      a[j] = q0 + j < rows && t < m ? amb_a[t] : nb * 1024;
+      const uint32_t q = q0 + j;
+      a[j] = q < rows && ((rejm >> q) & 1) ? amb_a[q * kGpThreads + threadIdx.x] : 0;
     }
 #pragma unroll
     for (uint32_t j = 0; j < 8; j++) {
@@ -764,35 +777,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
       const bool rj = (rejm >> q) & 1;
       const uint64_t b = __ballot(rj);
       if (rj) rej[s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below)] = a[j];
-      if (lane == 63) s_lastr[q * (kGpThreads / 64) + wv] = a[j];
     }
-  }
-  __syncthreads();
-  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
-    uint32_t a[8];
-#pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t t = (q0 + j) * kGpThreads + threadIdx.x;
-      a[j] = q0 + j < rows && t < m ? amb_a[t] : nb * 1024;
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t q = q0 + j;
-      if (q >= rows) break;
-      const uint32_t t = q * kGpThreads + threadIdx.x;
-      const uint32_t g = q * (kGpThreads / 64) + wv;
-      const uint64_t b = __ballot((rejm >> q) & 1);
-      const uint32_t before = s_cnt[g] + __builtin_popcountll(b & below);
-      uint32_t ap = __shfl_up(a[j], 1, 64);
-      if (lane == 0) ap = g > 0 ? s_lastr[g - 1] : 0;
-      const uint32_t b0 = t == 0 ? 0 : ap / 1024 + 1, b1 = a[j] / 1024;
-      for (uint32_t bb = b0; bb <= b1 && bb <= nb; bb++) rejblk[bb] = before;
-    }
-  }
-  if (m == rows * kGpThreads) {
-    // no sentinel item in the rows: the blocks past the last rank get R
-    const uint32_t b0 = m ? s_lastr[rows * (kGpThreads / 64) - 1] / 1024 + 1 : 0;
-    for (uint32_t bb = b0 + threadIdx.x; bb <= nb; bb += kGpThreads) rejblk[bb] = R;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1007,7 +992,6 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   B.amb_a = (uint32_t *)c->scratch_get("gp_amb_a", 3ull * kAmbCap * 4);
   B.amb_u = (int32_t *)c->scratch_get("gp_amb_u", 3ull * kAmbCap * 4);
   B.rej = (uint32_t *)c->scratch_get("gp_rej", 3ull * kAmbCap * 4);
-  B.rejblk = (uint32_t *)c->scratch_get("gp_rejblk", 3ull * (cand_cap / 1024 + 2) * 4);
   B.r_out = (uint32_t *)c->scratch_get("gp_r", 3ull * count * n * 4);
   B.D = (uint64_t *)c->scratch_get("gp_D", 3ull * count * 8);
   B.nonce_at = (uint64_t *)c->scratch_get("gp_nonce_at", 3ull * (count + 1) * 8);
