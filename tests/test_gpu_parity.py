@@ -162,6 +162,24 @@ def test_gen_perm_large(O, G, keys):
     _same_nonces(oc, gc)
 
 
+@pytest.mark.parametrize("n", [(1 << 20) + 3, 1 << 22, 1 << 24])
+def test_gen_perm_many_tiles(O, G, keys, n):
+    """Windows of hundreds of tiles: the dense look-back over long tile chains,
+    sparse super-tiles, several permutations in a row (the second starts at
+    the first one's refill boundary), a non-power-of-two size, and the sort
+    benchmark's size (2^24)."""
+    count = 2 if n < (1 << 22) else 1
+    oc, gc = _pair(O, G, keys)
+    oc.reserve(n, count)
+    gc.reserve(n, count)
+    for k in range(count):
+        for g in range(3):
+            members = [p for p in range(3) if p != 2 - g]
+            np.testing.assert_array_equal(gc.peek_perm(k, g, n), oc.peek_perm(k, members[0], g, n),
+                                          err_msg=f"perm {k} group {g}")
+    _same_nonces(oc, gc)
+
+
 @pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_apply_sharded(O, G, keys, width, inverse):
