@@ -223,9 +223,14 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
         uint32_t z = (S[pm][i] - S[p][i]) + f[i][p] * y[p] + f[i][p] * y[q] + f[i][q] * y[p];
         pr[p] = s0[p] + z;
       }
-      // into the element record's perm words
-      *reinterpret_cast<uint4 *>(R + (e0 + i) * kRec + kRecPerm) =
-          make_uint4(pr[0], pr[1], pr[2], 0u);
+      // into the element record's perm words, as one whole 32-byte piece
+      // (words 8..15: word 8 is v's last word for L = 3, 9..11 are zero) so
+      // that no partial write needs a read-modify-write
+      uint32_t *rec = R + (e0 + i) * kRec;
+      const uint32_t w8 = rec[8];
+      uint4 *o = reinterpret_cast<uint4 *>(rec + 8);
+      o[0] = make_uint4(w8, 0u, 0u, 0u);
+      o[1] = make_uint4(pr[0], pr[1], pr[2], 0u);
     }
   }
   }
@@ -304,8 +309,9 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       w[kRecPerm + 1] = pq[i].y + pm[1][i];
       w[kRecPerm + 2] = pq[i].z + pm[2][i];
       if (SCATTER) {
-        // open the permuted perm and put the permuted v at its position (the
-        // perm words there are rewritten by the next bit)
+        // open the permuted perm and put the permuted record at its position
+        // (whole: a partial write of its last 32-byte piece would cost a
+        // read-modify-write; the perm words are rewritten by the next bit)
         const uint32_t sigma = w[kRecPerm] + w[kRecPerm + 1] + w[kRecPerm + 2];
         if (sigma >= n) {
           atomicOr(err, 1u);
@@ -313,7 +319,7 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
         }
         uint4 *o = reinterpret_cast<uint4 *>(Rout + (uint64_t)sigma * kRec);
 #pragma unroll
-        for (int q = 0; q < kVq; q++)
+        for (int q = 0; q < kRec / 4; q++)
           o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
       } else {
         uint4 *o = reinterpret_cast<uint4 *>(Rout + e * kRec);
