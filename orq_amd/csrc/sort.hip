@@ -67,6 +67,38 @@ __device__ inline uint32_t bexcl(uint32_t x, uint32_t *s_w, uint32_t &total) {
   return base + inc - x;
 }
 
+// exclusive block scan of three values per thread at once (two barriers); s_w
+// holds 3 words per wave
+__device__ inline void bexcl3(const uint32_t (&x)[3], uint32_t *s_w, uint32_t (&ex)[3],
+                              uint32_t (&total)[3]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t inc[3] = {x[0], x[1], x[2]};
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      uint32_t t = __shfl_up(inc[k], o, 64);
+      if (lane >= o) inc[k] += t;
+    }
+  }
+  __syncthreads();
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 3; k++) s_w[3 * wv + k] = inc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    uint32_t base = 0, tot = 0;
+    for (int w = 0; w < nw; w++) {
+      const uint32_t v = s_w[3 * w + k];
+      if (w < wv) base += v;
+      tot += v;
+    }
+    ex[k] = base + inc[k] - x[k];
+    total[k] = tot;
+  }
+}
+
 // ---- pad_input ----------------------------------------------------------------
 // share k of element i: (sign-extended key << 32) | idx_k, idx_0 = i (ASC) or
 // (uint32)(-1-i) (DESC), idx_1 = idx_2 = 0 (public_share, replicated_3pc.h:456-470);
@@ -103,7 +135,7 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
                                                  int bit, uint32_t flip, uint32_t *__restrict__ F,
                                                  uint32_t *__restrict__ bsum) {
   AES_LDS;
-  __shared__ uint32_t s_w[kBT / 64];
+  __shared__ uint32_t s_w[3 * (kBT / 64)];
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
@@ -127,11 +159,12 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
       }
     }
   }
+  {
+    uint32_t ex[3], tot[3];
+    bexcl3(loc, s_w, ex, tot);
+    if (threadIdx.x == 0)
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    uint32_t tot;
-    bexcl(loc[k], s_w, tot);
-    if (threadIdx.x == 0) bsum[sb * 3 + k] = tot;
+      for (int k = 0; k < 3; k++) bsum[sb * 3 + k] = tot[k];
   }
   }
 }
@@ -171,7 +204,7 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
                                                   const uint32_t *__restrict__ tot,
                                                   uint32_t *__restrict__ R) {
   AES_LDS;
-  __shared__ uint32_t s_w[kBT / 64];
+  __shared__ uint32_t s_w[3 * (kBT / 64)];
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
@@ -187,10 +220,11 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
       loc[k] += f[i][k];
     }
   uint32_t pre[3];
+  {
+    uint32_t t[3];
+    bexcl3(loc, s_w, pre, t);
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    uint32_t t;
-    pre[k] = bexcl(loc[k], s_w, t) + boff[sb * 3 + k];
+    for (int k = 0; k < 3; k++) pre[k] += boff[sb * 3 + k];
   }
   uint32_t S[3][4];
   {
