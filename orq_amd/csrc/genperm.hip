@@ -1054,9 +1054,11 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
   ORQG_HIP(hipEventRecord(c->ev_aux, c->stream));
   ORQG_HIP(hipStreamWaitEvent(c->stream2, c->ev_aux, 0));
-  // two 4-wave workgroups per CU: enough random loads in flight, and room
-  // beside them for a resolve or window workgroup
-  const unsigned fgrid = grid_for(n, 256, std::max(1, c->num_cu / 2));
+  // while windows run: one 4-wave workgroup per two CUs, enough random loads
+  // in flight to finish in the windows' shadow and room beside them for the
+  // window and resolve workgroups, which need whole CUs
+  const unsigned fgrid_bg = grid_for(n, 256, std::max(1, c->num_cu / 2));
+  unsigned fgrid = fgrid_bg;
   const size_t chain_stride = (size_t)count * n;
   uint32_t launched[3] = {0, 0, 0};
   auto resolve_swaps = [&](int g, uint32_t upto) {
@@ -1095,6 +1097,8 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   ORQG_HIP(hipMemcpyAsync(fin, B.st, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
   for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] = fin[g].nonce;
+  // the windows are done: what is left runs alone on the whole GPU
+  fgrid = grid_for(n, 256, c->num_cu * 8);
   for (int g = 0; g < 3; g++) resolve_swaps(g, count);
   ORQG_HIP(hipGetLastError());
   uint32_t herr = 0;
