@@ -24,9 +24,10 @@
 //     launch first commits the previously resolved window: accepted candidate
 //     a becomes r_{i0 + a - #rejected before a};
 //   * k_gp_resolve: one workgroup per chain decides the ambiguous candidates
-//     exactly with alternating optimistic passes, builds the sorted list of
-//     rejected ranks, and advances the chain (next window, or the next
-//     permutation at the refill boundary after the last acceptance).
+//     exactly (a sure-reject test, then an in-order walk of the rest), builds
+//     the sorted list of rejected ranks, and advances the chain (next window,
+//     or the next permutation at the refill boundary after the last
+//     acceptance).
 // The window size keeps the expected number of ambiguous candidates within the
 // resolver's capacity; an overflowing window is retried at a quarter of its
 // size, down to kWinMin draws, which always fits.
@@ -569,27 +570,21 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
 // ---- exact resolution of the ambiguous candidates, chain advance -------------------
 // Ambiguous candidate t (list index t, rank a_t, T_t = k - v_t <= a_t) is
 // accepted iff fewer than T_t acceptances precede it, i.e. iff more than
-// U_t = a_t - T_t of the ambiguous candidates before it are rejected.
-//   type B list (accept iff rejected-before > U): index t <= U -> rejected;
-//     U < 0 -> accepted; otherwise accepted iff
-//     accepted-before-among-undecided < t - (sure accepts before) - U;
-//   type A list (accept iff accepted-before < T): t < T -> accepted;
-//     T <= 0 -> rejected; otherwise accepted iff
-//     rejected-before-among-undecided > t - (sure rejects before) - T.
-// The first pass (type B over all ambiguous candidates) streams global memory
-// row by row (item t = row * 1024 + thread, compacted with ballots); the
-// undecided rest, in order, is resolved in LDS.  Every pass decides at least
-// its first item, so the loop ends.
+// U_t = a_t - T_t >= 0 of the ambiguous candidates before it are rejected.
+// At most t of them are, so t <= U_t means rejected for sure.  The first
+// stage applies that test to all ambiguous candidates, streaming global
+// memory row by row (item t = row * 1024 + thread, compacted with ballots).
+// Every earlier candidate of an undecided one is either rejected for sure or
+// undecided, so an undecided item is accepted iff fewer than T' = t - U_t of
+// the undecided items before it are accepted: the second stage walks that
+// short list in order (in LDS, one wave, 64 items per step).
 __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t n,
                                                            uint32_t count, uint32_t maskN) {
   constexpr uint32_t kRows = kAmbCap / kGpThreads;  // 64
-  constexpr int PER = kL2Cap / kGpThreads;          // 16
-  __shared__ uint16_t s_idx[kL2Cap];  // current list -> undecided item
   __shared__ int32_t s_thr[kL2Cap];
   __shared__ uint8_t s_dec[kL2Cap];   // 1 accepted, 2 rejected
   __shared__ uint32_t s_cnt[kRows * (kGpThreads / 64)];
   __shared__ uint32_t s_w[kGpThreads / 64];
-  __shared__ uint64_t s_w64[kGpThreads / 64];
   __shared__ uint32_t s_l2;
   const int chain = blockIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -674,71 +669,38 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
       if (!ud) continue;
       const uint32_t t = q * kGpThreads + threadIdx.x;
       const uint32_t p = s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below);
-      s_idx[p] = (uint16_t)p;
       s_thr[p] = (int32_t)t - u[j];
       s_dec[p] = 0;
     }
   }
   __syncthreads();
-  uint32_t len = s_l2;
-  int type_b = 0;
-  while (len > 0) {
-    const uint32_t beg = threadIdx.x * PER;
-    uint8_t dec[PER];
-    uint32_t n_sa = 0, n_sr = 0, n_un = 0;
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      const uint32_t t = beg + q;
-      dec[q] = 3;
-      if (t < len) {
-        const int32_t thr = s_thr[t];
-        if (type_b) dec[q] = ((int32_t)t <= thr) ? 2 : (thr < 0 ? 1 : 0);
-        else dec[q] = ((int32_t)t < thr) ? 1 : (thr <= 0 ? 2 : 0);
-        n_sa += dec[q] == 1;
-        n_sr += dec[q] == 2;
-        n_un += dec[q] == 0;
+  // The undecided items, in order: item i is accepted iff fewer than T'_i of
+  // the undecided items before it are accepted.  One wave walks the list in
+  // chunks of 64: given the acceptances before the chunk (acc), the chunk's
+  // decisions are the fixed point of d_i = [acc + #(accepted before i in the
+  // chunk) < T'_i], reached by iterating from the optimistic guess with one
+  // ballot per round (item i is exact after round i, so at most 64 rounds; a
+  // repeated ballot is the unique fixed point, i.e. the sequential answer).
+  if (wv == 0) {
+    const uint32_t len = s_l2;
+    uint32_t acc = 0;
+    for (uint32_t c0 = 0; c0 < len; c0 += 64) {
+      const uint32_t i = c0 + lane;
+      const bool in = i < len;
+      const int32_t T = in ? s_thr[i] : 0;
+      uint64_t dm = __ballot(in && (int32_t)acc < T);
+      for (int round = 0; round < 65; round++) {
+        const uint32_t before = acc + (uint32_t)__builtin_popcountll(dm & below);
+        const uint64_t nm = __ballot(in && (int32_t)before < T);
+        if (nm == dm) break;
+        dm = nm;
       }
+      if (in) s_dec[i] = (dm >> lane) & 1 ? 1 : 2;
+      acc += (uint32_t)__builtin_popcountll(dm);
     }
-    // one scan of the three counts packed 21 bits apart
-    uint64_t tot;
-    const uint64_t ex = block_scan<uint64_t>((uint64_t)n_sa | ((uint64_t)n_sr << 21) |
-                                                 ((uint64_t)n_un << 42),
-                                             s_w64, tot);
-    uint32_t sa = (uint32_t)(ex & 0x1fffff), sr = (uint32_t)((ex >> 21) & 0x1fffff),
-             un = (uint32_t)(ex >> 42);
-    const uint32_t new_len = (uint32_t)(tot >> 42);
-    int32_t nthr[PER];
-    uint16_t nidx[PER];
-    uint32_t npos[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      const uint32_t t = beg + q;
-      npos[q] = 0xffffffffu;
-      if (dec[q] == 3) continue;
-      const uint16_t id = s_idx[t];
-      if (dec[q] != 0) {
-        s_dec[id] = dec[q];
-      } else {
-        const int32_t thr = s_thr[t];
-        nthr[q] = type_b ? (int32_t)t - (int32_t)sa - thr : (int32_t)t - (int32_t)sr - thr;
-        nidx[q] = id;
-        npos[q] = un++;
-      }
-      sa += dec[q] == 1;
-      sr += dec[q] == 2;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; q++)
-      if (npos[q] != 0xffffffffu) {
-        s_idx[npos[q]] = nidx[q];
-        s_thr[npos[q]] = nthr[q];
-      }
-    __syncthreads();
-    len = new_len;
-    type_b = !type_b;
   }
-  // rejected items: level-1 rejects and the undecided ones the passes rejected
+  __syncthreads();
+  // rejected items: level-1 rejects and the undecided ones the walk rejected
   for (uint32_t q = 0; q < rows; q++) {
     const bool ud = (undm >> q) & 1;
     const uint64_t b = __ballot(ud);
