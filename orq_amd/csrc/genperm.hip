@@ -96,16 +96,24 @@ struct ChainBufs {
 };
 
 // Window for k open positions: about W^2 k / (2 maskN^2) candidates are
-// ambiguous; 256 maskN / sqrt(k) keeps that near half of kAmbCap.
+// ambiguous; 256 maskN / sqrt(k) keeps that near half of kAmbCap.  Near the
+// end of a permutation that is far more than the maskN * H(k) draws the k
+// remaining positions take on average, so it is capped at three times those.
 __host__ __device__ inline uint32_t choose_window(uint32_t k, uint32_t maskN) {
-  double w = 256.0 * (double)maskN / sqrt((double)(k ? k : 1));
+  const double kk = (double)(k ? k : 1);
+  double w = 256.0 * (double)maskN / sqrt(kk);
+  const double rest = 3.0 * (double)maskN * (log(kk) + 0.58);
+  if (rest < w) w = rest;
   if (w >= (double)kWinMax) return kWinMax;
   uint32_t W = w >= kTile ? (uint32_t)(w / kTile) * kTile : (uint32_t)(w / kWinMin) * kWinMin;
   return W < kWinMin ? kWinMin : W;
 }
 
+// super-tiles when candidates are rare and the window still gives every
+// workgroup at least one (otherwise dense tiles spread it over more CUs)
 __device__ inline bool sparse_window(const ChainState &s, uint32_t n, uint32_t maskN) {
-  return !s.force_dense && (uint64_t)(n - s.i) * 256 <= maskN;
+  return !s.force_dense && (uint64_t)(n - s.i) * 256 <= maskN &&
+         (uint64_t)s.W >= (uint64_t)kSuperTile * gridDim.x;
 }
 
 __device__ inline uint32_t ntiles_of(const ChainState &s, uint32_t n, uint32_t maskN) {
