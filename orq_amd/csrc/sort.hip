@@ -273,6 +273,11 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
 // ---- sharded round on element records: gather through idx, then add the
 // resharing masks of v (binary, tszv-byte masks) and perm (arithmetic, 4-byte
 // masks), drawn from group PRG g ----------------------------------------------------
+// A pair of lanes takes a group of 4 elements, 2 each.  The group's two perm
+// mask blocks (calls 0 and 1, 4 elements per block) are computed one per lane
+// and exchanged; each lane computes its elements' two v mask blocks.  Gathers
+// run one group ahead of the AES work (indices two groups ahead), so the loads
+// are in flight while the keystream is computed.
 template <int L, bool SCATTER>
 __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
                                                 const uint32_t *__restrict__ rk, uint64_t nv,
@@ -284,33 +289,62 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
   AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
+  constexpr int kVq = (3 * L + 3) / 4;  // 16-byte pieces holding v
   const uint64_t n = sv.n;
   const uint64_t ngrp = (n + 3) / 4;
-  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < ngrp;
-       g += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t e0 = g * 4;
-    const int ne = (int)(n - e0 < 4 ? n - e0 : 4);
-    // issue every gather first so that the loads are in flight during the AES
-    // work: words 0..3L-1 (v) and kRecPerm.. (perm) of each source record
-    constexpr int kVq = (3 * L + 3) / 4;  // 16-byte pieces holding v
-    uint4 pv[4][kVq], pq[4];
+  const uint32_t half = threadIdx.x & 1;
+  const uint64_t gstride = ((uint64_t)gridDim.x * blockDim.x) >> 1;
+  uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 1;
+  // source row of this lane's element i of group gg, or -1
+  auto load_idx = [&](uint64_t gg, int32_t (&ix)[2]) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (i < ne) {
-        const uint64_t src = (uint64_t)(uint32_t)idx[e0 + i];
-        const uint4 *rs = reinterpret_cast<const uint4 *>(Rin + src * kRec);
+    for (int i = 0; i < 2; i++) {
+      const uint64_t e = gg * 4 + 2 * half + i;
+      ix[i] = gg < ngrp && e < n ? idx[e] : -1;
+    }
+  };
+  auto load_rec = [&](const int32_t (&ix)[2], uint4 (&pv)[2][kVq], uint4 (&pq)[2]) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      if (ix[i] >= 0) {
+        const uint4 *rs = reinterpret_cast<const uint4 *>(Rin + (uint64_t)(uint32_t)ix[i] * kRec);
 #pragma unroll
         for (int q = 0; q < kVq; q++) pv[i][q] = rs[q];
         pq[i] = rs[kRecPerm / 4];
       }
     }
-    // perm masks: one block per call covers the 4 elements
-    uint32_t pm[3][4];
-    group_masks<uint32_t, true, kCopies>(A, rk, np, sp, e0, pm);
+  };
+  int32_t ci[2], ni[2];
+  uint4 cv[2][kVq], cq[2];
+  load_idx(g, ci);
+  load_rec(ci, cv, cq);
+  load_idx(g + gstride, ni);
+  for (; g < ngrp; g += gstride) {
+    const uint64_t e0 = g * 4;
+    // next group's records and the index after it, in flight during the AES
+    uint4 xv[2][kVq], xq[2];
+    load_rec(ni, xv, xq);
+    int32_t xi[2];
+    load_idx(g + 2 * gstride, xi);
+    // perm masks: this lane's call (= half) for the group's 4 elements
+    uint4 mine;
+    {
+      uint64_t off;
+      uint32_t ctr, bib;
+      sched_locate(sp, e0, half, off, ctr, bib);
+      mine = aes_ctr_block<kCopies>(A, rk, np + off, ctr);
+    }
+    const uint4 other = make_uint4(__shfl_xor(mine.x, 1, 64), __shfl_xor(mine.y, 1, 64),
+                                   __shfl_xor(mine.z, 1, 64), __shfl_xor(mine.w, 1, 64));
+    const uint4 c0 = half ? other : mine, c1 = half ? mine : other;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (i >= ne) break;
-      const uint64_t e = e0 + i;
+    for (int i = 0; i < 2; i++) {
+      const uint64_t e = e0 + 2 * half + i;
+      if (e >= n) continue;
+      const uint32_t wsel = 2 * half + i;
+      const uint32_t m0 = wsel == 0 ? c0.x : wsel == 1 ? c0.y : wsel == 2 ? c0.z : c0.w;
+      const uint32_t m1 = wsel == 0 ? c1.x : wsel == 1 ? c1.y : wsel == 2 ? c1.z : c1.w;
+      const uint32_t m2 = 0u - m0 - m1;
       // v masks: element e's bytes of calls 0 and 1
       uint32_t r[2][L];
 #pragma unroll
@@ -326,10 +360,10 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       uint32_t w[kRec];
 #pragma unroll
       for (int q = 0; q < kVq; q++) {
-        w[4 * q] = pv[i][q].x;
-        w[4 * q + 1] = pv[i][q].y;
-        w[4 * q + 2] = pv[i][q].z;
-        w[4 * q + 3] = pv[i][q].w;
+        w[4 * q] = cv[i][q].x;
+        w[4 * q + 1] = cv[i][q].y;
+        w[4 * q + 2] = cv[i][q].z;
+        w[4 * q + 3] = cv[i][q].w;
       }
 #pragma unroll
       for (int q = 4 * kVq; q < kRec; q++) w[q] = 0u;
@@ -339,9 +373,9 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
         w[1 * L + q] ^= r[1][q];
         w[2 * L + q] ^= r[0][q] ^ r[1][q];
       }
-      w[kRecPerm + 0] = pq[i].x + pm[0][i];
-      w[kRecPerm + 1] = pq[i].y + pm[1][i];
-      w[kRecPerm + 2] = pq[i].z + pm[2][i];
+      w[kRecPerm + 0] = cq[i].x + m0;
+      w[kRecPerm + 1] = cq[i].y + m1;
+      w[kRecPerm + 2] = cq[i].z + m2;
       if (SCATTER) {
         // open the permuted perm and put the permuted record at its position
         // (whole: a partial write of its last 32-byte piece would cost a
@@ -361,6 +395,13 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
         for (int q = 0; q < kRec / 4; q++)
           o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
       }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+      for (int q = 0; q < kVq; q++) cv[i][q] = xv[i][q];
+      cq[i] = xq[i];
+      ni[i] = xi[i];
     }
   }
 }
