@@ -21,8 +21,8 @@
 //     rank order.  Candidate a is accepted for sure if v < k - a (at most a
 //     acceptances precede it); otherwise it is "ambiguous" and is written, in
 //     rank order too, to the resolver's list (a second look-back).  The same
-//     launch first commits the previously resolved window: accepted candidate
-//     a becomes r_{i0 + a - #rejected before a};
+//     launch also commits the previously resolved window (tasks taken after
+//     the tiles): accepted candidate a becomes r_{i0 + a - #rejected before a};
 //   * k_gp_resolve: one workgroup per chain decides the ambiguous candidates
 //     exactly (a sure-reject test, then an in-order walk of the rest), builds
 //     the sorted list of rejected ranks, and advances the chain (next window,
@@ -82,7 +82,9 @@ struct ChainState {
 
 struct ChainBufs {
   ChainState *st;      // [3]
-  uint64_t *cand;      // [2][3][cand_cap] candidates in rank order: (draw offset << 32) | v
+  uint32_t *cand_v;    // [2][3][cand_cap] candidates in rank order: v
+  uint32_t *cand_d;    // [2][3][cand_cap] ... and the draw offset in the window, written
+                       // only when the window can end the permutation (W >= k)
   uint32_t cand_cap;
   uint64_t *stat_c;    // [3][kMaxTiles] look-back status of the candidate counts
   uint64_t *stat_a;    // [3][kMaxTiles] ... of the ambiguous counts
@@ -259,7 +261,7 @@ __device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, u
   const ChainState &st = B.st[chain];
   const uint32_t C = st.c_C, R = st.c_R, i0 = st.c_i0, kstart = n - i0;
   if (b * 1024 >= C) return;
-  const uint64_t *cand = B.cand + ((size_t)st.c_buf * 3 + chain) * B.cand_cap;
+  const uint32_t *cand = B.cand_v + ((size_t)st.c_buf * 3 + chain) * B.cand_cap;
   const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
   uint32_t *r = B.r_out + ((size_t)chain * count + st.c_k) * (size_t)n;
   const uint32_t base = b * 1024;
@@ -267,7 +269,7 @@ __device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, u
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const uint32_t f = base + 64 * j + lane;
-    v[j] = f < C ? (uint32_t)cand[f] : 0;
+    v[j] = f < C ? cand[f] : 0;
   }
   const uint32_t lo0 = wave_lower_bound(rej, R, base);
   if (lane < 16) s_mask[lane] = 0;
@@ -343,15 +345,17 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
   const uint64_t below = lanes_below();
   const uint32_t maskN = mask + 1;
 
-  // tasks, in ticket order: commit chunks (16 blocks of 1024 ranks, one per
-  // wave) of the windows resolved last, then the tiles of the current windows
+  // tasks, in ticket order: the tiles of the current windows, then commit
+  // chunks (16 blocks of 1024 ranks, one per wave) of the windows resolved
+  // last, which workgroups without a tile (or done with theirs) take while
+  // the tiles' look-backs wait
   uint32_t nc[3], nt[3];
   for (int c = 0; c < 3; c++) {
     nc[c] = B.st[c].c_valid ? (B.st[c].c_C + 16383) / 16384 : 0u;
     nt[c] = draws_on ? ntiles_of(B.st[c], n, maskN) : 0u;
   }
-  const uint32_t ncom = nc[0] + nc[1] + nc[2];
-  const uint32_t total = ncom + nt[0] + nt[1] + nt[2];
+  const uint32_t ntile = nt[0] + nt[1] + nt[2];
+  const uint32_t total = ntile + nc[0] + nc[1] + nc[2];
   // s_next holds the next task, taken one task ahead: a tile takes its
   // successor's ticket while its first look-back waits anyway
   if (threadIdx.x == 0 && total > 0) s_next = atomicAdd(&B.ctr[0], 1u);
@@ -362,10 +366,10 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     const uint32_t task = s_next;
     __syncthreads();  // read by all before it is rewritten
     if (task >= total) break;
-    if (task < ncom) {
+    if (task >= ntile) {
       int chain;
       uint32_t chunk;
-      task_tile(nc, task, chain, chunk);
+      task_tile(nc, task - ntile, chain, chunk);
       commit_block(B, chain, chunk * (kGpThreads / 64) + wv, n, count, s_mask[wv]);
       if (threadIdx.x == 0) s_next = atomicAdd(&B.ctr[0], 1u);
       continue;
@@ -377,13 +381,15 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     AesCtx<COPIES> A(lds);
     int chain;
     uint32_t tile;
-    task_tile(nt, task - ncom, chain, tile);
+    task_tile(nt, task, chain, tile);
     ChainState &st = B.st[chain];
     const uint32_t kstart = n - st.i, W = st.W, seq = st.seq;
     const bool sparse = sparse_window(st, n, maskN);
     const uint32_t *rk = rk_groups + 60 * chain;
-    uint64_t *cand = B.cand + ((size_t)st.buf * 3 + chain) * B.cand_cap;
+    uint32_t *cand_v = B.cand_v + ((size_t)st.buf * 3 + chain) * B.cand_cap;
+    uint32_t *cand_d = B.cand_d + ((size_t)st.buf * 3 + chain) * B.cand_cap;
     const uint32_t cap = B.cand_cap;
+    const bool can_end = W >= kstart;
     uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
     int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
     uint32_t v[kDrawsPerThread];
@@ -456,7 +462,10 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
         const bool in = i < ns;
         const uint32_t d = in ? s_sd[wv][i] : 0u, x = in ? s_sv[wv][i] : 0u;
         const uint32_t a = run + i;
-        if (in && a < cap) cand[a] = ((uint64_t)d << 32) | x;
+        if (in && a < cap) {
+          cand_v[a] = x;
+          if (can_end) cand_d[a] = d;
+        }
         wa += __builtin_popcountll(__ballot(in && a + x >= kstart));
       }
     } else {
@@ -476,7 +485,10 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
         for (int q = 0; q < 4; q++) {
           const uint32_t x = v[4 * j + q];
           const bool is_c = x < kstart;
-          if (is_c && a < cap) cand[a] = ((uint64_t)(d + q) << 32) | x;
+          if (is_c && a < cap) {
+            cand_v[a] = x;
+            if (can_end) cand_d[a] = d + q;
+          }
           wa += __builtin_popcountll(__ballot(is_c && a + x >= kstart));
           a += is_c;
         }
@@ -776,8 +788,8 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t
         else qhi = mid;
       }
       const uint32_t a = kstart - 1 + qlo;
-      const uint64_t e = B.cand[((size_t)buf * 3 + chain) * B.cand_cap + a];
-      const uint64_t D_end = st->d0 + (e >> 32) + 1;
+      const uint32_t e = B.cand_d[((size_t)buf * 3 + chain) * B.cand_cap + a];
+      const uint64_t D_end = st->d0 + e + 1;
       const uint32_t k = st->k;
       B.D[(size_t)chain * count + k] = D_end;
       // 4 nonces per 2^20-word refill, one refill up front
@@ -955,7 +967,8 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   const uint32_t cand_cap = (uint32_t)std::min<double>(
       kWinMax, std::max<double>(1 << 16, 1024.0 * std::ceil(std::sqrt((double)n))));
   B.cand_cap = cand_cap;
-  B.cand = (uint64_t *)c->scratch_get("gp_cand", 6ull * cand_cap * 8);
+  B.cand_v = (uint32_t *)c->scratch_get("gp_cand_v", 6ull * cand_cap * 4);
+  B.cand_d = (uint32_t *)c->scratch_get("gp_cand_d", 6ull * cand_cap * 4);
   B.stat_c = (uint64_t *)c->scratch_get("gp_stat_c", 3ull * kMaxTiles * 8);
   B.stat_a = (uint64_t *)c->scratch_get("gp_stat_a", 3ull * kMaxTiles * 8);
   B.ctr = (uint32_t *)c->scratch_get("gp_ctr", 64);
