@@ -252,6 +252,7 @@ int orqg_ctx_rekey(orqg_ctx *c, const uint8_t keys[ORQG_NUM_KEYS][32]) {
   API_BEGIN(c)
   if (!keys) throw Error(ORQG_E_ARG, "ctx_rekey: NULL keys");
   ORQG_HIP(hipStreamSynchronize(c->stream));
+  check_swaps(c);
   ORQG_HIP(hipStreamSynchronize(c->stream2));
   std::memcpy(c->keys, keys, sizeof(c->keys));
   uint32_t rk[ORQG_NUM_KEYS * 60];
@@ -272,6 +273,7 @@ const char *orqg_last_error(orqg_ctx *c) { return c ? c->err.c_str() : "null con
 int orqg_sync(orqg_ctx *c) {
   API_BEGIN(c)
   ORQG_HIP(hipStreamSynchronize(c->stream));
+  check_swaps(c);
   c->timer_flush();
   API_END(c)
 }
@@ -300,6 +302,7 @@ int orqg_dev_to_host(orqg_ctx *c, void *host, const void *dev, size_t bytes) {
   API_BEGIN(c)
   ORQG_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
+  check_swaps(c);
   API_END(c)
 }
 
@@ -413,6 +416,7 @@ int orqg_vec_read(orqg_ctx *c, const orqg_vec *v, void *host) {
   std::vector<uint8_t> aos(3 * n * w);
   ORQG_HIP(hipMemcpyAsync(aos.data(), v->d, aos.size(), hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
+  check_swaps(c);
   uint8_t *h = (uint8_t *)host;
   for (size_t i = 0; i < n; i++)
     for (int k = 0; k < 3; k++) std::memcpy(h + (k * n + i) * w, &aos[(3 * i + k) * w], w);
@@ -443,6 +447,7 @@ int orqg_secret_share(orqg_ctx *c, const void *host_plain, orqg_vec *v) {
 int orqg_open(orqg_ctx *c, const orqg_vec *v, void *host_out) {
   API_BEGIN(c)
   open_vec(c, v, host_out);
+  check_swaps(c);
   API_END(c)
 }
 
@@ -741,9 +746,11 @@ size_t orqg_perm_queue_size(orqg_ctx *c) { return c ? c->queue.size() : 0; }
 int orqg_peek_perm(orqg_ctx *c, size_t index, int group, int32_t *host_out) {
   API_BEGIN(c)
   if (index >= c->queue.size() || group < 0 || group > 2) throw Error(ORQG_E_ARG, "peek_perm");
-  const ShardedPerm &p = c->queue[index];
+  ShardedPerm &p = c->queue[index];
+  perm_wait(c, p);
   ORQG_HIP(hipMemcpyAsync(host_out, p.fwd[group], p.n * 4, hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
+  check_swaps(c);
   API_END(c)
 }
 

@@ -13,22 +13,23 @@
 //   (2) swap resolution: the permutation those swaps produce.
 //
 // (1) walks the draw stream of each group PRG ({0,1}, {0,2}, {1,2}: three
-// independent chains, processed by the same launches) window by window.  In a
-// window that starts with k open positions:
-//   * k_gp_window: keystream -> draws; draws with v >= k can never be accepted
-//     (k only decreases); the others ("candidates") get their rank a in the
-//     window through a decoupled look-back over the tiles and are written in
-//     rank order.  Candidate a is accepted for sure if v < k - a (at most a
-//     acceptances precede it); otherwise it is "ambiguous" and is written, in
-//     rank order too, to the resolver's list (a second look-back).  The same
-//     launch also commits the previously resolved window (tasks taken after
-//     the tiles): accepted candidate a becomes r_{i0 + a - #rejected before a};
-//   * k_gp_resolve: one workgroup per chain decides the ambiguous candidates
-//     exactly (a sure-reject test, then an in-order walk of the rest), builds
-//     the sorted list of rejected ranks, and advances the chain (next window,
+// independent chains, processed by the same launches) window by window.  One
+// launch (k_gp_step) per step s:
+//   * its workgroups 3.. draw window s: keystream -> draws; draws with
+//     v >= k_hi can never be accepted (k only decreases); the others
+//     ("candidates") get their rank a in the window through a decoupled
+//     look-back over the tiles and are written in rank order.  Candidate a is
+//     accepted for sure if v < k_lo - a (at most a acceptances precede it);
+//     otherwise it is "listed" and written, in rank order too, to the
+//     resolver's list (a second look-back).  They also commit the window
+//     resolved at step s - 1 (tasks taken after the tiles): accepted candidate
+//     a becomes r_{i0 + a - #rejected before a};
+//   * its workgroups 0..2 (one per chain) resolve window s - 1 exactly (sure
+//     acceptances and rejections, then an in-order walk of the rest), build
+//     the sorted list of rejected ranks, and advance the chain (next window,
 //     or the next permutation at the refill boundary after the last
 //     acceptance).
-// The window size keeps the expected number of ambiguous candidates within the
+// The window size keeps the expected number of listed candidates within the
 // resolver's capacity; an overflowing window is retried at a quarter of its
 // size, down to kWinMin draws, which always fits.
 //
@@ -62,36 +63,61 @@ constexpr int kStash = 192;                             // candidates per wave
 constexpr uint64_t kRefillWords = 1ull << 20;  // gen_perm buffer: 2^20 uint32
 constexpr uint64_t kWordsPerNonce = 1ull << 18;
 
-struct ChainState {
-  uint64_t nonce;   // nonce where the current permutation's draws start
-  uint64_t d0;      // draw index (inside the current permutation) of the window start
-  uint32_t k;       // index of the permutation being generated
-  uint32_t i;       // acceptances so far in it
-  uint32_t W;       // draws in the current window
+// Windows are drawn and resolved in a two-stage pipeline: step s draws window
+// s while window s - 1 is resolved.  Window s is drawn before the exact number
+// of open positions at its start is known: window s - 1 accepts C - m + A of
+// its C candidates (m listed, A of them accepted), so k lies in
+// [k_lo, k_hi] = n - i - C + m - [L, 0] (i: exact acceptances before window
+// s - 1, L: a bound on A), and window s classifies with those bounds
+// (candidates: v < k_hi; sure acceptances: v < k_lo - a).  A window that may
+// end the permutation or needs a retry is not followed by a speculative one:
+// its chain skips a step and restarts from the exact state; a bound that
+// fails is caught by the resolver, which redraws the window exactly.
+
+// A chain's exact state before window s (ring slot s & 1), written by the
+// resolve launch of step s.
+struct ResState {
+  uint64_t nonce;        // nonce where the current permutation's draws start
+  uint64_t d0;           // restart: draw offset (in the permutation) of the next window
+  uint32_t k;            // index of the permutation being generated
+  uint32_t i;            // acceptances in it before window s
+  uint32_t W;            // restart: draws of the next window
+  uint32_t force_dense;  // restart: dense tiles (a sparse tile's stash overflowed)
   uint32_t done;
-  uint32_t err;
-  uint32_t seq;     // window sequence number (tags the look-back status words)
-  uint32_t buf;     // candidate buffer of the current window
-  uint32_t C_tot;   // candidates in the current window (written by its last tile)
-  uint32_t m_tot;   // ambiguous candidates in it
-  uint32_t ovf;     // a sparse tile's candidate stash overflowed
-  uint32_t force_dense;  // retry the current window with dense tiles
-  // the window just resolved, committed by the next k_gp_window launch
-  uint32_t c_valid, c_i0, c_k, c_C, c_R, c_buf;
+  uint32_t void_next;    // window s was drawn on a state that did not hold: discard it
+  // window s - 1 resolved, committed by the window launch of step s + 1
+  uint32_t c_valid, c_i0, c_k, c_C, c_R, c_cbuf, c_rbuf;
+};
+
+// Window s of a chain (ring slot s & 1), planned by every workgroup of the
+// window launch of step s from slot s - 1 (the same answer everywhere).
+struct WinDesc {
+  uint64_t d0;         // draw offset in the permutation
+  uint32_t active;     // drawn in this step
+  uint32_t W;          // draws
+  uint32_t klo, khi;   // bounds on the open positions at its start
+  uint32_t sparse;     // super-tiles
+  uint32_t nt;         // tiles
+  uint32_t seq;        // look-back tag (step + 1)
+  uint32_t can_end;    // W >= klo: the window may complete the permutation
+  uint32_t C, m;       // candidates, ambiguous candidates (written by the last tile)
+  uint32_t ovf;        // = seq when a sparse tile's stash overflowed
 };
 
 struct ChainBufs {
-  ChainState *st;      // [3]
-  uint32_t *cand_v;    // [2][3][cand_cap] candidates in rank order: v
-  uint32_t *cand_d;    // [2][3][cand_cap] ... and the draw offset in the window, written
-                       // only when the window can end the permutation (W >= k)
+  ResState *rs;        // [2][3]
+  WinDesc *wd;         // [2][3]
+  uint32_t *ctl;       // [0] step, [2] tile ticket, [3] finished workgroups
+  uint32_t *err;       // device-side check flags
+  uint32_t *cand_v;    // [3][3][cand_cap] candidates in rank order: v (buffer s % 3)
+  uint32_t *cand_d;    // [3][3][cand_cap] ... and the draw offset in the window, written
+                       // only when the window can end the permutation
   uint32_t cand_cap;
   uint64_t *stat_c;    // [3][kMaxTiles] look-back status of the candidate counts
   uint64_t *stat_a;    // [3][kMaxTiles] ... of the ambiguous counts
-  uint32_t *ctr;       // [2] tile ticket, finished workgroups
-  uint32_t *amb_a;     // [3][kAmbCap]   ambiguous candidates in rank order: rank
-  int32_t *amb_u;      // [3][kAmbCap]   ... and U = rank - (k - v)
-  uint32_t *rej;       // [3][kAmbCap]   ranks of rejected candidates, ascending
+  uint32_t *amb_a;     // [2][3][kAmbCap] ambiguous candidates in rank order: rank
+  int32_t *amb_u;      // [2][3][kAmbCap] ... and rank - (k_lo - v)
+  uint32_t *rej;       // [2][3][kAmbCap] ranks of rejected candidates, ascending
   uint32_t *r_out;     // [3][count][n]  accepted offsets r_i
   uint64_t *D;         // [3][count]     draws consumed per permutation
   uint64_t *nonce_at;  // [3][count+1]   nonce at the start of each permutation
@@ -111,16 +137,59 @@ __host__ __device__ inline uint32_t choose_window(uint32_t k, uint32_t maskN) {
   return W < kWinMin ? kWinMin : W;
 }
 
-// super-tiles when candidates are rare and the window still gives every
-// workgroup at least one (otherwise dense tiles spread it over more CUs)
-__device__ inline bool sparse_window(const ChainState &s, uint32_t n, uint32_t maskN) {
-  return !s.force_dense && (uint64_t)(n - s.i) * 256 <= maskN &&
-         (uint64_t)s.W >= (uint64_t)kSuperTile * gridDim.x;
-}
-
-__device__ inline uint32_t ntiles_of(const ChainState &s, uint32_t n, uint32_t maskN) {
-  const uint32_t td = sparse_window(s, n, maskN) ? kSuperTile : kTile;
-  return s.done ? 0u : (s.W + td - 1) / td;
+// Plan window s of `chain` from the state of step s - 1.  Speculates (window
+// s - 1 still being resolved) only when window s - 1 can neither complete the
+// permutation (acceptances <= C < k_lo) nor be retried for capacity; a
+// window the resolver discards or retries leaves an exact restart.
+__device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s, uint32_t n,
+                                      uint32_t maskN, uint32_t nwork) {
+  const ResState &r = B.rs[((s - 1) & 1) * 3 + chain];
+  const WinDesc &p = B.wd[((s - 1) & 1) * 3 + chain];
+  WinDesc w;
+  w.active = 0;
+  w.C = w.m = w.ovf = 0;
+  w.seq = s + 1;
+  w.nt = 0;
+  if (r.done) return w;
+  bool force_dense = false;
+  if (p.active && !r.void_next) {
+    if (p.ovf == p.seq || p.m > kAmbCap || p.C > B.cand_cap || p.C >= p.klo)
+      return w;  // wait for the exact state
+    // Window s - 1 accepts C - m + A of its candidates, A = its listed
+    // candidates that are accepted: its sure acceptances listed because of
+    // the bounds (about C (khi - klo) / k) and the accepted undecided ones (at
+    // most the undecided, about C^3 / (6 k^2)).  k_lo assumes A below a bound
+    // with ample margin; the resolver checks it and redraws if it fails.
+    const double kp = (double)(n - r.i), Cd = (double)p.C;
+    const double mu = Cd * Cd * Cd / (6.0 * kp * kp) + Cd * (double)(p.khi - p.klo) / kp;
+    const double Lf = mu + 8.0 * sqrt(mu) + 256.0;
+    const uint32_t L = Lf >= (double)p.m ? p.m : (uint32_t)Lf;
+    // the undecided items a bound L adds to the next resolve (about m L / k)
+    // must leave room in its list
+    if (mu + (double)p.m * (double)L / kp > (double)(kL2Cap / 2)) return w;
+    w.khi = n - (r.i + p.C) + p.m;
+    w.klo = w.khi - L;
+    w.d0 = p.d0 + p.W;
+    w.W = choose_window(w.khi, maskN);
+  } else {
+    w.klo = w.khi = n - r.i;
+    w.d0 = r.d0;
+    w.W = r.W;
+    force_dense = r.force_dense != 0;
+  }
+  w.active = 1;
+  // super-tiles when candidates are rare and the window still gives every
+  // workgroup at least one (otherwise dense tiles spread it over more CUs)
+  w.sparse = !force_dense && (uint64_t)w.khi * 256 <= maskN &&
+             (uint64_t)w.W >= (uint64_t)kSuperTile * nwork;
+  const uint32_t td = w.sparse ? kSuperTile : kTile;
+  // whole rounds of tiles over the drawing workgroups (a chain may draw
+  // alone in a step); retried windows below one round keep their size
+  const uint64_t round = (uint64_t)td * nwork;
+  if (w.W >= round) w.W = (uint32_t)(w.W / round * round);
+  w.can_end = w.W >= w.klo;
+  w.nt = (w.W + td - 1) / td;
+  return w;
 }
 
 // exclusive block scan of one value per thread (blockDim multiple of 64)
@@ -254,15 +323,14 @@ __device__ inline uint32_t wave_lower_bound(const uint32_t *__restrict__ a, uint
 // wave takes 1024 consecutive ranks: the rejected ones among them (a slice of
 // the ascending rejected list, found by a wave search) become a 1024-bit mask
 // in LDS, and lane l handles ranks base + 64 j + l, so loads and stores coalesce.
-__device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, uint32_t n,
-                                    uint32_t count, unsigned long long *s_mask) {
+__device__ inline void commit_block(const ChainBufs &B, const ResState &st, int chain, uint32_t b,
+                                    uint32_t n, uint32_t count, unsigned long long *s_mask) {
   const int lane = threadIdx.x & 63;
   const uint64_t below = lanes_below();
-  const ChainState &st = B.st[chain];
   const uint32_t C = st.c_C, R = st.c_R, i0 = st.c_i0, kstart = n - i0;
   if (b * 1024 >= C) return;
-  const uint32_t *cand = B.cand_v + ((size_t)st.c_buf * 3 + chain) * B.cand_cap;
-  const uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
+  const uint32_t *cand = B.cand_v + ((size_t)st.c_cbuf * 3 + chain) * B.cand_cap;
+  const uint32_t *rej = B.rej + ((size_t)st.c_rbuf * 3 + chain) * kAmbCap;
   uint32_t *r = B.r_out + ((size_t)chain * count + st.c_k) * (size_t)n;
   const uint32_t base = b * 1024;
   uint32_t v[16];
@@ -310,15 +378,15 @@ __device__ inline void commit_block(const ChainBufs &B, int chain, uint32_t b, u
 // acceptances precede it); otherwise it is ambiguous and goes to the resolver.
 template <int COPIES>
 __device__ inline void draw_round(const AesCtx<COPIES> &A, const uint32_t *__restrict__ rk,
-                                  const ChainState &st, uint32_t W, uint32_t mask, uint32_t base,
-                                  uint32_t (&v)[kDrawsPerThread]) {
+                                  uint64_t nonce, uint64_t d0, uint32_t W, uint32_t mask,
+                                  uint32_t base, uint32_t (&v)[kDrawsPerThread]) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     const uint32_t d = base + j * 256 + lane * 4;
     if (d < W) {
-      const uint64_t dabs = st.d0 + d;
-      const uint4 blk = aes_ctr_block<COPIES>(A, rk, st.nonce + dabs / kWordsPerNonce,
+      const uint64_t dabs = d0 + d;
+      const uint4 blk = aes_ctr_block<COPIES>(A, rk, nonce + dabs / kWordsPerNonce,
                                               2 + (uint32_t)((dabs % kWordsPerNonce) / 4));
       v[4 * j + 0] = blk.x & mask;
       v[4 * j + 1] = blk.y & mask;
@@ -331,8 +399,24 @@ __device__ inline void draw_round(const AesCtx<COPIES> &A, const uint32_t *__res
   }
 }
 
+constexpr uint32_t kRows = kAmbCap / kGpThreads;  // 64
+constexpr uint32_t kWaves = kGpThreads / 64;
+// LDS of a resolving workgroup (it uses the AES table space of the step kernel)
+struct ResolveLds {
+  int32_t thr[kL2Cap];
+  uint64_t cnt[kRows * kWaves];  // (sure acceptances << 32 | undecided) before each
+                                 // (row, wave), then rejected before each
+  uint64_t w[kWaves];
+  uint8_t dec[kL2Cap];           // 1 accepted, 2 rejected
+  uint32_t l2;
+};
+
+__device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uint32_t n,
+                               uint32_t count, uint32_t maskN, ResolveLds &L);
+static_assert(sizeof(ResolveLds) <= AesLds<kCopies>::kWords * 4, "resolver LDS exceeds the tables'");
+
 template <int COPIES>
-__global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__restrict__ tab,
+__global__ void __launch_bounds__(kGpThreads) k_gp_step(const uint32_t *__restrict__ tab,
                                                           const uint32_t *__restrict__ rk_groups,
                                                           ChainBufs B, uint32_t n, uint32_t mask,
                                                           uint32_t count, uint32_t draws_on) {
@@ -341,37 +425,71 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
   __shared__ uint32_t s_next, s_off, s_aoff;
   __shared__ unsigned long long s_mask[kGpThreads / 64][16];
   __shared__ uint32_t s_sd[kGpThreads / 64][kStash], s_sv[kGpThreads / 64][kStash];
+  __shared__ WinDesc s_plan[3];
+  __shared__ uint64_t s_nonce[3];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t below = lanes_below();
   const uint32_t maskN = mask + 1;
-
+  const uint32_t step = B.ctl[0];
+  const ResState *rs_prev = B.rs + ((step - 1) & 1) * 3;
+  // workgroups 0..2 resolve window step - 1 (one chain each) on their own
+  // CUs while the others draw window step (draws_on = 0: a commit-only
+  // launch, no step)
+  const uint32_t nres = draws_on ? 3u : 0u, nwork = gridDim.x - nres;
+  if (blockIdx.x < nres) {
+    resolve_window(B, blockIdx.x, step, n, count, maskN, *reinterpret_cast<ResolveLds *>(lds));
+  } else {
+  if (threadIdx.x < 3) {
+    const int c = threadIdx.x;
+    WinDesc w = plan_window(B, c, step, n, maskN, nwork);
+    if (!draws_on) {
+      w.active = 0;
+      w.nt = 0;
+    }
+    s_plan[c] = w;
+    s_nonce[c] = rs_prev[c].nonce;
+    if (draws_on && blockIdx.x == nres) {
+      WinDesc &o = B.wd[(step & 1) * 3 + c];
+      o.d0 = w.d0;
+      o.active = w.active;
+      o.W = w.W;
+      o.klo = w.klo;
+      o.khi = w.khi;
+      o.sparse = w.sparse;
+      o.nt = w.nt;
+      o.seq = w.seq;
+      o.can_end = w.can_end;
+    }
+  }
+  __syncthreads();
   // tasks, in ticket order: the tiles of the current windows, then commit
   // chunks (16 blocks of 1024 ranks, one per wave) of the windows resolved
-  // last, which workgroups without a tile (or done with theirs) take while
-  // the tiles' look-backs wait
+  // at the previous step, which workgroups without a tile (or done with
+  // theirs) take while the tiles' look-backs wait
   uint32_t nc[3], nt[3];
   for (int c = 0; c < 3; c++) {
-    nc[c] = B.st[c].c_valid ? (B.st[c].c_C + 16383) / 16384 : 0u;
-    nt[c] = draws_on ? ntiles_of(B.st[c], n, maskN) : 0u;
+    nc[c] = rs_prev[c].c_valid ? (rs_prev[c].c_C + 16383) / 16384 : 0u;
+    nt[c] = s_plan[c].nt;
   }
   const uint32_t ntile = nt[0] + nt[1] + nt[2];
   const uint32_t total = ntile + nc[0] + nc[1] + nc[2];
   // s_next holds the next task, taken one task ahead: a tile takes its
   // successor's ticket while its first look-back waits anyway
-  if (threadIdx.x == 0 && total > 0) s_next = atomicAdd(&B.ctr[0], 1u);
+  if (threadIdx.x == 0 && total > 0) s_next = atomicAdd(&B.ctl[2], 1u);
   bool tables = false;
   for (;;) {
     if (total == 0) break;
     __syncthreads();
-    const uint32_t task = s_next;
+    const uint32_t task = __builtin_amdgcn_readfirstlane(s_next);  // uniform: scalar registers
     __syncthreads();  // read by all before it is rewritten
     if (task >= total) break;
     if (task >= ntile) {
       int chain;
       uint32_t chunk;
       task_tile(nc, task - ntile, chain, chunk);
-      commit_block(B, chain, chunk * (kGpThreads / 64) + wv, n, count, s_mask[wv]);
-      if (threadIdx.x == 0) s_next = atomicAdd(&B.ctr[0], 1u);
+      commit_block(B, rs_prev[chain], chain, chunk * (kGpThreads / 64) + wv, n, count,
+                   s_mask[wv]);
+      if (threadIdx.x == 0) s_next = atomicAdd(&B.ctl[2], 1u);
       continue;
     }
     if (!tables) {
@@ -382,16 +500,27 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     int chain;
     uint32_t tile;
     task_tile(nt, task, chain, tile);
-    ChainState &st = B.st[chain];
-    const uint32_t kstart = n - st.i, W = st.W, seq = st.seq;
-    const bool sparse = sparse_window(st, n, maskN);
+    const WinDesc &pl = s_plan[chain];
+    // candidates: v < khi; sure acceptances: a + v < klo
+    const uint32_t khi = __builtin_amdgcn_readfirstlane(pl.khi);
+    const uint32_t klo = __builtin_amdgcn_readfirstlane(pl.klo);
+    const uint32_t W = __builtin_amdgcn_readfirstlane(pl.W);
+    const uint32_t seq = __builtin_amdgcn_readfirstlane(pl.seq);
+    const bool sparse = __builtin_amdgcn_readfirstlane(pl.sparse) != 0;
+    const uint64_t d0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pl.d0 >> 32)) << 32) |
+                        __builtin_amdgcn_readfirstlane((uint32_t)pl.d0);
+    const uint64_t nonce =
+        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(s_nonce[chain] >> 32)) << 32) |
+        __builtin_amdgcn_readfirstlane((uint32_t)s_nonce[chain]);
+    WinDesc &wd = B.wd[(step & 1) * 3 + chain];
     const uint32_t *rk = rk_groups + 60 * chain;
-    uint32_t *cand_v = B.cand_v + ((size_t)st.buf * 3 + chain) * B.cand_cap;
-    uint32_t *cand_d = B.cand_d + ((size_t)st.buf * 3 + chain) * B.cand_cap;
+    const size_t cb = (size_t)(step % 3) * 3 + chain;
+    uint32_t *cand_v = B.cand_v + cb * B.cand_cap;
+    uint32_t *cand_d = B.cand_d + cb * B.cand_cap;
     const uint32_t cap = B.cand_cap;
-    const bool can_end = W >= kstart;
-    uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
-    int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
+    const bool can_end = __builtin_amdgcn_readfirstlane(pl.can_end) != 0;
+    uint32_t *amb_a = B.amb_a + ((size_t)(step & 1) * 3 + chain) * kAmbCap;
+    int32_t *amb_u = B.amb_u + ((size_t)(step & 1) * 3 + chain) * kAmbCap;
     uint32_t v[kDrawsPerThread];
     uint32_t wc = 0;  // this wave's candidates
     uint32_t wbase;
@@ -399,14 +528,14 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
       wbase = tile * kSuperTile + (uint32_t)wv * (kSuperTile / (kGpThreads / 64));
       for (uint32_t r = 0; r < kSuperRounds; r++) {
         const uint32_t rbase = wbase + r * 2048;
-        draw_round<COPIES>(A, rk, st, W, mask, rbase, v);
+        draw_round<COPIES>(A, rk, nonce, d0, W, mask, rbase, v);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           uint64_t b[4];
           uint32_t pre = 0;
 #pragma unroll
           for (int q = 0; q < 4; q++) {
-            b[q] = __ballot(v[4 * j + q] < kstart);
+            b[q] = __ballot(v[4 * j + q] < khi);
             pre += __builtin_popcountll(b[q] & below);
           }
           if (!(b[0] | b[1] | b[2] | b[3])) continue;
@@ -415,7 +544,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
 #pragma unroll
           for (int q = 0; q < 4; q++) {
             const uint32_t x = v[4 * j + q];
-            if (x < kstart) {
+            if (x < khi) {
               if (i < (uint32_t)kStash) {
                 s_sd[wv][i] = d + q;
                 s_sv[wv][i] = x;
@@ -427,12 +556,12 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
           for (int q = 0; q < 4; q++) wc += __builtin_popcountll(b[q]);
         }
       }
-      if (wc > (uint32_t)kStash && lane == 0) atomicOr(&st.ovf, 1u);  // window retried dense
+      if (wc > (uint32_t)kStash && lane == 0) wd.ovf = seq;  // window retried dense
     } else {
       wbase = tile * kTile + (uint32_t)wv * 2048;
-      draw_round<COPIES>(A, rk, st, W, mask, wbase, v);
+      draw_round<COPIES>(A, rk, nonce, d0, W, mask, wbase, v);
 #pragma unroll
-      for (int e = 0; e < kDrawsPerThread; e++) wc += __builtin_popcountll(__ballot(v[e] < kstart));
+      for (int e = 0; e < kDrawsPerThread; e++) wc += __builtin_popcountll(__ballot(v[e] < khi));
     }
     // candidate counts per wave, tile offset by look-back (wave 0 also takes
     // the next ticket meanwhile)
@@ -440,17 +569,17 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     __syncthreads();
     if (wv == 0) {
       uint32_t tk = 0;
-      if (lane == 0) tk = atomicAdd(&B.ctr[0], 1u);
+      if (lane == 0) tk = atomicAdd(&B.ctl[2], 1u);
       uint32_t cnt = wave_sum(lane < kGpThreads / 64 ? s_wc[lane] : 0u);
-      uint32_t off = look_back(B.stat_c + (size_t)chain * kMaxTiles, tile, seq, cnt, &st.err);
+      uint32_t off = look_back(B.stat_c + (size_t)chain * kMaxTiles, tile, seq, cnt, B.err);
       if (lane == 0) {
         s_next = tk;
         s_off = off;
-        if (tile == nt[chain] - 1) st.C_tot = off + cnt;
+        if (tile == nt[chain] - 1) wd.C = off + cnt;
       }
     }
     __syncthreads();
-    uint32_t run = s_off;
+    uint32_t run = __builtin_amdgcn_readfirstlane(s_off);
     for (int w = 0; w < wv; w++) run += s_wc[w];
     const uint32_t run0 = run;
     // candidate records in rank order, ambiguous counts
@@ -466,7 +595,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
           cand_v[a] = x;
           if (can_end) cand_d[a] = d;
         }
-        wa += __builtin_popcountll(__ballot(in && a + x >= kstart));
+        wa += __builtin_popcountll(__ballot(in && a + x >= klo));
       }
     } else {
 #pragma unroll
@@ -475,7 +604,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
         uint32_t pre = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          b[q] = __ballot(v[4 * j + q] < kstart);
+          b[q] = __ballot(v[4 * j + q] < khi);
           pre += __builtin_popcountll(b[q] & below);
         }
         if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
@@ -484,12 +613,12 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const uint32_t x = v[4 * j + q];
-          const bool is_c = x < kstart;
+          const bool is_c = x < khi;
           if (is_c && a < cap) {
             cand_v[a] = x;
             if (can_end) cand_d[a] = d + q;
           }
-          wa += __builtin_popcountll(__ballot(is_c && a + x >= kstart));
+          wa += __builtin_popcountll(__ballot(is_c && a + x >= klo));
           a += is_c;
         }
 #pragma unroll
@@ -500,15 +629,15 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
     __syncthreads();
     if (wv == 0) {
       uint32_t cnt = wave_sum(lane < kGpThreads / 64 ? s_wa[lane] : 0u);
-      uint32_t off = look_back(B.stat_a + (size_t)chain * kMaxTiles, tile, seq, cnt, &st.err);
+      uint32_t off = look_back(B.stat_a + (size_t)chain * kMaxTiles, tile, seq, cnt, B.err);
       if (lane == 0) {
         s_aoff = off;
-        if (tile == nt[chain] - 1) st.m_tot = off + cnt;
+        if (tile == nt[chain] - 1) wd.m = off + cnt;
       }
     }
     __syncthreads();
     // ambiguous records, in rank order
-    uint32_t arun = s_aoff;
+    uint32_t arun = __builtin_amdgcn_readfirstlane(s_aoff);
     for (int w = 0; w < wv; w++) arun += s_wa[w];
     run = run0;
     if (sparse) {
@@ -518,12 +647,12 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
         const bool in = i < ns;
         const uint32_t x = in ? s_sv[wv][i] : 0u;
         const uint32_t a = run + i;
-        const bool amb = in && a + x >= kstart;
+        const bool amb = in && a + x >= klo;
         const uint64_t ab = __ballot(amb);
         const uint32_t r = arun + __builtin_popcountll(ab & below);
         if (amb && r < kAmbCap) {
           amb_a[r] = a;
-          amb_u[r] = (int32_t)(a + x - kstart);
+          amb_u[r] = (int32_t)(a + x - klo);
         }
         arun += __builtin_popcountll(ab);
       }
@@ -534,7 +663,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
         uint32_t pre = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          b[q] = __ballot(v[4 * j + q] < kstart);
+          b[q] = __ballot(v[4 * j + q] < khi);
           pre += __builtin_popcountll(b[q] & below);
         }
         if (!(b[0] | b[1] | b[2] | b[3])) continue;  // no candidate in this row
@@ -544,8 +673,8 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const uint32_t x = v[4 * j + q];
-          const bool is_c = x < kstart;
-          amb[q] = is_c && a + x >= kstart;
+          const bool is_c = x < khi;
+          amb[q] = is_c && a + x >= klo;
           const uint64_t ab = __ballot(amb[q]);
           apre += __builtin_popcountll(ab & below);
           a += is_c;
@@ -559,11 +688,11 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
           if (amb[q]) {
             if (r < kAmbCap) {
               amb_a[r] = a;
-              amb_u[r] = (int32_t)(a + x - kstart);
+              amb_u[r] = (int32_t)(a + x - klo);
             }
             r++;
           }
-          a += x < kstart;
+          a += x < khi;
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -573,246 +702,292 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_window(const uint32_t *__rest
       }
     }
   }
-  // the last workgroup out resets the ticket and retires the committed windows
+  }
+  // the last workgroup out resets the ticket, retires the committed windows
+  // and advances the step
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    const uint32_t fin = atomicAdd(&B.ctr[1], 1u);
+    const uint32_t fin = atomicAdd(&B.ctl[3], 1u);
     if (fin == gridDim.x - 1) {
-      B.ctr[0] = 0;
-      B.ctr[1] = 0;
-      for (int c = 0; c < 3; c++) B.st[c].c_valid = 0;
+      B.ctl[2] = 0;
+      B.ctl[3] = 0;
+      for (int c = 0; c < 3; c++) B.rs[((step - 1) & 1) * 3 + c].c_valid = 0;
+      if (draws_on) B.ctl[0] = step + 1;
       __threadfence();
     }
   }
 }
 
 // ---- exact resolution of the ambiguous candidates, chain advance -------------------
-// Ambiguous candidate t (list index t, rank a_t, T_t = k - v_t <= a_t) is
-// accepted iff fewer than T_t acceptances precede it, i.e. iff more than
-// U_t = a_t - T_t >= 0 of the ambiguous candidates before it are rejected.
-// At most t of them are, so t <= U_t means rejected for sure.  The first
-// stage applies that test to all ambiguous candidates, streaming global
-// memory row by row (item t = row * 1024 + thread, compacted with ballots).
-// Every earlier candidate of an undecided one is either rejected for sure or
-// undecided, so an undecided item is accepted iff fewer than T' = t - U_t of
-// the undecided items before it are accepted: the second stage walks that
-// short list in order (in LDS, one wave, 64 items per step).
-__global__ void __launch_bounds__(kGpThreads) k_gp_resolve(ChainBufs B, uint32_t n,
-                                                           uint32_t count, uint32_t maskN) {
-  constexpr uint32_t kRows = kAmbCap / kGpThreads;  // 64
-  __shared__ int32_t s_thr[kL2Cap];
-  __shared__ uint8_t s_dec[kL2Cap];   // 1 accepted, 2 rejected
-  __shared__ uint32_t s_cnt[kRows * (kGpThreads / 64)];
-  __shared__ uint32_t s_w[kGpThreads / 64];
-  __shared__ uint32_t s_l2;
-  const int chain = blockIdx.x;
+// With k the exact open positions at the window start, listed candidate t
+// (list index t, rank a_t, T_t = k - v_t) is accepted iff fewer than T_t
+// acceptances precede it, i.e. iff more than U_t = a_t - T_t of the
+// candidates before it are rejected; only listed ones can be.  U_t < 0 means
+// accepted for sure (listed because the window was drawn with k_lo < k), and
+// at most t of the listed items before t are rejected, so t <= U_t means
+// rejected for sure.  An undecided item t is accepted iff fewer than
+// T' = t - SA_t - U_t of the undecided items before it are accepted (SA_t:
+// sure acceptances listed before it; T' <= 0 rejects): the walk decides that
+// short list in order (in LDS, one wave, 64 items per step).  Item t of the
+// list is handled by thread t % 1024 as its row t / 1024; loads go out 16
+// rows at a time.
+__device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uint32_t n,
+                               uint32_t count, uint32_t maskN, ResolveLds &L) {
+  constexpr uint32_t kHalf = 16;  // rows per batch of loads
+  constexpr uint32_t kNb = kRows / kHalf;
+  int32_t *s_thr = L.thr;
+  uint8_t *s_dec = L.dec;
+  uint64_t *s_cnt = L.cnt;
+  uint64_t *s_w = L.w;
+  uint32_t &s_l2 = L.l2;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t below = lanes_below();
-  ChainState *st = B.st + chain;
-  if (st->done) return;
-  const uint32_t ovf = st->ovf;
-  __syncthreads();
-  if (ovf) {
-    // a sparse tile's stash overflowed: the same window again, dense
-    if (threadIdx.x == 0) {
-      st->ovf = 0;
-      st->force_dense = 1;
-      st->c_valid = 0;
-      st->seq++;
+  const ResState &r = B.rs[((step - 1) & 1) * 3 + chain];
+  const WinDesc &p = B.wd[((step - 1) & 1) * 3 + chain];
+  ResState &o = B.rs[(step & 1) * 3 + chain];
+  const uint32_t kx = n - r.i;  // exact open positions at the window start
+  // nothing drawn, or drawn on a state that did not hold: carry the state over
+  bool skip = r.done || !p.active || r.void_next;
+  if (threadIdx.x == 0) {
+    o = r;
+    o.c_valid = 0;
+    o.void_next = 0;
+    if (!skip && (p.ovf == p.seq || p.klo > kx || kx > p.khi)) {
+      // a sparse tile's stash overflowed (the same window again, dense), or
+      // the bounds it was drawn with do not hold (the same window again,
+      // exact); the window drawn after it is discarded
+      o.d0 = p.d0;
+      o.W = p.W;
+      o.force_dense = p.ovf == p.seq;
+      o.void_next = 1;
     }
-    return;
   }
-  const uint32_t kstart = n - st->i;
-  const uint32_t *amb_a = B.amb_a + (size_t)chain * kAmbCap;
-  const int32_t *amb_u = B.amb_u + (size_t)chain * kAmbCap;
-  uint32_t *rej = B.rej + (size_t)chain * kAmbCap;
-  const uint32_t m = st->m_tot, C = st->C_tot;
+  if (!skip && (p.ovf == p.seq || p.klo > kx || kx > p.khi)) skip = true;
+  const int32_t delta = (int32_t)(kx - p.klo);
+  const size_t ab = ((size_t)((step - 1) & 1) * 3 + chain) * kAmbCap;
+  const uint32_t *amb_a = B.amb_a + ab;
+  const int32_t *amb_u = B.amb_u + ab;
+  uint32_t *rej = B.rej + ab;
+  const uint32_t m = p.m, C = p.C;
   const bool over = m > kAmbCap || C > B.cand_cap;
-  const uint32_t rows = over ? 0 : (m + kGpThreads - 1) / kGpThreads;
-  // level 1: bit q of undm = item q * 1024 + threadIdx.x undecided, of rejm rejected
-  uint64_t undm = 0, rejm = 0;
-  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
-    int32_t u[8];
+  const uint32_t rows = skip || over ? 0 : (m + kGpThreads - 1) / kGpThreads;
+  const uint32_t nent = rows * kWaves;
+  // level 1: bit q of undm = item q * 1024 + threadIdx.x undecided, of rejm
+  // rejected, of sam accepted for sure
+  uint64_t undm = 0, rejm = 0, sam = 0;
+#pragma unroll 1
+  for (uint32_t h = 0; h < kNb; h++) {
+    if (h * kHalf >= rows) break;
+    int32_t u[kHalf];
 #pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t t = (q0 + j) * kGpThreads + threadIdx.x;
-      u[j] = t < m ? amb_u[t] : 0;
+    for (uint32_t j = 0; j < kHalf; j++) {
+      const uint32_t q = h * kHalf + j, t = q * kGpThreads + threadIdx.x;
+      u[j] = q < rows && t < m ? amb_u[t] - delta : 0;
     }
 #pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t q = q0 + j, t = q * kGpThreads + threadIdx.x;
+    for (uint32_t j = 0; j < kHalf; j++) {
+      const uint32_t q = h * kHalf + j, t = q * kGpThreads + threadIdx.x;
       if (q >= rows) break;
-      const bool in = t < m, rj = in && (int32_t)t <= u[j], ud = in && !rj;
+      const bool in = t < m, sa = in && u[j] < 0, rj = in && !sa && (int32_t)t <= u[j],
+                 ud = in && !sa && !rj;
       rejm |= (uint64_t)rj << q;
       undm |= (uint64_t)ud << q;
-      const uint64_t b = __ballot(ud);
-      if (lane == 0) s_cnt[q * (kGpThreads / 64) + wv] = __builtin_popcountll(b);
+      sam |= (uint64_t)sa << q;
+      const uint64_t bu = __ballot(ud), bs = __ballot(sa);
+      if (lane == 0)
+        s_cnt[q * kWaves + wv] =
+            ((uint64_t)__builtin_popcountll(bs) << 32) | (uint64_t)__builtin_popcountll(bu);
     }
   }
-  // undecided positions: exclusive scan over (row, wave) counts
+  // undecided positions and sure acceptances before each (row, wave): one
+  // exclusive scan over both counts
   __syncthreads();
-  const uint32_t nent = rows * (kGpThreads / 64);
   {
-    const uint32_t x = threadIdx.x < nent ? s_cnt[threadIdx.x] : 0u;
-    uint32_t l2;
-    const uint32_t ex = block_scan(x, s_w, l2);
+    const uint64_t x = threadIdx.x < nent ? s_cnt[threadIdx.x] : 0ull;
+    uint64_t tot;
+    const uint64_t ex = block_scan<uint64_t>(x, s_w, tot);
     __syncthreads();
     if (threadIdx.x < nent) s_cnt[threadIdx.x] = ex;
-    if (threadIdx.x == 0) s_l2 = l2;
+    if (threadIdx.x == 0) s_l2 = (uint32_t)tot;
     __syncthreads();
   }
-  if (over || s_l2 > kL2Cap) {
+  if (!skip && (over || s_l2 > kL2Cap)) {
     // too many candidates or ambiguous ones: retry this window at a quarter size
     if (threadIdx.x == 0) {
-      uint32_t w = st->W / 4;
-      st->W = w >= kTile ? w / kTile * kTile : (w >= kWinMin ? w / kWinMin * kWinMin : kWinMin);
-      st->c_valid = 0;
-      st->seq++;
+      const uint32_t w = p.W / 4;
+      o.d0 = p.d0;
+      o.W = w >= kTile ? w / kTile * kTile : (w >= kWinMin ? w / kWinMin * kWinMin : kWinMin);
+      o.force_dense = 0;
+      o.void_next = 1;
     }
-    return;
+    skip = true;
   }
-  // undecided items continue as a type A list with T' = t - U
-  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
-    int32_t u[8];
+  if (!skip) {
+    // undecided items with their thresholds T' = t - SA_t - U_t
+#pragma unroll 1
+    for (uint32_t h = 0; h < kNb; h++) {
+      if (h * kHalf >= rows) break;
+      int32_t u[kHalf];
 #pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t q = q0 + j;
-      u[j] = q < rows && ((undm >> q) & 1) ? amb_u[q * kGpThreads + threadIdx.x] : 0;
+      for (uint32_t j = 0; j < kHalf; j++) {
+        const uint32_t q = h * kHalf + j, t = q * kGpThreads + threadIdx.x;
+        u[j] = q < rows && ((undm >> q) & 1) ? amb_u[t] - delta : 0;
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kHalf; j++) {
+        const uint32_t q = h * kHalf + j;
+        if (q >= rows) break;
+        const bool ud = (undm >> q) & 1;
+        const uint64_t bu = __ballot(ud), bs = __ballot((sam >> q) & 1);
+        if (ud) {
+          const uint32_t t = q * kGpThreads + threadIdx.x;
+          const uint64_t e = s_cnt[q * kWaves + wv];
+          const uint32_t pos = (uint32_t)e + __builtin_popcountll(bu & below);
+          const uint32_t sat = (uint32_t)(e >> 32) + __builtin_popcountll(bs & below);
+          s_thr[pos] = (int32_t)(t - sat) - u[j];
+          s_dec[pos] = 0;
+        }
+      }
     }
+    // ranks of the first batch's items that may be rejected, in flight
+    // during the walk
+    uint32_t a[kHalf];
 #pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t q = q0 + j;
+    for (uint32_t q = 0; q < kHalf; q++) {
+      const uint32_t t = q * kGpThreads + threadIdx.x;
+      a[q] = q < rows && (((rejm | undm) >> q) & 1) ? amb_a[t] : 0u;
+    }
+    __syncthreads();
+    // The undecided items, in order: item i is accepted iff fewer than T'_i of
+    // the undecided items before it are accepted.  One wave walks the list in
+    // chunks of 64: given the acceptances before the chunk (acc), the chunk's
+    // decisions are the fixed point of d_i = [acc + #(accepted before i in the
+    // chunk) < T'_i], reached by iterating from the optimistic guess with one
+    // ballot per round (item i is exact after round i, so at most 64 rounds; a
+    // repeated ballot is the unique fixed point, i.e. the sequential answer).
+    if (wv == 0) {
+      const uint32_t len = s_l2;
+      uint32_t acc = 0;
+      for (uint32_t c0 = 0; c0 < len; c0 += 64) {
+        const uint32_t i = c0 + lane;
+        const bool in = i < len;
+        const int32_t T = in ? s_thr[i] : 0;
+        uint64_t dm = __ballot(in && (int32_t)acc < T);
+        for (int round = 0; round < 65; round++) {
+          const uint32_t before = acc + (uint32_t)__builtin_popcountll(dm & below);
+          const uint64_t nm = __ballot(in && (int32_t)before < T);
+          if (nm == dm) break;
+          dm = nm;
+        }
+        if (in) s_dec[i] = (dm >> lane) & 1 ? 1 : 2;
+        acc += (uint32_t)__builtin_popcountll(dm);
+      }
+    }
+    __syncthreads();
+    // rejected items: the sure ones and the undecided ones the walk rejected
+#pragma unroll
+    for (uint32_t q = 0; q < kRows; q++) {
       if (q >= rows) break;
       const bool ud = (undm >> q) & 1;
-      const uint64_t b = __ballot(ud);
-      if (!ud) continue;
-      const uint32_t t = q * kGpThreads + threadIdx.x;
-      const uint32_t p = s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below);
-      s_thr[p] = (int32_t)t - u[j];
-      s_dec[p] = 0;
+      const uint64_t bu = __ballot(ud);
+      if (ud && s_dec[(uint32_t)s_cnt[q * kWaves + wv] + __builtin_popcountll(bu & below)] == 2)
+        rejm |= 1ull << q;
     }
-  }
-  __syncthreads();
-  // The undecided items, in order: item i is accepted iff fewer than T'_i of
-  // the undecided items before it are accepted.  One wave walks the list in
-  // chunks of 64: given the acceptances before the chunk (acc), the chunk's
-  // decisions are the fixed point of d_i = [acc + #(accepted before i in the
-  // chunk) < T'_i], reached by iterating from the optimistic guess with one
-  // ballot per round (item i is exact after round i, so at most 64 rounds; a
-  // repeated ballot is the unique fixed point, i.e. the sequential answer).
-  if (wv == 0) {
-    const uint32_t len = s_l2;
-    uint32_t acc = 0;
-    for (uint32_t c0 = 0; c0 < len; c0 += 64) {
-      const uint32_t i = c0 + lane;
-      const bool in = i < len;
-      const int32_t T = in ? s_thr[i] : 0;
-      uint64_t dm = __ballot(in && (int32_t)acc < T);
-      for (int round = 0; round < 65; round++) {
-        const uint32_t before = acc + (uint32_t)__builtin_popcountll(dm & below);
-        const uint64_t nm = __ballot(in && (int32_t)before < T);
-        if (nm == dm) break;
-        dm = nm;
-      }
-      if (in) s_dec[i] = (dm >> lane) & 1 ? 1 : 2;
-      acc += (uint32_t)__builtin_popcountll(dm);
-    }
-  }
-  __syncthreads();
-  // rejected items: level-1 rejects and the undecided ones the walk rejected
-  for (uint32_t q = 0; q < rows; q++) {
-    const bool ud = (undm >> q) & 1;
-    const uint64_t b = __ballot(ud);
-    if (ud) {
-      const uint32_t p = s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below);
-      if (s_dec[p] == 2) rejm |= 1ull << q;
-    }
-  }
-  __syncthreads();
-  for (uint32_t q = 0; q < rows; q++) {
-    const uint64_t b = __ballot((rejm >> q) & 1);
-    if (lane == 0) s_cnt[q * (kGpThreads / 64) + wv] = __builtin_popcountll(b);
-  }
-  __syncthreads();
-  uint32_t R;
-  {
-    const uint32_t x = threadIdx.x < nent ? s_cnt[threadIdx.x] : 0u;
-    const uint32_t ex = block_scan(x, s_w, R);
     __syncthreads();
-    if (threadIdx.x < nent) s_cnt[threadIdx.x] = ex;
-    __syncthreads();
-  }
-  // rejected ranks in ascending order (the ambiguous list is in rank order);
-  // loads go out 8 rows at a time
-  for (uint32_t q0 = 0; q0 < rows; q0 += 8) {
-    uint32_t a[8];
 #pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t q = q0 + j;
-      a[j] = q < rows && ((rejm >> q) & 1) ? amb_a[q * kGpThreads + threadIdx.x] : 0;
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < 8; j++) {
-      const uint32_t q = q0 + j;
+    for (uint32_t q = 0; q < kRows; q++) {
       if (q >= rows) break;
-      const bool rj = (rejm >> q) & 1;
-      const uint64_t b = __ballot(rj);
-      if (rj) rej[s_cnt[q * (kGpThreads / 64) + wv] + __builtin_popcountll(b & below)] = a[j];
+      const uint64_t b = __ballot((rejm >> q) & 1);
+      if (lane == 0) s_cnt[q * kWaves + wv] = __builtin_popcountll(b);
     }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t acc = C - R;
-    if (acc > kstart) atomicOr(&st->err, 4u);
-    const uint32_t buf = st->buf;
-    st->c_valid = 1;
-    st->c_i0 = st->i;
-    st->c_k = st->k;
-    st->c_C = C;
-    st->c_R = R;
-    st->c_buf = buf;
-    st->buf = buf ^ 1u;
-    st->seq++;
-    st->force_dense = 0;
-    if (acc >= kstart) {
-      // the permutation ends here: rank a* of the kstart-th acceptance is
-      // kstart-1 shifted past every rejected rank at or before it (rej is
-      // strictly increasing, so rej[q] - q is non-decreasing: the rejected
-      // ranks passed are those with rej[q] - q <= kstart - 1)
-      uint32_t qlo = 0, qhi = R;
-      while (qlo < qhi) {
-        const uint32_t mid = (qlo + qhi) >> 1;
-        if (rej[mid] - mid <= kstart - 1) qlo = mid + 1;
-        else qhi = mid;
+    __syncthreads();
+    uint64_t R64;
+    {
+      const uint64_t x = threadIdx.x < nent ? s_cnt[threadIdx.x] : 0ull;
+      const uint64_t ex = block_scan<uint64_t>(x, s_w, R64);
+      __syncthreads();
+      if (threadIdx.x < nent) s_cnt[threadIdx.x] = ex;
+      __syncthreads();
+    }
+    const uint32_t R = (uint32_t)R64;
+    // rejected ranks in ascending order (the list is in rank order); the
+    // later rows load their ranks now
+#pragma unroll 1
+    for (uint32_t h = 0; h < kNb; h++) {
+      if (h * kHalf >= rows) break;
+      if (h > 0) {
+#pragma unroll
+        for (uint32_t q = 0; q < kHalf; q++) {
+          const uint32_t qq = h * kHalf + q, t = qq * kGpThreads + threadIdx.x;
+          a[q] = qq < rows && ((rejm >> qq) & 1) ? amb_a[t] : 0u;
+        }
       }
-      const uint32_t a = kstart - 1 + qlo;
-      const uint32_t e = B.cand_d[((size_t)buf * 3 + chain) * B.cand_cap + a];
-      const uint64_t D_end = st->d0 + e + 1;
-      const uint32_t k = st->k;
-      B.D[(size_t)chain * count + k] = D_end;
-      // 4 nonces per 2^20-word refill, one refill up front
-      st->nonce += 4 * (1 + D_end / kRefillWords);
-      st->k = k + 1;
-      st->i = 0;
-      st->d0 = 0;
-      B.nonce_at[(size_t)chain * (count + 1) + k + 1] = st->nonce;
-      if (st->k >= count) st->done = 1;
-      st->W = choose_window(n, maskN);
-    } else {
-      st->i += acc;
-      st->d0 += st->W;
-      st->W = choose_window(n - st->i, maskN);
+#pragma unroll
+      for (uint32_t q = 0; q < kHalf; q++) {
+        const uint32_t qq = h * kHalf + q;
+        if (qq >= rows) break;
+        const bool rj = (rejm >> qq) & 1;
+        const uint64_t b = __ballot(rj);
+        if (rj) rej[(uint32_t)s_cnt[qq * kWaves + wv] + __builtin_popcountll(b & below)] = a[q];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const uint32_t acc = C - R;
+      if (acc > kx) atomicOr(B.err, 4u);
+      o.c_valid = 1;
+      o.c_i0 = r.i;
+      o.c_k = r.k;
+      o.c_C = C;
+      o.c_R = R;
+      o.c_cbuf = (step - 1) % 3;
+      o.c_rbuf = (step - 1) & 1;
+      o.force_dense = 0;
+      if (acc >= kx) {
+        // the permutation ends here: rank a* of the kx-th acceptance is kx-1
+        // shifted past every rejected rank at or before it (rej is strictly
+        // increasing, so rej[q] - q is non-decreasing: the rejected ranks
+        // passed are those with rej[q] - q <= kx - 1)
+        uint32_t qlo = 0, qhi = R;
+        while (qlo < qhi) {
+          const uint32_t mid = (qlo + qhi) >> 1;
+          if (rej[mid] - mid <= kx - 1) qlo = mid + 1;
+          else qhi = mid;
+        }
+        const uint32_t a = kx - 1 + qlo;
+        const uint32_t e = B.cand_d[((size_t)((step - 1) % 3) * 3 + chain) * B.cand_cap + a];
+        const uint64_t D_end = p.d0 + e + 1;
+        const uint32_t k = r.k;
+        B.D[(size_t)chain * count + k] = D_end;
+        // 4 nonces per 2^20-word refill, one refill up front
+        o.nonce = r.nonce + 4 * (1 + D_end / kRefillWords);
+        o.k = k + 1;
+        o.i = 0;
+        o.d0 = 0;
+        B.nonce_at[(size_t)chain * (count + 1) + k + 1] = o.nonce;
+        if (o.k >= count) o.done = 1;
+        o.W = choose_window(n, maskN);
+        o.void_next = 1;
+      } else {
+        o.i = r.i + acc;
+        o.d0 = p.d0 + p.W;
+        o.W = choose_window(n - o.i, maskN);
+      }
     }
   }
 }
 
+// after a batch: all done, check flags, permutations completed per chain (the
+// state the last resolve launch wrote)
 __global__ void k_gp_done(ChainBufs B, uint32_t *flag) {
   if (threadIdx.x == 0) {
-    flag[0] = B.st[0].done & B.st[1].done & B.st[2].done;
-    flag[1] = B.st[0].err | B.st[1].err | B.st[2].err;
-    for (int g = 0; g < 3; g++) flag[2 + g] = B.st[g].k;  // permutations completed
+    const ResState *r = B.rs + ((B.ctl[0] - 1) & 1) * 3;
+    flag[0] = r[0].done & r[1].done & r[2].done;
+    flag[1] = *B.err;
+    for (int g = 0; g < 3; g++) flag[2 + g] = r[g].k;
+    for (int g = 0; g < 3; g++) reinterpret_cast<uint64_t *>(flag + 6)[g] = r[g].nonce;
   }
 }
 
@@ -915,6 +1090,8 @@ __global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r_all, size_t
 }  // namespace
 
 void free_perm(ShardedPerm &p) {
+  if (p.ready) (void)hipEventDestroy(p.ready);
+  p.ready = nullptr;
   for (int g = 0; g < 3; g++) {
     dev_free(p.fwd[g]);
     dev_free(p.inv[g]);
@@ -930,6 +1107,7 @@ void free_perm(ShardedPerm &p) {
 // permutation for the {0,1,2} group, which no operator on this path applies and
 // whose PRG feeds nothing else here, so it is not generated.
 void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
+  check_swaps(c);  // the previous call's swap resolution (its scratch is reused)
   if (!c->queue.empty() && c->stored_size != n_) {
     for (auto &p : c->queue) free_perm(p);
     c->queue.clear();
@@ -961,57 +1139,63 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
 
   TimedRegion tr(c, "gen_perm");
   ChainBufs B;
-  B.st = (ChainState *)c->scratch_get("gp_state", 3 * sizeof(ChainState));
+  B.rs = (ResState *)c->scratch_get("gp_rs", 6 * sizeof(ResState));
+  B.wd = (WinDesc *)c->scratch_get("gp_wd", 6 * sizeof(WinDesc));
+  B.ctl = (uint32_t *)c->scratch_get("gp_ctl", 64);
+  B.err = (uint32_t *)c->scratch_get("gp_err", 16);
   // candidates per window: about 256 sqrt(k) <= 256 sqrt(n); 4x headroom (a
   // window over capacity is retried smaller)
   const uint32_t cand_cap = (uint32_t)std::min<double>(
       kWinMax, std::max<double>(1 << 16, 1024.0 * std::ceil(std::sqrt((double)n))));
   B.cand_cap = cand_cap;
-  B.cand_v = (uint32_t *)c->scratch_get("gp_cand_v", 6ull * cand_cap * 4);
-  B.cand_d = (uint32_t *)c->scratch_get("gp_cand_d", 6ull * cand_cap * 4);
+  B.cand_v = (uint32_t *)c->scratch_get("gp_cand_v", 9ull * cand_cap * 4);
+  B.cand_d = (uint32_t *)c->scratch_get("gp_cand_d", 9ull * cand_cap * 4);
   B.stat_c = (uint64_t *)c->scratch_get("gp_stat_c", 3ull * kMaxTiles * 8);
   B.stat_a = (uint64_t *)c->scratch_get("gp_stat_a", 3ull * kMaxTiles * 8);
-  B.ctr = (uint32_t *)c->scratch_get("gp_ctr", 64);
-  B.amb_a = (uint32_t *)c->scratch_get("gp_amb_a", 3ull * kAmbCap * 4);
-  B.amb_u = (int32_t *)c->scratch_get("gp_amb_u", 3ull * kAmbCap * 4);
-  B.rej = (uint32_t *)c->scratch_get("gp_rej", 3ull * kAmbCap * 4);
+  B.amb_a = (uint32_t *)c->scratch_get("gp_amb_a", 6ull * kAmbCap * 4);
+  B.amb_u = (int32_t *)c->scratch_get("gp_amb_u", 6ull * kAmbCap * 4);
+  B.rej = (uint32_t *)c->scratch_get("gp_rej", 6ull * kAmbCap * 4);
   B.r_out = (uint32_t *)c->scratch_get("gp_r", 3ull * count * n * 4);
   B.D = (uint64_t *)c->scratch_get("gp_D", 3ull * count * 8);
   B.nonce_at = (uint64_t *)c->scratch_get("gp_nonce_at", 3ull * (count + 1) * 8);
-  uint32_t *flag = (uint32_t *)c->scratch_get("gp_flag", 32);
+  uint32_t *flag = (uint32_t *)c->scratch_get("gp_flag", 64);
 
-  ChainState init[3];
+  // the exact state before window 0 sits in ring slot 1 (step -1), with no
+  // window drawn at step -1
+  ResState init[6];
+  WinDesc wd0[6];
   std::memset(init, 0, sizeof(init));
+  std::memset(wd0, 0, sizeof(wd0));
   std::vector<uint64_t> nat(3ull * (count + 1), 0);
   for (int g = 0; g < 3; g++) {
-    init[g].nonce = c->nonce[kGroup0 + g];
-    init[g].W = choose_window(n, maskN);
-    init[g].seq = 1;  // the status words are cleared to seq 0 below
-    nat[(size_t)g * (count + 1)] = init[g].nonce;
+    ResState &r = init[3 + g];
+    r.nonce = c->nonce[kGroup0 + g];
+    r.W = choose_window(n, maskN);
+    nat[(size_t)g * (count + 1)] = r.nonce;
   }
   ORQG_HIP(hipMemsetAsync(B.stat_c, 0, 3ull * kMaxTiles * 8, c->stream));
   ORQG_HIP(hipMemsetAsync(B.stat_a, 0, 3ull * kMaxTiles * 8, c->stream));
-  ORQG_HIP(hipMemsetAsync(B.ctr, 0, 64, c->stream));
-  ORQG_HIP(hipMemcpyAsync(B.st, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  ORQG_HIP(hipMemsetAsync(B.ctl, 0, 64, c->stream));
+  ORQG_HIP(hipMemsetAsync(B.err, 0, 16, c->stream));
+  ORQG_HIP(hipMemcpyAsync(B.rs, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  ORQG_HIP(hipMemcpyAsync(B.wd, wd0, sizeof(wd0), hipMemcpyHostToDevice, c->stream));
   ORQG_HIP(hipMemcpyAsync(B.nonce_at, nat.data(), nat.size() * 8, hipMemcpyHostToDevice, c->stream));
 
-  const unsigned grid = std::max(1, c->num_cu);
-  uint32_t host_flag[5] = {0, 0, 0, 0, 0};
+  // one workgroup per CU: three resolve, the others draw (a workgroup fills a
+  // CU's LDS)
+  const unsigned grid = std::max(4, c->num_cu);
+  uint32_t host_flag[12] = {0};
   const int kBatch = 16;
-  // The window kernels take all their state from device memory, so a batch of
-  // windows is one static launch sequence: it is captured once as a graph.
-  // Each step commits the previously resolved windows, draws and classifies
-  // the current ones; a last commit-only step leaves every resolved window of
-  // the batch committed, so the completed-permutation count read after a
-  // batch covers committed offsets only.
+  // The kernel takes all its state from device memory, so a batch of steps
+  // is one static launch sequence, captured once as a graph: each step draws
+  // window s (committing the windows resolved at step s - 1 as well) and
+  // resolves window s - 1; a last commit-only launch leaves every resolved
+  // window of the batch committed, so the completed-permutation count read
+  // after a batch covers committed offsets.
   auto enqueue_batch = [&]() {
-    for (int w = 0; w <= kBatch; w++) {
-      hipLaunchKernelGGL(k_gp_window<kCopies>, dim3(grid), dim3(kGpThreads), 0, c->stream,
-                         c->d_t0, c->rk(kGroup0), B, n, mask, count, (uint32_t)(w < kBatch));
-      if (w < kBatch)
-        hipLaunchKernelGGL(k_gp_resolve, dim3(3), dim3(kGpThreads), 0, c->stream, B, n, count,
-                           maskN);
-    }
+    for (int w = 0; w <= kBatch; w++)
+      hipLaunchKernelGGL(k_gp_step<kCopies>, dim3(grid), dim3(kGpThreads), 0, c->stream, c->d_t0,
+                         c->rk(kGroup0), B, n, mask, count, (uint32_t)(w < kBatch));
   };
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1058,11 +1242,20 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
     }
     if (upto > launched[g]) launched[g] = upto;
   };
+  // a permutation is ready once all three of its swap resolutions are queued
+  uint32_t marked = 0;
+  auto mark_ready = [&]() {
+    const uint32_t all = std::min(launched[0], std::min(launched[1], launched[2]));
+    for (; marked < all; marked++) {
+      ORQG_HIP(hipEventCreateWithFlags(&fresh[marked].ready, hipEventDisableTiming));
+      ORQG_HIP(hipEventRecord(fresh[marked].ready, c->stream2));
+    }
+  };
   // every window either resolves or is retried at a quarter size down to
   // kWinMin draws, which always resolves; the cap only guards logic errors
   const uint64_t max_windows = 64ull * count * (1 + (uint64_t)n / 1024) + (1u << 20);
   for (uint64_t it = 0;; it++) {
-    if (it * kBatch > max_windows) throw Error(ORQG_E_PARITY, "gen_perm: window limit exceeded");
+    if (it * kBatch > 2 * max_windows) throw Error(ORQG_E_PARITY, "gen_perm: window limit exceeded");
     ORQG_HIP(hipGraphLaunch(gexec, c->stream));
     hipLaunchKernelGGL(k_gp_done, dim3(1), dim3(64), 0, c->stream, B, flag);
     ORQG_HIP(hipGetLastError());
@@ -1074,27 +1267,44 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
     // r of permutations below host_flag[2+g] is complete (the commits ran
     // before the synchronisation above)
     for (int g = 0; g < 3; g++) resolve_swaps(g, std::min(host_flag[2 + g], count));
+    mark_ready();
     if (host_flag[0]) break;
   }
-  ChainState fin[3];
-  ORQG_HIP(hipMemcpyAsync(fin, B.st, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
-  ORQG_HIP(hipStreamSynchronize(c->stream));
-  for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] = fin[g].nonce;
-  // the windows are done: what is left runs alone on the whole GPU
+  for (int g = 0; g < 3; g++) {
+    uint64_t nonce;
+    std::memcpy(&nonce, host_flag + 6 + 2 * g, 8);
+    c->nonce[kGroup0 + g] = nonce;
+  }
+  // the windows are done: the rest of the swap resolution gets a full grid
+  // and runs on while the caller goes on (a consumer of a permutation waits
+  // for its event; the check flags are read by check_swaps)
   fgrid = grid_for(n, 256, c->num_cu * 8);
   for (int g = 0; g < 3; g++) resolve_swaps(g, count);
+  mark_ready();
   ORQG_HIP(hipGetLastError());
+  c->fy_err = err;
+  c->fy_pending = true;
+  for (auto &p : fresh) c->queue.push_back(p);
+}
+
+void perm_wait(orqg_ctx *c, ShardedPerm &p) {
+  if (p.ready) ORQG_HIP(hipStreamWaitEvent(c->stream, p.ready, 0));
+}
+
+void check_swaps(orqg_ctx *c) {
+  if (!c->fy_pending) return;
+  c->fy_pending = false;
   uint32_t herr = 0;
-  ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream2));
+  ORQG_HIP(hipMemcpyAsync(&herr, c->fy_err, 4, hipMemcpyDeviceToHost, c->stream2));
   ORQG_HIP(hipStreamSynchronize(c->stream2));
   if (herr) throw Error(ORQG_E_PARITY, "gen_perm: swap resolution check failed");
-  for (auto &p : fresh) c->queue.push_back(p);
 }
 
 ShardedPerm next_perm(orqg_ctx *c, size_t n) {
   if (c->queue.empty() || c->stored_size != n) reserve_perms(c, n, 1);
   ShardedPerm p = c->queue.front();
   c->queue.pop_front();
+  perm_wait(c, p);
   return p;
 }
 

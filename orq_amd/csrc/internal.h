@@ -101,6 +101,7 @@ struct ShardedPerm {
   size_t n = 0;
   int32_t *fwd[3] = {nullptr, nullptr, nullptr};  // pi_g      (scatter form)
   int32_t *inv[3] = {nullptr, nullptr, nullptr};  // pi_g^{-1} (gather form)
+  hipEvent_t ready = nullptr;  // recorded after its swap resolution (second stream)
 };
 
 struct Timer {
@@ -127,6 +128,8 @@ struct orqg_ctx {
   std::map<std::string, orqg::Scratch> scratch;
   std::multimap<size_t, void *> pool;  // free device blocks by size (allocator lock held)
   std::deque<orqg::ShardedPerm> queue;
+  uint32_t *fy_err = nullptr;       // swap-resolution check flags, read by check_swaps
+  bool fy_pending = false;
   size_t stored_size = 0;
   int num_cu = 256;
   bool timing = false;
@@ -202,6 +205,11 @@ void reshare(orqg_ctx *c, orqg_vec *v, int group);
 void reserve_perms(orqg_ctx *c, size_t n, size_t count);
 ShardedPerm next_perm(orqg_ctx *c, size_t n);
 void free_perm(ShardedPerm &p);
+// the main stream waits (on the device) until p's swap resolution is done
+void perm_wait(orqg_ctx *c, ShardedPerm &p);
+// swap resolution runs on the second stream after reserve_perms returns; its
+// device-side check is read here (waits for that stream), throws if it failed
+void check_swaps(orqg_ctx *c);
 void apply_sharded(orqg_ctx *c, orqg_vec *x, const ShardedPerm &pi, bool inverse);
 void apply_elementwise(orqg_ctx *c, orqg_vec *x, const orqg_vec *perm);
 void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out);

@@ -504,6 +504,7 @@ uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_wi
   ORQG_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
   if (herr) throw Error(ORQG_E_PARITY, "radix_sort: opened bit permutation out of range");
+  check_swaps(c);
   return cur;
 }
 
