@@ -1013,18 +1013,43 @@ __global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r_all, size_t
 
 // Each target's list sorted by step index, descending, in place: the lists
 // come out of k_fy_link nearly sorted (steps are linked in roughly increasing
-// order), so a list is only checked, and the rare unsorted one gets an
-// insertion sort.  Then parent(t) = the largest step < t that targeted t:
-// the head, or the one after it if the head is step t itself (r_t = 0).
+// order).  A list's first kSortRegs members are read into registers while it
+// is checked; an unsorted list that short (the usual case: the late targets'
+// lists) is sorted there by a sorting network and relinked, a longer one gets
+// an insertion sort in place.  Then parent(t) = the largest step < t that
+// targeted t: the head, or the one after it if the head is step t itself
+// (r_t = 0).
+constexpr int kSortRegs = 16;
+
+__device__ inline void cswap_desc(int32_t &a, int32_t &b) {
+  const int32_t hi = a > b ? a : b, lo = a > b ? b : a;
+  a = hi;
+  b = lo;
+}
+
 __global__ void k_fy_sort(uint32_t n, int32_t *__restrict__ head, int32_t *__restrict__ next,
                           int32_t *__restrict__ parent, uint32_t *__restrict__ err) {
   int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
   int32_t *par = parent + (size_t)blockIdx.y * n;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     int32_t h = hd[t];
+    int32_t mem[kSortRegs];
     bool sorted = true;
     uint32_t len = 0;
-    for (int32_t q = h; q >= 0; len++) {
+    int32_t q = h;
+#pragma unroll
+    for (int k = 0; k < kSortRegs; k++) {
+      mem[k] = -1;  // -1 sorts last (descending)
+      if (q >= 0) {
+        mem[k] = q;
+        const int32_t nq = nx[q];
+        if (nq > q) sorted = false;
+        q = nq;
+        len++;
+      }
+    }
+    // the rest of a longer list is only checked
+    for (; q >= 0 && sorted; len++) {
       const int32_t nq = nx[q];
       if (nq > q || len >= n) {  // (len >= n: a cycle, never for valid links)
         sorted = false;
@@ -1032,21 +1057,33 @@ __global__ void k_fy_sort(uint32_t n, int32_t *__restrict__ head, int32_t *__res
       }
       q = nq;
     }
-    if (!sorted) {
+    if (!sorted && q < 0 && len <= (uint32_t)kSortRegs) {
+      // odd-even transposition network, descending; the -1 padding stays last
+#pragma unroll
+      for (int r = 0; r < kSortRegs; r++) {
+#pragma unroll
+        for (int k = r & 1; k + 1 < kSortRegs; k += 2) cswap_desc(mem[k], mem[k + 1]);
+      }
+#pragma unroll
+      for (int k = 0; k < kSortRegs; k++)
+        if (mem[k] >= 0) nx[mem[k]] = k + 1 < kSortRegs ? mem[k + 1] : -1;
+      h = mem[0];
+      hd[t] = h;
+    } else if (!sorted) {
       int32_t sh = -1;  // sorted list built so far (descending)
       uint32_t guard = 0;
-      for (int32_t q = h; q >= 0 && guard < n; guard++) {
-        const int32_t nq = nx[q];
-        if (sh < 0 || q > sh) {
-          nx[q] = sh;
-          sh = q;
+      for (int32_t q2 = h; q2 >= 0 && guard < n; guard++) {
+        const int32_t nq = nx[q2];
+        if (sh < 0 || q2 > sh) {
+          nx[q2] = sh;
+          sh = q2;
         } else {
           int32_t p = sh;
-          for (int32_t pn = nx[p]; pn > q; pn = nx[p]) p = pn;
-          nx[q] = nx[p];
-          nx[p] = q;
+          for (int32_t pn = nx[p]; pn > q2; pn = nx[p]) p = pn;
+          nx[q2] = nx[p];
+          nx[p] = q2;
         }
-        q = nq;
+        q2 = nq;
       }
       if (guard >= n) atomicOr(err, 32u);
       h = sh;
