@@ -993,11 +993,13 @@ __global__ void k_gp_done(ChainBufs B, uint32_t *flag) {
 
 // ---- swap resolution ----------------------------------------------------------
 // Step j swaps positions j and t_j = j + r_j.  Steps are chained per target
-// position (head[t] -> most recently linked step -> next[...]); list order is
-// arbitrary, the walks take maxima.
+// position (head[t] -> most recently linked step -> next[...]).  A list is
+// sorted (descending) iff every link put a larger step in front of a smaller
+// one; a link that did not marks its target dirty, and only dirty lists are
+// sorted.
 __global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r_all, size_t chain_stride,
                           int32_t *__restrict__ head, int32_t *__restrict__ next,
-                          uint32_t *__restrict__ err) {
+                          uint8_t *__restrict__ dirty, uint32_t *__restrict__ err) {
   const uint32_t *r = r_all + blockIdx.y * chain_stride;
   int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
@@ -1007,18 +1009,20 @@ __global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r_all, size_t
       nx[j] = -1;
       continue;
     }
-    nx[j] = atomicExch(&hd[t], (int32_t)j);
+    const int32_t old = atomicExch(&hd[t], (int32_t)j);
+    nx[j] = old;
+    if (old > (int32_t)j) dirty[t] = 1;
   }
 }
 
-// Each target's list sorted by step index, descending, in place: the lists
-// come out of k_fy_link nearly sorted (steps are linked in roughly increasing
-// order).  A list's first kSortRegs members are read into registers while it
-// is checked; an unsorted list that short (the usual case: the late targets'
-// lists) is sorted there by a sorting network and relinked, a longer one gets
-// an insertion sort in place.  Then parent(t) = the largest step < t that
-// targeted t: the head, or the one after it if the head is step t itself
-// (r_t = 0).
+// Each dirty target's list sorted by step index, descending, in place (the
+// flag is cleared for the next permutation): the lists come out of k_fy_link
+// nearly sorted, since steps are linked in roughly increasing order.  A
+// list's first kSortRegs members are read into registers; an unsorted list
+// that short (the usual case: the late targets' lists) is sorted there by a
+// sorting network and relinked, a longer one gets an insertion sort in place.
+// Then parent(t) = the largest step < t that targeted t: the head, or the one
+// after it if the head is step t itself (r_t = 0).
 constexpr int kSortRegs = 16;
 
 __device__ inline void cswap_desc(int32_t &a, int32_t &b) {
@@ -1028,11 +1032,17 @@ __device__ inline void cswap_desc(int32_t &a, int32_t &b) {
 }
 
 __global__ void k_fy_sort(uint32_t n, int32_t *__restrict__ head, int32_t *__restrict__ next,
-                          int32_t *__restrict__ parent, uint32_t *__restrict__ err) {
+                          int32_t *__restrict__ parent, uint8_t *__restrict__ dirty,
+                          uint32_t *__restrict__ err) {
   int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
   int32_t *par = parent + (size_t)blockIdx.y * n;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     int32_t h = hd[t];
+    if (!dirty[t]) {
+      par[t] = h == (int32_t)t ? nx[t] : h;
+      continue;
+    }
+    dirty[t] = 0;
     int32_t mem[kSortRegs];
     bool sorted = true;
     uint32_t len = 0;
@@ -1254,8 +1264,10 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   int32_t *head = (int32_t *)c->scratch_get("fy_head", (size_t)n * 4 + 64);
   int32_t *next = (int32_t *)c->scratch_get("fy_next", (size_t)n * 4 + 64);
   int32_t *parent = (int32_t *)c->scratch_get("fy_parent", (size_t)n * 4 + 64);
+  uint8_t *dirty = (uint8_t *)c->scratch_get("fy_dirty", (size_t)n + 64);
   uint32_t *err = (uint32_t *)c->scratch_get("fy_err", 16);
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
+  ORQG_HIP(hipMemsetAsync(dirty, 0, n, c->stream));  // then cleared by k_fy_sort
   ORQG_HIP(hipEventRecord(c->ev_aux, c->stream));
   ORQG_HIP(hipStreamWaitEvent(c->stream2, c->ev_aux, 0));
   // while windows run: one 4-wave workgroup per two CUs, enough random loads
@@ -1270,9 +1282,9 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
       const uint32_t *r = B.r_out + (size_t)g * chain_stride + (size_t)k * n;
       ORQG_HIP(hipMemsetAsync(head, 0xff, (size_t)n * 4, c->stream2));
       hipLaunchKernelGGL(k_fy_link, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
-                         head, next, err);
+                         head, next, dirty, err);
       hipLaunchKernelGGL(k_fy_sort, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, head, next,
-                         parent, err);
+                         parent, dirty, err);
       hipLaunchKernelGGL(k_fy_walk, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
                          next, parent, fresh[k].fwd[g], fresh[k].fwd[g], fresh[k].fwd[g],
                          fresh[k].inv[g], fresh[k].inv[g], fresh[k].inv[g]);
