@@ -55,7 +55,7 @@ constexpr uint32_t kWinMin = 1024;
 constexpr uint32_t kMaxTiles = kWinMax / kTile;           // 2048
 constexpr uint32_t kAmbCap = 65536;  // ambiguous candidates per window (level 1)
 constexpr uint32_t kL2Cap = 16384;   // undecided after the first exact pass (in LDS)
-// sparse windows (candidate density <= 1/256) use super-tiles of 4 AES rounds
+// sparse windows (candidate density <= 1/128) use super-tiles of 4 AES rounds
 // per workgroup, their few candidates stashed in LDS between the rounds
 constexpr uint32_t kSuperRounds = 4;
 constexpr uint32_t kSuperTile = kTile * kSuperRounds;  // 131072 draws
@@ -180,7 +180,7 @@ __device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s,
   w.active = 1;
   // super-tiles when candidates are rare and the window still gives every
   // workgroup at least one (otherwise dense tiles spread it over more CUs)
-  w.sparse = !force_dense && (uint64_t)w.khi * 256 <= maskN &&
+  w.sparse = !force_dense && (uint64_t)w.khi * 128 <= maskN &&
              (uint64_t)w.W >= (uint64_t)kSuperTile * nwork;
   const uint32_t td = w.sparse ? kSuperTile : kTile;
   // whole rounds of tiles over the drawing workgroups (a chain may draw
