@@ -56,9 +56,11 @@ constexpr uint32_t kMaxTiles = kWinMax / kTile;           // 2048
 constexpr uint32_t kAmbCap = 65536;  // ambiguous candidates per window (level 1)
 constexpr uint32_t kL2Cap = 16384;   // undecided after the first exact pass (in LDS)
 // sparse windows (candidate density <= 1/128) use super-tiles of 4 AES rounds
-// per workgroup, their few candidates stashed in LDS between the rounds
+// per workgroup, their few candidates stashed in LDS between the rounds; 8
+// rounds when the density is <= 1/256 and the window gives every workgroup one
 constexpr uint32_t kSuperRounds = 4;
 constexpr uint32_t kSuperTile = kTile * kSuperRounds;  // 131072 draws
+constexpr uint32_t kSuperRounds2 = 8;
 constexpr int kStash = 192;                             // candidates per wave
 constexpr uint64_t kRefillWords = 1ull << 20;  // gen_perm buffer: 2^20 uint32
 constexpr uint64_t kWordsPerNonce = 1ull << 18;
@@ -182,7 +184,12 @@ __device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s,
   // workgroup at least one (otherwise dense tiles spread it over more CUs)
   w.sparse = !force_dense && (uint64_t)w.khi * 128 <= maskN &&
              (uint64_t)w.W >= (uint64_t)kSuperTile * nwork;
-  const uint32_t td = w.sparse ? kSuperTile : kTile;
+  if (w.sparse)
+    w.sparse = (uint64_t)w.khi * 256 <= maskN &&
+                       (uint64_t)w.W >= (uint64_t)kTile * kSuperRounds2 * nwork
+                   ? kSuperRounds2
+                   : kSuperRounds;
+  const uint32_t td = w.sparse ? kTile * w.sparse : kTile;
   // whole rounds of tiles over the drawing workgroups (a chain may draw
   // alone in a step); retried windows below one round keep their size
   const uint64_t round = (uint64_t)td * nwork;
@@ -371,11 +378,12 @@ __device__ inline void commit_block(const ChainBufs &B, const ResState &st, int 
 // l computes AES blocks j * 64 + l (j < 8) of that range, i.e. draws
 // j * 256 + 4 l + q, so rank order inside a wave is (j, l, q) and candidates
 // are compacted with ballots into consecutive addresses.
-// Sparse tile (131072 draws): wave w covers [w * 8192, (w + 1) * 8192) in four
-// rounds of 2048 laid out as above; its candidates go to an LDS stash in rank
-// order, so one ticket and one pair of look-backs serve four AES rounds.
-// Candidate a (rank in the window) is accepted for sure if v < k - a (at most a
-// acceptances precede it); otherwise it is ambiguous and goes to the resolver.
+// Sparse tile (R = 4 or 8 rounds, R * 32768 draws): wave w covers
+// [w * R * 2048, (w + 1) * R * 2048) in R rounds of 2048 laid out as above; its
+// candidates go to an LDS stash in rank order, so one ticket and one pair of
+// look-backs serve R AES rounds.
+// Candidate a (rank in the window) is accepted for sure if v < k_lo - a (at
+// most a acceptances precede it); otherwise it is listed for the resolver.
 template <int COPIES>
 __device__ inline void draw_round(const AesCtx<COPIES> &A, const uint32_t *__restrict__ rk,
                                   uint64_t nonce, uint64_t d0, uint32_t W, uint32_t mask,
@@ -506,7 +514,8 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_step(const uint32_t *__restri
     const uint32_t klo = __builtin_amdgcn_readfirstlane(pl.klo);
     const uint32_t W = __builtin_amdgcn_readfirstlane(pl.W);
     const uint32_t seq = __builtin_amdgcn_readfirstlane(pl.seq);
-    const bool sparse = __builtin_amdgcn_readfirstlane(pl.sparse) != 0;
+    const uint32_t srounds = __builtin_amdgcn_readfirstlane(pl.sparse);  // 0: dense
+    const bool sparse = srounds != 0;
     const uint64_t d0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(pl.d0 >> 32)) << 32) |
                         __builtin_amdgcn_readfirstlane((uint32_t)pl.d0);
     const uint64_t nonce =
@@ -525,8 +534,8 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_step(const uint32_t *__restri
     uint32_t wc = 0;  // this wave's candidates
     uint32_t wbase;
     if (sparse) {
-      wbase = tile * kSuperTile + (uint32_t)wv * (kSuperTile / (kGpThreads / 64));
-      for (uint32_t r = 0; r < kSuperRounds; r++) {
+      wbase = tile * srounds * kTile + (uint32_t)wv * srounds * 2048u;
+      for (uint32_t r = 0; r < srounds; r++) {
         const uint32_t rbase = wbase + r * 2048;
         draw_round<COPIES>(A, rk, nonce, d0, W, mask, rbase, v);
 #pragma unroll
