@@ -273,11 +273,42 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
 // ---- sharded round on element records: gather through idx, then add the
 // resharing masks of v (binary, tszv-byte masks) and perm (arithmetic, 4-byte
 // masks), drawn from group PRG g ----------------------------------------------------
-// A pair of lanes takes a group of 4 elements, 2 each.  The group's two perm
-// mask blocks (calls 0 and 1, 4 elements per block) are computed one per lane
-// and exchanged; each lane computes its elements' two v mask blocks.  Gathers
-// run one group ahead of the AES work (indices two groups ahead), so the loads
-// are in flight while the keystream is computed.
+// A wave takes 128 consecutive elements per step, lane l elements base + l
+// and base + 64 + l, so a quad of lanes holds the 4 elements one perm mask
+// block covers.  Each lane computes its elements' two v mask blocks and one of
+// the quad's four perm blocks (call l & 1 for the block of 64 (l >> 1) & 1),
+// broadcast in the quad with DPP.  Gathers run one step ahead of the AES work
+// (indices two steps ahead).  Records are written after a 4 x 4 transpose of
+// 16-byte quarters inside the quad, so one store instruction writes 16 whole
+// 64-byte records instead of a quarter of 64.
+template <int CTRL>
+__device__ inline uint32_t quad_dpp(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ inline uint4 quad_dpp4(const uint4 &x) {
+  return make_uint4(quad_dpp<CTRL>(x.x), quad_dpp<CTRL>(x.y), quad_dpp<CTRL>(x.z), quad_dpp<CTRL>(x.w));
+}
+__device__ inline uint32_t word_of(const uint4 &b, uint32_t s) {
+  return s == 0 ? b.x : s == 1 ? b.y : s == 2 ? b.z : b.w;
+}
+// lane s of a quad holds quarters X[0..3] of record s; afterwards it holds
+// quarter s of records 0..3 (two butterfly stages, xor 1 then xor 2)
+__device__ inline void quad_transpose(uint4 (&X)[4], uint32_t s) {
+#pragma unroll
+  for (int h = 0; h < 2; h++) {  // quarters 2h, 2h + 1 with lane s ^ 1
+    const uint4 p0 = quad_dpp4<0xB1>(X[2 * h]), p1 = quad_dpp4<0xB1>(X[2 * h + 1]);
+    if (s & 1u) X[2 * h] = p1;
+    else X[2 * h + 1] = p0;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; h++) {  // quarters h, h + 2 with lane s ^ 2
+    const uint4 p0 = quad_dpp4<0x4E>(X[h]), p2 = quad_dpp4<0x4E>(X[h + 2]);
+    if (s & 2u) X[h] = p2;
+    else X[h + 2] = p0;
+  }
+}
+
 template <int L, bool SCATTER>
 __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
                                                 const uint32_t *__restrict__ rk, uint64_t nv,
@@ -291,16 +322,16 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
   AesCtx<kCopies> A(lds);
   constexpr int kVq = (3 * L + 3) / 4;  // 16-byte pieces holding v
   const uint64_t n = sv.n;
-  const uint64_t ngrp = (n + 3) / 4;
-  const uint32_t half = threadIdx.x & 1;
-  const uint64_t gstride = ((uint64_t)gridDim.x * blockDim.x) >> 1;
-  uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 1;
-  // source row of this lane's element i of group gg, or -1
-  auto load_idx = [&](uint64_t gg, int32_t (&ix)[2]) {
+  const uint32_t lane = threadIdx.x & 63, s = lane & 3;
+  const uint64_t nstep = (n + 127) / 128;
+  const uint64_t wstride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // source row of this lane's element i of step ww, or -1
+  auto load_idx = [&](uint64_t ww, int32_t (&ix)[2]) {
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-      const uint64_t e = gg * 4 + 2 * half + i;
-      ix[i] = gg < ngrp && e < n ? idx[e] : -1;
+      const uint64_t e = ww * 128 + 64 * i + lane;
+      ix[i] = ww < nstep && e < n ? idx[e] : -1;
     }
   };
   auto load_rec = [&](const int32_t (&ix)[2], uint4 (&pv)[2][kVq], uint4 (&pq)[2]) {
@@ -316,34 +347,41 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
   };
   int32_t ci[2], ni[2];
   uint4 cv[2][kVq], cq[2];
-  load_idx(g, ci);
+  load_idx(w, ci);
   load_rec(ci, cv, cq);
-  load_idx(g + gstride, ni);
-  for (; g < ngrp; g += gstride) {
-    const uint64_t e0 = g * 4;
-    // next group's records and the index after it, in flight during the AES
+  load_idx(w + wstride, ni);
+  for (; w < nstep; w += wstride) {
+    const uint64_t base = w * 128;
+    // next step's records and the index after it, in flight during the AES
     uint4 xv[2][kVq], xq[2];
     load_rec(ni, xv, xq);
     int32_t xi[2];
-    load_idx(g + 2 * gstride, xi);
-    // perm masks: this lane's call (= half) for the group's 4 elements
-    uint4 mine;
+    load_idx(w + 2 * wstride, xi);
+    // this lane's perm block: call lane & 1 of the group of block (lane >> 1) & 1
+    uint4 pb;
     {
       uint64_t off;
       uint32_t ctr, bib;
-      sched_locate(sp, e0, half, off, ctr, bib);
-      mine = aes_ctr_block<kCopies>(A, rk, np + off, ctr);
+      sched_locate(sp, base + 64 * ((lane >> 1) & 1) + (lane & ~3u), lane & 1, off, ctr, bib);
+      pb = aes_ctr_block<kCopies>(A, rk, np + off, ctr);
     }
-    const uint4 other = make_uint4(__shfl_xor(mine.x, 1, 64), __shfl_xor(mine.y, 1, 64),
-                                   __shfl_xor(mine.z, 1, 64), __shfl_xor(mine.w, 1, 64));
-    const uint4 c0 = half ? other : mine, c1 = half ? mine : other;
+    // word s of quad lane c's block, for each c: calls 0 and 1 of blocks 0 and 1
+    uint32_t pm[4];
+    {
+      // every lane takes word s: each word is broadcast from quad lane c (all
+      // lanes active), then selected
+      const uint4 b0 = quad_dpp4<0x00>(pb), b1 = quad_dpp4<0x55>(pb);
+      pm[0] = word_of(b0, s);
+      pm[1] = word_of(b1, s);
+      const uint4 b2 = quad_dpp4<0xAA>(pb), b3 = quad_dpp4<0xFF>(pb);
+      pm[2] = word_of(b2, s);
+      pm[3] = word_of(b3, s);
+    }
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-      const uint64_t e = e0 + 2 * half + i;
-      if (e >= n) continue;
-      const uint32_t wsel = 2 * half + i;
-      const uint32_t m0 = wsel == 0 ? c0.x : wsel == 1 ? c0.y : wsel == 2 ? c0.z : c0.w;
-      const uint32_t m1 = wsel == 0 ? c1.x : wsel == 1 ? c1.y : wsel == 2 ? c1.z : c1.w;
+      const uint64_t e = base + 64 * i + lane;
+      const bool live = e < n;
+      const uint32_t m0 = pm[2 * i], m1 = pm[2 * i + 1];
       const uint32_t m2 = 0u - m0 - m1;
       // v masks: element e's bytes of calls 0 and 1
       uint32_t r[2][L];
@@ -351,50 +389,58 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       for (int j = 0; j < 2; j++) {
         uint64_t off;
         uint32_t ctr, bib;
-        sched_locate(sv, e, j, off, ctr, bib);
+        sched_locate(sv, live ? e : 0, j, off, ctr, bib);
         uint4 blk = aes_ctr_block<kCopies>(A, rk, nv + off, ctr);
         uint32_t w4[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
         for (int q = 0; q < L; q++) r[j][q] = w4[bib / 4 + q];
       }
-      uint32_t w[kRec];
+      uint32_t wd[kRec];
 #pragma unroll
       for (int q = 0; q < kVq; q++) {
-        w[4 * q] = cv[i][q].x;
-        w[4 * q + 1] = cv[i][q].y;
-        w[4 * q + 2] = cv[i][q].z;
-        w[4 * q + 3] = cv[i][q].w;
+        wd[4 * q] = cv[i][q].x;
+        wd[4 * q + 1] = cv[i][q].y;
+        wd[4 * q + 2] = cv[i][q].z;
+        wd[4 * q + 3] = cv[i][q].w;
       }
 #pragma unroll
-      for (int q = 4 * kVq; q < kRec; q++) w[q] = 0u;
+      for (int q = 4 * kVq; q < kRec; q++) wd[q] = 0u;
 #pragma unroll
       for (int q = 0; q < L; q++) {
-        w[0 * L + q] ^= r[0][q];
-        w[1 * L + q] ^= r[1][q];
-        w[2 * L + q] ^= r[0][q] ^ r[1][q];
+        wd[0 * L + q] ^= r[0][q];
+        wd[1 * L + q] ^= r[1][q];
+        wd[2 * L + q] ^= r[0][q] ^ r[1][q];
       }
-      w[kRecPerm + 0] = cq[i].x + m0;
-      w[kRecPerm + 1] = cq[i].y + m1;
-      w[kRecPerm + 2] = cq[i].z + m2;
+      wd[kRecPerm + 0] = cq[i].x + m0;
+      wd[kRecPerm + 1] = cq[i].y + m1;
+      wd[kRecPerm + 2] = cq[i].z + m2;
+      // destination row: the element itself, or (scatter) its opened perm
+      uint32_t dst = (uint32_t)e;
+      bool ok = live;
       if (SCATTER) {
-        // open the permuted perm and put the permuted record at its position
-        // (whole: a partial write of its last 32-byte piece would cost a
-        // read-modify-write; the perm words are rewritten by the next bit)
-        const uint32_t sigma = w[kRecPerm] + w[kRecPerm + 1] + w[kRecPerm + 2];
-        if (sigma >= n) {
+        dst = wd[kRecPerm] + wd[kRecPerm + 1] + wd[kRecPerm + 2];
+        if (live && dst >= n) {
           atomicOr(err, 1u);
-          continue;
+          ok = false;
         }
-        uint4 *o = reinterpret_cast<uint4 *>(Rout + (uint64_t)sigma * kRec);
-#pragma unroll
-        for (int q = 0; q < kRec / 4; q++)
-          o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-      } else {
-        uint4 *o = reinterpret_cast<uint4 *>(Rout + e * kRec);
-#pragma unroll
-        for (int q = 0; q < kRec / 4; q++)
-          o[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
       }
+      uint4 X[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) X[q] = make_uint4(wd[4 * q], wd[4 * q + 1], wd[4 * q + 2], wd[4 * q + 3]);
+      quad_transpose(X, s);
+      // store t: quarter s of the record of quad lane t (whole records per
+      // instruction); a dropped row is marked by an out-of-range destination
+      const uint32_t dk = ok ? dst : 0xffffffffu;
+#define ORQG_STORE(t, CTRL)                                                          \
+  {                                                                                  \
+    const uint32_t dt = quad_dpp<CTRL>(dk);                                          \
+    if (dt != 0xffffffffu) reinterpret_cast<uint4 *>(Rout + (uint64_t)dt * kRec)[s] = X[t]; \
+  }
+      ORQG_STORE(0, 0x00)
+      ORQG_STORE(1, 0x55)
+      ORQG_STORE(2, 0xAA)
+      ORQG_STORE(3, 0xFF)
+#undef ORQG_STORE
     }
 #pragma unroll
     for (int i = 0; i < 2; i++) {
