@@ -88,22 +88,20 @@ struct AesLds {
   static constexpr int kBytes = kWords * 4;
 };
 
-// Host-built LDS image of the replicated tables (word layout above).
-inline void aes_make_lds_image(const AesTables &t, int copies, uint32_t *img) {
-  for (int j = 0; j < 4; j++)
-    for (int x = 0; x < 256; x++) {
-      uint32_t v = t.t0[x];
-      uint32_t r = j == 0 ? v : ((v << (8 * j)) | (v >> (32 - 8 * j)));
-      for (int c = 0; c < copies; c++) img[(j >> 1) * 16384 + x * 64 + (j & 1) * 32 + c] = r;
-    }
-}
-
-// Copy the table image into LDS with 16-byte loads (lds must be 16-byte aligned).
+// Build the replicated tables in LDS (word layout above) from T0 in global
+// memory (1 KiB, so a launch's workgroups read 1 KiB each, not the 128 KiB
+// image).  Consecutive lanes write consecutive 16-byte chunks; chunk w holds
+// copies 4 (w & 7) .. + 3 of entry x = (w >> 4) & 255 of table
+// j = 2 (w >> 12) + ((w >> 3) & 1).  lds must be 16-byte aligned.
 template <int COPIES>
-__device__ inline void aes_load_tables(uint32_t *lds, const uint32_t *__restrict__ g_img) {
-  const uint4 *src = reinterpret_cast<const uint4 *>(g_img);
+__device__ inline void aes_load_tables(uint32_t *lds, const uint32_t *__restrict__ g_t0) {
   uint4 *dst = reinterpret_cast<uint4 *>(lds);
-  for (int w = threadIdx.x; w < AesLds<COPIES>::kWords / 4; w += blockDim.x) dst[w] = src[w];
+  for (int w = threadIdx.x; w < AesLds<COPIES>::kWords / 4; w += blockDim.x) {
+    const uint32_t j = ((uint32_t)(w >> 12) << 1) | ((uint32_t)(w >> 3) & 1u);
+    const uint32_t t = g_t0[(w >> 4) & 255];
+    const uint32_t v = j ? (t << (8 * j)) | (t >> (32 - 8 * j)) : t;
+    dst[w] = make_uint4(v, v, v, v);
+  }
   __syncthreads();
 }
 

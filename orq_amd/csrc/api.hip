@@ -213,11 +213,9 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     uint32_t rk[ORQG_NUM_KEYS * 60];
     for (int k = 0; k < ORQG_NUM_KEYS; k++) aes256_expand(c->tabs, c->keys[k], rk + 60 * k);
     c->d_rk = (uint32_t *)dev_alloc(c, sizeof(rk));
-    std::vector<uint32_t> img(AesLds<32>::kWords);
-    aes_make_lds_image(c->tabs, 32, img.data());
-    c->d_t0 = (uint32_t *)dev_alloc(c, img.size() * 4);
+    c->d_t0 = (uint32_t *)dev_alloc(c, sizeof(c->tabs.t0));  // T0; kernels replicate it in LDS
     ORQG_HIP(hipMemcpy(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice));
-    ORQG_HIP(hipMemcpy(c->d_t0, img.data(), img.size() * 4, hipMemcpyHostToDevice));
+    ORQG_HIP(hipMemcpy(c->d_t0, c->tabs.t0, sizeof(c->tabs.t0), hipMemcpyHostToDevice));
   } catch (const std::exception &e) {
     std::fprintf(stderr, "orqg_ctx_create: %s\n", e.what());
     unregister_ctx(c);
