@@ -345,18 +345,28 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       }
     }
   };
+  // the scatter variant (short of registers) loads a step's records at its
+  // start instead of one step ahead; they are still in flight during its AES
+  constexpr bool kAhead = !SCATTER;
   int32_t ci[2], ni[2];
   uint4 cv[2][kVq], cq[2];
-  load_idx(w, ci);
-  load_rec(ci, cv, cq);
-  load_idx(w + wstride, ni);
+  if (kAhead) {
+    load_idx(w, ci);
+    load_rec(ci, cv, cq);
+    load_idx(w + wstride, ni);
+  }
   for (; w < nstep; w += wstride) {
     const uint64_t base = w * 128;
     // next step's records and the index after it, in flight during the AES
     uint4 xv[2][kVq], xq[2];
-    load_rec(ni, xv, xq);
     int32_t xi[2];
-    load_idx(w + 2 * wstride, xi);
+    if (kAhead) {
+      load_rec(ni, xv, xq);
+      load_idx(w + 2 * wstride, xi);
+    } else {
+      load_idx(w, ci);
+      load_rec(ci, cv, cq);
+    }
     // this lane's perm block: call lane & 1 of the group of block (lane >> 1) & 1
     uint4 pb;
     {
@@ -442,12 +452,14 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       ORQG_STORE(3, 0xFF)
 #undef ORQG_STORE
     }
+    if (kAhead) {
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
+      for (int i = 0; i < 2; i++) {
 #pragma unroll
-      for (int q = 0; q < kVq; q++) cv[i][q] = xv[i][q];
-      cq[i] = xq[i];
-      ni[i] = xi[i];
+        for (int q = 0; q < kVq; q++) cv[i][q] = xv[i][q];
+        cq[i] = xq[i];
+        ni[i] = xi[i];
+      }
     }
   }
 }
