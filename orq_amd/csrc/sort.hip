@@ -277,8 +277,8 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
 // and base + 64 + l, so a quad of lanes holds the 4 elements one perm mask
 // block covers.  Each lane computes its elements' two v mask blocks and one of
 // the quad's four perm blocks (call l & 1 for the block of 64 (l >> 1) & 1),
-// broadcast in the quad with DPP.  Gathers run one step ahead of the AES work
-// (indices two steps ahead).  Records are written after a 4 x 4 transpose of
+// broadcast in the quad with DPP.  A step's gathers are issued before its AES
+// work and consumed after it.  Records are written after a 4 x 4 transpose of
 // 16-byte quarters inside the quad, so one store instruction writes 16 whole
 // 64-byte records instead of a quarter of 64.
 template <int CTRL>
@@ -345,28 +345,14 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       }
     }
   };
-  // the scatter variant (short of registers) loads a step's records at its
-  // start instead of one step ahead; they are still in flight during its AES
-  constexpr bool kAhead = !SCATTER;
-  int32_t ci[2], ni[2];
-  uint4 cv[2][kVq], cq[2];
-  if (kAhead) {
-    load_idx(w, ci);
-    load_rec(ci, cv, cq);
-    load_idx(w + wstride, ni);
-  }
   for (; w < nstep; w += wstride) {
     const uint64_t base = w * 128;
-    // next step's records and the index after it, in flight during the AES
-    uint4 xv[2][kVq], xq[2];
-    int32_t xi[2];
-    if (kAhead) {
-      load_rec(ni, xv, xq);
-      load_idx(w + 2 * wstride, xi);
-    } else {
-      load_idx(w, ci);
-      load_rec(ci, cv, cq);
-    }
+    // the step's records, in flight during its AES work (loading a step
+    // ahead measured slower: the registers it holds cost more than it hides)
+    int32_t ci[2];
+    uint4 cv[2][kVq], cq[2];
+    load_idx(w, ci);
+    load_rec(ci, cv, cq);
     // this lane's perm block: call lane & 1 of the group of block (lane >> 1) & 1
     uint4 pb;
     {
@@ -451,15 +437,6 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       ORQG_STORE(2, 0xAA)
       ORQG_STORE(3, 0xFF)
 #undef ORQG_STORE
-    }
-    if (kAhead) {
-#pragma unroll
-      for (int i = 0; i < 2; i++) {
-#pragma unroll
-        for (int q = 0; q < kVq; q++) cv[i][q] = xv[i][q];
-        cq[i] = xq[i];
-        ni[i] = xi[i];
-      }
     }
   }
 }
