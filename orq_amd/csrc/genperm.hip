@@ -38,6 +38,7 @@
 // earlier step that targeted this position" links.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "internal.h"
@@ -111,6 +112,8 @@ struct ChainBufs {
   WinDesc *wd;         // [2][3]
   uint32_t *ctl;       // [0] step, [2] tile ticket, [3] finished workgroups
   uint32_t *err;       // device-side check flags
+  uint32_t l2cap;      // undecided items a resolve takes (kL2Cap; smaller only to test retries)
+  uint32_t bound_shift;  // k_lo margin divided by 2^bound_shift (0; > 0 only to test redraws)
   uint32_t *cand_v;    // [3][3][cand_cap] candidates in rank order: v (buffer s % 3)
   uint32_t *cand_d;    // [3][3][cand_cap] ... and the draw offset in the window, written
                        // only when the window can end the permutation
@@ -165,10 +168,10 @@ __device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s,
     const double kp = (double)(n - r.i), Cd = (double)p.C;
     const double mu = Cd * Cd * Cd / (6.0 * kp * kp) + Cd * (double)(p.khi - p.klo) / kp;
     const double Lf = mu + 8.0 * sqrt(mu) + 256.0;
-    const uint32_t L = Lf >= (double)p.m ? p.m : (uint32_t)Lf;
+    const uint32_t L = (Lf >= (double)p.m ? p.m : (uint32_t)Lf) >> B.bound_shift;
     // the undecided items a bound L adds to the next resolve (about m L / k)
     // must leave room in its list
-    if (mu + (double)p.m * (double)L / kp > (double)(kL2Cap / 2)) return w;
+    if (mu + (double)p.m * (double)L / kp > (double)(B.l2cap / 2)) return w;
     w.khi = n - (r.i + p.C) + p.m;
     w.klo = w.khi - L;
     w.d0 = p.d0 + p.W;
@@ -821,7 +824,7 @@ __device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uin
     if (threadIdx.x == 0) s_l2 = (uint32_t)tot;
     __syncthreads();
   }
-  if (!skip && (over || s_l2 > kL2Cap)) {
+  if (!skip && (over || s_l2 > B.l2cap)) {
     // too many candidates or ambiguous ones: retry this window at a quarter size
     if (threadIdx.x == 0) {
       const uint32_t w = p.W / 4;
@@ -1204,6 +1207,13 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   const uint32_t cand_cap = (uint32_t)std::min<double>(
       kWinMax, std::max<double>(1 << 16, 1024.0 * std::ceil(std::sqrt((double)n))));
   B.cand_cap = cand_cap;
+  // test hooks (environment, read per call): a smaller undecided-list cap
+  // forces window retries, a smaller k_lo margin forces bound failures and
+  // redraws; the permutations stay exact either way
+  B.l2cap = kL2Cap;
+  B.bound_shift = 0;
+  if (const char *e = std::getenv("ORQG_GP_L2CAP")) B.l2cap = std::max(64, std::min<int>((int)kL2Cap, std::atoi(e)));
+  if (const char *e = std::getenv("ORQG_GP_BOUND_SHIFT")) B.bound_shift = (uint32_t)std::max(0, std::min(31, std::atoi(e)));
   B.cand_v = (uint32_t *)c->scratch_get("gp_cand_v", 9ull * cand_cap * 4);
   B.cand_d = (uint32_t *)c->scratch_get("gp_cand_d", 9ull * cand_cap * 4);
   B.stat_c = (uint64_t *)c->scratch_get("gp_stat_c", 3ull * kMaxTiles * 8);

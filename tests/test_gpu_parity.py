@@ -180,6 +180,30 @@ def test_gen_perm_many_tiles(O, G, keys, n):
     _same_nonces(oc, gc)
 
 
+@pytest.mark.parametrize("n,count,l2cap,shift", [
+    (50000, 2, 64, 0),          # undecided lists over a tiny cap: windows retried at W/4
+    ((1 << 20) + 3, 2, 0, 3),   # k_lo margin / 8: some speculative windows redrawn
+    ((1 << 20) + 3, 1, 0, 12),  # no margin: nearly every speculative window redrawn
+])
+def test_gen_perm_forced_redraws(O, G, keys, monkeypatch, n, count, l2cap, shift):
+    """The retry and redraw paths of the window pipeline, forced through the
+    test hooks (a smaller undecided-list cap, a smaller k_lo margin), still
+    give the oracle's permutations and nonces."""
+    if l2cap:
+        monkeypatch.setenv("ORQG_GP_L2CAP", str(l2cap))
+    if shift:
+        monkeypatch.setenv("ORQG_GP_BOUND_SHIFT", str(shift))
+    oc, gc = _pair(O, G, keys)
+    oc.reserve(n, count)
+    gc.reserve(n, count)
+    for k in range(count):
+        for g in range(3):
+            members = [p for p in range(3) if p != 2 - g]
+            np.testing.assert_array_equal(gc.peek_perm(k, g, n), oc.peek_perm(k, members[0], g, n),
+                                          err_msg=f"perm {k} group {g}")
+    _same_nonces(oc, gc)
+
+
 @pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_apply_sharded(O, G, keys, width, inverse):
