@@ -150,14 +150,24 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
   b2a_bit_group<uint32_t>(A, rk_pair, nb, s, e0, bits, f);
   uint32_t loc[3] = {0, 0, 0};
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    if (e0 + i < s.n) {
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        F[(e0 + i) * 3 + k] = f[i][k];
-        loc[k] += f[i][k];
-      }
+    for (int k = 0; k < 3; k++) {
+      if (e0 + i >= s.n) f[i][k] = 0u;
+      loc[k] += f[i][k];
     }
+  // the thread's 12 words are contiguous and 16-byte aligned: three vector stores
+  if (e0 + 4 <= s.n) {
+    uint4 *o = reinterpret_cast<uint4 *>(F + e0 * 3);
+    o[0] = make_uint4(f[0][0], f[0][1], f[0][2], f[1][0]);
+    o[1] = make_uint4(f[1][1], f[1][2], f[2][0], f[2][1]);
+    o[2] = make_uint4(f[2][2], f[3][0], f[3][1], f[3][2]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if (e0 + i < s.n)
+#pragma unroll
+        for (int k = 0; k < 3; k++) F[(e0 + i) * 3 + k] = f[i][k];
   }
   {
     uint32_t ex[3], tot[3];
@@ -212,13 +222,22 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
   const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
   uint32_t f[4][3];
   uint32_t loc[3] = {0, 0, 0};
+  if (e0 + 4 <= s.n) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(F + e0 * 3);
+    const uint4 a = src[0], b = src[1], c = src[2];
+    f[0][0] = a.x; f[0][1] = a.y; f[0][2] = a.z; f[1][0] = a.w;
+    f[1][1] = b.x; f[1][2] = b.y; f[2][0] = b.z; f[2][1] = b.w;
+    f[2][2] = c.x; f[3][0] = c.y; f[3][1] = c.z; f[3][2] = c.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) f[i][k] = (e0 + i < s.n) ? F[(e0 + i) * 3 + k] : 0u;
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      f[i][k] = (e0 + i < s.n) ? F[(e0 + i) * 3 + k] : 0u;
-      loc[k] += f[i][k];
-    }
+    for (int k = 0; k < 3; k++) loc[k] += f[i][k];
   uint32_t pre[3];
   {
     uint32_t t[3];
