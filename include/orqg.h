@@ -170,7 +170,11 @@ int orqg_reshare(orqg_ctx *ctx, orqg_vec *v, int group);
 /* ---- permutations ---------------------------------------------------- */
 /* PermutationManager::reserve (permutation_manager.h:102-181): generate
  * `count` honest-majority sharded permutations of size n with gen_perm
- * (hm_sharded_permutation_generator.h:70-108) and queue them               */
+ * (hm_sharded_permutation_generator.h:70-108) and queue them.  Returns once
+ * the draws are done; the last swap resolutions may still run on a second
+ * stream.  Every consumer of a permutation waits for it on the device, and
+ * a failed device-side check of the swap resolution is reported by the next
+ * orqg_sync / host read / reserve (ORQG_E_PARITY).                          */
 int orqg_reserve_perms(orqg_ctx *ctx, size_t n, size_t count);
 size_t orqg_perm_queue_size(orqg_ctx *ctx);
 /* local permutation of queued sharded permutation `index` for group 0..2
