@@ -56,12 +56,9 @@ constexpr uint32_t kWinMin = 1024;
 constexpr uint32_t kMaxTiles = kWinMax / kTile;           // 2048
 constexpr uint32_t kAmbCap = 65536;  // ambiguous candidates per window (level 1)
 constexpr uint32_t kL2Cap = 16384;   // undecided after the first exact pass (in LDS)
-// sparse windows (candidate density <= 1/128) use super-tiles of 4 AES rounds
-// per workgroup, their few candidates stashed in LDS between the rounds; 8
-// rounds when the density is <= 1/256 and the window gives every workgroup one
-constexpr uint32_t kSuperRounds = 4;
-constexpr uint32_t kSuperTile = kTile * kSuperRounds;  // 131072 draws
-constexpr uint32_t kSuperRounds2 = 8;
+// sparse windows (candidate density <= 1/64) use super-tiles of 2, 4 or 8 AES
+// rounds per workgroup, their few candidates stashed in LDS between the rounds
+constexpr uint32_t kSuperRounds2 = 8;  // the most rounds per super-tile
 constexpr int kStash = 192;                             // candidates per wave
 constexpr uint64_t kRefillWords = 1ull << 20;  // gen_perm buffer: 2^20 uint32
 constexpr uint64_t kWordsPerNonce = 1ull << 18;
@@ -185,13 +182,16 @@ __device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s,
   w.active = 1;
   // super-tiles when candidates are rare and the window still gives every
   // workgroup at least one (otherwise dense tiles spread it over more CUs)
-  w.sparse = !force_dense && (uint64_t)w.khi * 128 <= maskN &&
-             (uint64_t)w.W >= (uint64_t)kSuperTile * nwork;
-  if (w.sparse)
-    w.sparse = (uint64_t)w.khi * 256 <= maskN &&
-                       (uint64_t)w.W >= (uint64_t)kTile * kSuperRounds2 * nwork
-                   ? kSuperRounds2
-                   : kSuperRounds;
+  // R rounds per super-tile: the most of 8, 4, 2 for which a wave expects at
+  // most 64 candidates (R * 2048 * k_hi / maskN, a third of its stash) and the
+  // window still gives every drawing workgroup a super-tile
+  w.sparse = 0;
+  if (!force_dense)
+    for (uint32_t R = kSuperRounds2; R >= 2; R >>= 1)
+      if ((uint64_t)w.khi * 32 * R <= maskN && (uint64_t)w.W >= (uint64_t)kTile * R * nwork) {
+        w.sparse = R;
+        break;
+      }
   const uint32_t td = w.sparse ? kTile * w.sparse : kTile;
   // whole rounds of tiles over the drawing workgroups (a chain may draw
   // alone in a step); retried windows below one round keep their size
