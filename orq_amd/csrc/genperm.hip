@@ -56,7 +56,7 @@ constexpr uint32_t kWinMin = 1024;
 constexpr uint32_t kMaxTiles = kWinMax / kTile;           // 2048
 constexpr uint32_t kAmbCap = 65536;  // ambiguous candidates per window (level 1)
 constexpr uint32_t kL2Cap = 16384;   // undecided after the first exact pass (in LDS)
-// sparse windows (candidate density <= 1/64) use super-tiles of 2, 4 or 8 AES
+// sparse windows (candidate density <= 3/128) use super-tiles of 2, 4 or 8 AES
 // rounds per workgroup, their few candidates stashed in LDS between the rounds
 constexpr uint32_t kSuperRounds2 = 8;  // the most rounds per super-tile
 constexpr int kStash = 192;                             // candidates per wave
@@ -183,12 +183,14 @@ __device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s,
   // super-tiles when candidates are rare and the window still gives every
   // workgroup at least one (otherwise dense tiles spread it over more CUs)
   // R rounds per super-tile: the most of 8, 4, 2 for which a wave expects at
-  // most 64 candidates (R * 2048 * k_hi / maskN, a third of its stash) and the
-  // window still gives every drawing workgroup a super-tile
+  // most 96 candidates (R * 2048 * k_hi / maskN; its stash of 192 is then ten
+  // standard deviations above) and the window still gives every drawing
+  // workgroup a super-tile
   w.sparse = 0;
   if (!force_dense)
     for (uint32_t R = kSuperRounds2; R >= 2; R >>= 1)
-      if ((uint64_t)w.khi * 32 * R <= maskN && (uint64_t)w.W >= (uint64_t)kTile * R * nwork) {
+      if ((uint64_t)w.khi * 2048 * R <= 96ull * maskN &&
+          (uint64_t)w.W >= (uint64_t)kTile * R * nwork) {
         w.sparse = R;
         break;
       }
