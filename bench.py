@@ -23,6 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# keystream rate of the T-table AES-256 CTR core with its tables in LDS
+# (measured, 2^30 counter blocks): the ceiling of every keystream-bound kernel
+AES_CEILING_GBS = 1100.0
+GROUP_ROLE0 = 6  # PRG roles 6..8: the pair groups {0,1}, {0,2}, {1,2}
 
 
 def zipf_keys(n, s=1.1, domain=1 << 20, seed=0x0A11CE03):
@@ -388,6 +392,16 @@ def main():
     got = ctx.open(work).view(np.int64)
     ok = bool(np.array_equal(got, np.sort(keys_plain.view(np.int64), kind="stable")))
 
+    # gen_perm's keystream: a sharded permutation's three local permutations
+    # draw from the three pair-group PRGs, whose nonces advance by one per MiB
+    # of keystream (four per refill of 2^20 draws).  Measured on one extra
+    # reserve of the same 66 permutations, outside the timed region.
+    before = [ctx.nonce(GROUP_ROLE0 + g) for g in range(3)]
+    ctx.reserve(n, 66)
+    ctx.sync()
+    gen_ks_bytes = sum(ctx.nonce(GROUP_ROLE0 + g) - before[g] for g in range(3)) << 20
+    gen_ks_gbs = gen_ks_bytes / (fams["gen_perm"]["ms_per_step"] / 1e3) / 1e9
+
     # roofline of the dominant online kernel, k_round2 (64 bits x 2 plain rounds per step;
     # the third round of each bit, which also scatters, is timed as k_round2_scatter)
     r_ms, r_cnt = ctx.timer("k_round2")
@@ -422,6 +436,12 @@ def main():
                      "avg_launch_us": round(per_launch_ms * 1e3, 1),
                      "bytes_per_launch": n * ROUND_BYTES_PER_ROW,
                      "keystream_GBps": round(ks_gbs, 1)},
+        # the step's dominant kernel, k_gp_step (gen_perm's windows), is bound by
+        # AES T-table lookups in LDS, not by HBM: its rate against the AES ceiling
+        "gen_perm_keystream": {"kernel": "k_gp_step", "bound": "lds",
+                               "keystream_bytes": gen_ks_bytes,
+                               "achieved": round(gen_ks_gbs, 1), "ceiling": AES_CEILING_GBS,
+                               "unit": "GB/s", "frac": round(gen_ks_gbs / AES_CEILING_GBS, 4)},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample)

@@ -142,11 +142,20 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
   for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
   const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
   uint32_t bits[4][3], f[4][3];
+  // the 12 loads are issued together: a row past the end reads row n-1 (no
+  // branch around a load, which would serialize them) and is zeroed below
+  uint32_t raw[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint64_t e = e0 + i < s.n ? e0 + i : s.n - 1;
+#pragma unroll
+    for (int k = 0; k < 3; k++) raw[i][k] = P[e * kRec + k * L + word];
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++)
-      bits[i][k] = (e0 + i < s.n) ? (((P[(e0 + i) * kRec + k * L + word] >> bit) & 1u) ^ flip) : 0u;
+      bits[i][k] = (e0 + i < s.n) ? (((raw[i][k] >> bit) & 1u) ^ flip) : 0u;
   b2a_bit_group<uint32_t>(A, rk_pair, nb, s, e0, bits, f);
   uint32_t loc[3] = {0, 0, 0};
 #pragma unroll
