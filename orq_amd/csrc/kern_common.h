@@ -98,63 +98,105 @@ __device__ inline void group_masks(const AesCtx<COPIES> &A, const uint32_t *__re
 // to the second, both are exchanged, d = first - second, y = d*d via
 // multiply_a (call 2).  Party p's zero share of a call is R_{p-1} - R_p, R_k
 // being pairwise key k's keystream, so d_p = D_{p-1} - D_p (+ the bits) with
-// D_k = R_k(call 0) - R_k(call 1).  The blocks are computed key by key (one
-// key schedule live at a time): first calls 0 and 1 of each key, then call 2.
+// D_k = R_k(call 0) - R_k(call 1), and call 2 adds R2_{p-1} - R2_p to y_p.
+// For 4-byte elements the blocks are computed key by key, the three calls of
+// a key together: one key schedule live at a time, three independent blocks
+// in flight.
 template <typename T>
 __device__ inline void b2a_bit_group(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
                                      const NonceTriple &nb, const CallSched &s, uint64_t e0,
                                      const T (&bits)[16 / sizeof(T)][3],
                                      T (&y)[16 / sizeof(T)][3]) {
   constexpr int E = 16 / sizeof(T);
-  uint64_t off[3];
-  uint32_t ctr[3];
+  if constexpr (sizeof(T) == 4) {
+    uint64_t off[3];
+    uint32_t ctr[3];
 #pragma unroll
-  for (int j = 0; j < 3; j++) {
-    uint32_t bib;
-    sched_locate(s, e0, j, off[j], ctr[j], bib);
-  }
-  T d[E][3];
-#pragma unroll
-  for (int i = 0; i < E; i++) {
-    d[i][0] = (T)(bits[i][0] ^ bits[i][1]);
-    d[i][1] = (T)(0 - bits[i][2]);
-    d[i][2] = 0;
-  }
-#pragma unroll 1
-  for (int k = 0; k < 3; k++) {
-    const uint32_t *rk = rk_pair + 60 * k;
-    T R0[E], R1[E];
-    unpack_block<T>(aes_ctr_block<kCopies>(A, rk, nb.v[k] + off[0], ctr[0]), R0);
-    unpack_block<T>(aes_ctr_block<kCopies>(A, rk, nb.v[k] + off[1], ctr[1]), R1);
+    for (int j = 0; j < 3; j++) {
+      uint32_t bib;
+      sched_locate(s, e0, j, off[j], ctr[j], bib);
+    }
+    T d[E][3], m[E][3];
 #pragma unroll
     for (int i = 0; i < E; i++) {
-      const T D = (T)(R0[i] - R1[i]);
-#pragma unroll
-      for (int p = 0; p < 3; p++) {
-        // key k is party k's "next" key (-D) and party k+1's "prev" key (+D)
-        if (p == k) d[i][p] -= D;
-        if (p == (k + 1) % 3) d[i][p] += D;
-      }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < E; i++)
-#pragma unroll
-    for (int p = 0; p < 3; p++) {
-      const int q = (p + 1) % 3;
-      y[i][p] = (T)(d[i][p] * d[i][p] + d[i][p] * d[i][q] + d[i][q] * d[i][p]);
+      d[i][0] = (T)(bits[i][0] ^ bits[i][1]);
+      d[i][1] = (T)(0 - bits[i][2]);
+      d[i][2] = 0;
+      m[i][0] = m[i][1] = m[i][2] = 0;
     }
 #pragma unroll 1
-  for (int k = 0; k < 3; k++) {
-    T R2[E];
-    unpack_block<T>(aes_ctr_block<kCopies>(A, rk_pair + 60 * k, nb.v[k] + off[2], ctr[2]), R2);
+    for (int k = 0; k < 3; k++) {
+      const uint32_t *rk = rk_pair + 60 * k;
+      T R0[E], R1[E], R2[E];
+      unpack_block<T>(aes_ctr_block<kCopies>(A, rk, nb.v[k] + off[0], ctr[0]), R0);
+      unpack_block<T>(aes_ctr_block<kCopies>(A, rk, nb.v[k] + off[1], ctr[1]), R1);
+      unpack_block<T>(aes_ctr_block<kCopies>(A, rk, nb.v[k] + off[2], ctr[2]), R2);
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const T D = (T)(R0[i] - R1[i]);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+          // key k is party k's "next" key (-) and party k+1's "prev" key (+)
+          if (p == k) {
+            d[i][p] -= D;
+            m[i][p] -= R2[i];
+          }
+          if (p == (k + 1) % 3) {
+            d[i][p] += D;
+            m[i][p] += R2[i];
+          }
+        }
+      }
+    }
 #pragma unroll
     for (int i = 0; i < E; i++)
 #pragma unroll
       for (int p = 0; p < 3; p++) {
-        if (p == k) y[i][p] -= R2[i];
-        if (p == (k + 1) % 3) y[i][p] += R2[i];
+        const int q = (p + 1) % 3;
+        y[i][p] = (T)(m[i][p] + d[i][p] * d[i][p] + d[i][p] * d[i][q] + d[i][q] * d[i][p]);
       }
+  } else {
+    // 8-byte elements (two per block): call by call, the three keys' blocks
+    // together, which measured faster for them
+    T d[E][3];
+    // the three calls are consumed one after the other to bound register use
+#pragma unroll 1
+    for (int j = 0; j < 3; j++) {
+      uint64_t off;
+      uint32_t ctr, bib;
+      sched_locate(s, e0, j, off, ctr, bib);
+      T R[3][E];  // [key][elem]
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        uint4 blk = aes_ctr_block<kCopies>(A, rk_pair + 60 * k, nb.v[k] + off, ctr);
+        unpack_block<T>(blk, R[k]);
+      }
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        if (j == 0) {
+#pragma unroll
+          for (int p = 0; p < 3; p++) {
+            T a = (T)(R[(p + 2) % 3][i] - R[p][i]);
+            if (p == 0) a += bits[i][0] ^ bits[i][1];
+            d[i][p] = a;
+          }
+        } else if (j == 1) {
+#pragma unroll
+          for (int p = 0; p < 3; p++) {
+            T b = (T)(R[(p + 2) % 3][i] - R[p][i]);
+            if (p == 1) b += bits[i][2];
+            d[i][p] -= b;
+          }
+        } else {
+#pragma unroll
+          for (int p = 0; p < 3; p++) {
+            int q = (p + 1) % 3, pm = (p + 2) % 3;
+            y[i][p] = (T)(R[pm][i] - R[p][i]) + d[i][p] * d[i][p] + d[i][p] * d[i][q] +
+                      d[i][q] * d[i][p];
+          }
+        }
+      }
+    }
   }
 }
 
