@@ -145,8 +145,10 @@ __host__ __device__ inline uint32_t choose_window(uint32_t k, uint32_t maskN) {
 // window the resolver discards or retries leaves an exact restart.
 __device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s, uint32_t n,
                                       uint32_t maskN, uint32_t nwork) {
-  const ResState &r = B.rs[((s - 1) & 1) * 3 + chain];
-  const WinDesc &p = B.wd[((s - 1) & 1) * 3 + chain];
+  // copies, so that every field is loaded at once rather than one load per
+  // short-circuited test (slot s - 1 is not written during this step)
+  const ResState r = B.rs[((s - 1) & 1) * 3 + chain];
+  const WinDesc p = B.wd[((s - 1) & 1) * 3 + chain];
   WinDesc w;
   w.active = 0;
   w.C = w.m = w.ovf = 0;
@@ -414,6 +416,10 @@ __device__ inline void draw_round(const AesCtx<COPIES> &A, const uint32_t *__res
 
 constexpr uint32_t kRows = kAmbCap / kGpThreads;  // 64
 constexpr uint32_t kWaves = kGpThreads / 64;
+// the resolver loads items (row, thread) of its ambiguous lists without a
+// per-item condition (a load behind a branch is waited for on its own): every
+// row < kRows indexes inside the list's kAmbCap entries
+static_assert(kRows * kGpThreads == kAmbCap, "resolver rows must tile the ambiguous lists");
 // LDS of a resolving workgroup (it uses the AES table space of the step kernel)
 struct ResolveLds {
   int32_t thr[kL2Cap];
@@ -452,6 +458,14 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_step(const uint32_t *__restri
   if (blockIdx.x < nres) {
     resolve_window(B, blockIdx.x, step, n, count, maskN, *reinterpret_cast<ResolveLds *>(lds));
   } else {
+  // the windows resolved at the previous step (committed below): loaded here,
+  // unconditionally, so that their latency overlaps the planning
+  uint32_t cvalid[3], cC[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    cvalid[c] = rs_prev[c].c_valid;
+    cC[c] = rs_prev[c].c_C;
+  }
   if (threadIdx.x < 3) {
     const int c = threadIdx.x;
     WinDesc w = plan_window(B, c, step, n, maskN, nwork);
@@ -481,7 +495,7 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_step(const uint32_t *__restri
   // theirs) take while the tiles' look-backs wait
   uint32_t nc[3], nt[3];
   for (int c = 0; c < 3; c++) {
-    nc[c] = rs_prev[c].c_valid ? (rs_prev[c].c_C + 16383) / 16384 : 0u;
+    nc[c] = cvalid[c] ? (cC[c] + 16383) / 16384 : 0u;
     nt[c] = s_plan[c].nt;
   }
   const uint32_t ntile = nt[0] + nt[1] + nt[2];
@@ -757,8 +771,10 @@ __device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uin
   uint32_t &s_l2 = L.l2;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t below = lanes_below();
-  const ResState &r = B.rs[((step - 1) & 1) * 3 + chain];
-  const WinDesc &p = B.wd[((step - 1) & 1) * 3 + chain];
+  // copies of slot step - 1 (not written during this step): every field is
+  // loaded at once, not one load per short-circuited test
+  const ResState r = B.rs[((step - 1) & 1) * 3 + chain];
+  const WinDesc p = B.wd[((step - 1) & 1) * 3 + chain];
   ResState &o = B.rs[(step & 1) * 3 + chain];
   const uint32_t kx = n - r.i;  // exact open positions at the window start
   // nothing drawn, or drawn on a state that did not hold: carry the state over
@@ -797,7 +813,8 @@ __device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uin
 #pragma unroll
     for (uint32_t j = 0; j < kHalf; j++) {
       const uint32_t q = h * kHalf + j, t = q * kGpThreads + threadIdx.x;
-      u[j] = q < rows && t < m ? amb_u[t] - delta : 0;
+      const int32_t x = amb_u[t];  // unconditional (t < kAmbCap): loads in flight together
+      u[j] = q < rows && t < m ? x - delta : 0;
     }
 #pragma unroll
     for (uint32_t j = 0; j < kHalf; j++) {
@@ -846,7 +863,8 @@ __device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uin
 #pragma unroll
       for (uint32_t j = 0; j < kHalf; j++) {
         const uint32_t q = h * kHalf + j, t = q * kGpThreads + threadIdx.x;
-        u[j] = q < rows && ((undm >> q) & 1) ? amb_u[t] - delta : 0;
+        const int32_t x = amb_u[t];
+        u[j] = q < rows && ((undm >> q) & 1) ? x - delta : 0;
       }
 #pragma unroll
       for (uint32_t j = 0; j < kHalf; j++) {
@@ -870,7 +888,8 @@ __device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uin
 #pragma unroll
     for (uint32_t q = 0; q < kHalf; q++) {
       const uint32_t t = q * kGpThreads + threadIdx.x;
-      a[q] = q < rows && (((rejm | undm) >> q) & 1) ? amb_a[t] : 0u;
+      const uint32_t x = amb_a[t];
+      a[q] = ((rejm | undm) >> q) & 1 ? x : 0u;  // (no bit is set at or past rows)
     }
     __syncthreads();
     // The undecided items, in order: item i is accepted iff fewer than T'_i of
@@ -934,7 +953,8 @@ __device__ void resolve_window(const ChainBufs &B, int chain, uint32_t step, uin
 #pragma unroll
         for (uint32_t q = 0; q < kHalf; q++) {
           const uint32_t qq = h * kHalf + q, t = qq * kGpThreads + threadIdx.x;
-          a[q] = qq < rows && ((rejm >> qq) & 1) ? amb_a[t] : 0u;
+          const uint32_t x = amb_a[t];
+          a[q] = (rejm >> qq) & 1 ? x : 0u;
         }
       }
 #pragma unroll
