@@ -27,10 +27,18 @@ struct PairArgs {
 template <typename T, int E>
 __device__ inline void load_run(const T *__restrict__ col, uint64_t row0, uint64_t m, uint64_t q0,
                                 T (&v)[E][3]) {
+  // rows past the run read its last row (callers run only with m >= 1) and
+  // are zeroed: no branch around a load, so the 3E loads are in flight together
 #pragma unroll
-  for (int i = 0; i < E; i++)
+  for (int i = 0; i < E; i++) {
+    const bool in = q0 + i < m;
+    const uint64_t q = in ? q0 + i : m - 1;
 #pragma unroll
-    for (int p = 0; p < 3; p++) v[i][p] = q0 + i < m ? col[3 * (row0 + q0 + i) + p] : (T)0;
+    for (int p = 0; p < 3; p++) {
+      const T x = col[3 * (row0 + q) + p];
+      v[i][p] = in ? x : (T)0;
+    }
+  }
 }
 template <typename T, int E>
 __device__ inline void store_run(T *__restrict__ col, uint64_t m, uint64_t q0, const T (&v)[E][3]) {
@@ -225,7 +233,10 @@ __global__ void __launch_bounds__(kAesThreads)
         } else {  // max / min: inner = mux(bg, acc, in)
           T B2[3];
 #pragma unroll
-          for (int p = 0; p < 3; p++) B2[p] = q0 + i < a.m ? bg[3 * (q0 + i) + p] : (T)0;
+          for (int p = 0; p < 3; p++) {
+            const T x = bg[3 * (q0 + i < a.m ? q0 + i : a.m - 1) + p];
+            B2[p] = q0 + i < a.m ? x : (T)0;
+          }
           mux3<T>(B2, AC[i], IN[i], ks, F[i]);
         }
       }
