@@ -71,10 +71,10 @@ ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
 # HBM bytes one k_round2 launch at 2^24 rows actually moves, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the factors a plain
 # 384 MiB device copy gives (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE):
-# (1080417.8 * 2.0 + 1049030.3 * 1.0) KiB; profiles/radix_sort_2p24_int64_kernels.txt.
+# (1080534.0 * 2.0 + 1049126.9 * 1.0) KiB; profiles/radix_sort_2p24_int64_kernels.txt.
 # The gather reads one 128-byte line per 64-byte element record; the writes
 # are the 64-byte records exactly.
-ROUND_TRAFFIC_2P24 = int((1080417.8 * 2.0 + 1049030.3 * 1.0) * 1024)
+ROUND_TRAFFIC_2P24 = int((1080534.0 * 2.0 + 1049126.9 * 1.0) * 1024)
 
 
 def _timed(ctx, fn, reps):
@@ -335,6 +335,8 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gen-keystream", action="store_true",
+                    help="skip the extra reserve that measures gen_perm's keystream rate")
     ap.add_argument("--workload", choices=["radix", "primitives", "table", "join", "serve"],
                     default="radix")
     ap.add_argument("--requests", type=int, default=120)
@@ -396,11 +398,17 @@ def main():
     # draw from the three pair-group PRGs, whose nonces advance by one per MiB
     # of keystream (four per refill of 2^20 draws).  Measured on one extra
     # reserve of the same 66 permutations, outside the timed region.
-    before = [ctx.nonce(GROUP_ROLE0 + g) for g in range(3)]
-    ctx.reserve(n, 66)
-    ctx.sync()
-    gen_ks_bytes = sum(ctx.nonce(GROUP_ROLE0 + g) - before[g] for g in range(3)) << 20
-    gen_ks_gbs = gen_ks_bytes / (fams["gen_perm"]["ms_per_step"] / 1e3) / 1e9
+    # (--no-gen-keystream skips it, so that a profiled run holds one step)
+    gen_ks = None
+    if not args.no_gen_keystream:
+        before = [ctx.nonce(GROUP_ROLE0 + g) for g in range(3)]
+        ctx.reserve(n, 66)
+        ctx.sync()
+        gen_ks_bytes = sum(ctx.nonce(GROUP_ROLE0 + g) - before[g] for g in range(3)) << 20
+        gen_ks_gbs = gen_ks_bytes / (fams["gen_perm"]["ms_per_step"] / 1e3) / 1e9
+        gen_ks = {"kernel": "k_gp_step", "bound": "lds", "keystream_bytes": gen_ks_bytes,
+                  "achieved": round(gen_ks_gbs, 1), "ceiling": AES_CEILING_GBS, "unit": "GB/s",
+                  "frac": round(gen_ks_gbs / AES_CEILING_GBS, 4)}
 
     # roofline of the dominant online kernel, k_round2 (64 bits x 2 plain rounds per step;
     # the third round of each bit, which also scatters, is timed as k_round2_scatter)
@@ -438,10 +446,7 @@ def main():
                      "keystream_GBps": round(ks_gbs, 1)},
         # the step's dominant kernel, k_gp_step (gen_perm's windows), is bound by
         # AES T-table lookups in LDS, not by HBM: its rate against the AES ceiling
-        "gen_perm_keystream": {"kernel": "k_gp_step", "bound": "lds",
-                               "keystream_bytes": gen_ks_bytes,
-                               "achieved": round(gen_ks_gbs, 1), "ceiling": AES_CEILING_GBS,
-                               "unit": "GB/s", "frac": round(gen_ks_gbs / AES_CEILING_GBS, 4)},
+        "gen_perm_keystream": gen_ks,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample)
