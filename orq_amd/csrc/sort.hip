@@ -139,23 +139,31 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
+  // a superblock's 12 words per thread, issued together: a row past the end
+  // reads row n-1 (no branch around a load, which would serialize them) and
+  // is zeroed when used
+  uint32_t raw[4][3];
+  auto load_raw = [&](uint32_t sb) {
+    const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint64_t e = e0 + i < s.n ? e0 + i : s.n - 1;
+#pragma unroll
+      for (int k = 0; k < 3; k++) raw[i][k] = P[e * kRec + k * L + word];
+    }
+  };
+  if (blockIdx.x < nsb) load_raw(blockIdx.x);
   for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
   const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
   uint32_t bits[4][3], f[4][3];
-  // the 12 loads are issued together: a row past the end reads row n-1 (no
-  // branch around a load, which would serialize them) and is zeroed below
-  uint32_t raw[4][3];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const uint64_t e = e0 + i < s.n ? e0 + i : s.n - 1;
-#pragma unroll
-    for (int k = 0; k < 3; k++) raw[i][k] = P[e * kRec + k * L + word];
-  }
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++)
       bits[i][k] = (e0 + i < s.n) ? (((raw[i][k] >> bit) & 1u) ^ flip) : 0u;
+  // the next superblock's words are in flight during this one's keystream
+  // (the last superblock reloads itself rather than branch)
+  load_raw(sb + gridDim.x < nsb ? sb + gridDim.x : sb);
   b2a_bit_group<uint32_t>(A, rk_pair, nb, s, e0, bits, f);
   uint32_t loc[3] = {0, 0, 0};
 #pragma unroll
