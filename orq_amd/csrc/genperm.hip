@@ -1276,7 +1276,7 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   // steps per graph batch: large permutations take ~50 steps each, so long
   // batches save host round trips; small ones finish in a few steps, and the
   // steps of a batch left after the last permutation are empty launches
-  const int kBatch = n >= (1u << 20) ? 64 : 16;
+  const int kBatch = n >= (1u << 22) ? 128 : n >= (1u << 20) ? 64 : 16;
   // The kernel takes all its state from device memory, so a batch of steps
   // is one static launch sequence, captured once as a graph: each step draws
   // window s (committing the windows resolved at step s - 1 as well) and
