@@ -288,3 +288,23 @@ def test_radix_sort_sizes(O, G, keys, n):
     rng = np.random.default_rng(n)
     k = rng.integers(0, 50, n).astype(np.uint32)
     _sort_case(O, G, keys, k, 4, False)
+
+
+def test_radix_sort_many_superblocks(G, keys):
+    """2^20 + 5 int64 rows: 257 superblocks of 4096 rows, more than the per-bit
+    kernels' one workgroup per CU, so a workgroup takes several (the next
+    one's words are prefetched) and the last is ragged.  Checked against the
+    plaintext stable sort and its ranks; the oracle is too slow at this size."""
+    n = (1 << 20) + 5
+    rng = np.random.default_rng(9)
+    k = rng.integers(-2**40, 2**40, n, dtype=np.int64)
+    k[::97] = k[5]  # duplicates: stability decides their order
+    gc = G.Ctx(keys, -12)
+    gv = gc.secret_share(k.view(np.uint64), 8)
+    gperm = gc.radix_sort(gv)
+    order = np.argsort(k, kind="stable")
+    np.testing.assert_array_equal(gc.open(gv).view(np.int64), k[order])
+    ranks = np.empty(n, dtype=np.int64)
+    ranks[order] = np.arange(n)
+    np.testing.assert_array_equal(gc.open(gperm).view(np.int32), ranks)
+    assert gc.queue_size() == 0
