@@ -294,6 +294,7 @@ typedef struct {
   int status;                        /* ORQG_OK or an ORQG_E_* code */
   int slot;                          /* worker that ran it, -1 if rejected */
   double t_submit_ms, t_start_ms, t_done_ms; /* server clock */
+  char msg[192];                     /* error text when status != ORQG_OK */
 } orqg_completion;
 typedef struct {
   int device;
@@ -303,6 +304,9 @@ typedef struct {
 } orqg_server_config;
 typedef struct {
   uint64_t submitted, completed, rejected, max_running, peak_reserved, peak_live_bytes;
+  uint64_t failed;      /* completed with a status other than ORQG_OK */
+  uint64_t mallocs;     /* hipMalloc calls of the process's allocator */
+  uint64_t pool_trims;  /* pooled blocks it released (pool cap, or memory pressure) */
 } orqg_server_stats;
 typedef struct orqg_server orqg_server;
 int orqg_server_create(const orqg_server_config *cfg, orqg_server **out);

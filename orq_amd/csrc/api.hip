@@ -77,6 +77,12 @@ namespace orqg {
 // hipFree (which waits for the whole device) sits on the steady-state path.
 // g_live_bytes counts everything held, pooled blocks included, and is what a
 // context's HBM budget is checked against.
+//
+// A pool holds at most kPoolCap bytes: a served stream of many sizes would
+// otherwise keep one set of blocks per size class it has seen.  Over the cap
+// the oldest blocks are released.  Released blocks are handed to hipFree only
+// after the allocator lock is dropped: hipFree waits for all device work, and
+// other contexts' threads must not queue behind that wait for the lock.
 struct Block {
   size_t bytes;
   orqg_ctx *owner;
@@ -85,10 +91,18 @@ static size_t g_live_bytes = 0;
 static std::unordered_map<void *, Block> g_blocks;
 static std::unordered_set<orqg_ctx *> g_alive;
 static std::mutex g_alloc_mu;
+static uint64_t g_age = 0, g_mallocs = 0, g_trimmed = 0;
+constexpr size_t kPoolCap = (size_t)24 << 30;
 
 size_t live_bytes() {
   std::lock_guard<std::mutex> lk(g_alloc_mu);
   return g_live_bytes;
+}
+
+void alloc_counters(uint64_t *mallocs, uint64_t *trimmed) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  if (mallocs) *mallocs = g_mallocs;
+  if (trimmed) *trimmed = g_trimmed;
 }
 
 static size_t size_class(size_t b) {
@@ -98,16 +112,35 @@ static size_t size_class(size_t b) {
   return (b + step - 1) / step * step;
 }
 
-// hipFree every pooled block of every context (caller holds g_alloc_mu)
-static void release_pools_locked() {
-  for (orqg_ctx *c : g_alive) {
-    for (auto &kv : c->pool) {
-      (void)hipFree(kv.second);
-      g_live_bytes -= kv.first;
-      g_blocks.erase(kv.second);
+// take the oldest pooled block of c out of the books (lock held); the caller
+// hipFrees it after dropping the lock
+static void *evict_oldest_locked(orqg_ctx *c) {
+  Pool &P = c->pool;
+  auto it = P.by_age.begin();
+  const size_t cls = it->second.first;
+  void *p = it->second.second;
+  auto range = P.by_size.equal_range(cls);
+  for (auto j = range.first; j != range.second; ++j)
+    if (j->second.first == p) {
+      P.by_size.erase(j);
+      break;
     }
-    c->pool.clear();
-  }
+  P.by_age.erase(it);
+  P.bytes -= cls;
+  g_live_bytes -= cls;
+  g_blocks.erase(p);
+  g_trimmed++;
+  return p;
+}
+
+// every pooled block of every context (lock held)
+static void take_all_pools_locked(std::vector<void *> &out) {
+  for (orqg_ctx *c : g_alive)
+    while (!c->pool.by_age.empty()) out.push_back(evict_oldest_locked(c));
+}
+
+static void free_blocks(const std::vector<void *> &v) {
+  for (void *p : v) (void)hipFree(p);
 }
 
 void register_ctx(orqg_ctx *c) {
@@ -118,61 +151,87 @@ void register_ctx(orqg_ctx *c) {
 // The context's pooled blocks are released; blocks it still lends out (vectors
 // the caller has not freed) are released by hipFree when they are freed.
 void unregister_ctx(orqg_ctx *c) {
-  std::lock_guard<std::mutex> lk(g_alloc_mu);
-  for (auto &kv : c->pool) {
-    (void)hipFree(kv.second);
-    g_live_bytes -= kv.first;
-    g_blocks.erase(kv.second);
+  std::vector<void *> victims;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    while (!c->pool.by_age.empty()) victims.push_back(evict_oldest_locked(c));
+    for (auto &kv : g_blocks)
+      if (kv.second.owner == c) kv.second.owner = nullptr;
+    g_alive.erase(c);
   }
-  c->pool.clear();
-  for (auto &kv : g_blocks)
-    if (kv.second.owner == c) kv.second.owner = nullptr;
-  g_alive.erase(c);
+  free_blocks(victims);
 }
 
 void *dev_alloc(orqg_ctx *c, size_t bytes) {
   const size_t cls = size_class(bytes ? bytes : 16);
-  std::lock_guard<std::mutex> lk(g_alloc_mu);
-  auto it = c->pool.find(cls);
-  if (it != c->pool.end()) {
-    void *p = it->second;
-    c->pool.erase(it);
-    return p;
+  std::vector<void *> victims;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    Pool &P = c->pool;
+    auto it = P.by_size.find(cls);
+    if (it != P.by_size.end()) {
+      void *p = it->second.first;
+      P.by_age.erase(it->second.second);
+      P.by_size.erase(it);
+      P.bytes -= cls;
+      return p;
+    }
+    if (c->budget && g_live_bytes + cls > c->budget) take_all_pools_locked(victims);
   }
-  if (c->budget && g_live_bytes + cls > c->budget) {
-    release_pools_locked();
-    if (g_live_bytes + cls > c->budget)
+  free_blocks(victims);
+  victims.clear();
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    if (c->budget && g_live_bytes + cls > c->budget)
       throw Error(ORQG_E_BUDGET, "allocation exceeds the configured HBM budget");
   }
   void *p = nullptr;
   if (hipMalloc(&p, cls) != hipSuccess) {
     (void)hipGetLastError();
-    release_pools_locked();
+    {
+      std::lock_guard<std::mutex> lk(g_alloc_mu);
+      take_all_pools_locked(victims);
+    }
+    free_blocks(victims);
     if (hipMalloc(&p, cls) != hipSuccess) {
       (void)hipGetLastError();
-      throw Error(ORQG_E_OOM, "hipMalloc failed");
+      throw Error(ORQG_E_OOM, "hipMalloc of " + std::to_string(cls) + " bytes failed");
     }
   }
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_mallocs++;
   g_live_bytes += cls;
   g_blocks[p] = {cls, c};
   return p;
 }
+
 void dev_free(void *p) {
   if (!p) return;
-  std::lock_guard<std::mutex> lk(g_alloc_mu);
-  auto it = g_blocks.find(p);
-  if (it == g_blocks.end()) {
-    (void)hipFree(p);
-    return;
+  std::vector<void *> victims;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_blocks.find(p);
+    if (it == g_blocks.end()) {
+      victims.push_back(p);  // not ours (never expected)
+    } else {
+      orqg_ctx *owner = it->second.owner;
+      const size_t cls = it->second.bytes;
+      if (owner && g_alive.count(owner)) {
+        Pool &P = owner->pool;
+        const uint64_t age = ++g_age;
+        auto pos = P.by_size.emplace(cls, std::make_pair(p, age));
+        (void)pos;
+        P.by_age.emplace(age, std::make_pair(cls, p));
+        P.bytes += cls;
+        while (P.bytes > kPoolCap) victims.push_back(evict_oldest_locked(owner));
+      } else {
+        g_live_bytes -= cls;
+        g_blocks.erase(it);
+        victims.push_back(p);
+      }
+    }
   }
-  orqg_ctx *owner = it->second.owner;
-  if (owner && g_alive.count(owner)) {
-    owner->pool.emplace(it->second.bytes, p);
-    return;
-  }
-  (void)hipFree(p);
-  g_live_bytes -= it->second.bytes;
-  g_blocks.erase(it);
+  free_blocks(victims);
 }
 }  // namespace orqg
 

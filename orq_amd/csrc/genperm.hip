@@ -268,7 +268,10 @@ __device__ inline bool task_tile(const uint32_t (&nt)[3], uint32_t task, int &ch
 // Tiles are taken in increasing order from a ticket, so every tile waited on
 // belongs to a running (or finished) workgroup and publishes its count before
 // waiting itself.  Run by one whole wave; returns the exclusive prefix.
-constexpr uint32_t kSpinMax = 1u << 20;
+// A wait is bounded in time (the 100 MHz constant clock), not in polls, so
+// that a GPU shared with other streams' kernels cannot trip it early: a wait
+// longer than kSpinTicks reports error flag 16 instead of hanging.
+constexpr uint64_t kSpinTicks = 400000000ull;  // 4 s
 
 __device__ inline uint32_t look_back(uint64_t *stat, uint32_t tile, uint32_t seq, uint32_t cnt,
                                      uint32_t *err) {
@@ -286,7 +289,8 @@ __device__ inline uint32_t look_back(uint64_t *stat, uint32_t tile, uint32_t seq
     const int32_t idx = base - lane;
     uint32_t kind = 2, val = 0;  // before tile 0: an inclusive prefix of 0
     if (idx >= 0) {
-      for (uint32_t spin = 0;; spin++) {
+      const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+      for (;;) {
         const uint64_t s =
             __hip_atomic_load(stat + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t hi = (uint32_t)(s >> 32);
@@ -295,7 +299,7 @@ __device__ inline uint32_t look_back(uint64_t *stat, uint32_t tile, uint32_t seq
           val = (uint32_t)s;
           break;
         }
-        if (spin >= kSpinMax) {  // never expected: give up loudly instead of hanging
+        if (__builtin_amdgcn_s_memrealtime() - t0 > kSpinTicks) {  // never expected
           atomicOr(err, 16u);
           kind = 2;
           break;

@@ -97,6 +97,14 @@ struct Scratch {
   size_t bytes = 0;
 };
 
+// A context's free device blocks (caching allocator, guarded by the allocator
+// lock): by size class for reuse, and by age for trimming the oldest first.
+struct Pool {
+  std::multimap<size_t, std::pair<void *, uint64_t>> by_size;  // class -> (block, age)
+  std::map<uint64_t, std::pair<size_t, void *>> by_age;        // age -> (class, block)
+  size_t bytes = 0;
+};
+
 struct ShardedPerm {
   size_t n = 0;
   int32_t *fwd[3] = {nullptr, nullptr, nullptr};  // pi_g      (scatter form)
@@ -126,7 +134,7 @@ struct orqg_ctx {
   std::string err;
   std::vector<orqg_msg> log;
   std::map<std::string, orqg::Scratch> scratch;
-  std::multimap<size_t, void *> pool;  // free device blocks by size (allocator lock held)
+  orqg::Pool pool;  // free device blocks of this context
   std::deque<orqg::ShardedPerm> queue;
   uint32_t *fy_err = nullptr;       // swap-resolution check flags, read by check_swaps
   bool fy_pending = false;
@@ -165,6 +173,10 @@ namespace orqg {
 void *dev_alloc(orqg_ctx *c, size_t bytes);
 void dev_free(void *p);
 size_t live_bytes();
+// allocator counters since load: hipMalloc calls, pooled blocks released by
+// trimming (a context's pool over its cap, or every pool after a failed
+// hipMalloc / an allocation over the budget)
+void alloc_counters(uint64_t *mallocs, uint64_t *trimmed);
 
 // Scoped timing of one kernel family on the context stream: two events
 // bracket the region's launches; nothing waits for them here.

@@ -17,6 +17,7 @@
 // bits).
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -147,13 +148,27 @@ void worker(orqg_server *s, int slot) {
     done.id = p.req.id;
     done.t_submit_ms = p.t_submit;
     done.t_start_ms = s->now_ms();
+    auto note = [&](const char *m) { std::snprintf(done.msg, sizeof(done.msg), "%s", m); };
     try {
       run_request(c, p.req);
       done.status = ORQG_OK;
     } catch (const Error &e) {
       done.status = e.code;
+      note(e.what());
+    } catch (const std::exception &e) {
+      done.status = ORQG_E_DEVICE;
+      note(e.what());
     } catch (...) {
       done.status = ORQG_E_DEVICE;
+      note("unknown exception");
+    }
+    if (done.status != ORQG_OK) {
+      // leave the slot's context clean for the next request: wait for both
+      // streams and read (and so clear) a pending swap-resolution check, so
+      // that this failure is not reported again by the next request's rekey
+      (void)orqg_sync(c);
+      (void)hipStreamSynchronize(c->stream2);
+      (void)hipGetLastError();
     }
     done.t_done_ms = s->now_ms();
     done.slot = slot;
@@ -162,6 +177,7 @@ void worker(orqg_server *s, int slot) {
       s->reserved -= p.need;
       s->running--;
       s->stats.completed++;
+      if (done.status != ORQG_OK) s->stats.failed++;
       s->stats.peak_live_bytes = std::max<uint64_t>(s->stats.peak_live_bytes, live_bytes());
       s->done.push_back(done);
     }
@@ -252,6 +268,7 @@ int orqg_server_get_stats(orqg_server *s, orqg_server_stats *st) {
   if (!s || !st) return ORQG_E_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
   *st = s->stats;
+  orqg::alloc_counters(&st->mallocs, &st->pool_trims);
   return ORQG_OK;
 }
 

@@ -20,7 +20,7 @@ class Request(ctypes.Structure):
 class Completion(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("slot", ctypes.c_int),
                 ("t_submit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double),
-                ("t_done_ms", ctypes.c_double)]
+                ("t_done_ms", ctypes.c_double), ("msg", ctypes.c_char * 192)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -31,7 +31,9 @@ class ServerConfig(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
                 ("rejected", ctypes.c_uint64), ("max_running", ctypes.c_uint64),
-                ("peak_reserved", ctypes.c_uint64), ("peak_live_bytes", ctypes.c_uint64)]
+                ("peak_reserved", ctypes.c_uint64), ("peak_live_bytes", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("mallocs", ctypes.c_uint64),
+                ("pool_trims", ctypes.c_uint64)]
 
 
 class Job:
@@ -85,6 +87,7 @@ class Server:
             c = out[i]
             job = self.jobs.pop(c.id)
             job.completion = (c.status, c.slot, c.t_submit_ms, c.t_start_ms, c.t_done_ms)
+            job.error = c.msg.decode(errors="replace") if c.status else ""
             res.append(job)
         return res
 
