@@ -1177,11 +1177,8 @@ __global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r_all, size_t
 void free_perm(ShardedPerm &p) {
   if (p.ready) (void)hipEventDestroy(p.ready);
   p.ready = nullptr;
-  for (int g = 0; g < 3; g++) {
-    dev_free(p.fwd[g]);
-    dev_free(p.inv[g]);
-    p.fwd[g] = p.inv[g] = nullptr;
-  }
+  for (int g = 0; g < 3; g++) p.fwd[g] = p.inv[g] = nullptr;
+  p.slab.reset();
 }
 
 // PermutationManager::reserve for 3PC (permutation_manager.h:102-181):
@@ -1207,12 +1204,21 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   const uint32_t maskN = 1u << bits;
   const uint32_t mask = maskN - 1;
 
+  // the six arrays of every new permutation in one allocation (a served
+  // stream of many sizes then costs one allocator entry per reserve, not six
+  // per permutation); a consumer of the last of them waits for that one's
+  // swap resolution, which is the last second-stream work on the slab
   std::vector<ShardedPerm> fresh(count);
-  for (auto &p : fresh) {
+  const size_t stride = ((size_t)n * 4 + 16 + 255) / 256 * 256;
+  void *slab_p = dev_alloc(c, stride * 6 * count);
+  std::shared_ptr<void> slab(slab_p, [](void *q) { dev_free(q); });
+  for (uint32_t k = 0; k < count; k++) {
+    ShardedPerm &p = fresh[k];
     p.n = n;
+    p.slab = slab;
     for (int g = 0; g < 3; g++) {
-      p.fwd[g] = (int32_t *)dev_alloc(c, (size_t)n * 4 + 16);
-      p.inv[g] = (int32_t *)dev_alloc(c, (size_t)n * 4 + 16);
+      p.fwd[g] = (int32_t *)((char *)slab_p + stride * (6ull * k + g));
+      p.inv[g] = (int32_t *)((char *)slab_p + stride * (6ull * k + 3 + g));
     }
   }
   if (n == 0) {

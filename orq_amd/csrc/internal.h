@@ -5,6 +5,7 @@
 
 #include <deque>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -110,6 +111,9 @@ struct ShardedPerm {
   int32_t *fwd[3] = {nullptr, nullptr, nullptr};  // pi_g      (scatter form)
   int32_t *inv[3] = {nullptr, nullptr, nullptr};  // pi_g^{-1} (gather form)
   hipEvent_t ready = nullptr;  // recorded after its swap resolution (second stream)
+  // one device allocation holds every permutation of a reserve call; it goes
+  // back to the allocator when the last of them is freed
+  std::shared_ptr<void> slab;
 };
 
 struct Timer {
