@@ -23,10 +23,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X LDS read bandwidth: 256 CUs x 128 B/clk x 2.4 GHz
+LDS_PEAK_GBS = 78643.2
 # keystream rate of the T-table AES-256 CTR core with its tables in LDS
 # (measured, 2^30 counter blocks): the ceiling of every keystream-bound kernel
 AES_CEILING_GBS = 1100.0
+# LDS bytes the AES core reads per byte of keystream: 224 four-byte table
+# lookups (13 full rounds x 16 + the last round's 16) per 16-byte block
+LDS_BYTES_PER_KS_BYTE = 224 * 4 / 16
 GROUP_ROLE0 = 6  # PRG roles 6..8: the pair groups {0,1}, {0,2}, {1,2}
+# HBM bytes per row of one 64-bit radix sort in the lower-bound model of one
+# read and one write per share buffer at each global dependency: per bit
+# 48 W + 304 (W = 16, the reference's __int128 padded element), plus the
+# unpadding tail 6 W + 560
+SORT_MODEL_W = 16
+SORT_MODEL_BYTES_PER_ROW = 64 * (48 * SORT_MODEL_W + 304) + 6 * SORT_MODEL_W + 560
+# keystream the online phase draws per row per bit: the b2a_bit and
+# multiply_a zero sharings and the six resharing rounds' masks (10.5 AES
+# blocks)
+ONLINE_KEYSTREAM_PER_ROW_BIT = 168
 
 
 def zipf_keys(n, s=1.1, domain=1 << 20, seed=0x0A11CE03):
@@ -75,6 +90,10 @@ ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
 # The gather reads one 128-byte line per 64-byte element record; the writes
 # are the 64-byte records exactly.
 ROUND_TRAFFIC_2P24 = int((1080559.4 * 2.0 + 1049075.5 * 1.0) * 1024)
+# HBM bytes per k_gp_step launch averaged over the 3250 launches of one 2^24
+# sort, the same way (profiles/gen_perm_k_gp_step_2p24_counters.txt); None
+# until that profile exists
+GP_STEP_TRAFFIC_2P24 = None
 
 
 def _timed(ctx, fn, reps):
@@ -430,6 +449,25 @@ def main():
                   "achieved": round(gen_ks_gbs, 1), "ceiling": AES_CEILING_GBS, "unit": "GB/s",
                   "frac": round(gen_ks_gbs / AES_CEILING_GBS, 4)}
 
+    # the dominant kernel, k_gp_step (gen_perm's windows): bound by the AES
+    # T-table lookups it reads from LDS.  Its launches are timed in graph
+    # batches (kBatch steps + one commit-only launch per event pair).
+    gp_ms, gp_cnt = ctx.timer("k_gp_step")
+    gp_launch_ms = gp_ms / max(gp_cnt, 1)
+    roofline = None
+    if gen_ks is not None and gp_cnt:
+        lds_bytes = gen_ks["keystream_bytes"] * LDS_BYTES_PER_KS_BYTE  # per step (one sort)
+        lds_per_launch = lds_bytes / (gp_cnt / args.steps)
+        lds_gbs = lds_per_launch / (gp_launch_ms / 1e3) / 1e9
+        roofline = {"kernel": "k_gp_step", "bound": "lds", "achieved": round(lds_gbs, 1),
+                    "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4),
+                    "traffic": GP_STEP_TRAFFIC_2P24 if n == 1 << 24 else None,
+                    "avg_launch_us": round(gp_launch_ms * 1e3, 1),
+                    "launches_per_step": gp_cnt // args.steps,
+                    "bytes_per_launch": int(lds_per_launch),
+                    "keystream_GBps": round(gen_ks["keystream_bytes"] * args.steps / (gp_ms / 1e3)
+                                            / 1e9, 1),
+                    "aes_ceiling_GBps": AES_CEILING_GBS}
     # roofline of the dominant online kernel, k_round2 (64 bits x 2 plain rounds per step;
     # the third round of each bit, which also scatters, is timed as k_round2_scatter)
     r_ms, r_cnt = ctx.timer("k_round2")
@@ -437,6 +475,23 @@ def main():
     achieved = n * ROUND_BYTES_PER_ROW / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
     ks_gbs = n * ROUND_KEYSTREAM_PER_ROW / (per_launch_ms / 1e3) / 1e9 if per_launch_ms > 0 else 0.0
 
+    # the whole step under the lower-bound HBM model (declared W), end to end
+    # and online only, beside the keystream it must generate: at the AES
+    # ceiling that keystream alone takes ks_floor_ms
+    model_bytes = n * SORT_MODEL_BYTES_PER_ROW
+    online_ks = n * 64 * ONLINE_KEYSTREAM_PER_ROW_BIT
+    step_ks = online_ks + (gen_ks["keystream_bytes"] if gen_ks else 0)
+    step_model = {
+        "model": f"(48W+304) B/row/bit x 64 + (6W+560) B/row tail, W={SORT_MODEL_W}",
+        "bytes_per_step": model_bytes, "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "achieved": round(model_bytes / t_step / 1e9, 1),
+        "frac": round(model_bytes / t_step / 1e9 / HBM_PEAK_GBS, 4),
+        "online_achieved": round(model_bytes / (online_ms / 1e3) / 1e9, 1) if online_ms else None,
+        "online_frac": round(model_bytes / (online_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+        if online_ms else None,
+        "keystream_bytes_per_step": step_ks if gen_ks else None,
+        "keystream_floor_ms": round(step_ks / (AES_CEILING_GBS * 1e9) * 1e3, 1) if gen_ks else None,
+    }
     line = {
         "metric": "Oblivious radix sort Mrows/s @2^24x64b 3PC",
         "value": round(n / t_step / 1e6, 4),
@@ -457,15 +512,16 @@ def main():
         "online_ms": round(online_ms, 2),
         "gen_perm_ms": fams["gen_perm"]["ms_per_step"],
         "families": fams,
-        "roofline": {"kernel": "k_round2", "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": roofline,
+        "roofline_step": step_model,
+        "roofline_online": {"kernel": "k_round2", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": ROUND_TRAFFIC_2P24 if n == 1 << 24 else None,
                      "avg_launch_us": round(per_launch_ms * 1e3, 1),
                      "bytes_per_launch": n * ROUND_BYTES_PER_ROW,
                      "keystream_GBps": round(ks_gbs, 1)},
-        # the step's dominant kernel, k_gp_step (gen_perm's windows), is bound by
-        # AES T-table lookups in LDS, not by HBM: its rate against the AES ceiling
+        # gen_perm's keystream rate against the AES ceiling
         "gen_perm_keystream": gen_ks,
     }
     if not args.no_cpu:

@@ -68,6 +68,7 @@ def lib():
             ("oc_bitonic_sort", [vp, vp, sz, vp, vp, sz, vp, sz]),
             ("oc_aggregate", [vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, sz, i32, vp]),
             ("oc_distinct", [vp, vp, sz, vp]),
+            ("oc_local", [vp, i32, vp, vp, vp]),
             ("oc_vec_resize", [vp, sz]),
             ("oc_adder", [vp, i32, vp, vp, vp]),
             ("oc_secret_share_from", [vp, vp, vp, i32, i32]),
@@ -175,6 +176,26 @@ class Ctx:
         out = np.zeros(v.n, dtype=_DT[v.width])
         _check(lib().oc_open(self.h, v.h, int(arith), _ptr(out)), "open")
         return out
+
+    LOCAL = {"xor_b": 0, "add_a": 1, "sub_a": 2, "not_b": 3, "neg_a": 4}
+
+    def local(self, op, x, y=None):
+        """Message-free share ops: xor_b, add_a, sub_a, not_b, neg_a."""
+        z = Vec(x.n, x.width)
+        _check(lib().oc_local(self.h, self.LOCAL[op], x.h, (y or x).h, z.h), op)
+        return z
+
+    def xor_b(self, x, y):
+        return self.local("xor_b", x, y)
+
+    def add_a(self, x, y):
+        return self.local("add_a", x, y)
+
+    def sub_a(self, x, y):
+        return self.local("sub_a", x, y)
+
+    def not_b(self, x):
+        return self.local("not_b", x)
 
     def and_b(self, x, y):
         z = Vec(x.n, x.width)

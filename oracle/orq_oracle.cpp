@@ -1566,6 +1566,29 @@ void a2b(oc_ctx *c, const oc_vec *x, oc_vec *y) {
 }  // namespace
 
 // =================================================================== C ABI
+// Local (message-free) share operations, each party on its two share copies:
+// op 0 xor_b (z = x ^ y, protocol.h:649), 1 add_a (z = x + y, protocol.h:561),
+// 2 sub_a (z = x - y, protocol.h:570), 3 not_b (z = ~x, replicated_3pc.h:199),
+// 4 neg_a (z = -x, protocol.h:587).  y is not read by ops 3 and 4.
+template <typename T>
+static void local_op(int op, const oc_vec *x, const oc_vec *y, oc_vec *z) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = (const T *)x->sh[p][k].data();
+      const T *b = (const T *)y->sh[p][k].data();
+      T *d = (T *)z->sh[p][k].data();
+      for (size_t i = 0; i < x->n; i++) {
+        switch (op) {
+          case 0: d[i] = a[i] ^ b[i]; break;
+          case 1: d[i] = (T)(a[i] + b[i]); break;
+          case 2: d[i] = (T)(a[i] - b[i]); break;
+          case 3: d[i] = (T)~a[i]; break;
+          default: d[i] = (T)(T(0) - a[i]); break;
+        }
+      }
+    }
+}
+
 extern "C" {
 
 oc_ctx *oc_create(const uint8_t *keys /* 10 x 32 bytes */, long batch_size, int gen_everyone) {
@@ -1682,6 +1705,20 @@ int oc_open(oc_ctx *c, oc_vec *v, int arith, void *out) {
       std::memcpy(out, o.data(), o.size() * 8);
     } else
       throw std::runtime_error("open: width");
+  })
+}
+
+int oc_local(oc_ctx *, int op, oc_vec *x, oc_vec *y, oc_vec *z) {
+  OC_TRY({
+    if (op < 0 || op > 4 || x->n != z->n || x->width != z->width ||
+        (op < 3 && (y->n != x->n || y->width != x->width)))
+      throw std::runtime_error("local op: argument");
+    if (x->width == 4)
+      local_op<uint32_t>(op, x, y, z);
+    else if (x->width == 8)
+      local_op<uint64_t>(op, x, y, z);
+    else
+      throw std::runtime_error("local op: width");
   })
 }
 

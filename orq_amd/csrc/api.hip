@@ -31,7 +31,7 @@ void orqg_ctx::timer_flush() {
     ORQG_HIP(hipEventElapsedTime(&ms, ev_pool[t.a], ev_pool[t.b]));
     auto &tm = timers[t.fam];
     tm.ms += ms;
-    tm.launches += 1;
+    tm.launches += t.launches;
   }
   pending.clear();
   ev_used = 0;

@@ -1359,7 +1359,12 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   const uint64_t max_windows = 64ull * count * (1 + (uint64_t)n / 1024) + (1u << 20);
   for (uint64_t it = 0;; it++) {
     if (it * kBatch > 2 * max_windows) throw Error(ORQG_E_PARITY, "gen_perm: window limit exceeded");
-    ORQG_HIP(hipGraphLaunch(gexec, c->stream));
+    {
+      // family k_gp_step: the batch's kBatch step launches and its
+      // commit-only launch, back to back in one graph
+      TimedRegion ts(c, "k_gp_step", kBatch + 1);
+      ORQG_HIP(hipGraphLaunch(gexec, c->stream));
+    }
     hipLaunchKernelGGL(k_gp_done, dim3(1), dim3(64), 0, c->stream, B, flag);
     ORQG_HIP(hipGetLastError());
     ORQG_HIP(hipMemcpyAsync(host_flag, flag, sizeof(host_flag), hipMemcpyDeviceToHost, c->stream));

@@ -153,6 +153,7 @@ struct orqg_ctx {
   struct PendingTimer {
     std::string fam;
     size_t a, b;
+    uint64_t launches;  // kernel launches the region holds
   };
   std::vector<PendingTimer> pending;
   size_t timer_event();
@@ -188,7 +189,8 @@ struct TimedRegion {
   orqg_ctx *c;
   const char *fam;
   size_t a = 0;
-  TimedRegion(orqg_ctx *c_, const char *f) : c(c_), fam(f) {
+  uint64_t launches;
+  TimedRegion(orqg_ctx *c_, const char *f, uint64_t nl = 1) : c(c_), fam(f), launches(nl) {
     if (c->timing) {
       a = c->timer_event();
       ORQG_HIP(hipEventRecord(c->ev_pool[a], c->stream));
@@ -198,7 +200,7 @@ struct TimedRegion {
     if (c->timing) {
       size_t b = c->timer_event();
       ORQG_HIP(hipEventRecord(c->ev_pool[b], c->stream));
-      c->pending.push_back({fam, a, b});
+      c->pending.push_back({fam, a, b, launches});
     }
   }
 };

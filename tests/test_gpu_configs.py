@@ -94,16 +94,16 @@ def test_c2_primitives_2p24_uint64(O, G):
     ox, gx = oc.secret_share(x, 8), gc.secret_share(x, 8)
     oy, gy = oc.secret_share(y, 8), gc.secret_share(y, 8)
     _same_shares(ox, gx)
-    oz, gz = oc.xor_b(ox, oy), gc.xor_b(gx, gy)
-    _same_shares(oz, gz)
-    np.testing.assert_array_equal(gc.open(gz), x ^ y)
+    oz, gx_y = oc.xor_b(ox, oy), gc.xor_b(gx, gy)
+    _same_shares(oz, gx_y)
     del oz
     oz, gz = oc.and_b(ox, oy), gc.and_b(gx, gy)
     _same_shares(oz, gz)
-    np.testing.assert_array_equal(gc.open(gz), x & y)
     _same_nonces(oc, gc)
-    _same_log(oc, gc)
+    _same_log(oc, gc)  # before any open, which only the GPU side does below
     del ox, oy, oz, oc
+    np.testing.assert_array_equal(gc.open(gx_y), x ^ y)
+    np.testing.assert_array_equal(gc.open(gz), x & y)
     sx, sy = x.view(np.int64), y.view(np.int64)
     np.testing.assert_array_equal(gc.open(gc.cmp("==", gx, gy)), (x == y).astype(np.uint64))
     np.testing.assert_array_equal(gc.open(gc.cmp("<", gx, gy)), (sx < sy).astype(np.uint64))

@@ -308,3 +308,32 @@ def test_radix_sort_many_superblocks(G, keys):
     ranks[order] = np.arange(n)
     np.testing.assert_array_equal(gc.open(gperm).view(np.int32), ranks)
     assert gc.queue_size() == 0
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("n", [1, 4097, 70001])
+def test_local_ops(O, G, keys, width, n):
+    """xor_b, not_b (boolean) and add_a, sub_a (arithmetic): shares equal the
+    oracle's, values equal the closed forms, and no message or randomness is
+    used (nonces and log unchanged)."""
+    rng = np.random.default_rng(n * width)
+    dt = np.uint32 if width == 4 else np.uint64
+    x = rng.integers(0, 2**63, n, dtype=np.uint64).astype(dt)
+    y = rng.integers(0, 2**63, n, dtype=np.uint64).astype(dt)
+    oc, gc = _pair(O, G, keys)
+    ox, gx = oc.secret_share(x, width), gc.secret_share(x, width)
+    oy, gy = oc.secret_share(y, width), gc.secret_share(y, width)
+    oa, ga = oc.secret_share(x, width, arith=True), gc.secret_share(x, width, arith=True)
+    ob, gb = oc.secret_share(y, width, arith=True), gc.secret_share(y, width, arith=True)
+    log0 = gc.msglog()
+    with np.errstate(over="ignore"):
+        for name, ov, gv, want in [
+            ("xor_b", oc.xor_b(ox, oy), gc.xor_b(gx, gy), x ^ y),
+            ("not_b", oc.not_b(ox), gc.not_b(gx), ~x),
+            ("add_a", oc.add_a(oa, ob), gc.add_a(ga, gb), x + y),
+            ("sub_a", oc.sub_a(oa, ob), gc.sub_a(ga, gb), x - y),
+        ]:
+            _same_shares(ov, gv)
+            np.testing.assert_array_equal(gc.open(gv), want, err_msg=name)
+    _same_nonces(oc, gc)
+    assert [m for m in gc.msglog() if m[0] != "open"] == [m for m in log0 if m[0] != "open"]

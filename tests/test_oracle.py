@@ -278,3 +278,28 @@ def test_radix_sort_partial_bits(ctx):
     ctx.radix_sort(v, bits=8)
     order = np.argsort(k & 0xFF, kind="stable")
     np.testing.assert_array_equal(ctx.open(v), k[order])
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_local_ops_closed_forms(keys, width):
+    """xor_b, add_a, sub_a, not_b, neg_a (protocol.h:561-649, replicated_3pc.h:199):
+    every party applies the op to both of its shares, so the opened value is
+    the op on the plaintexts (mod 2^bits), and each of the three distinct
+    shares is the op on the inputs' shares."""
+    dt = np.uint32 if width == 4 else np.uint64
+    rng = np.random.default_rng(width)
+    x = rng.integers(0, 2**63, 1000, dtype=np.uint64).astype(dt)
+    y = rng.integers(0, 2**63, 1000, dtype=np.uint64).astype(dt)
+    c = O.Ctx(keys)
+    X, Y = c.secret_share(x, width), c.secret_share(y, width)
+    A, B = c.secret_share(x, width, arith=True), c.secret_share(y, width, arith=True)
+    with np.errstate(over="ignore"):
+        np.testing.assert_array_equal(c.open(c.xor_b(X, Y)), x ^ y)
+        np.testing.assert_array_equal(c.open(c.not_b(X)), ~x)
+        np.testing.assert_array_equal(c.open(c.add_a(A, B), arith=True), x + y)
+        np.testing.assert_array_equal(c.open(c.sub_a(A, B), arith=True), x - y)
+        np.testing.assert_array_equal(c.open(c.local("neg_a", A), arith=True), dt(0) - x)
+        z = c.sub_a(A, B)
+        for p in range(3):
+            np.testing.assert_array_equal(z.get(p, 0), A.get(p, 0) - B.get(p, 0))
+            np.testing.assert_array_equal(z.get(p, 1), A.get(p, 1) - B.get(p, 1))
