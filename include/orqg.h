@@ -230,6 +230,17 @@ int orqg_aggregate(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys,
 /* operators::distinct (operators/distinct.h:19-44): res[0] = 1 and
  * res[i] = (row i differs from row i-1 on some key), as boolean shares      */
 int orqg_distinct(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, orqg_vec *res);
+/* orqg_distinct without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its one
+ * scratch buffer (3 * n * width bytes of comparison results) is a fixed
+ * 48 MiB static device array per GPU, held until the process ends, instead
+ * of context memory, so n is at most 2^21 rows of width 8 (2^22 of width 4);
+ * a larger input returns ORQG_E_ARG.  res is zeroed by a kernel, not a memset.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from different contexts or streams) race.  Results, nonces and message log
+ * are those of orqg_distinct.                                               */
+int orqg_distinct_noalloc(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, orqg_vec *res);
 /* public_share of one value in every row (replicated_3pc.h:456-470)         */
 int orqg_public_fill(orqg_ctx *ctx, orqg_vec *v, uint64_t value, int enc);
 

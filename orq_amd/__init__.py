@@ -51,7 +51,7 @@ EXPORTS = [
     "orqg_perm_identity", "orqg_perm_invert", "orqg_perm_b2a", "orqg_perm_negate",
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
-    "orqg_aggregate", "orqg_distinct", "orqg_public_fill", "orqg_vec_copy_rows",
+    "orqg_aggregate", "orqg_distinct", "orqg_distinct_noalloc", "orqg_public_fill", "orqg_vec_copy_rows",
     "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
     "orqg_a2b", "orqg_secret_share_from", "orqg_server_create", "orqg_server_submit",
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
@@ -116,6 +116,7 @@ def lib():
             "orqg_vec_tail": (i32, [vp, vp, sz]),
             "orqg_aggregate": (i32, [vp, vp, sz, vp, sz, vp, sz, i32, vp]),
             "orqg_distinct": (i32, [vp, vp, sz, vp]),
+            "orqg_distinct_noalloc": (i32, [vp, vp, sz, vp]),
             "orqg_public_fill": (i32, [vp, vp, u64, i32]),
             "orqg_vec_copy_rows": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_vec_broadcast_row": (i32, [vp, vp, sz, vp, sz, sz]),

@@ -9,6 +9,10 @@
 // call of the reference (the equality circuits, the and_b / multiply_a of
 // the aggregation functions, b2a_bit of the group bits) draws its keystream
 // at its own nonce offset, so shares, nonces and messages match it.
+#include <map>
+#include <mutex>
+#include <string>
+
 #include "internal.h"
 #include "kern_common.h"
 #include "cmp_core.h"
@@ -497,16 +501,24 @@ void aggregate_t(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const orqg_agg_s
   }
 }
 
+// ne_buf: the comparison results (3 n w bytes); NULL takes the context's
+// scratch.  The caller-supplied form zeroes res with a kernel instead of a
+// memset node, so that it can sit in a captured graph.
 template <typename T>
-void distinct_t(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res) {
+void distinct_t(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res, T *ne_buf) {
   constexpr int w = sizeof(T);
   const uint64_t n = res->n;
-  ORQG_HIP(hipMemsetAsync(res->d, 0, 3 * n * w, c->stream));  // res->zero()
+  if (ne_buf)
+    public_fill_raw(c, res->d, n, 0, w);  // res->zero()
+  else
+    ORQG_HIP(hipMemsetAsync(res->d, 0, 3 * n * w, c->stream));
   res->enc = ORQG_BSHARED;
   if (n == 0) return;
   public_fill_raw(c, res->d, 1, 1, w);  // res[0] = public 1
   if (n == 1) return;
-  T *ne = (T *)c->scratch_get("distinct_ne", 3 * n * w + 64);
+  // every row of ne that distinct_or reads (0 .. n-2) is written by the
+  // comparison first, so the buffer needs no clearing
+  T *ne = ne_buf ? ne_buf : (T *)c->scratch_get("distinct_ne", 3 * n * w + 64);
   TimedRegion tr(c, "distinct");
   for (size_t k = 0; k < nk; k++) {
     Launcher L(c, w, n - 1);
@@ -555,9 +567,58 @@ void distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res) {
     if (!keys[k] || keys[k]->n != res->n || keys[k]->width != res->width)
       throw Error(ORQG_E_ARG, "distinct: columns differ in size or width");
   if (res->width == 4)
-    distinct_t<uint32_t>(c, keys, nk, res);
+    distinct_t<uint32_t>(c, keys, nk, res, nullptr);
   else
-    distinct_t<uint64_t>(c, keys, nk, res);
+    distinct_t<uint64_t>(c, keys, nk, res, nullptr);
+}
+
+}  // namespace orqg
+
+// ---- distinct without allocation ---------------------------------------------
+// The comparison buffer of orqg_distinct_noalloc lives in this static device
+// array: every GPU the library runs on holds one copy of it until the process
+// ends, and every call on that GPU shares it (calls that overlap on one GPU
+// race on it).  In host code `scratch` names a host-side stand-in, not device
+// memory: the device address of each GPU's copy comes from
+// hipGetSymbolAddress, taken once per device at context creation.
+static __device__ __align__(256) uint8_t scratch[orqg::kNoAllocScratchBytes];
+
+namespace orqg {
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device ordinal -> its copy of `scratch`
+}  // namespace
+
+void noalloc_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr.count(device)) return;
+  void *p = nullptr;
+  ORQG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = p;
+}
+
+void distinct_noalloc(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res) {
+  for (size_t k = 0; k < nk; k++)
+    if (!keys[k] || keys[k]->n != res->n || keys[k]->width != res->width)
+      throw Error(ORQG_E_ARG, "distinct: columns differ in size or width");
+  const size_t need = 3 * res->n * (size_t)res->width;
+  if (need > kNoAllocScratchBytes)
+    throw Error(ORQG_E_ARG, "distinct_noalloc: " + std::to_string(res->n) + " rows of width " +
+                                std::to_string(res->width) + " need " + std::to_string(need) +
+                                " bytes of scratch, over its fixed " +
+                                std::to_string(kNoAllocScratchBytes) +
+                                " (at most 2^21 rows of width 8, 2^22 of width 4)");
+  void *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(c->device);
+    if (it != g_scratch_ptr.end()) ptr = it->second;
+  }
+  if (!ptr) throw Error(ORQG_E_DEVICE, "distinct_noalloc: scratch of this device not set up");
+  if (res->width == 4)
+    distinct_t<uint32_t>(c, keys, nk, res, (uint32_t *)ptr);
+  else
+    distinct_t<uint64_t>(c, keys, nk, res, (uint64_t *)ptr);
 }
 
 }  // namespace orqg

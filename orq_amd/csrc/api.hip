@@ -261,6 +261,7 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     c->budget = cfg->hbm_budget;
     std::memcpy(c->keys, cfg->keys, sizeof(c->keys));
     ORQG_HIP(hipSetDevice(c->device));
+    noalloc_setup(c->device);
     register_ctx(c);
     ORQG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     ORQG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
@@ -783,6 +784,12 @@ int orqg_distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, orqg_vec *re
   API_BEGIN(c)
   if (!res || (nkeys && !keys)) throw Error(ORQG_E_ARG, "distinct: NULL argument");
   distinct(c, keys, nkeys, res);
+  API_END(c)
+}
+int orqg_distinct_noalloc(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, orqg_vec *res) {
+  API_BEGIN(c)
+  if (!res || (nkeys && !keys)) throw Error(ORQG_E_ARG, "distinct_noalloc: NULL argument");
+  distinct_noalloc(c, keys, nkeys, res);
   API_END(c)
 }
 int orqg_public_fill(orqg_ctx *c, orqg_vec *v, uint64_t value, int enc) {

@@ -242,6 +242,12 @@ void broadcast_row(orqg_ctx *c, const orqg_vec *src, uint64_t row, orqg_vec *dst
 void aggregate(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const orqg_agg_spec *bs, size_t nbs,
                const orqg_agg_spec *as, size_t nas, int reverse, const orqg_vec *sel);
 void distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res);
+// the allocation-free distinct: its one scratch buffer is a fixed static
+// device array of kNoAllocScratchBytes (set up per device by noalloc_setup,
+// which context creation calls with the device current)
+constexpr size_t kNoAllocScratchBytes = (size_t)48 << 20;
+void noalloc_setup(int device);
+void distinct_noalloc(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res);
 void adder(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z);
 void a2b(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
 void perm_b2a(orqg_ctx *c, orqg_vec *perm);
