@@ -54,24 +54,67 @@ def zipf_keys(n, s=1.1, domain=1 << 20, seed=0x0A11CE03):
     return np.searchsorted(cdf, u, side="right").astype(np.uint64)  # rank - 1 in [0, domain)
 
 
-def cpu_baseline(n_sample):
-    """Oracle radix sort (single thread, 3 simulated parties) on a bounded sample."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cap_s, sizes=(1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)):
+    """The oracle's 3PC radix sort of int64 Zipf keys on the host CPU: the
+    three parties on three threads, one worker thread per party (the
+    reference's default), the reference's batch schedule (batch_size -12).
+    Sizes 2^16, 2^18, ... up to 2^24, each run once, while the next size's
+    projected time fits the cap; the others are reported as cut by the cap.
+    The value is the end-to-end rate (permutation generation included, as
+    micro_sorting times it) at the largest size run; online-only is beside it."""
     from oracle import oracle as O
     from orq_amd import keys_from_seed
 
-    keys = zipf_keys(n_sample)
-    ctx = O.Ctx(keys_from_seed(0x0A11CE03))
-    v = ctx.secret_share(keys, 8)
-    t0 = time.perf_counter()
-    ctx.radix_sort(v)
-    dt = time.perf_counter() - t0
+    runs, spent, last = [], 0.0, None
+    for n in sizes:
+        if last is not None:
+            est = last[1] * n / last[0] * 1.6  # per-row cost grows with n (caches)
+            if spent + est > cap_s:
+                runs.append({"rows": n, "run": False,
+                             "why": f"projected {est:.0f} s would pass the {cap_s:.0f} s cap"})
+                continue
+        keys = zipf_keys(n)
+        ctx = O.Ctx(keys_from_seed(0x0A11CE03), -12)
+        v = ctx.secret_share(keys, 8)
+        t0 = time.perf_counter()
+        ctx.reserve(n, 66)  # PermutationManager::reserve of a 64-bit radix sort
+        t1 = time.perf_counter()
+        ctx.radix_sort(v)
+        t2 = time.perf_counter()
+        ok = bool(np.array_equal(ctx.open(v).view(np.int64), np.sort(keys.view(np.int64))))
+        spent += t2 - t0
+        last = (n, t2 - t0)
+        runs.append({"rows": n, "run": True, "s": round(t2 - t0, 3),
+                     "Mrows_per_s": round(n / (t2 - t0) / 1e6, 4),
+                     "online_Mrows_per_s": round(n / (t2 - t1) / 1e6, 4),
+                     "reserve_s": round(t1 - t0, 3), "correct": ok})
+        del ctx, v
+    done = [r for r in runs if r["run"]]
+    top = done[-1]
     return {
-        "value": n_sample / dt / 1e6,
+        "value": top["Mrows_per_s"],
         "unit": "Mrows/s",
-        "cores": 1,
+        "cores": 3,
         "kind": "port",
-        "sample": f"oracle radix_sort of {n_sample} int64 Zipf keys incl. permutation "
-                  f"generation, 1 thread, {dt:.2f} s",
+        "sample": f"oracle radix_sort of {top['rows']} int64 Zipf keys incl. permutation "
+                  f"generation ({top['s']:.2f} s); 3 parties x 1 thread",
+        "threads_per_party": 1,
+        "batch_size": -12,
+        "nproc": os.cpu_count(),
+        "cpu_model": _cpu_model(),
+        "cap_s": cap_s,
+        "online_Mrows_per_s": top["online_Mrows_per_s"],
+        "sizes": runs,
     }
 
 
@@ -323,8 +366,11 @@ def run_stream(jobs, slots, rate, seed=7):
 
 def bench_serve(args):
     """Served mixed-size stream: continuous batching against the plain
-    one-request-at-a-time path on the same requests and arrivals.  Any failed
-    request makes the line invalid (value null) and the exit status 1."""
+    one-request-at-a-time path on the same requests and arrival times, at a
+    ladder of Poisson arrival rates.  For each mode the headline takes the
+    highest rate whose p99 latency stays within the declared target, and the
+    rows/s sustained there.  Any failed request makes the line invalid
+    (value null) and the exit status 1."""
     jobs = make_requests(args.requests)
     rows = sum(j.n for j in jobs)
     res = {}
@@ -335,6 +381,7 @@ def bench_serve(args):
             st = r["stats"]
             n_failed += len(r["failed"])
             res[f"rate{rate}_slots{slots}"] = {
+                "rate_per_s": rate, "slots": slots,
                 "rows_per_s": round(r["rows_per_s"], 1), "p50_ms": round(r["p50_ms"], 1),
                 "p99_ms": round(r["p99_ms"], 1), "wall_s": round(r["wall_s"], 2),
                 "max_running": st["max_running"], "failed": len(r["failed"]),
@@ -342,23 +389,27 @@ def bench_serve(args):
             print(f"# rate {rate}/s slots {slots}: {r['rows_per_s'] / 1e6:.3f} Mrows/s "
                   f"p50 {r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms failed {len(r['failed'])}",
                   flush=True)
-    # the headline: highest sustained rows/s of the batched server whose p99
-    # is within the one-at-a-time path's p99 at the lowest rate
-    target = res[f"rate{args.rates[0]}_slots1"]["p99_ms"]
-    ok = [v["rows_per_s"] for k, v in res.items()
-          if k.endswith(f"slots{args.slots}") and v["p99_ms"] <= target]
-    base = [v["rows_per_s"] for k, v in res.items() if k.endswith("slots1") and v["p99_ms"] <= target]
+    target = args.p99_target_ms
+
+    def best(slots):
+        ok = [v for v in res.values() if v["slots"] == slots and v["p99_ms"] <= target]
+        return max(ok, key=lambda v: v["rate_per_s"]) if ok else None
+
+    batched, plain = best(args.slots), best(1)
     valid = n_failed == 0
     line = {"metric": "served-query p50/p99 latency and Mrows/s (mixed 2^10..2^22 stream)",
-            "value": round(max(ok) / 1e6, 4) if ok and valid else None,
-            "unit": "Mrows/s at p99 <= target",
+            "value": round(batched["rows_per_s"] / 1e6, 4) if batched and valid else None,
+            "unit": f"Mrows/s at the highest arrival rate with p99 <= {target:.0f} ms",
             "n_gpus": 1, "steps": len(jobs), "higher_is_better": True, "dtype": "int64",
             "data": "synthetic: log-uniform sizes, Zipf(1.1) keys over 2^20, uint32 values, "
                     "Poisson arrivals",
             "config": {"workload": f"{len(jobs)} requests ({rows} rows), sort / GROUP BY SUM "
-                                   f"50/50, slots {args.slots} vs 1",
+                                   f"50/50, {args.slots} slots vs one at a time",
                        "rates_per_s": args.rates, "p99_target_ms": target},
-            "one_at_a_time_Mrows_s_at_target": round(max(base) / 1e6, 4) if base else None,
+            "batched": {k: batched[k] for k in ("rate_per_s", "rows_per_s", "p50_ms", "p99_ms")}
+            if batched else None,
+            "one_at_a_time": {k: plain[k] for k in ("rate_per_s", "rows_per_s", "p50_ms", "p99_ms")}
+            if plain else None,
             "failed_requests": n_failed,
             "runs": res}
     print(json.dumps(line), flush=True)
@@ -372,7 +423,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 24)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--cpu-cap", type=float, default=30.0,
+                    help="seconds of CPU-baseline work (sizes past the cap are reported, not run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gen-keystream", action="store_true",
                     help="skip the extra reserve that measures gen_perm's keystream rate")
@@ -380,7 +432,9 @@ def main():
                     default="radix")
     ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--slots", type=int, default=4)
-    ap.add_argument("--rates", type=float, nargs="+", default=[4.0, 8.0, 16.0])
+    ap.add_argument("--rates", type=float, nargs="+", default=[4.0, 6.0, 8.0, 12.0, 16.0])
+    ap.add_argument("--p99-target-ms", type=float, default=1000.0,
+                    help="serve workload: the p99 latency both modes are held to")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: all three parties share "
@@ -525,7 +579,7 @@ def main():
         "gen_perm_keystream": gen_ks,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_cap)
     print(json.dumps(line), flush=True)
 
 

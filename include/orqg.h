@@ -108,6 +108,25 @@ int orqg_vec_copy_rows(orqg_ctx *ctx, const orqg_vec *src, size_t src_start, orq
 int orqg_vec_broadcast_row(orqg_ctx *ctx, const orqg_vec *src, size_t row, orqg_vec *dst,
                            size_t dst_start, size_t count);
 
+/* ---- share-local Vector operations (no messages, no randomness) ------ */
+/* z = op(x) on every share; z may be x.  Each map is linear over the
+ * sharing, so it acts on the shared value:
+ *   ORQG_SHL / ORQG_ASR / ORQG_LSR   x << arg, arithmetic and logical x >> arg
+ *     (bit_left_shift / bit_arithmetic_right_shift / bit_logical_right_shift,
+ *     include/core/containers/mapping_access_vector.h:376-421, as
+ *     BSharedVector operator<< / >>, b_shared_vector.h:100-127,384-414)
+ *   ORQG_PARITY      XOR of all bits (bit_xor, mapping_access_vector.h:430-439)
+ *   ORQG_EXTEND_LSB  every bit = the LSB (extend_lsb, :1541-1547)
+ *   ORQG_LTZ         the sign bit as 0/1 (ltz, :1528-1535; protocol.h:682)
+ *   ORQG_MASK        x & arg (mask, :1312-1316; b_shared_vector.h:462-464)  */
+enum { ORQG_SHL = 0, ORQG_ASR = 1, ORQG_LSR = 2, ORQG_PARITY = 3, ORQG_EXTEND_LSB = 4,
+       ORQG_LTZ = 5, ORQG_MASK = 6 };
+int orqg_vec_unary(orqg_ctx *ctx, int op, const orqg_vec *x, uint64_t arg, orqg_vec *z);
+/* SharedVector::prefix_sum (include/core/containers/shared_vector.h:249):
+ * in place inclusive scan of every share (std::inclusive_scan,
+ * mapping_access_vector.h:446), so of the A-shared value                   */
+int orqg_prefix_sum(orqg_ctx *ctx, orqg_vec *v);
+
 /* ---- randomness ------------------------------------------------------ */
 /* AESPRGAlgorithm::fillBytes of PRG `role` into device memory, advancing its
  * nonce (prg_algorithm.h:161-231)                                            */

@@ -69,6 +69,8 @@ def lib():
             ("oc_aggregate", [vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, sz, i32, vp]),
             ("oc_distinct", [vp, vp, sz, vp]),
             ("oc_local", [vp, i32, vp, vp, vp]),
+            ("oc_unary", [vp, i32, vp, u64, vp]),
+            ("oc_prefix_sum", [vp, vp]),
             ("oc_vec_resize", [vp, sz]),
             ("oc_adder", [vp, i32, vp, vp, vp]),
             ("oc_secret_share_from", [vp, vp, vp, i32, i32]),
@@ -184,6 +186,18 @@ class Ctx:
         z = Vec(x.n, x.width)
         _check(lib().oc_local(self.h, self.LOCAL[op], x.h, (y or x).h, z.h), op)
         return z
+
+    UNARY = {"<<": 0, ">>": 1, "lsr": 2, "parity": 3, "extend_lsb": 4, "ltz": 5, "mask": 6}
+
+    def unary(self, op, x, arg=0):
+        """Share-local maps: '<<', '>>' (arithmetic), 'lsr', 'parity',
+        'extend_lsb', 'ltz', 'mask' (x & arg)."""
+        z = Vec(x.n, x.width)
+        _check(lib().oc_unary(self.h, self.UNARY[op], x.h, arg, z.h), op)
+        return z
+
+    def prefix_sum(self, v):
+        _check(lib().oc_prefix_sum(self.h, v.h), "prefix_sum")
 
     def xor_b(self, x, y):
         return self.local("xor_b", x, y)

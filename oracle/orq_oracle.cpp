@@ -33,10 +33,18 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 typedef unsigned __int128 u128;
 typedef __int128 i128;
+
+// The three parties run on three threads where each one only touches its own
+// shares and its own PRG objects (the reference runs each party as its own
+// process with one worker thread): PARTIES_PARALLEL in front of a
+// `for (int p = 0; p < 3; p++)` loop.  Results do not depend on it.
+#define ORQ_PRAGMA(x) _Pragma(#x)
+#define PARTIES_PARALLEL ORQ_PRAGMA(omp parallel for num_threads(3) schedule(static, 1))
 
 namespace {
 
@@ -249,6 +257,7 @@ void mult_batch(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z, size_t s, size_t e,
                 const char *op) {
   size_t nb = e - s;
   std::vector<T> local[3];
+  PARTIES_PARALLEL
   for (int p = 0; p < 3; p++) {
     local[p].resize(nb);
     if (kArith)
@@ -283,6 +292,7 @@ void b2a_bit_batch(oc_ctx *c, oc_vec *x, oc_vec *y, size_t s, size_t e) {
   size_t nb = e - s;
   oc_vec *s0 = new_vec(nb, sizeof(T)), *s1 = new_vec(nb, sizeof(T));
   std::vector<T> send0[3], send1[3];
+  PARTIES_PARALLEL
   for (int p = 0; p < 3; p++) {
     T *a0 = S<T>(s0, p, 0), *a1 = S<T>(s0, p, 1), *b0 = S<T>(s1, p, 0);
     const T *x0 = S<T>(x, p, 0) + s, *x1 = S<T>(x, p, 1) + s;
@@ -392,6 +402,7 @@ void reshare(oc_ctx *c, oc_vec *v, int g) {
     if (!in_group(g, p)) outsider = p;
   for (auto b : batches_for(v->n, c->batch_size)) {
     size_t nb = b.e - b.s;
+    PARTIES_PARALLEL
     for (int p = 0; p < 3; p++) {
       if (p == outsider) continue;
       std::vector<T> r[3];
@@ -459,18 +470,22 @@ void reserve(oc_ctx *c, size_t n, size_t count) {
   c->stored_size = n;
   if (count <= c->queue.size()) return;
   count -= c->queue.size();
-  for (size_t k = 0; k < count; k++) {
-    oc_ctx::Sharded sp;
-    sp.n = n;
-    for (int p = 0; p < 3; p++)
+  std::vector<oc_ctx::Sharded> fresh(count);
+  // each party generates its groups' permutations in queue order from its
+  // own group PRGs
+  PARTIES_PARALLEL
+  for (int p = 0; p < 3; p++)
+    for (size_t k = 0; k < count; k++) {
+      oc_ctx::Sharded &sp = fresh[k];
+      sp.n = n;
       for (int g = 0; g < 4; g++) {
         if (!in_group(g, p)) continue;
         if (g == 3 && !c->gen_everyone) continue;
         sp.perm[p][g].resize(n);
         gen_perm(sp.perm[p][g], c->P[p].group[g], nullptr);
       }
-    c->queue.push_back(std::move(sp));
-  }
+    }
+  for (auto &sp : fresh) c->queue.push_back(std::move(sp));
 }
 
 oc_ctx::Sharded next_perm(oc_ctx *c, size_t n) {
@@ -500,6 +515,7 @@ template <typename T, bool kArith>
 void apply_sharded(oc_ctx *c, oc_vec *x, const oc_ctx::Sharded &sp, bool inverse) {
   for (int gi = 0; gi < 3; gi++) {
     int g = inverse ? 2 - gi : gi;
+    PARTIES_PARALLEL
     for (int p = 0; p < 3; p++) {
       if (!in_group(g, p)) continue;
       for (int k = 0; k < 2; k++) {
@@ -533,6 +549,7 @@ void apply_elementwise(oc_ctx *c, oc_vec *x, oc_vec *perm, bool perm_arith) {
   for (size_t i = 0; i < x->n; i++)
     if (opened[i] < 0 || (size_t)opened[i] >= x->n)
       throw std::runtime_error("opened permutation out of range");
+  PARTIES_PARALLEL
   for (int p = 0; p < 3; p++)
     for (int k = 0; k < 2; k++) scatter<T>(S<T>(x, p, k), opened.data(), x->n);
   delete cp;
@@ -787,6 +804,7 @@ void radix_sort(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
   oc_vec *vprime = new_vec(n, 4), *f1 = new_vec(n, 4), *s = new_vec(2 * n, 4);
   oc_vec *s1 = new_vec(n, 4), *perm = new_vec(n, 4);
   for (int i = 0; i < bits; i++) {
+    PARTIES_PARALLEL
     for (int p = 0; p < 3; p++)
       for (int k = 0; k < 2; k++) {
         const PT *src = S<PT>(pad, p, k);
@@ -799,6 +817,7 @@ void radix_sort(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
       }
     b2a_bit<uint32_t>(c, vprime, f1);
     // s0 = one - f1 ; s1 = f1 ; s[0] -= 1 ; prefix_sum over 2n
+    PARTIES_PARALLEL
     for (int p = 0; p < 3; p++)
       for (int k = 0; k < 2; k++) {
         uint32_t *sp = S<uint32_t>(s, p, k);
@@ -816,12 +835,14 @@ void radix_sort(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
         }
       }
     // s1 -= s0 ; f1 *= s1 ; s0 += f1
+    PARTIES_PARALLEL
     for (int p = 0; p < 3; p++)
       for (int k = 0; k < 2; k++) {
         uint32_t *sp = S<uint32_t>(s, p, k), *d = S<uint32_t>(s1, p, k);
         for (size_t j = 0; j < n; j++) d[j] = sp[n + j] - sp[j];
       }
     mult<uint32_t, true>(c, f1, s1, f1, "mul_a");
+    PARTIES_PARALLEL
     for (int p = 0; p < 3; p++)
       for (int k = 0; k < 2; k++) {
         uint32_t *sp = S<uint32_t>(s, p, k), *d = S<uint32_t>(perm, p, k);
@@ -1589,6 +1610,48 @@ static void local_op(int op, const oc_vec *x, const oc_vec *y, oc_vec *z) {
     }
 }
 
+// Share-local Vector maps, each party on both of its shares
+// (mapping_access_vector.h): 0 bit_left_shift (:413-421), 1
+// bit_arithmetic_right_shift (:376-384), 2 bit_logical_right_shift, 3 bit_xor
+// (:430-439, popcount parity), 4 extend_lsb (:1541-1547), 5 ltz (:1528-1535),
+// 6 mask (:1312-1316).  Shift amounts at or over the width give 0 (sign fill
+// for the arithmetic shift), where the reference's expression is undefined.
+template <typename T>
+static void unary_op(int op, const oc_vec *x, uint64_t arg, oc_vec *z) {
+  typedef typename std::make_signed<T>::type S;
+  const int B = 8 * sizeof(T);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const T *a = (const T *)x->sh[p][k].data();
+      T *d = (T *)z->sh[p][k].data();
+      for (size_t i = 0; i < x->n; i++) {
+        const T v = a[i];
+        T r;
+        switch (op) {
+          case 0: r = arg >= (uint64_t)B ? 0 : (T)(v << arg); break;
+          case 1: r = (T)((S)v >> (arg >= (uint64_t)B ? B - 1 : (int)arg)); break;
+          case 2: r = arg >= (uint64_t)B ? 0 : (T)(v >> arg); break;
+          case 3: r = (T)(__builtin_popcountll((unsigned long long)v) & 1); break;
+          case 4: r = (T)(-(v & 1)); break;
+          case 5: r = (T)((S)v < 0); break;
+          default: r = v & (T)arg; break;
+        }
+        d[i] = r;
+      }
+    }
+}
+
+// SharedVector::prefix_sum (shared_vector.h:249): std::inclusive_scan of
+// each share (mapping_access_vector.h:446)
+template <typename T>
+static void prefix_sum_op(oc_vec *v) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      T *d = (T *)v->sh[p][k].data();
+      std::inclusive_scan(d, d + v->n, d);
+    }
+}
+
 extern "C" {
 
 oc_ctx *oc_create(const uint8_t *keys /* 10 x 32 bytes */, long batch_size, int gen_everyone) {
@@ -1719,6 +1782,30 @@ int oc_local(oc_ctx *, int op, oc_vec *x, oc_vec *y, oc_vec *z) {
       local_op<uint64_t>(op, x, y, z);
     else
       throw std::runtime_error("local op: width");
+  })
+}
+
+int oc_unary(oc_ctx *, int op, oc_vec *x, uint64_t arg, oc_vec *z) {
+  OC_TRY({
+    if (op < 0 || op > 6 || x->n != z->n || x->width != z->width)
+      throw std::runtime_error("unary: argument");
+    if (x->width == 4)
+      unary_op<uint32_t>(op, x, arg, z);
+    else if (x->width == 8)
+      unary_op<uint64_t>(op, x, arg, z);
+    else
+      throw std::runtime_error("unary: width");
+  })
+}
+
+int oc_prefix_sum(oc_ctx *, oc_vec *v) {
+  OC_TRY({
+    if (v->width == 4)
+      prefix_sum_op<uint32_t>(v);
+    else if (v->width == 8)
+      prefix_sum_op<uint64_t>(v);
+    else
+      throw std::runtime_error("prefix_sum: width");
   })
 }
 

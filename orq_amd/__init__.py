@@ -51,7 +51,8 @@ EXPORTS = [
     "orqg_perm_identity", "orqg_perm_invert", "orqg_perm_b2a", "orqg_perm_negate",
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
-    "orqg_aggregate", "orqg_distinct", "orqg_distinct_noalloc", "orqg_public_fill", "orqg_vec_copy_rows",
+    "orqg_aggregate", "orqg_distinct", "orqg_distinct_noalloc", "orqg_vec_unary",
+    "orqg_prefix_sum", "orqg_public_fill", "orqg_vec_copy_rows",
     "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
     "orqg_a2b", "orqg_secret_share_from", "orqg_server_create", "orqg_server_submit",
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
@@ -117,6 +118,8 @@ def lib():
             "orqg_aggregate": (i32, [vp, vp, sz, vp, sz, vp, sz, i32, vp]),
             "orqg_distinct": (i32, [vp, vp, sz, vp]),
             "orqg_distinct_noalloc": (i32, [vp, vp, sz, vp]),
+            "orqg_vec_unary": (i32, [vp, i32, vp, u64, vp]),
+            "orqg_prefix_sum": (i32, [vp, vp]),
             "orqg_public_fill": (i32, [vp, vp, u64, i32]),
             "orqg_vec_copy_rows": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_vec_broadcast_row": (i32, [vp, vp, sz, vp, sz, sz]),
@@ -322,6 +325,19 @@ class Ctx:
         y = Vec(self, x.n, x.width, ORQG_ASHARED)
         self._check(lib().orqg_b2a(self.h, x.h, y.h), "b2a")
         return y
+
+    UNARY = {"<<": 0, ">>": 1, "lsr": 2, "parity": 3, "extend_lsb": 4, "ltz": 5, "mask": 6}
+
+    def unary(self, op, x, arg=0, out=None):
+        """Share-local map: '<<', '>>' (arithmetic), 'lsr', 'parity',
+        'extend_lsb', 'ltz', 'mask' (x & arg); out may be x."""
+        z = out if out is not None else Vec(self, x.n, x.width, x.encoding)
+        self._check(lib().orqg_vec_unary(self.h, self.UNARY[op], x.h, arg, z.h), "vec_unary")
+        return z
+
+    def prefix_sum(self, v):
+        """SharedVector::prefix_sum, in place."""
+        self._check(lib().orqg_prefix_sum(self.h, v.h), "prefix_sum")
 
     def reshare(self, v, group):
         self._check(lib().orqg_reshare(self.h, v.h, group), "reshare")

@@ -274,8 +274,13 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     for (int k = 0; k < ORQG_NUM_KEYS; k++) aes256_expand(c->tabs, c->keys[k], rk + 60 * k);
     c->d_rk = (uint32_t *)dev_alloc(c, sizeof(rk));
     c->d_t0 = (uint32_t *)dev_alloc(c, sizeof(c->tabs.t0));  // T0; kernels replicate it in LDS
-    ORQG_HIP(hipMemcpy(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice));
-    ORQG_HIP(hipMemcpy(c->d_t0, c->tabs.t0, sizeof(c->tabs.t0), hipMemcpyHostToDevice));
+    // copies on the context's own stream, never the legacy null stream: a
+    // null-stream operation would be ordered against every other context's
+    // streams (and is refused while any of them is capturing a graph)
+    ORQG_HIP(hipMemcpyAsync(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice, c->stream));
+    ORQG_HIP(hipMemcpyAsync(c->d_t0, c->tabs.t0, sizeof(c->tabs.t0), hipMemcpyHostToDevice,
+                            c->stream));
+    ORQG_HIP(hipStreamSynchronize(c->stream));
   } catch (const std::exception &e) {
     std::fprintf(stderr, "orqg_ctx_create: %s\n", e.what());
     unregister_ctx(c);
@@ -315,7 +320,8 @@ int orqg_ctx_rekey(orqg_ctx *c, const uint8_t keys[ORQG_NUM_KEYS][32]) {
   std::memcpy(c->keys, keys, sizeof(c->keys));
   uint32_t rk[ORQG_NUM_KEYS * 60];
   for (int k = 0; k < ORQG_NUM_KEYS; k++) aes256_expand(c->tabs, c->keys[k], rk + 60 * k);
-  ORQG_HIP(hipMemcpy(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice));
+  ORQG_HIP(hipMemcpyAsync(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice, c->stream));
+  ORQG_HIP(hipStreamSynchronize(c->stream));
   for (int k = 0; k < ORQG_NUM_KEYS; k++) c->nonce[k] = 0;
   for (auto &p : c->queue) free_perm(p);
   c->queue.clear();
@@ -784,6 +790,16 @@ int orqg_distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, orqg_vec *re
   API_BEGIN(c)
   if (!res || (nkeys && !keys)) throw Error(ORQG_E_ARG, "distinct: NULL argument");
   distinct(c, keys, nkeys, res);
+  API_END(c)
+}
+int orqg_vec_unary(orqg_ctx *c, int op, const orqg_vec *x, uint64_t arg, orqg_vec *z) {
+  API_BEGIN(c)
+  vec_unary(c, op, x, arg, z);
+  API_END(c)
+}
+int orqg_prefix_sum(orqg_ctx *c, orqg_vec *v) {
+  API_BEGIN(c)
+  prefix_sum(c, v);
   API_END(c)
 }
 int orqg_distinct_noalloc(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, orqg_vec *res) {

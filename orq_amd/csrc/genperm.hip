@@ -1288,30 +1288,48 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   // steps of a batch left after the last permutation are empty launches
   const int kBatch = n >= (1u << 22) ? 128 : n >= (1u << 20) ? 64 : 16;
   // The kernel takes all its state from device memory, so a batch of steps
-  // is one static launch sequence, captured once as a graph: each step draws
+  // is one static launch sequence, built once as a graph: each step draws
   // window s (committing the windows resolved at step s - 1 as well) and
   // resolves window s - 1; a last commit-only launch leaves every resolved
   // window of the batch committed, so the completed-permutation count read
-  // after a batch covers committed offsets.
-  auto enqueue_batch = [&]() {
-    for (int w = 0; w <= kBatch; w++)
-      hipLaunchKernelGGL(k_gp_step<kCopies>, dim3(grid), dim3(kGpThreads), 0, c->stream, c->d_t0,
-                         c->rk(kGroup0), B, n, mask, count, (uint32_t)(w < kBatch));
-  };
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  ORQG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_batch();
-  ORQG_HIP(hipStreamEndCapture(c->stream, &graph));
-  ORQG_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  // after a batch covers committed offsets.  The graph is built node by node
+  // rather than captured from a stream: a capture in one context's thread
+  // breaks (and leaves its stream stuck capturing) when another thread makes
+  // a null-stream call meanwhile.
   struct GraphGuard {
-    hipGraph_t g;
-    hipGraphExec_t e;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
     ~GraphGuard() {
       if (e) (void)hipGraphExecDestroy(e);
       if (g) (void)hipGraphDestroy(g);
     }
-  } guard{graph, gexec};
+  } guard;
+  ORQG_HIP(hipGraphCreate(&guard.g, 0));
+  {
+    const uint32_t *tab = c->d_t0;
+    const uint32_t *rkg = c->rk(kGroup0);
+    uint32_t on = 1, off = 0;
+    void *args_on[] = {(void *)&tab, (void *)&rkg, (void *)&B, (void *)&n, (void *)&mask,
+                       (void *)&count, (void *)&on};
+    void *args_off[] = {(void *)&tab, (void *)&rkg, (void *)&B, (void *)&n, (void *)&mask,
+                        (void *)&count, (void *)&off};
+    hipGraphNode_t prev = nullptr;
+    for (int w = 0; w <= kBatch; w++) {
+      hipKernelNodeParams kp;
+      std::memset(&kp, 0, sizeof(kp));
+      kp.func = reinterpret_cast<void *>(&k_gp_step<kCopies>);
+      kp.gridDim = dim3(grid);
+      kp.blockDim = dim3(kGpThreads);
+      kp.sharedMemBytes = 0;
+      kp.kernelParams = w < kBatch ? args_on : args_off;
+      kp.extra = nullptr;
+      hipGraphNode_t node;
+      ORQG_HIP(hipGraphAddKernelNode(&node, guard.g, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+      prev = node;
+    }
+  }
+  ORQG_HIP(hipGraphInstantiate(&guard.e, guard.g, nullptr, nullptr, 0));
+  hipGraphExec_t gexec = guard.e;
   // Swap resolution of a finished local permutation runs on a second stream
   // while the windows of the next ones proceed: it is memory-latency bound,
   // the windows are AES (VALU/LDS) bound.

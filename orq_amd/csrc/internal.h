@@ -264,5 +264,7 @@ void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *ord
                 orqg_vec *const *b_cols, size_t nb);
 
 enum LocalOp { kXor = 0, kAdd = 1, kSub = 2, kNot = 3 };
+void vec_unary(orqg_ctx *c, int op, const orqg_vec *x, uint64_t arg, orqg_vec *z);
+void prefix_sum(orqg_ctx *c, orqg_vec *v);
 
 }  // namespace orqg

@@ -337,3 +337,42 @@ def test_local_ops(O, G, keys, width, n):
             np.testing.assert_array_equal(gc.open(gv), want, err_msg=name)
     _same_nonces(oc, gc)
     assert [m for m in gc.msglog() if m[0] != "open"] == [m for m in log0 if m[0] != "open"]
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_unary_ops(O, G, keys, width):
+    """orqg_vec_unary: shares equal the oracle's for every map and shift
+    amount (the oracle's maps are pinned by closed forms); in place too."""
+    n = 5003
+    rng = np.random.default_rng(17 + width)
+    dt = np.uint32 if width == 4 else np.uint64
+    x = rng.integers(0, 2**63, n, dtype=np.uint64).astype(dt)
+    oc, gc = _pair(O, G, keys)
+    ox, gx = oc.secret_share(x, width), gc.secret_share(x, width)
+    bits = 8 * width
+    cases = [("<<", s) for s in (0, 3, bits - 1, bits + 1)] + \
+            [(">>", s) for s in (1, bits - 1, bits + 2)] + [("lsr", 7), ("lsr", bits)] + \
+            [("parity", 0), ("extend_lsb", 0), ("ltz", 0), ("mask", 0x5555_5555_5555_5555)]
+    for op, arg in cases:
+        arg = arg & ((1 << 64) - 1)
+        _same_shares(oc.unary(op, ox, arg), gc.unary(op, gx, arg))
+    gc.unary("mask", gx, 0xFF, out=gx)  # in place
+    _same_shares(oc.unary("mask", ox, 0xFF), gx)
+    _same_nonces(oc, gc)
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("n", [1, 4095, 4096, 4097, 70001, (1 << 20) + 3])
+def test_prefix_sum(O, G, keys, width, n):
+    """orqg_prefix_sum (one tile, tile edges, many tiles): shares equal the
+    oracle's inclusive scans, the value is the running sum mod 2^bits."""
+    rng = np.random.default_rng(n + width)
+    dt = np.uint32 if width == 4 else np.uint64
+    x = rng.integers(0, 2**63, n, dtype=np.uint64).astype(dt)
+    oc, gc = _pair(O, G, keys)
+    oa, ga = oc.secret_share(x, width, arith=True), gc.secret_share(x, width, arith=True)
+    oc.prefix_sum(oa)
+    gc.prefix_sum(ga)
+    _same_shares(oa, ga)
+    with np.errstate(over="ignore"):
+        np.testing.assert_array_equal(gc.open(ga), np.cumsum(x, dtype=dt))

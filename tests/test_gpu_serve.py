@@ -150,3 +150,37 @@ def test_admission_budget(G):
     assert all(status[i] == 0 for i in range(6))
     assert st["max_running"] <= 2
     assert st["peak_reserved"] <= 250 << 20
+
+
+def test_many_overlapping_small_requests(G):
+    """48 small requests over four slots: every slot keeps re-keying its
+    context (host-to-device copies) while the others generate permutations
+    (graph launches of gen_perm steps).  Every request must complete OK,
+    and each result must equal the same request served alone.  (With a
+    stream-captured graph and null-stream copies this mix failed requests
+    and left a slot's stream stuck capturing.)"""
+    from orq_amd.serve import Server
+
+    rng = np.random.default_rng(11)
+    sizes = [int(x) for x in rng.integers(1 << 10, 1 << 15, 48)]
+    mix, plain = _make_jobs(G, sizes, seed=5)
+    alone, _ = _make_jobs(G, sizes, seed=5)
+    srv = Server(slots=4)
+    for j in mix:
+        srv.submit(j)
+    done = srv.drain()
+    st = srv.stats()
+    srv.close()
+    bad = [(j.id, j.completion[0], j.error) for j in done if j.completion[0] != 0]
+    assert not bad, bad
+    assert st["failed"] == 0 and st["max_running"] > 1
+    srv1 = Server(slots=1)
+    for j in alone[:8]:
+        srv1.submit(j)
+        srv1.drain()
+    srv1.close()
+    for a, b, (op, k, v) in zip(mix[:8], alone[:8], plain[:8]):
+        assert b.completion[0] == 0, b.error
+        for x, y in ((a.out_key, b.out_key), (a.out_val, b.out_val), (a.out_valid, b.out_valid)):
+            np.testing.assert_array_equal(x, y)
+        _check_plain(a, op, k, v)

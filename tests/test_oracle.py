@@ -303,3 +303,36 @@ def test_local_ops_closed_forms(keys, width):
         for p in range(3):
             np.testing.assert_array_equal(z.get(p, 0), A.get(p, 0) - B.get(p, 0))
             np.testing.assert_array_equal(z.get(p, 1), A.get(p, 1) - B.get(p, 1))
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_unary_and_prefix_sum_closed_forms(keys, width):
+    """Share-local maps (mapping_access_vector.h:376-446,1312-1316,1528-1547)
+    open to the same map of the plaintext, and prefix_sum opens to the
+    running sum mod 2^bits."""
+    dt = np.uint32 if width == 4 else np.uint64
+    sdt = np.int32 if width == 4 else np.int64
+    bits = 8 * width
+    rng = np.random.default_rng(3 + width)
+    x = rng.integers(0, 2**63, 777, dtype=np.uint64).astype(dt)
+    x[:4] = [0, 1, dt(1) << dt(bits - 1), ~dt(0)]
+    c = O.Ctx(keys)
+    X = c.secret_share(x, width)
+    with np.errstate(over="ignore"):
+        for s in (0, 1, 5, bits - 1, bits, bits + 3):
+            sh = dt(min(s, bits - 1))
+            want_shl = np.zeros_like(x) if s >= bits else (x << dt(s))
+            want_lsr = np.zeros_like(x) if s >= bits else (x >> dt(s))
+            np.testing.assert_array_equal(c.open(c.unary("<<", X, s)), want_shl)
+            np.testing.assert_array_equal(c.open(c.unary("lsr", X, s)), want_lsr)
+            np.testing.assert_array_equal(c.open(c.unary(">>", X, s)).view(sdt),
+                                          x.view(sdt) >> sdt(sh))
+        par = np.array([bin(int(v)).count("1") & 1 for v in x], dtype=dt)
+        np.testing.assert_array_equal(c.open(c.unary("parity", X)), par)
+        np.testing.assert_array_equal(c.open(c.unary("extend_lsb", X)), dt(0) - (x & dt(1)))
+        np.testing.assert_array_equal(c.open(c.unary("ltz", X)), (x.view(sdt) < 0).astype(dt))
+        m = 0x00F0F0F0F0F0F0F0 & ((1 << bits) - 1)
+        np.testing.assert_array_equal(c.open(c.unary("mask", X, m)), x & dt(m))
+        A = c.secret_share(x, width, arith=True)
+        c.prefix_sum(A)
+        np.testing.assert_array_equal(c.open(A, arith=True), np.cumsum(x, dtype=dt))
