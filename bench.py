@@ -133,10 +133,10 @@ ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
 # The gather reads one 128-byte line per 64-byte element record; the writes
 # are the 64-byte records exactly.
 ROUND_TRAFFIC_2P24 = int((1080559.4 * 2.0 + 1049075.5 * 1.0) * 1024)
-# HBM bytes per k_gp_step launch averaged over the 3250 launches of one 2^24
-# sort, the same way (profiles/gen_perm_k_gp_step_2p24_counters.txt); None
-# until that profile exists
-GP_STEP_TRAFFIC_2P24 = None
+# HBM bytes per k_gp_step launch averaged over the 3225 launches of one 2^24
+# sort, the same way: (3386.8 * 2.0 + 10421.4 * 1.0) KiB
+# (profiles/gen_perm_k_gp_step_2p24_counters.txt)
+GP_STEP_TRAFFIC_2P24 = int((3386.8 * 2.0 + 10421.4 * 1.0) * 1024)
 
 
 def _timed(ctx, fn, reps):
