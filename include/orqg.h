@@ -275,6 +275,21 @@ int orqg_bitonic_sort(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, const 
 int orqg_table_sort(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, const int *orders,
                     const int *single_bit, orqg_vec *const *a_cols, size_t na,
                     orqg_vec *const *b_cols, size_t nb);
+/* SortingProtocol (sorting.h:15-22); QUICKSORT is the reference's DEFAULT   */
+enum { ORQG_RADIXSORT = 0, ORQG_QUICKSORT = 1 };
+/* operators::table_sort with an explicit protocol: RADIXSORT as above, or
+ * QUICKSORT for the multi-bit sort columns (32-bit columns only; single-bit
+ * columns use 1-bit radix sort under either), with the reserve counts of
+ * sorting.h:531-550 for that protocol.                                      */
+int orqg_table_sort_protocol(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys,
+                             const int *orders, const int *single_bit, orqg_vec *const *a_cols,
+                             size_t na, orqg_vec *const *b_cols, size_t nb, int protocol);
+/* operators::quicksort (include/core/operators/quicksort.h:212-244) of a
+ * B-shared 32-bit vector in place (signed order; stable, through the index
+ * padding): shuffle, rounds of pivot comparisons whose bits are opened,
+ * public partitions; perm_out (width 4) gets the A-shared sorting
+ * permutation.  Consumes 3 permutations (reserved if the queue lacks them). */
+int orqg_quicksort(orqg_ctx *ctx, orqg_vec *v, int order, orqg_vec *perm_out);
 
 /* ---- operators ------------------------------------------------------- */
 /* radix_sort (include/core/operators/radixsort.h:205-229): sorts the

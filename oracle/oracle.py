@@ -64,7 +64,8 @@ def lib():
             ("oc_perm_compose", [vp, vp, vp, i32]),
             ("oc_shuffle", [vp, vp, i32]),
             ("oc_table_shuffle", [vp, vp, sz, vp, sz]),
-            ("oc_table_sort", [vp, vp, sz, vp, vp, vp, sz, vp, sz]),
+            ("oc_table_sort", [vp, vp, sz, vp, vp, vp, sz, vp, sz, i32]),
+            ("oc_quicksort", [vp, vp, i32, vp]),
             ("oc_bitonic_sort", [vp, vp, sz, vp, vp, sz, vp, sz]),
             ("oc_aggregate", [vp, vp, sz, vp, vp, vp, sz, vp, vp, vp, sz, i32, vp]),
             ("oc_distinct", [vp, vp, sz, vp]),
@@ -308,13 +309,24 @@ class Ctx:
         b, nb = self._vec_array(b_cols)
         _check(lib().oc_table_shuffle(self.h, a, na, b, nb), "table_shuffle")
 
-    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=()):
+    PROTOCOLS = {"RADIXSORT": 0, "QUICKSORT": 1}
+
+    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=(),
+                   protocol="RADIXSORT"):
         k, nk = self._vec_array(keys)
         o = (ctypes.c_int * nk)(*[1 if x in (1, "DESC") else 0 for x in (orders or [0] * nk)])
         sb = (ctypes.c_int * nk)(*[int(bool(x)) for x in (single_bit or [0] * nk)])
         a, na = self._vec_array(list(a_cols))
         b, nb = self._vec_array(list(b_cols))
-        _check(lib().oc_table_sort(self.h, k, nk, o, sb, a, na, b, nb), "table_sort")
+        _check(lib().oc_table_sort(self.h, k, nk, o, sb, a, na, b, nb, self.PROTOCOLS[protocol]),
+               "table_sort")
+
+    def quicksort(self, v, desc=False):
+        """operators::quicksort of 32-bit B-shared keys; returns the A-shared
+        sorting permutation (quicksort.h:212-244)."""
+        perm = Vec(v.n, 4)
+        _check(lib().oc_quicksort(self.h, v.h, int(desc), perm.h), "quicksort")
+        return perm
 
     def bitonic_sort(self, keys, orders=None, a_cols=(), b_cols=()):
         k, nk = self._vec_array(keys)

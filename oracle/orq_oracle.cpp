@@ -875,6 +875,122 @@ void radix_sort(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
   delete pad;
 }
 
+// ---------------------------------------------------------- quicksort ---
+// quicksort (include/core/operators/quicksort.h:212-244) of 32-bit keys:
+// pad_input to int64 (sorting.h:431-462), quicksort_body
+// (quicksort.h:62-198), remove_padding (sorting.h:474-495) and invert.  The
+// comparison bits are opened each round and the partition is public, as in
+// the reference.
+
+template <typename T>
+oc_vec *gather_rows(const oc_vec *v, const std::vector<size_t> &idx);
+
+// swap rows a and b in every party's share copies
+void swap_rows64(oc_vec *v, size_t a, size_t b) {
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) std::swap(S<uint64_t>(v, p, k)[a], S<uint64_t>(v, p, k)[b]);
+}
+
+// partition (quicksort.h:28-50): Lomuto over [start, end] with the element
+// at `start` as pivot; cmp[j] == 0 (not above the pivot) moves j left.  The
+// pivot lands at i, which is marked, and i + 1 heads the right part.
+void qs_partition(oc_vec *v, const std::vector<uint64_t> &cmp, long start, long end,
+                  std::vector<long> &pivots) {
+  long i = start;
+  for (long j = start + 1; j <= end; j++)
+    if (cmp[j] == 0) {
+      i++;
+      swap_rows64(v, (size_t)j, (size_t)i);
+    }
+  swap_rows64(v, (size_t)start, (size_t)i);
+  pivots[i] = i;
+  if (i + 1 <= end) pivots[i + 1] = i + 1;
+}
+
+// quicksort_body (quicksort.h:62-198) on the padded (unique) values
+void quicksort_body(oc_ctx *c, oc_vec *v) {
+  const size_t N = v->n;
+  {  // v.shuffle(): one sharded permutation (shuffle.h:752-757)
+    oc_ctx::Sharded sp = next_perm(c, N);
+    apply_sharded<uint64_t, false>(c, v, sp, false);
+  }
+  std::vector<long> pivots(N, -1);
+  pivots[0] = 0;
+  std::vector<uint64_t> exp_cmp(N, 0);
+  for (;;) {
+    // non-pivots, and the pivot of each: the last marked position before it
+    // (pivot_temp = prefix max of pivots)
+    std::vector<size_t> red_idx, piv_idx;
+    long run = -1;
+    for (size_t j = 0; j < N; j++) {
+      run = std::max(run, pivots[j]);
+      if (pivots[j] < 0) {
+        red_idx.push_back(j);
+        piv_idx.push_back((size_t)run);
+      }
+    }
+    const size_t size = red_idx.size();
+    oc_vec *red = gather_rows<uint64_t>(v, red_idx), *piv = gather_rows<uint64_t>(v, piv_idx);
+    oc_vec *eq = new_vec(size, 8), *gt = new_vec(size, 8);
+    compare<uint64_t>(c, red, piv, eq, gt);  // reduced_vec._compare(pivot_vec, temp, comparisons)
+    std::vector<uint64_t> cmp = open_vec<uint64_t, false>(c, gt);
+    for (size_t k = 0; k < size; k++) exp_cmp[red_idx[k]] = cmp[k];
+    delete red;
+    delete piv;
+    delete eq;
+    delete gt;
+    long index = -1;
+    for (long p = 0; p < (long)N; p++) {
+      if (pivots[p] < 0) continue;
+      if (p - index == 1) {  // adjacent pivots
+        index++;
+        continue;
+      }
+      qs_partition(v, exp_cmp, index, p - 1, pivots);
+      index = p;
+    }
+    qs_partition(v, exp_cmp, index, (long)N - 1, pivots);
+    size_t num_pivots = 0;
+    for (long x : pivots) num_pivots += x >= 0;
+    if (num_pivots >= N) return;
+  }
+}
+
+void quicksort32(oc_ctx *c, oc_vec *v, bool desc, oc_vec *perm_out) {
+  const size_t n = v->n;
+  reserve(c, n, 3);  // 1 for the shuffle, 1 for remove_padding's b2a, 1 pair for invert
+  if (n == 0) return;
+  oc_vec *pad = new_vec(n, 8);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const uint32_t *src = S<uint32_t>(v, p, k);
+      uint64_t *dst = S<uint64_t>(pad, p, k);
+      for (size_t i = 0; i < n; i++) dst[i] = (uint64_t)(int64_t)(int32_t)src[i] << 32;
+    }
+  for (size_t i = 0; i < n; i++) {
+    uint32_t idx = desc ? (uint32_t)(-1 - (int64_t)i) : (uint32_t)i;
+    S<uint64_t>(pad, 0, 0)[i] |= idx;
+    S<uint64_t>(pad, 2, 1)[i] |= idx;
+  }
+  if (desc) reverse_vec(pad);
+  quicksort_body(c, pad);
+  if (desc) reverse_vec(pad);
+  for (int p = 0; p < 3; p++)
+    for (int k = 0; k < 2; k++) {
+      const uint64_t *src = S<uint64_t>(pad, p, k);
+      uint32_t *pd = S<uint32_t>(perm_out, p, k);
+      uint32_t *vd = S<uint32_t>(v, p, k);
+      for (size_t j = 0; j < n; j++) {
+        pd[j] = (uint32_t)src[j];
+        vd[j] = (uint32_t)(uint64_t)((int64_t)src[j] >> 32);
+      }
+    }
+  delete pad;
+  perm_b2a(c, perm_out);
+  if (desc) perm_negate(c, perm_out);
+  perm_invert(c, perm_out);
+}
+
 // ElementwisePermutation::reverse (permutation.h:177-198): p * (-1) + (n - 1)
 void perm_reverse(oc_ctx *c, oc_vec *perm) {
   const size_t n = perm->n;
@@ -949,20 +1065,32 @@ void radix_any(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
     radix_sort<uint64_t, u128, i128>(c, v, bits, desc, perm_out);
 }
 
-// operators::table_sort with SortingProtocol::RADIXSORT (sorting.h:509-612)
+// operators::table_sort (sorting.h:509-612) with protocol 0 RADIXSORT or
+// 1 QUICKSORT (32-bit keys); single-bit columns use 1-bit radix sort either way
 void table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, const int *single_bit,
-                oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb) {
+                oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb, int protocol) {
   const size_t n = keys[0]->n;
   int ns = 0;
   for (size_t j = 0; j < nkeys; j++) ns += single_bit && single_bit[j] ? 1 : 0;
   const int nk = (int)nkeys - ns, nc = (int)(na + nb), L = 8 * keys[0]->width;
   long perms_required = nk + ns - 1 + nk + ns;
-  long pairs_required = 4L * ns + 3L * nk + nc - 1 + (long)nk * L;
+  long pairs_required = 4L * ns + 3L * nk + nc - 1;
+  if (protocol == 1)
+    perms_required += nk;
+  else
+    pairs_required += (long)nk * L;
   if (n == 0) return;
   reserve(c, n, (size_t)(perms_required + pairs_required));
   auto sort_sub = [&](int col, oc_vec *perm_out) {
-    int bits = single_bit && single_bit[col] ? 1 : 8 * keys[col]->width;
-    radix_any(c, keys[col], bits, orders && orders[col] == 1, perm_out);
+    const bool desc = orders && orders[col] == 1;
+    if (single_bit && single_bit[col]) {
+      radix_any(c, keys[col], 1, desc, perm_out);
+    } else if (protocol == 1) {
+      if (keys[col]->width != 4) throw std::runtime_error("quicksort: 32-bit keys only");
+      quicksort32(c, keys[col], desc, perm_out);
+    } else {
+      radix_any(c, keys[col], 8 * keys[col]->width, desc, perm_out);
+    }
   };
   int C = (int)nkeys - 1;
   oc_vec *sort_perm = new_vec(n, 4);
@@ -1977,8 +2105,15 @@ int oc_table_shuffle(oc_ctx *c, oc_vec **a_cols, size_t na, oc_vec **b_cols, siz
   OC_TRY(table_shuffle(c, a_cols, na, b_cols, nb))
 }
 int oc_table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, const int *single_bit,
-                  oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb) {
-  OC_TRY(table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb))
+                  oc_vec **a_cols, size_t na, oc_vec **b_cols, size_t nb, int protocol) {
+  OC_TRY(table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb, protocol))
+}
+
+int oc_quicksort(oc_ctx *c, oc_vec *v, int desc, oc_vec *perm_out) {
+  OC_TRY({
+    if (v->width != 4) throw std::runtime_error("quicksort: 32-bit keys only");
+    quicksort32(c, v, desc != 0, perm_out);
+  })
 }
 
 int oc_bitonic_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, oc_vec **a_cols,

@@ -52,7 +52,7 @@ EXPORTS = [
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
     "orqg_aggregate", "orqg_distinct", "orqg_distinct_noalloc", "orqg_vec_unary",
-    "orqg_prefix_sum", "orqg_public_fill", "orqg_vec_copy_rows",
+    "orqg_prefix_sum", "orqg_quicksort", "orqg_table_sort_protocol", "orqg_public_fill", "orqg_vec_copy_rows",
     "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
     "orqg_a2b", "orqg_secret_share_from", "orqg_server_create", "orqg_server_submit",
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
@@ -120,6 +120,8 @@ def lib():
             "orqg_distinct_noalloc": (i32, [vp, vp, sz, vp]),
             "orqg_vec_unary": (i32, [vp, i32, vp, u64, vp]),
             "orqg_prefix_sum": (i32, [vp, vp]),
+            "orqg_quicksort": (i32, [vp, vp, i32, vp]),
+            "orqg_table_sort_protocol": (i32, [vp, vp, sz, vp, vp, vp, sz, vp, sz, i32]),
             "orqg_public_fill": (i32, [vp, vp, u64, i32]),
             "orqg_vec_copy_rows": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_vec_broadcast_row": (i32, [vp, vp, sz, vp, sz, sz]),
@@ -407,14 +409,25 @@ class Ctx:
         b, nb = self._vec_array(b_cols)
         self._check(lib().orqg_table_shuffle(self.h, a, na, b, nb), "table_shuffle")
 
-    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=()):
+    PROTOCOLS = {"RADIXSORT": 0, "QUICKSORT": 1}
+
+    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=(),
+                   protocol="RADIXSORT"):
         k, nk = self._vec_array(keys)
         o = (ctypes.c_int * nk)(*[(DESC if x in (DESC, "DESC") else ASC) for x in
                                   (orders or [ASC] * nk)])
         sb = (ctypes.c_int * nk)(*[int(bool(x)) for x in (single_bit or [0] * nk)])
         a, na = self._vec_array(list(a_cols))
         b, nb = self._vec_array(list(b_cols))
-        self._check(lib().orqg_table_sort(self.h, k, nk, o, sb, a, na, b, nb), "table_sort")
+        self._check(lib().orqg_table_sort_protocol(self.h, k, nk, o, sb, a, na, b, nb,
+                                                   self.PROTOCOLS[protocol]), "table_sort")
+
+    def quicksort(self, v, desc=False):
+        """operators::quicksort of 32-bit B-shared keys, in place; returns the
+        A-shared sorting permutation."""
+        perm = Vec(self, v.n, 4, ORQG_ASHARED)
+        self._check(lib().orqg_quicksort(self.h, v.h, DESC if desc else ASC, perm.h), "quicksort")
+        return perm
 
     def bitonic_sort(self, keys, orders=None, a_cols=(), b_cols=()):
         k, nk = self._vec_array(keys)

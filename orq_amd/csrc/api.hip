@@ -759,6 +759,26 @@ int orqg_table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int 
   table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb);
   API_END(c)
 }
+int orqg_table_sort_protocol(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
+                             const int *single_bit, orqg_vec *const *a_cols, size_t na,
+                             orqg_vec *const *b_cols, size_t nb, int protocol) {
+  API_BEGIN(c)
+  if (nkeys == 0 || !keys || !keys[0]) throw Error(ORQG_E_ARG, "table_sort: no sort keys");
+  const size_t n = keys[0]->n;
+  check_cols(keys, nkeys, n);
+  check_cols(a_cols, na, n);
+  check_cols(b_cols, nb, n);
+  for (size_t j = 0; j < nkeys; j++)
+    if (keys[j]->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "table_sort: keys must be B-shared");
+  table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb, protocol);
+  API_END(c)
+}
+int orqg_quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out) {
+  API_BEGIN(c)
+  if (!v || v->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "quicksort: v must be B-shared");
+  quicksort(c, v, order, perm_out);
+  API_END(c)
+}
 
 int orqg_bitonic_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
                       orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols, size_t nb) {

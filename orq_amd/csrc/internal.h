@@ -261,7 +261,8 @@ void bitonic_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const int *orde
                   orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols, size_t nb);
 void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
                 const int *single_bit, orqg_vec *const *a_cols, size_t na,
-                orqg_vec *const *b_cols, size_t nb);
+                orqg_vec *const *b_cols, size_t nb, int protocol = ORQG_RADIXSORT);
+void quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out);
 
 enum LocalOp { kXor = 0, kAdd = 1, kSub = 2, kNot = 3 };
 void vec_unary(orqg_ctx *c, int op, const orqg_vec *x, uint64_t arg, orqg_vec *z);

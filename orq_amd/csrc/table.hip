@@ -244,17 +244,29 @@ void table_shuffle(orqg_ctx *c, orqg_vec *const *a_cols, size_t na, orqg_vec *co
 // operators::table_sort with SortingProtocol::RADIXSORT (sorting.h:509-612).
 void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *orders,
                 const int *single_bit, orqg_vec *const *a_cols, size_t na,
-                orqg_vec *const *b_cols, size_t nb) {
+                orqg_vec *const *b_cols, size_t nb, int protocol) {
   if (nkeys == 0) throw Error(ORQG_E_ARG, "table_sort: no sort keys");
+  if (protocol != ORQG_RADIXSORT && protocol != ORQG_QUICKSORT)
+    throw Error(ORQG_E_ARG, "table_sort: unknown protocol");
   const uint64_t n = keys[0]->n;
   int ns = 0;
   for (size_t j = 0; j < nkeys; j++) ns += single_bit && single_bit[j] ? 1 : 0;
   const int nk = (int)nkeys - ns;
   const int nc = (int)(na + nb);
   const int L = 8 * keys[0]->width;
-  // sorting.h:531-550 for 3PC and RADIXSORT (pairs are single permutations)
+  if (protocol == ORQG_QUICKSORT)
+    for (size_t j = 0; j < nkeys; j++)
+      if (!(single_bit && single_bit[j]) && keys[j]->width != 4)
+        throw Error(ORQG_E_ARG, "table_sort: QUICKSORT takes 32-bit sort columns");
+  // sorting.h:531-550 for 3PC (pairs are single permutations): quicksort
+  // needs one more permutation per multi-bit key (its shuffle), radix sort
+  // one more pair per key bit
   long perms_required = nk + ns - 1 + nk + ns;
-  long pairs_required = 4L * ns + 3L * nk + nc - 1 + (long)nk * L;
+  long pairs_required = 4L * ns + 3L * nk + nc - 1;
+  if (protocol == ORQG_QUICKSORT)
+    perms_required += nk;
+  else
+    pairs_required += (long)nk * L;
   if (n == 0) return;
   reserve_perms(c, n, (size_t)(perms_required + pairs_required));
 
@@ -270,8 +282,13 @@ void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *ord
   };
   try {
     auto sort_sub = [&](int col, orqg_vec *perm_out) {
-      const int bits = single_bit && single_bit[col] ? 1 : 8 * keys[col]->width;
-      radix_sort(c, keys[col], orders ? orders[col] : ORQG_ASC, bits, perm_out);
+      const int ord = orders ? orders[col] : ORQG_ASC;
+      if (single_bit && single_bit[col])
+        radix_sort(c, keys[col], ord, 1, perm_out);
+      else if (protocol == ORQG_QUICKSORT)
+        quicksort(c, keys[col], ord, perm_out);
+      else
+        radix_sort(c, keys[col], ord, 8 * keys[col]->width, perm_out);
     };
     int C = (int)nkeys - 1;
     orqg_vec sort_perm = make(4), next = make(4);

@@ -23,6 +23,11 @@ RESERVED = (JOIN_ID, VALID, UNIQ)  # encoded_table.h:38
 NON_AGGREGATIONS = ("copy", "valid")  # AggregationSelector::isAggregation
 RADIXSORT = "RADIXSORT"
 BITONICSORT = "BITONICSORT"
+QUICKSORT = "QUICKSORT"
+# SortingProtocol::DEFAULT is QUICKSORT (sorting.h:5-22).  Quicksort pads a
+# 32-bit key to 64 bits; for 64-bit tables (128-bit padded values) DEFAULT
+# resolves to RADIXSORT here.
+DEFAULT = "DEFAULT"
 
 
 def is_bshared(name):
@@ -83,10 +88,12 @@ class EncodedTable:
         """encoded_table.h:1446-1466 with pad value 0."""
         self.resize(1 << max(self.rows - 1, 0).bit_length())
 
-    def sort(self, spec, to_be_sorted=None, protocol=RADIXSORT):
+    def sort(self, spec, to_be_sorted=None, protocol=DEFAULT):
         """EncodedTable::sort (encoded_table.h:900-998).  `spec` is a list of
         column names (all ascending) or of (name, order) pairs."""
-        if protocol not in (RADIXSORT, BITONICSORT):
+        if protocol == DEFAULT:
+            protocol = QUICKSORT if self.width == 4 else RADIXSORT
+        if protocol not in (RADIXSORT, BITONICSORT, QUICKSORT):
             raise ValueError(f"sorting protocol {protocol!r} is not provided")
         spec = [(s, ASC) if isinstance(s, str) else (s[0], s[1]) for s in spec]
         keys = [k for k, _ in spec]
@@ -109,7 +116,8 @@ class EncodedTable:
                 else:
                     self.resize(original)
             return self
-        self.ctx.table_sort([self.schema[k] for k in keys], orders, single, a, b)
+        self.ctx.table_sort([self.schema[k] for k in keys], orders, single, a, b,
+                            protocol=protocol)
         return self
 
     # ---- columns (encoded_table.h:724-803) ----

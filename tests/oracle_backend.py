@@ -64,8 +64,9 @@ class OracleCtx:
     def table_shuffle(self, a_cols, b_cols):
         self.c.table_shuffle(a_cols, b_cols)
 
-    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=()):
-        self.c.table_sort(keys, orders, single_bit, a_cols, b_cols)
+    def table_sort(self, keys, orders=None, single_bit=None, a_cols=(), b_cols=(),
+                   protocol="RADIXSORT"):
+        self.c.table_sort(keys, orders, single_bit, a_cols, b_cols, protocol)
 
     def bitonic_sort(self, keys, orders=None, a_cols=(), b_cols=()):
         self.c.bitonic_sort(keys, orders, a_cols, b_cols)

@@ -76,8 +76,9 @@ def _gpu_table(G, gc, data, schema, width=4):
     return EncodedTable.secret_share(gc, data, schema, width)
 
 
+@pytest.mark.parametrize("protocol", ["RADIXSORT", "QUICKSORT"])
 @pytest.mark.parametrize("desc", [False, True])
-def test_reference_table_sort_single(O, G, keys, desc):
+def test_reference_table_sort_single(O, G, keys, desc, protocol):
     # tests/test_sorting.cpp:214-252: shuffle, then sort on "[0]"
     n, ncol = 1000, 4
     schema = [f"[{i}]" if i % 2 == 0 else str(i) for i in range(ncol)]
@@ -90,8 +91,8 @@ def test_reference_table_sort_single(O, G, keys, desc):
     for name in ot:
         _same_shares(ot[name], gt[name])
     a, b = _split(ot, keys=("[0]",))
-    oc.table_sort([ot["[0]"]], ["DESC" if desc else "ASC"], None, a, b)
-    gt.sort([("[0]", "DESC" if desc else "ASC")])
+    oc.table_sort([ot["[0]"]], ["DESC" if desc else "ASC"], None, a, b, protocol)
+    gt.sort([("[0]", "DESC" if desc else "ASC")], protocol=protocol)
     for name in ot:
         _same_shares(ot[name], gt[name])
     _same_nonces(oc, gc)
@@ -102,8 +103,9 @@ def test_reference_table_sort_single(O, G, keys, desc):
         np.testing.assert_array_equal(got[name], want, err_msg=name)
 
 
+@pytest.mark.parametrize("protocol", ["RADIXSORT", "QUICKSORT"])
 @pytest.mark.parametrize("tiebreak", [False, True])
-def test_reference_table_sort_multi(O, G, keys, tiebreak):
+def test_reference_table_sort_multi(O, G, keys, tiebreak, protocol):
     # tests/test_sorting.cpp:258-318, spec [0] ASC, [2] DESC, [4] ASC
     n, ncol = 3000, 6
     rng = np.random.default_rng(9)
@@ -115,8 +117,8 @@ def test_reference_table_sort_multi(O, G, keys, tiebreak):
     gt = _gpu_table(G, gc, data, schema)
     ks = [k for k, _ in spec]
     a, b = _split(ot, keys=ks)
-    oc.table_sort([ot[k] for k in ks], [o for _, o in spec], None, a, b)
-    gt.sort(spec)
+    oc.table_sort([ot[k] for k in ks], [o for _, o in spec], None, a, b, protocol)
+    gt.sort(spec, protocol=protocol)
     assert gc.queue_size() == 0
     for name in ot:
         _same_shares(ot[name], gt[name])

@@ -103,8 +103,9 @@ def _open(c, cols):
     return {k: c.open(v, arith="[" not in k).view(np.int32) for k, v in cols.items()}
 
 
+@pytest.mark.parametrize("protocol", ["RADIXSORT", "QUICKSORT"])
 @pytest.mark.parametrize("desc", [False, True])
-def test_reference_table_sort_single(keys, desc):
+def test_reference_table_sort_single(keys, desc, protocol):
     # tests/test_sorting.cpp:214-252 with num_columns = 4
     n, ncol = 100, 4
     schema = [f"[{i}]" if i % 2 == 0 else str(i) for i in range(ncol)]
@@ -115,7 +116,7 @@ def test_reference_table_sort_single(keys, desc):
     shuffled = _open(c, cols)
     assert not np.array_equal(shuffled["[0]"], np.arange(n))
     a, b = _split(cols, keys=("[0]",))
-    c.table_sort([cols["[0]"]], ["DESC" if desc else "ASC"], None, a, b)
+    c.table_sort([cols["[0]"]], ["DESC" if desc else "ASC"], None, a, b, protocol)
     got = _open(c, cols)
     want = np.arange(n)[::-1] if desc else np.arange(n)
     for name in schema:
@@ -124,8 +125,9 @@ def test_reference_table_sort_single(keys, desc):
     assert c.queue_size() == 0
 
 
+@pytest.mark.parametrize("protocol", ["RADIXSORT", "QUICKSORT"])
 @pytest.mark.parametrize("tiebreak", [False, True])
-def test_reference_table_sort_multi(keys, tiebreak):
+def test_reference_table_sort_multi(keys, tiebreak, protocol):
     # tests/test_sorting.cpp:258-318 with num_columns = 6: spec [0] ASC, [2] DESC, [4] ASC
     n, ncol = 300, 6
     rng = np.random.default_rng(5)
@@ -136,11 +138,11 @@ def test_reference_table_sort_multi(keys, tiebreak):
     t1, t2 = _table(c1, data, schema), _table(c2, data, schema)
     ks = [k for k, _ in spec]
     a, b = _split(t1, keys=ks)
-    c1.table_sort([t1[k] for k in ks], [o for _, o in spec], None, a, b)
+    c1.table_sort([t1[k] for k in ks], [o for _, o in spec], None, a, b, protocol)
     assert c1.queue_size() == 0
     for k, o in reversed(spec):  # one column at a time, last key first
         a, b = _split(t2, keys=(k,))
-        c2.table_sort([t2[k]], [o], None, a, b)
+        c2.table_sort([t2[k]], [o], None, a, b, protocol)
     g1, g2 = _open(c1, t1), _open(c2, t2)
     order = np.lexsort((data[4], -np.asarray(data[2]), data[0]))
     for name in schema:
