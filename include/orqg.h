@@ -297,6 +297,34 @@ int orqg_quicksort(orqg_ctx *ctx, orqg_vec *v, int order, orqg_vec *perm_out);
  * signed), returns the A-shared sorting permutation in perm_out (width 4). */
 int orqg_radix_sort(orqg_ctx *ctx, orqg_vec *v, int order, int bits, orqg_vec *perm_out);
 
+/* ---- share files (the reference's CSV formats) ------------------------ */
+/* Each call covers the three parties' files: party_files[p] holds party p's
+ * replicated pair (s_p, s_{p+1}); a pair whose two holders disagree is
+ * refused (ORQG_E_PARITY).  Values are signed decimals of the share width.  */
+/* EVector(size, file) (include/core/containers/e_vector.h:316-348): one row
+ * "a,b" (or "a b") per element; a first line that is not numeric is skipped;
+ * rows past v's size are ignored, missing rows are zero shares.             */
+int orqg_vec_read_share_files(orqg_ctx *ctx, orqg_vec *v, const char *const party_files[3]);
+/* EVector::output (e_vector.h:249-271)                                      */
+int orqg_vec_write_share_files(orqg_ctx *ctx, const orqg_vec *v,
+                               const char *const party_files[3]);
+/* EncodedTable::inputCSVTableSecretShares (encoded_table.h:491-560): header
+ * "[COL]_0,[COL]_1,..."; names[j] names cols[j] (all of one size n).  When the
+ * table has "[##V]" and the files lack it, party 0 secret-shares ones into it
+ * over the rows read.                                                        */
+int orqg_table_read_share_csv(orqg_ctx *ctx, const char *const party_files[3],
+                              const char *const *names, orqg_vec *const *cols, size_t ncols);
+/* EncodedTable::outputCSVTableSecretShares (encoded_table.h:571-628), columns
+ * in the order given (the reference writes its schema, i.e. sorted names)   */
+int orqg_table_write_share_csv(orqg_ctx *ctx, const char *const party_files[3],
+                               const char *const *names, orqg_vec *const *cols, size_t ncols);
+/* EncodedTable::inputCSVTableData (encoded_table.h:391-488): the input
+ * party's plaintext CSV (header of column names, all in the table); every
+ * column is then secret-shared from input_party in the order given, "[##V]"
+ * being 1 on the rows read when the file lacks it.                          */
+int orqg_table_read_csv_data(orqg_ctx *ctx, const char *path, const char *const *names,
+                             orqg_vec *const *cols, size_t ncols, int input_party);
+
 /* ---- message log (Communicator byte accounting, communicator.h:14-183) - */
 typedef struct {
   char op[24];

@@ -52,7 +52,9 @@ EXPORTS = [
     "orqg_perm_reverse", "orqg_perm_compose", "orqg_shuffle", "orqg_table_shuffle",
     "orqg_table_sort", "orqg_bitonic_sort", "orqg_vec_resize", "orqg_vec_tail",
     "orqg_aggregate", "orqg_distinct", "orqg_distinct_noalloc", "orqg_vec_unary",
-    "orqg_prefix_sum", "orqg_quicksort", "orqg_table_sort_protocol", "orqg_public_fill", "orqg_vec_copy_rows",
+    "orqg_prefix_sum", "orqg_quicksort", "orqg_table_sort_protocol",
+    "orqg_vec_read_share_files", "orqg_vec_write_share_files", "orqg_table_read_share_csv",
+    "orqg_table_write_share_csv", "orqg_table_read_csv_data", "orqg_public_fill", "orqg_vec_copy_rows",
     "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
     "orqg_a2b", "orqg_secret_share_from", "orqg_server_create", "orqg_server_submit",
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
@@ -121,6 +123,11 @@ def lib():
             "orqg_vec_unary": (i32, [vp, i32, vp, u64, vp]),
             "orqg_prefix_sum": (i32, [vp, vp]),
             "orqg_quicksort": (i32, [vp, vp, i32, vp]),
+            "orqg_vec_read_share_files": (i32, [vp, vp, vp]),
+            "orqg_vec_write_share_files": (i32, [vp, vp, vp]),
+            "orqg_table_read_share_csv": (i32, [vp, vp, vp, vp, sz]),
+            "orqg_table_write_share_csv": (i32, [vp, vp, vp, vp, sz]),
+            "orqg_table_read_csv_data": (i32, [vp, ctypes.c_char_p, vp, vp, sz, i32]),
             "orqg_table_sort_protocol": (i32, [vp, vp, sz, vp, vp, vp, sz, vp, sz, i32]),
             "orqg_public_fill": (i32, [vp, vp, u64, i32]),
             "orqg_vec_copy_rows": (i32, [vp, vp, sz, vp, sz, sz]),
@@ -421,6 +428,47 @@ class Ctx:
         b, nb = self._vec_array(list(b_cols))
         self._check(lib().orqg_table_sort_protocol(self.h, k, nk, o, sb, a, na, b, nb,
                                                    self.PROTOCOLS[protocol]), "table_sort")
+
+    # ---- share files (the reference's CSV formats) ----
+    @staticmethod
+    def _paths(files):
+        assert len(files) == 3, "one file per party"
+        bufs = [str(f).encode() for f in files]
+        return (ctypes.c_char_p * 3)(*bufs), bufs
+
+    @staticmethod
+    def _names(names):
+        bufs = [n.encode() for n in names]
+        return (ctypes.c_char_p * max(len(bufs), 1))(*bufs), bufs
+
+    def read_share_files(self, v, files):
+        """EVector(size, file) for the three parties' files."""
+        arr, _keep = self._paths(files)
+        self._check(lib().orqg_vec_read_share_files(self.h, v.h, arr), "vec_read_share_files")
+
+    def write_share_files(self, v, files):
+        """EVector::output to the three parties' files."""
+        arr, _keep = self._paths(files)
+        self._check(lib().orqg_vec_write_share_files(self.h, v.h, arr), "vec_write_share_files")
+
+    def table_read_share_csv(self, files, names, cols):
+        arr, _k1 = self._paths(files)
+        nm, _k2 = self._names(names)
+        c, nc = self._vec_array(list(cols))
+        self._check(lib().orqg_table_read_share_csv(self.h, arr, nm, c, nc), "table_read_share_csv")
+
+    def table_write_share_csv(self, files, names, cols):
+        arr, _k1 = self._paths(files)
+        nm, _k2 = self._names(names)
+        c, nc = self._vec_array(list(cols))
+        self._check(lib().orqg_table_write_share_csv(self.h, arr, nm, c, nc),
+                    "table_write_share_csv")
+
+    def table_read_csv_data(self, path, names, cols, input_party=0):
+        nm, _k = self._names(names)
+        c, nc = self._vec_array(list(cols))
+        self._check(lib().orqg_table_read_csv_data(self.h, str(path).encode(), nm, c, nc,
+                                                   input_party), "table_read_csv_data")
 
     def quicksort(self, v, desc=False):
         """operators::quicksort of 32-bit B-shared keys, in place; returns the

@@ -773,6 +773,49 @@ int orqg_table_sort_protocol(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, c
   table_sort(c, keys, nkeys, orders, single_bit, a_cols, na, b_cols, nb, protocol);
   API_END(c)
 }
+static void check_files(const char *const f[3]) {
+  if (!f || !f[0] || !f[1] || !f[2]) throw Error(ORQG_E_ARG, "csv: three party file paths needed");
+}
+int orqg_vec_read_share_files(orqg_ctx *c, orqg_vec *v, const char *const files[3]) {
+  API_BEGIN(c)
+  if (!v) throw Error(ORQG_E_ARG, "vec_read_share_files: NULL vector");
+  check_files(files);
+  vec_read_share_files(c, v, files);
+  API_END(c)
+}
+int orqg_vec_write_share_files(orqg_ctx *c, const orqg_vec *v, const char *const files[3]) {
+  API_BEGIN(c)
+  if (!v) throw Error(ORQG_E_ARG, "vec_write_share_files: NULL vector");
+  check_files(files);
+  vec_write_share_files(c, v, files);
+  API_END(c)
+}
+int orqg_table_read_share_csv(orqg_ctx *c, const char *const files[3], const char *const *names,
+                              orqg_vec *const *cols, size_t ncols) {
+  API_BEGIN(c)
+  check_files(files);
+  if (ncols && (!names || !cols)) throw Error(ORQG_E_ARG, "table_read_share_csv: NULL columns");
+  check_cols(cols, ncols, ncols ? cols[0]->n : 0);
+  table_read_share_csv(c, files, names, cols, ncols);
+  API_END(c)
+}
+int orqg_table_write_share_csv(orqg_ctx *c, const char *const files[3], const char *const *names,
+                               orqg_vec *const *cols, size_t ncols) {
+  API_BEGIN(c)
+  check_files(files);
+  if (ncols && (!names || !cols)) throw Error(ORQG_E_ARG, "table_write_share_csv: NULL columns");
+  check_cols(cols, ncols, ncols ? cols[0]->n : 0);
+  table_write_share_csv(c, files, names, cols, ncols);
+  API_END(c)
+}
+int orqg_table_read_csv_data(orqg_ctx *c, const char *path, const char *const *names,
+                             orqg_vec *const *cols, size_t ncols, int input_party) {
+  API_BEGIN(c)
+  if (!path || (ncols && (!names || !cols))) throw Error(ORQG_E_ARG, "table_read_csv_data: NULL");
+  check_cols(cols, ncols, ncols ? cols[0]->n : 0);
+  table_read_csv_data(c, path, names, cols, ncols, input_party);
+  API_END(c)
+}
 int orqg_quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out) {
   API_BEGIN(c)
   if (!v || v->enc != ORQG_BSHARED) throw Error(ORQG_E_ARG, "quicksort: v must be B-shared");

@@ -263,6 +263,14 @@ void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *ord
                 const int *single_bit, orqg_vec *const *a_cols, size_t na,
                 orqg_vec *const *b_cols, size_t nb, int protocol = ORQG_RADIXSORT);
 void quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out);
+void vec_read_share_files(orqg_ctx *c, orqg_vec *v, const char *const files[3]);
+void vec_write_share_files(orqg_ctx *c, const orqg_vec *v, const char *const files[3]);
+void table_read_share_csv(orqg_ctx *c, const char *const files[3], const char *const *names,
+                          orqg_vec *const *cols, size_t ncols);
+void table_write_share_csv(orqg_ctx *c, const char *const files[3], const char *const *names,
+                           orqg_vec *const *cols, size_t ncols);
+void table_read_csv_data(orqg_ctx *c, const char *path, const char *const *names,
+                         orqg_vec *const *cols, size_t ncols, int input_party);
 
 enum LocalOp { kXor = 0, kAdd = 1, kSub = 2, kNot = 3 };
 void vec_unary(orqg_ctx *c, int op, const orqg_vec *x, uint64_t arg, orqg_vec *z);

@@ -58,6 +58,43 @@ class EncodedTable:
             cols[col] = ctx.secret_share(a, width, arith=not is_bshared(col))
         return cls(ctx, name, cols, width)
 
+    @classmethod
+    def with_schema(cls, ctx, name, schema, rows, width=4):
+        """EncodedTable(name, schema, rows) (encoded_table.h:244-249): zero
+        columns, B-shared when the name has '[', and the valid column set to
+        ones (configureValid)."""
+        cols = {c: ctx.vec(rows, width, ORQG_BSHARED if is_bshared(c) else ORQG_ASHARED)
+                for c in schema}
+        t = cls(ctx, name, cols, width)
+        if VALID in schema:
+            ctx.public_fill(t.schema[VALID], 1, ORQG_BSHARED)
+        return t
+
+    # ---- share files (encoded_table.h:382-628) ----
+    def input_csv_table_data(self, path, input_party=0):
+        """inputCSVTableData: the input party's plaintext CSV, then every
+        column secret-shared from it in schema order."""
+        names = self.column_names()
+        self.ctx.table_read_csv_data(path, names, [self.schema[c] for c in names], input_party)
+
+    def input_csv_table_secret_shares(self, files):
+        """inputCSVTableSecretShares from the three parties' files."""
+        names = self.column_names()
+        self.ctx.table_read_share_csv(files, names, [self.schema[c] for c in names])
+
+    def output_csv_table_secret_shares(self, files):
+        """outputCSVTableSecretShares to the three parties' files."""
+        names = self.column_names()
+        self.ctx.table_write_share_csv(files, names, [self.schema[c] for c in names])
+
+    def input_secret_shares(self, col, files):
+        """inputSecretShares: one column from the three parties' files."""
+        self.ctx.read_share_files(self.schema[col], files)
+
+    def output_secret_shares(self, col, files):
+        """outputSecretShares: one column to the three parties' files."""
+        self.ctx.write_share_files(self.schema[col], files)
+
     def column_names(self):
         return sorted(self.schema)
 
