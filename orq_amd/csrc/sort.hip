@@ -127,43 +127,35 @@ __global__ void k_pad(uint64_t n, bool desc, const K *__restrict__ v, uint32_t *
 }
 
 // ---- one bit: b2a_bit + block sums -------------------------------------------------
+// A superblock's 12 words per thread (4 rows x 3 shares of the key word that
+// holds the bit), issued together: a row past the end reads row n-1 (no
+// branch around a load, which would serialize them) and is zeroed when used.
 template <int L>
-__global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0,
-                                                 const uint32_t *__restrict__ rk_pair,
-                                                 NonceTriple nb, CallSched s,
-                                                 const uint32_t *__restrict__ P, int word,
-                                                 int bit, uint32_t flip, uint32_t *__restrict__ F,
-                                                 uint32_t *__restrict__ bsum) {
-  AES_LDS;
-  __shared__ uint32_t s_w[3 * (kBT / 64)];
-  aes_load_tables<kCopies>(lds, t0);
-  AesCtx<kCopies> A(lds);
-  const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
-  // a superblock's 12 words per thread, issued together: a row past the end
-  // reads row n-1 (no branch around a load, which would serialize them) and
-  // is zeroed when used
-  uint32_t raw[4][3];
-  auto load_raw = [&](uint32_t sb) {
-    const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
+__device__ inline void bit_load_raw(const uint32_t *__restrict__ P, uint64_t n, int word,
+                                    uint64_t e0, uint32_t (&raw)[4][3]) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const uint64_t e = e0 + i < s.n ? e0 + i : s.n - 1;
+  for (int i = 0; i < 4; i++) {
+    const uint64_t e = e0 + i < n ? e0 + i : n - 1;
 #pragma unroll
-      for (int k = 0; k < 3; k++) raw[i][k] = P[e * kRec + k * L + word];
-    }
-  };
-  if (blockIdx.x < nsb) load_raw(blockIdx.x);
-  for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
-  const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
-  uint32_t bits[4][3], f[4][3];
+    for (int k = 0; k < 3; k++) raw[i][k] = P[e * kRec + k * L + word];
+  }
+}
+
+__device__ inline void bit_extract(const uint32_t (&raw)[4][3], uint64_t e0, uint64_t n, int bit,
+                                   uint32_t flip, uint32_t (&bits)[4][3]) {
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-      bits[i][k] = (e0 + i < s.n) ? (((raw[i][k] >> bit) & 1u) ^ flip) : 0u;
-  // the next superblock's words are in flight during this one's keystream
-  // (the last superblock reloads itself rather than branch)
-  load_raw(sb + gridDim.x < nsb ? sb + gridDim.x : sb);
+    for (int k = 0; k < 3; k++) bits[i][k] = (e0 + i < n) ? (((raw[i][k] >> bit) & 1u) ^ flip) : 0u;
+}
+
+// b2a_bit of one superblock's bits: f1 to F, the superblock's share sums to
+// bsum3[0..2]
+__device__ inline void bit_b2a_block(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
+                                     const NonceTriple &nb, const CallSched &s, uint64_t e0,
+                                     const uint32_t (&bits)[4][3], uint32_t *__restrict__ F,
+                                     uint32_t *__restrict__ bsum3, uint32_t *s_w) {
+  uint32_t f[4][3];
   b2a_bit_group<uint32_t>(A, rk_pair, nb, s, e0, bits, f);
   uint32_t loc[3] = {0, 0, 0};
 #pragma unroll
@@ -186,13 +178,36 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
 #pragma unroll
         for (int k = 0; k < 3; k++) F[(e0 + i) * 3 + k] = f[i][k];
   }
-  {
-    uint32_t ex[3], tot[3];
-    bexcl3(loc, s_w, ex, tot);
-    if (threadIdx.x == 0)
+  uint32_t ex[3], tot[3];
+  bexcl3(loc, s_w, ex, tot);
+  if (threadIdx.x == 0)
 #pragma unroll
-      for (int k = 0; k < 3; k++) bsum[sb * 3 + k] = tot[k];
-  }
+    for (int k = 0; k < 3; k++) bsum3[k] = tot[k];
+}
+
+template <int L>
+__global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0,
+                                                 const uint32_t *__restrict__ rk_pair,
+                                                 NonceTriple nb, CallSched s,
+                                                 const uint32_t *__restrict__ P, int word,
+                                                 int bit, uint32_t flip, uint32_t *__restrict__ F,
+                                                 uint32_t *__restrict__ bsum) {
+  AES_LDS;
+  __shared__ uint32_t s_w[3 * (kBT / 64)];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
+  uint32_t raw[4][3];
+  if (blockIdx.x < nsb) bit_load_raw<L>(P, s.n, word, (uint64_t)blockIdx.x * kEPB + threadIdx.x * 4, raw);
+  for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
+    const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
+    uint32_t bits[4][3];
+    bit_extract(raw, e0, s.n, bit, flip, bits);
+    // the next superblock's words are in flight during this one's keystream
+    // (the last superblock reloads itself rather than branch)
+    const uint32_t nx = sb + gridDim.x < nsb ? sb + gridDim.x : sb;
+    bit_load_raw<L>(P, s.n, word, (uint64_t)nx * kEPB + threadIdx.x * 4, raw);
+    bit_b2a_block(A, rk_pair, nb, s, e0, bits, F, bsum + sb * 3, s_w);
   }
 }
 
@@ -221,22 +236,14 @@ __global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restric
 }
 
 // s0_k(e) = one_k*e - F_k(e), s1_k(e) = one_k*(n-1) - Ftot_k + F_k(e) with F the
-// inclusive prefix of f1 and one = (1,0,0); perm = s0 + f1*(s1 - s0).
-template <int L>
-__global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t0,
-                                                  const uint32_t *__restrict__ rk_pair,
-                                                  NonceTriple nb, CallSched s,
-                                                  const uint32_t *__restrict__ F,
-                                                  const uint32_t *__restrict__ boff,
-                                                  const uint32_t *__restrict__ tot,
-                                                  uint32_t *__restrict__ R) {
-  AES_LDS;
-  __shared__ uint32_t s_w[3 * (kBT / 64)];
-  aes_load_tables<kCopies>(lds, t0);
-  AesCtx<kCopies> A(lds);
-  const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
-  for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
-  const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
+// inclusive prefix of f1 and one = (1,0,0); perm = s0 + f1*(s1 - s0).  One
+// superblock: boff3 = its exclusive offsets, tot = the totals (3 words each).
+__device__ inline void bit_perm_block(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
+                                      const NonceTriple &nb, const CallSched &s, uint64_t e0,
+                                      const uint32_t *__restrict__ F,
+                                      const uint32_t *__restrict__ boff3,
+                                      const uint32_t *__restrict__ tot, uint32_t *__restrict__ R,
+                                      uint32_t *s_w) {
   uint32_t f[4][3];
   uint32_t loc[3] = {0, 0, 0};
   if (e0 + 4 <= s.n) {
@@ -260,7 +267,7 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
     uint32_t t[3];
     bexcl3(loc, s_w, pre, t);
 #pragma unroll
-    for (int k = 0; k < 3; k++) pre[k] += boff[sb * 3 + k];
+    for (int k = 0; k < 3; k++) pre[k] += boff3[k];
   }
   uint32_t S[3][4];
   {
@@ -303,7 +310,24 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
       o[1] = make_uint4(pr[0], pr[1], pr[2], 0u);
     }
   }
-  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t0,
+                                                  const uint32_t *__restrict__ rk_pair,
+                                                  NonceTriple nb, CallSched s,
+                                                  const uint32_t *__restrict__ F,
+                                                  const uint32_t *__restrict__ boff,
+                                                  const uint32_t *__restrict__ tot,
+                                                  uint32_t *__restrict__ R) {
+  AES_LDS;
+  __shared__ uint32_t s_w[3 * (kBT / 64)];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
+  for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x)
+    bit_perm_block(A, rk_pair, nb, s, (uint64_t)sb * kEPB + threadIdx.x * 4, F, boff + sb * 3, tot,
+                   R, s_w);
 }
 
 // ---- sharded round on element records: gather through idx, then add the
@@ -345,23 +369,18 @@ __device__ inline void quad_transpose(uint4 (&X)[4], uint32_t s) {
   }
 }
 
+// One wave step of a sharded round: rows [128 w, 128 w + 128) of the n = sv.n
+// rows (every wave of the launch takes its own steps).
 template <int L, bool SCATTER>
-__global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
-                                                const uint32_t *__restrict__ rk, uint64_t nv,
-                                                CallSched sv, uint64_t np, CallSched sp,
-                                                const int32_t *__restrict__ idx,
-                                                const uint32_t *__restrict__ Rin,
-                                                uint32_t *__restrict__ Rout,
-                                                uint32_t *__restrict__ err) {
-  AES_LDS;
-  aes_load_tables<kCopies>(lds, t0);
-  AesCtx<kCopies> A(lds);
+__device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk,
+                                   uint64_t nv, const CallSched &sv, uint64_t np,
+                                   const CallSched &sp, const int32_t *__restrict__ idx,
+                                   const uint32_t *__restrict__ Rin, uint32_t *__restrict__ Rout,
+                                   uint32_t *__restrict__ err, uint64_t w) {
   constexpr int kVq = (3 * L + 3) / 4;  // 16-byte pieces holding v
   const uint64_t n = sv.n;
   const uint32_t lane = threadIdx.x & 63, s = lane & 3;
   const uint64_t nstep = (n + 127) / 128;
-  const uint64_t wstride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   // source row of this lane's element i of step ww, or -1
   auto load_idx = [&](uint64_t ww, int32_t (&ix)[2]) {
 #pragma unroll
@@ -381,100 +400,115 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
       }
     }
   };
-  for (; w < nstep; w += wstride) {
-    const uint64_t base = w * 128;
-    // the step's records, in flight during its AES work (loading a step
-    // ahead measured slower: the registers it holds cost more than it hides)
-    int32_t ci[2];
-    uint4 cv[2][kVq], cq[2];
-    load_idx(w, ci);
-    load_rec(ci, cv, cq);
-    // this lane's perm block: call lane & 1 of the group of block (lane >> 1) & 1
-    uint4 pb;
-    {
+  const uint64_t base = w * 128;
+  // the step's records, in flight during its AES work (loading a step
+  // ahead measured slower: the registers it holds cost more than it hides)
+  int32_t ci[2];
+  uint4 cv[2][kVq], cq[2];
+  load_idx(w, ci);
+  load_rec(ci, cv, cq);
+  // this lane's perm block: call lane & 1 of the group of block (lane >> 1) & 1
+  uint4 pb;
+  {
+    uint64_t off;
+    uint32_t ctr, bib;
+    sched_locate(sp, base + 64 * ((lane >> 1) & 1) + (lane & ~3u), lane & 1, off, ctr, bib);
+    pb = aes_ctr_block<kCopies>(A, rk, np + off, ctr);
+  }
+  // word s of quad lane c's block, for each c: calls 0 and 1 of blocks 0 and 1
+  uint32_t pm[4];
+  {
+    // every lane takes word s: each word is broadcast from quad lane c (all
+    // lanes active), then selected
+    const uint4 b0 = quad_dpp4<0x00>(pb), b1 = quad_dpp4<0x55>(pb);
+    pm[0] = word_of(b0, s);
+    pm[1] = word_of(b1, s);
+    const uint4 b2 = quad_dpp4<0xAA>(pb), b3 = quad_dpp4<0xFF>(pb);
+    pm[2] = word_of(b2, s);
+    pm[3] = word_of(b3, s);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const uint64_t e = base + 64 * i + lane;
+    const bool live = e < n;
+    const uint32_t m0 = pm[2 * i], m1 = pm[2 * i + 1];
+    const uint32_t m2 = 0u - m0 - m1;
+    // v masks: element e's bytes of calls 0 and 1
+    uint32_t r[2][L];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
       uint64_t off;
       uint32_t ctr, bib;
-      sched_locate(sp, base + 64 * ((lane >> 1) & 1) + (lane & ~3u), lane & 1, off, ctr, bib);
-      pb = aes_ctr_block<kCopies>(A, rk, np + off, ctr);
+      sched_locate(sv, live ? e : 0, j, off, ctr, bib);
+      uint4 blk = aes_ctr_block<kCopies>(A, rk, nv + off, ctr);
+      uint32_t w4[4] = {blk.x, blk.y, blk.z, blk.w};
+#pragma unroll
+      for (int q = 0; q < L; q++) r[j][q] = w4[bib / 4 + q];
     }
-    // word s of quad lane c's block, for each c: calls 0 and 1 of blocks 0 and 1
-    uint32_t pm[4];
-    {
-      // every lane takes word s: each word is broadcast from quad lane c (all
-      // lanes active), then selected
-      const uint4 b0 = quad_dpp4<0x00>(pb), b1 = quad_dpp4<0x55>(pb);
-      pm[0] = word_of(b0, s);
-      pm[1] = word_of(b1, s);
-      const uint4 b2 = quad_dpp4<0xAA>(pb), b3 = quad_dpp4<0xFF>(pb);
-      pm[2] = word_of(b2, s);
-      pm[3] = word_of(b3, s);
+    uint32_t wd[kRec];
+#pragma unroll
+    for (int q = 0; q < kVq; q++) {
+      wd[4 * q] = cv[i][q].x;
+      wd[4 * q + 1] = cv[i][q].y;
+      wd[4 * q + 2] = cv[i][q].z;
+      wd[4 * q + 3] = cv[i][q].w;
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      const uint64_t e = base + 64 * i + lane;
-      const bool live = e < n;
-      const uint32_t m0 = pm[2 * i], m1 = pm[2 * i + 1];
-      const uint32_t m2 = 0u - m0 - m1;
-      // v masks: element e's bytes of calls 0 and 1
-      uint32_t r[2][L];
+    for (int q = 4 * kVq; q < kRec; q++) wd[q] = 0u;
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        uint64_t off;
-        uint32_t ctr, bib;
-        sched_locate(sv, live ? e : 0, j, off, ctr, bib);
-        uint4 blk = aes_ctr_block<kCopies>(A, rk, nv + off, ctr);
-        uint32_t w4[4] = {blk.x, blk.y, blk.z, blk.w};
-#pragma unroll
-        for (int q = 0; q < L; q++) r[j][q] = w4[bib / 4 + q];
+    for (int q = 0; q < L; q++) {
+      wd[0 * L + q] ^= r[0][q];
+      wd[1 * L + q] ^= r[1][q];
+      wd[2 * L + q] ^= r[0][q] ^ r[1][q];
+    }
+    wd[kRecPerm + 0] = cq[i].x + m0;
+    wd[kRecPerm + 1] = cq[i].y + m1;
+    wd[kRecPerm + 2] = cq[i].z + m2;
+    // destination row: the element itself, or (scatter) its opened perm
+    uint32_t dst = (uint32_t)e;
+    bool ok = live;
+    if (SCATTER) {
+      dst = wd[kRecPerm] + wd[kRecPerm + 1] + wd[kRecPerm + 2];
+      if (live && dst >= n) {
+        atomicOr(err, 1u);
+        ok = false;
       }
-      uint32_t wd[kRec];
+    }
+    uint4 X[4];
 #pragma unroll
-      for (int q = 0; q < kVq; q++) {
-        wd[4 * q] = cv[i][q].x;
-        wd[4 * q + 1] = cv[i][q].y;
-        wd[4 * q + 2] = cv[i][q].z;
-        wd[4 * q + 3] = cv[i][q].w;
-      }
-#pragma unroll
-      for (int q = 4 * kVq; q < kRec; q++) wd[q] = 0u;
-#pragma unroll
-      for (int q = 0; q < L; q++) {
-        wd[0 * L + q] ^= r[0][q];
-        wd[1 * L + q] ^= r[1][q];
-        wd[2 * L + q] ^= r[0][q] ^ r[1][q];
-      }
-      wd[kRecPerm + 0] = cq[i].x + m0;
-      wd[kRecPerm + 1] = cq[i].y + m1;
-      wd[kRecPerm + 2] = cq[i].z + m2;
-      // destination row: the element itself, or (scatter) its opened perm
-      uint32_t dst = (uint32_t)e;
-      bool ok = live;
-      if (SCATTER) {
-        dst = wd[kRecPerm] + wd[kRecPerm + 1] + wd[kRecPerm + 2];
-        if (live && dst >= n) {
-          atomicOr(err, 1u);
-          ok = false;
-        }
-      }
-      uint4 X[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) X[q] = make_uint4(wd[4 * q], wd[4 * q + 1], wd[4 * q + 2], wd[4 * q + 3]);
-      quad_transpose(X, s);
-      // store t: quarter s of the record of quad lane t (whole records per
-      // instruction); a dropped row is marked by an out-of-range destination
-      const uint32_t dk = ok ? dst : 0xffffffffu;
+    for (int q = 0; q < 4; q++) X[q] = make_uint4(wd[4 * q], wd[4 * q + 1], wd[4 * q + 2], wd[4 * q + 3]);
+    quad_transpose(X, s);
+    // store t: quarter s of the record of quad lane t (whole records per
+    // instruction); a dropped row is marked by an out-of-range destination
+    const uint32_t dk = ok ? dst : 0xffffffffu;
 #define ORQG_STORE(t, CTRL)                                                          \
-  {                                                                                  \
-    const uint32_t dt = quad_dpp<CTRL>(dk);                                          \
-    if (dt != 0xffffffffu) reinterpret_cast<uint4 *>(Rout + (uint64_t)dt * kRec)[s] = X[t]; \
-  }
-      ORQG_STORE(0, 0x00)
-      ORQG_STORE(1, 0x55)
-      ORQG_STORE(2, 0xAA)
-      ORQG_STORE(3, 0xFF)
+{                                                                                  \
+  const uint32_t dt = quad_dpp<CTRL>(dk);                                          \
+  if (dt != 0xffffffffu) reinterpret_cast<uint4 *>(Rout + (uint64_t)dt * kRec)[s] = X[t]; \
+}
+    ORQG_STORE(0, 0x00)
+    ORQG_STORE(1, 0x55)
+    ORQG_STORE(2, 0xAA)
+    ORQG_STORE(3, 0xFF)
 #undef ORQG_STORE
-    }
   }
+}
+
+template <int L, bool SCATTER>
+__global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
+                                                const uint32_t *__restrict__ rk, uint64_t nv,
+                                                CallSched sv, uint64_t np, CallSched sp,
+                                                const int32_t *__restrict__ idx,
+                                                const uint32_t *__restrict__ Rin,
+                                                uint32_t *__restrict__ Rout,
+                                                uint32_t *__restrict__ err) {
+  AES_LDS;
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const uint64_t nstep = (sv.n + 127) / 128;
+  const uint64_t wstride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nstep; w += wstride)
+    round2_step<L, SCATTER>(A, rk, nv, sv, np, sp, idx, Rin, Rout, err, w);
 }
 
 // remove_padding (sorting.h:475-495): perm_b = low 32 bits (B-shared int32),
