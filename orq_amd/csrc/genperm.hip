@@ -1035,11 +1035,9 @@ __global__ void k_gp_done(ChainBufs B, uint32_t *flag) {
 // sorted (descending) iff every link put a larger step in front of a smaller
 // one; a link that did not marks its target dirty, and only dirty lists are
 // sorted.
-__global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r_all, size_t chain_stride,
-                          int32_t *__restrict__ head, int32_t *__restrict__ next,
-                          uint8_t *__restrict__ dirty, uint32_t *__restrict__ err) {
-  const uint32_t *r = r_all + blockIdx.y * chain_stride;
-  int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
+__device__ inline void fy_link(uint32_t n, const uint32_t *__restrict__ r, int32_t *__restrict__ hd,
+                               int32_t *__restrict__ nx, uint8_t *__restrict__ dirty,
+                               uint32_t *__restrict__ err) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     uint32_t t = j + r[j];
     if (t >= n) {
@@ -1051,6 +1049,12 @@ __global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r_all, size_t
     nx[j] = old;
     if (old > (int32_t)j) dirty[t] = 1;
   }
+}
+
+__global__ void k_fy_link(uint32_t n, const uint32_t *__restrict__ r, int32_t *__restrict__ head,
+                          int32_t *__restrict__ next, uint8_t *__restrict__ dirty,
+                          uint32_t *__restrict__ err) {
+  fy_link(n, r, head, next, dirty, err);
 }
 
 // Each dirty target's list sorted by step index, descending, in place (the
@@ -1069,11 +1073,9 @@ __device__ inline void cswap_desc(int32_t &a, int32_t &b) {
   b = lo;
 }
 
-__global__ void k_fy_sort(uint32_t n, int32_t *__restrict__ head, int32_t *__restrict__ next,
-                          int32_t *__restrict__ parent, uint8_t *__restrict__ dirty,
-                          uint32_t *__restrict__ err) {
-  int32_t *hd = head + (size_t)blockIdx.y * n, *nx = next + (size_t)blockIdx.y * n;
-  int32_t *par = parent + (size_t)blockIdx.y * n;
+__device__ inline void fy_sort(uint32_t n, int32_t *__restrict__ hd, int32_t *__restrict__ nx,
+                               int32_t *__restrict__ par, uint8_t *__restrict__ dirty,
+                               uint32_t *__restrict__ err) {
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     int32_t h = hd[t];
     if (!dirty[t]) {
@@ -1141,21 +1143,22 @@ __global__ void k_fy_sort(uint32_t n, int32_t *__restrict__ head, int32_t *__res
   }
 }
 
+__global__ void k_fy_sort(uint32_t n, int32_t *__restrict__ head, int32_t *__restrict__ next,
+                          int32_t *__restrict__ parent, uint8_t *__restrict__ dirty,
+                          uint32_t *__restrict__ err) {
+  fy_sort(n, head, next, parent, dirty, err);
+}
+
 // Position j is final after step j; it receives what position t_j held just
 // before step j: V(q) for q = the last earlier step that targeted t_j (the next
 // one in t_j's sorted list), or t_j itself.  V(m), the value at position m just
 // before step m, is V(parent(m)), or m when no earlier step targeted m.
-__global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r_all, size_t chain_stride,
-                          const int32_t *__restrict__ next, const int32_t *__restrict__ parent,
-                          int32_t *fwd0, int32_t *fwd1, int32_t *fwd2, int32_t *inv0, int32_t *inv1,
-                          int32_t *inv2) {
-  const uint32_t *r = r_all + blockIdx.y * chain_stride;
-  const int32_t *nx = next + (size_t)blockIdx.y * n, *par = parent + (size_t)blockIdx.y * n;
-  int32_t *fwd = blockIdx.y == 0 ? fwd0 : blockIdx.y == 1 ? fwd1 : fwd2;
-  int32_t *inv = blockIdx.y == 0 ? inv0 : blockIdx.y == 1 ? inv1 : inv2;
+__device__ inline void fy_walk(uint32_t n, const uint32_t *__restrict__ r,
+                               const int32_t *__restrict__ nx, const int32_t *__restrict__ par,
+                               int32_t *__restrict__ fwd, int32_t *__restrict__ inv) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const uint32_t t = j + r[j];
-    if (t >= n) continue;  // flagged by k_fy_link
+    if (t >= n) continue;  // flagged by fy_link
     const int32_t q = nx[j];
     uint32_t val = t;
     if (q >= 0) {
@@ -1172,6 +1175,139 @@ __global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r_all, size_t
   }
 }
 
+__global__ void k_fy_walk(uint32_t n, const uint32_t *__restrict__ r, const int32_t *__restrict__ next,
+                          const int32_t *__restrict__ parent, int32_t *fwd, int32_t *inv) {
+  fy_walk(n, r, next, parent, fwd, inv);
+}
+
+// ---- small permutations ----------------------------------------------------------
+// A permutation whose draws fit one refill (fewer than 2^20: the usual case up
+// to a few times 10^4 rows) starts exactly 4 nonces after the previous one, so
+// every local permutation of a reserve -- of several contexts at once -- is
+// drawn by its own workgroup in one launch, from a start nonce known in
+// advance (a permutation that needed more is detected afterwards from its
+// draw count and the later ones of its chain are drawn again).  The workgroup
+// draws kSmallDraws words per pass (one AES block per thread) into LDS; then
+// its first wave scans them 64 at a time: with k open positions, the draws
+// v < k are candidates, and candidate a (rank among them) is accepted for
+// sure if v < k - a; a group of 64 with any other candidate is walked lane by
+// lane.  The swap resolution then runs for every local permutation in one
+// launch sequence (grid y = permutation).
+// draws per pass: one AES block per thread of waves 1.. (wave 0 scans)
+constexpr uint32_t kSmallDraws = (kGpThreads - 64) * 4;
+constexpr uint32_t kScanGroup = 20;  // 64-draw chunks a scanning lane holds at once
+static_assert(kSmallDraws % (64 * kScanGroup) == 0, "scan groups must tile a pass");
+
+struct SmallTask {
+  const uint32_t *rk;  // the group PRG's round keys
+  uint64_t nonce;      // nonce where the permutation's draws start
+  uint32_t *r;         // [n] accepted offsets r_i
+  uint64_t *D;         // draws consumed
+  uint64_t max_draws;  // guard: a permutation drawing more sets an error flag
+  uint32_t n, mask;
+};
+
+// Pass p's draws go to buffer p & 1: waves 1.. draw pass p + 1 while wave 0
+// scans pass p.  Wave 0 leaves the open count after pass p in s_k[p & 1],
+// which every wave reads after the pass's barrier (two slots: wave 0 may
+// already be scanning pass p + 1 while a slower wave still reads slot p & 1).
+template <int COPIES>
+__global__ void __launch_bounds__(kGpThreads) k_gp_small(const uint32_t *__restrict__ tab,
+                                                           const SmallTask *__restrict__ tasks,
+                                                           uint32_t *__restrict__ err) {
+  __shared__ __align__(16) uint32_t lds[AesLds<COPIES>::kWords];
+  __shared__ __align__(16) uint32_t buf[2][kSmallDraws];
+  __shared__ uint32_t s_k[2];
+  aes_load_tables<COPIES>(lds, tab);
+  AesCtx<COPIES> A(lds);
+  const SmallTask t = tasks[blockIdx.x];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t below = lanes_below();
+  auto draw = [&](uint64_t base, uint32_t *dst) {
+    const uint32_t q = threadIdx.x - 64;
+    const uint64_t d = base + q * 4;
+    const uint4 blk = aes_ctr_block<COPIES>(A, t.rk, t.nonce + d / kWordsPerNonce,
+                                            2 + (uint32_t)((d % kWordsPerNonce) / 4));
+    reinterpret_cast<uint4 *>(dst)[q] =
+        make_uint4(blk.x & t.mask, blk.y & t.mask, blk.z & t.mask, blk.w & t.mask);
+  };
+  uint32_t k = t.n, i = 0;  // open positions, acceptances (wave 0)
+  uint64_t D = 0;
+  if (threadIdx.x >= 64) draw(0, buf[0]);
+  __syncthreads();
+  for (uint64_t p = 0;; p++) {
+    const uint64_t base = p * kSmallDraws;
+    if (threadIdx.x >= 64) {
+      draw(base + kSmallDraws, buf[(p + 1) & 1]);
+    } else {
+      const uint32_t *b = buf[p & 1];
+      for (uint32_t g0 = 0; g0 < kSmallDraws / 64 && k > 0; g0 += kScanGroup) {
+        uint32_t vv[kScanGroup];
+#pragma unroll
+        for (uint32_t u = 0; u < kScanGroup; u++) vv[u] = b[(g0 + u) * 64 + lane];
+#pragma unroll
+        for (uint32_t u = 0; u < kScanGroup; u++) {
+          const uint32_t v = vv[u];
+          const uint64_t cand = __ballot(v < k);
+          if (!cand) continue;
+          const uint32_t a = (uint32_t)__popcll(cand & below);
+          uint64_t acc = __ballot(v < k && v + a < k);
+          if (acc != cand) {
+            acc = 0;
+            uint32_t kk = k;
+            for (uint64_t rest = cand; rest && kk;) {
+              const int L = __ffsll((long long)rest) - 1;
+              rest &= rest - 1;
+              const uint32_t vL = (uint32_t)__builtin_amdgcn_readlane((int)v, L);
+              if (vL < kk) {
+                acc |= 1ull << L;
+                kk--;
+              }
+            }
+          }
+          if ((acc >> lane) & 1ull) t.r[i + (uint32_t)__popcll(acc & below)] = v;
+          const uint32_t na = (uint32_t)__popcll(acc);
+          i += na;
+          k -= na;
+          if (k == 0) {
+            D = base + (g0 + u) * 64 + (uint64_t)(63 - __clzll((long long)acc)) + 1;
+            break;
+          }
+        }
+      }
+      if (threadIdx.x == 0) s_k[p & 1] = k;
+    }
+    __syncthreads();
+    if (s_k[p & 1] == 0) break;
+    if (base + 2 * kSmallDraws >= t.max_draws) {
+      if (threadIdx.x == 0) atomicOr(err, 64u);
+      return;
+    }
+  }
+  if (threadIdx.x == 0) *t.D = D;
+}
+
+struct FyTask {
+  const uint32_t *r;
+  int32_t *head, *next, *parent;
+  uint8_t *dirty;
+  int32_t *fwd, *inv;
+  uint32_t n;
+};
+
+__global__ void k_fyb_link(const FyTask *__restrict__ tasks, uint32_t *__restrict__ err) {
+  const FyTask t = tasks[blockIdx.y];
+  fy_link(t.n, t.r, t.head, t.next, t.dirty, err);
+}
+__global__ void k_fyb_sort(const FyTask *__restrict__ tasks, uint32_t *__restrict__ err) {
+  const FyTask t = tasks[blockIdx.y];
+  fy_sort(t.n, t.head, t.next, t.parent, t.dirty, err);
+}
+__global__ void k_fyb_walk(const FyTask *__restrict__ tasks) {
+  const FyTask t = tasks[blockIdx.y];
+  fy_walk(t.n, t.r, t.next, t.parent, t.fwd, t.inv);
+}
+
 }  // namespace
 
 void free_perm(ShardedPerm &p) {
@@ -1181,39 +1317,41 @@ void free_perm(ShardedPerm &p) {
   p.slab.reset();
 }
 
-// PermutationManager::reserve for 3PC (permutation_manager.h:102-181):
-// `count` new sharded permutations, each with one local permutation per pair
-// group, generated in queue order from each group's PRG
-// (HMShardedPermutationGenerator::generateBatch,
-// hm_sharded_permutation_generator.h:178-194).  The reference also generates a
-// permutation for the {0,1,2} group, which no operator on this path applies and
-// whose PRG feeds nothing else here, so it is not generated.
-void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
+namespace {
+
+// One context's share of a reserve: the permutations it still needs.
+struct Reserve {
+  orqg_ctx *c = nullptr;
+  uint32_t n = 0, count = 0;
+  std::vector<ShardedPerm> fresh;
+};
+
+// The queue bookkeeping of PermutationManager::reserve and the device arrays
+// of the new permutations; false when nothing is left to generate.
+bool reserve_begin(orqg_ctx *c, size_t n_, size_t count_, Reserve &R) {
   check_swaps(c);  // the previous call's swap resolution (its scratch is reused)
   if (!c->queue.empty() && c->stored_size != n_) {
     for (auto &p : c->queue) free_perm(p);
     c->queue.clear();
   }
   c->stored_size = n_;
-  if (count_ <= c->queue.size()) return;
+  if (count_ <= c->queue.size()) return false;
   count_ -= c->queue.size();
   if (n_ >= (1ull << 31)) throw Error(ORQG_E_ARG, "reserve: n must be < 2^31");
   const uint32_t n = (uint32_t)n_, count = (uint32_t)count_;
-  uint32_t bits = 0;
-  for (uint32_t t = n; t; t >>= 1) bits++;
-  const uint32_t maskN = 1u << bits;
-  const uint32_t mask = maskN - 1;
-
+  R.c = c;
+  R.n = n;
+  R.count = count;
   // the six arrays of every new permutation in one allocation (a served
   // stream of many sizes then costs one allocator entry per reserve, not six
   // per permutation); a consumer of the last of them waits for that one's
   // swap resolution, which is the last second-stream work on the slab
-  std::vector<ShardedPerm> fresh(count);
+  R.fresh.assign(count, ShardedPerm());
   const size_t stride = ((size_t)n * 4 + 16 + 255) / 256 * 256;
   void *slab_p = dev_alloc(c, stride * 6 * count);
   std::shared_ptr<void> slab(slab_p, [](void *q) { dev_free(q); });
   for (uint32_t k = 0; k < count; k++) {
-    ShardedPerm &p = fresh[k];
+    ShardedPerm &p = R.fresh[k];
     p.n = n;
     p.slab = slab;
     for (int g = 0; g < 3; g++) {
@@ -1224,9 +1362,25 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   if (n == 0) {
     // gen_perm still fills its buffer once: 4 nonces per local permutation
     for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] += 4ull * count;
-    for (auto &p : fresh) c->queue.push_back(p);
-    return;
+    for (auto &p : R.fresh) c->queue.push_back(p);
+    return false;
   }
+  return true;
+}
+
+void reserve_end(Reserve &R) {
+  for (auto &p : R.fresh) R.c->queue.push_back(p);
+  R.fresh.clear();
+}
+
+// The windowed pipeline over the whole GPU, for one context's reserve.
+void gen_windows(orqg_ctx *c, Reserve &R) {
+  const uint32_t n = R.n, count = R.count;
+  std::vector<ShardedPerm> &fresh = R.fresh;
+  uint32_t bits = 0;
+  for (uint32_t t = n; t; t >>= 1) bits++;
+  const uint32_t maskN = 1u << bits;
+  const uint32_t mask = maskN - 1;
 
   TimedRegion tr(c, "gen_perm");
   ChainBufs B;
@@ -1353,13 +1507,12 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
     for (uint32_t k = launched[g]; k < upto; k++) {
       const uint32_t *r = B.r_out + (size_t)g * chain_stride + (size_t)k * n;
       ORQG_HIP(hipMemsetAsync(head, 0xff, (size_t)n * 4, c->stream2));
-      hipLaunchKernelGGL(k_fy_link, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
-                         head, next, dirty, err);
-      hipLaunchKernelGGL(k_fy_sort, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, head, next,
-                         parent, dirty, err);
-      hipLaunchKernelGGL(k_fy_walk, dim3(fgrid, 1), dim3(256), 0, c->stream2, n, r, chain_stride,
-                         next, parent, fresh[k].fwd[g], fresh[k].fwd[g], fresh[k].fwd[g],
-                         fresh[k].inv[g], fresh[k].inv[g], fresh[k].inv[g]);
+      hipLaunchKernelGGL(k_fy_link, dim3(fgrid), dim3(256), 0, c->stream2, n, r, head, next, dirty,
+                         err);
+      hipLaunchKernelGGL(k_fy_sort, dim3(fgrid), dim3(256), 0, c->stream2, n, head, next, parent,
+                         dirty, err);
+      hipLaunchKernelGGL(k_fy_walk, dim3(fgrid), dim3(256), 0, c->stream2, n, r, next, parent,
+                         fresh[k].fwd[g], fresh[k].inv[g]);
     }
     if (upto > launched[g]) launched[g] = upto;
   };
@@ -1410,7 +1563,206 @@ void reserve_perms(orqg_ctx *c, size_t n_, size_t count_) {
   ORQG_HIP(hipGetLastError());
   c->fy_err = err;
   c->fy_pending = true;
-  for (auto &p : fresh) c->queue.push_back(p);
+}
+
+// Sizes up to this take the small path (ORQG_GP_SMALL_MAX overrides it; 0
+// turns the small path off).  At 2^16 rows a permutation draws about 1.5 * 2^20
+// words, so its nonce count (8) is still predicted right nearly always.
+uint32_t small_max() {
+  if (const char *e = std::getenv("ORQG_GP_SMALL_MAX")) return (uint32_t)std::max(0L, std::atol(e));
+  // the window pipeline's test hooks act on that path only
+  if (std::getenv("ORQG_GP_L2CAP") || std::getenv("ORQG_GP_BOUND_SHIFT")) return 0;
+  return 1u << 16;
+}
+
+// Nonces a permutation of n rows most likely consumes: 4 per refill of 2^20
+// draws, with the expected draw count maskN * H(n) (position i takes
+// maskN / (n - i) draws on average).
+uint64_t likely_nonces(uint32_t n, uint32_t maskN) {
+  double h = 0;
+  for (uint32_t k = n; k >= 1; k--) h += 1.0 / k;
+  const double draws = (double)maskN * h;
+  return 4ull * (1 + (uint64_t)(draws / (double)kRefillWords));
+}
+
+// Every local permutation of the reserves in `rs` (contexts of the same
+// device, n <= small_max()), drawn in one launch on lead's stream and resolved
+// in one launch sequence.  The contexts' streams are ordered before and after.
+void gen_small(orqg_ctx *lead, const std::vector<Reserve *> &rs) {
+  TimedRegion tr(lead, "gen_perm");
+  struct Chain {
+    Reserve *R;
+    int g;
+    size_t first;  // task index of its permutation 0
+    uint64_t q;    // predicted nonces per permutation
+  };
+  std::vector<Chain> chains;
+  std::vector<SmallTask> task;
+  std::vector<uint32_t *> rbuf(rs.size());
+  size_t ntask = 0;
+  for (Reserve *R : rs) ntask += 3ull * R->count;
+  uint64_t *dD = (uint64_t *)lead->scratch_get("gps_D", ntask * 8 + 64);
+  uint32_t *derr = (uint32_t *)lead->scratch_get("gps_err", 16);
+  for (size_t q = 0; q < rs.size(); q++) {
+    Reserve &R = *rs[q];
+    uint32_t bits = 0;
+    for (uint32_t t = R.n; t; t >>= 1) bits++;
+    const uint32_t maskN = 1u << bits;
+    uint64_t per = likely_nonces(R.n, maskN);
+    // test hook: a fixed (wrong) prediction forces the redraws below
+    if (const char *e = std::getenv("ORQG_GP_SMALL_PREDICT")) per = (uint64_t)std::max(0L, std::atol(e));
+    rbuf[q] = (uint32_t *)R.c->scratch_get("gp_r", 3ull * R.count * R.n * 4 + 64);
+    for (int g = 0; g < 3; g++) {
+      chains.push_back({&R, g, task.size(), per});
+      for (uint32_t k = 0; k < R.count; k++) {
+        SmallTask t;
+        t.rk = R.c->rk(kGroup0 + g);
+        t.nonce = R.c->nonce[kGroup0 + g] + per * k;
+        t.r = rbuf[q] + ((size_t)g * R.count + k) * R.n;
+        t.D = dD + task.size();
+        // far beyond any real draw count (expected maskN * H(n))
+        t.max_draws = 1024ull * maskN * (bits + 1);
+        t.n = R.n;
+        t.mask = maskN - 1;
+        task.push_back(t);
+      }
+    }
+  }
+  // every context's earlier work (its queue, its scratch) precedes the draws
+  for (Reserve *R : rs)
+    if (R->c != lead) {
+      ORQG_HIP(hipEventRecord(R->c->ev_aux, R->c->stream));
+      ORQG_HIP(hipStreamWaitEvent(lead->stream, R->c->ev_aux, 0));
+    }
+  SmallTask *dtask = (SmallTask *)lead->scratch_get("gps_tasks", ntask * sizeof(SmallTask) + 64);
+  ORQG_HIP(hipMemsetAsync(derr, 0, 4, lead->stream));
+  std::vector<uint64_t> D(ntask, 0);
+  std::vector<SmallTask> todo = task;
+  for (int round = 0; !todo.empty(); round++) {
+    if (round > (int)ntask + 1) throw Error(ORQG_E_PARITY, "gen_perm: start nonces did not settle");
+    ORQG_HIP(hipMemcpyAsync(dtask, todo.data(), todo.size() * sizeof(SmallTask),
+                            hipMemcpyHostToDevice, lead->stream));
+    {
+      TimedRegion tk(lead, "k_gp_small");
+      hipLaunchKernelGGL(k_gp_small<kCopies>, dim3((unsigned)todo.size()), dim3(kGpThreads), 0,
+                         lead->stream, lead->d_t0, dtask, derr);
+    }
+    ORQG_HIP(hipGetLastError());
+    uint32_t herr = 0;
+    ORQG_HIP(hipMemcpyAsync(D.data(), dD, ntask * 8, hipMemcpyDeviceToHost, lead->stream));
+    ORQG_HIP(hipMemcpyAsync(&herr, derr, 4, hipMemcpyDeviceToHost, lead->stream));
+    ORQG_HIP(hipStreamSynchronize(lead->stream));
+    if (herr) throw Error(ORQG_E_PARITY, "gen_perm: draw limit exceeded");
+    // exact start nonces from the draw counts; a chain whose permutation k
+    // started elsewhere is drawn again from k on
+    todo.clear();
+    for (const Chain &ch : chains) {
+      uint64_t e = ch.R->c->nonce[kGroup0 + ch.g];
+      for (uint32_t k = 0; k < ch.R->count; k++) {
+        SmallTask &t = task[ch.first + k];
+        if (t.nonce != e) {
+          for (uint32_t k2 = k; k2 < ch.R->count; k2++) {
+            task[ch.first + k2].nonce = e + ch.q * (k2 - k);
+            todo.push_back(task[ch.first + k2]);
+          }
+          break;
+        }
+        e += 4ull * (1 + D[ch.first + k] / kRefillWords);
+      }
+    }
+  }
+  for (const Chain &ch : chains) {
+    const SmallTask &last = task[ch.first + ch.R->count - 1];
+    ch.R->c->nonce[kGroup0 + ch.g] = last.nonce + 4ull * (1 + D[ch.first + ch.R->count - 1] / kRefillWords);
+  }
+  // swap resolution of all of them
+  size_t tot = 0;
+  uint32_t nmax = 0;
+  for (Reserve *R : rs) {
+    tot += 3ull * R->count * R->n;
+    nmax = std::max(nmax, R->n);
+  }
+  int32_t *head = (int32_t *)lead->scratch_get("gps_head", tot * 4 + 64);
+  int32_t *next = (int32_t *)lead->scratch_get("gps_next", tot * 4 + 64);
+  int32_t *parent = (int32_t *)lead->scratch_get("gps_parent", tot * 4 + 64);
+  uint8_t *dirty = (uint8_t *)lead->scratch_get("gps_dirty", tot + 64);
+  std::vector<FyTask> fy;
+  fy.reserve(ntask);
+  size_t off = 0;
+  for (const Chain &ch : chains)
+    for (uint32_t k = 0; k < ch.R->count; k++) {
+      FyTask f;
+      f.r = task[ch.first + k].r;
+      f.head = head + off;
+      f.next = next + off;
+      f.parent = parent + off;
+      f.dirty = dirty + off;
+      f.fwd = ch.R->fresh[k].fwd[ch.g];
+      f.inv = ch.R->fresh[k].inv[ch.g];
+      f.n = ch.R->n;
+      fy.push_back(f);
+      off += ch.R->n;
+    }
+  FyTask *dfy = (FyTask *)lead->scratch_get("gps_fy", fy.size() * sizeof(FyTask) + 64);
+  ORQG_HIP(hipMemcpyAsync(dfy, fy.data(), fy.size() * sizeof(FyTask), hipMemcpyHostToDevice,
+                          lead->stream));
+  ORQG_HIP(hipMemsetAsync(head, 0xff, tot * 4, lead->stream));
+  ORQG_HIP(hipMemsetAsync(dirty, 0, tot, lead->stream));
+  const unsigned gx = grid_for(nmax, 256, 1u << 16);
+  for (size_t t0 = 0; t0 < fy.size(); t0 += 65535) {
+    const unsigned ny = (unsigned)std::min<size_t>(65535, fy.size() - t0);
+    hipLaunchKernelGGL(k_fyb_link, dim3(gx, ny), dim3(256), 0, lead->stream, dfy + t0, derr);
+    hipLaunchKernelGGL(k_fyb_sort, dim3(gx, ny), dim3(256), 0, lead->stream, dfy + t0, derr);
+    hipLaunchKernelGGL(k_fyb_walk, dim3(gx, ny), dim3(256), 0, lead->stream, dfy + t0);
+  }
+  ORQG_HIP(hipGetLastError());
+  // (the wait also keeps the host task tables alive until they are copied)
+  uint32_t herr = 0;
+  ORQG_HIP(hipMemcpyAsync(&herr, derr, 4, hipMemcpyDeviceToHost, lead->stream));
+  ORQG_HIP(hipStreamSynchronize(lead->stream));
+  if (herr) throw Error(ORQG_E_PARITY, "gen_perm: swap resolution check failed");
+}
+
+}  // namespace
+
+// PermutationManager::reserve for 3PC (permutation_manager.h:102-181):
+// `count` new sharded permutations, each with one local permutation per pair
+// group, generated in queue order from each group's PRG
+// (HMShardedPermutationGenerator::generateBatch,
+// hm_sharded_permutation_generator.h:178-194).  The reference also generates a
+// permutation for the {0,1,2} group, which no operator on this path applies and
+// whose PRG feeds nothing else here, so it is not generated.
+void reserve_perms(orqg_ctx *c, size_t n, size_t count) {
+  Reserve R;
+  if (!reserve_begin(c, n, count, R)) return;
+  if (R.n <= small_max())
+    gen_small(c, {&R});
+  else
+    gen_windows(c, R);
+  reserve_end(R);
+}
+
+// reserve_perms of several contexts (distinct, one device): the small ones
+// share one launch sequence on the first one's stream.
+void reserve_perms_multi(const std::vector<orqg_ctx *> &cs, const std::vector<size_t> &ns,
+                         const std::vector<size_t> &counts) {
+  std::vector<Reserve> R(cs.size());
+  std::vector<Reserve *> small;
+  for (size_t q = 0; q < cs.size(); q++) {
+    if (!reserve_begin(cs[q], ns[q], counts[q], R[q])) continue;
+    if (R[q].n <= small_max()) {
+      small.push_back(&R[q]);
+    } else {
+      gen_windows(cs[q], R[q]);
+      reserve_end(R[q]);
+    }
+  }
+  if (small.empty()) return;
+  gen_small(small[0]->c, small);
+  for (Reserve *r : small) {
+    // the lead's stream finished the generation (gen_small synchronised it)
+    reserve_end(*r);
+  }
 }
 
 void perm_wait(orqg_ctx *c, ShardedPerm &p) {

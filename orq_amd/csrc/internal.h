@@ -221,6 +221,10 @@ void mult(orqg_ctx *c, bool arith, const orqg_vec *x, const orqg_vec *y, orqg_ve
 void b2a_bit(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);
 void reshare(orqg_ctx *c, orqg_vec *v, int group);
 void reserve_perms(orqg_ctx *c, size_t n, size_t count);
+// reserve_perms of several distinct contexts on one device; the small sizes
+// are generated together
+void reserve_perms_multi(const std::vector<orqg_ctx *> &cs, const std::vector<size_t> &ns,
+                         const std::vector<size_t> &counts);
 ShardedPerm next_perm(orqg_ctx *c, size_t n);
 void free_perm(ShardedPerm &p);
 // the main stream waits (on the device) until p's swap resolution is done

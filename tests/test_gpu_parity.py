@@ -133,8 +133,13 @@ def test_reshare(O, G, keys, width):
     _same_nonces(oc, gc)
 
 
+@pytest.mark.parametrize("path", ["small", "windows"])
 @pytest.mark.parametrize("n", [1, 2, 3, 100, 1000, 4099])
-def test_gen_perm_matches_oracle(O, G, keys, n):
+def test_gen_perm_matches_oracle(O, G, keys, monkeypatch, n, path):
+    """Both generators: one workgroup per local permutation (the default at
+    these sizes) and the windowed pipeline (forced by a small-path limit of 0)."""
+    if path == "windows":
+        monkeypatch.setenv("ORQG_GP_SMALL_MAX", "0")
     oc, gc = _pair(O, G, keys)
     count = 3
     oc.reserve(n, count)
@@ -147,6 +152,28 @@ def test_gen_perm_matches_oracle(O, G, keys, n):
             got = gc.peek_perm(k, g, n)
             np.testing.assert_array_equal(got, want, err_msg=f"perm {k} group {g}")
             assert np.array_equal(np.sort(got), np.arange(n))
+    _same_nonces(oc, gc)
+
+
+@pytest.mark.parametrize("n,count,predict", [
+    ((1 << 16) + 1, 3, 4),    # every permutation takes 8 nonces, 4 predicted
+    (1 << 17, 4, None),       # about 3.1 refills each: the prediction misses often
+    (5000, 3, 12),            # 4 each, 12 predicted
+])
+def test_gen_perm_small_path_renonce(O, G, keys, monkeypatch, n, count, predict):
+    """The small path when the predicted start nonces are wrong for some
+    permutations: those are drawn again from the exact ones."""
+    monkeypatch.setenv("ORQG_GP_SMALL_MAX", str(1 << 17))
+    if predict is not None:
+        monkeypatch.setenv("ORQG_GP_SMALL_PREDICT", str(predict))
+    oc, gc = _pair(O, G, keys)
+    oc.reserve(n, count)
+    gc.reserve(n, count)
+    for k in range(count):
+        for g in range(3):
+            members = [p for p in range(3) if p != 2 - g]
+            np.testing.assert_array_equal(gc.peek_perm(k, g, n), oc.peek_perm(k, members[0], g, n),
+                                          err_msg=f"perm {k} group {g}")
     _same_nonces(oc, gc)
 
 
