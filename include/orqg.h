@@ -296,6 +296,16 @@ int orqg_quicksort(orqg_ctx *ctx, orqg_vec *v, int order, orqg_vec *perm_out);
  * B-shared vector v in place on its low `bits` bits (bits == 8*width sorts
  * signed), returns the A-shared sorting permutation in perm_out (width 4). */
 int orqg_radix_sort(orqg_ctx *ctx, orqg_vec *v, int order, int bits, orqg_vec *perm_out);
+/* radix_sort of m independent requests at once (a served batch).  Request q
+ * sorts vs[q] on its own context ctxs[q] (all distinct, one device) with
+ * orders[q] and bits[q], into perms_out[q], and gets exactly what
+ * orqg_radix_sort on that context gives: the same shares, nonces,
+ * permutation queue and message log.  The permutations of the small requests
+ * are generated in shared launches, and the bit iterations
+ * (radixsort.h:141-192) of all requests run in shared, segmented launches,
+ * whose shapes are tuned online.  The first context carries the error text. */
+int orqg_radix_sort_batch(orqg_ctx *const *ctxs, orqg_vec *const *vs, const int *orders,
+                          const int *bits, orqg_vec *const *perms_out, size_t m);
 
 /* ---- share files (the reference's CSV formats) ------------------------ */
 /* Each call covers the three parties' files: party_files[p] holds party p's

@@ -60,7 +60,8 @@ EXPORTS = [
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
     "orqg_server_destroy",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
-    "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_msglog_count", "orqg_msglog_get",
+    "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_radix_sort_batch",
+    "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
 ]
 
@@ -153,6 +154,9 @@ def lib():
             "orqg_apply_sharded_perm": (i32, [vp, vp, i32]),
             "orqg_apply_elementwise_perm": (i32, [vp, vp, vp]),
             "orqg_radix_sort": (i32, [vp, vp, i32, i32, vp]),
+            "orqg_radix_sort_batch": (i32, [ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                            ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                            ctypes.POINTER(vp), sz]),
             "orqg_msglog_count": (sz, [vp]),
             "orqg_msglog_get": (i32, [vp, sz, ctypes.POINTER(_Msg)]),
             "orqg_msglog_clear": (i32, [vp]),
@@ -589,3 +593,26 @@ class Ctx:
 
     def timer_reset(self):
         lib().orqg_timer_reset(self.h)
+
+
+def radix_sort_batch(vecs, bits=None, desc=None):
+    """radix_sort of several requests at once, each vector on its own context
+    (orqg_radix_sort_batch): vecs[q] is sorted in place on its context, as
+    Ctx.radix_sort would, and the A-shared sorting permutations are returned
+    in order.  bits / desc: per request (default: full width, ascending)."""
+    m = len(vecs)
+    if bits is None:
+        bits = [8 * v.width for v in vecs]
+    if desc is None:
+        desc = [False] * m
+    perms = [Vec(v.ctx, v.n, 4, ORQG_ASHARED) for v in vecs]
+    vp = ctypes.c_void_p
+    ctxs = (vp * m)(*[v.ctx.h for v in vecs])
+    hv = (vp * m)(*[v.h for v in vecs])
+    hp = (vp * m)(*[p.h for p in perms])
+    orders = (ctypes.c_int * m)(*[DESC if d else ASC for d in desc])
+    nb = (ctypes.c_int * m)(*bits)
+    rc = lib().orqg_radix_sort_batch(ctxs, hv, orders, nb, hp, m)
+    if rc != 0:
+        vecs[0].ctx._check(rc, "radix_sort_batch")
+    return perms

@@ -2,6 +2,9 @@
 // context / vector / message-log plumbing behind them.
 #include <cstring>
 #include <mutex>
+#include <set>
+#include <string>
+#include <vector>
 #include <unordered_map>
 #include <unordered_set>
 
@@ -926,6 +929,34 @@ int orqg_radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *per
   radix_sort(c, v, order, bits, perm_out);
   perm_out->enc = ORQG_ASHARED;
   API_END(c)
+}
+
+int orqg_radix_sort_batch(orqg_ctx *const *ctxs, orqg_vec *const *vs, const int *orders,
+                          const int *bits, orqg_vec *const *perms_out, size_t m) {
+  orqg_ctx *c0 = (ctxs && m) ? ctxs[0] : nullptr;
+  API_BEGIN(c0)
+  if (m == 0) return ORQG_OK;
+  if (!ctxs || !vs || !orders || !bits || !perms_out)
+    throw Error(ORQG_E_ARG, "radix_sort_batch: NULL array");
+  std::vector<orqg_ctx *> cs(ctxs, ctxs + m);
+  std::vector<orqg_vec *> v(vs, vs + m), p(perms_out, perms_out + m);
+  std::vector<int> o(orders, orders + m), b(bits, bits + m);
+  std::set<orqg_ctx *> seen;
+  for (size_t q = 0; q < m; q++) {
+    const std::string at = "radix_sort_batch: request " + std::to_string(q) + ": ";
+    if (!cs[q] || !v[q] || !p[q]) throw Error(ORQG_E_ARG, at + "NULL context or vector");
+    if (!seen.insert(cs[q]).second) throw Error(ORQG_E_ARG, at + "a context appears twice");
+    if (cs[q]->device != cs[0]->device) throw Error(ORQG_E_ARG, at + "contexts on different devices");
+    if (v[q]->ctx != cs[q] || p[q]->ctx != cs[q])
+      throw Error(ORQG_E_ARG, at + "vectors must belong to the request's context");
+    if (p[q]->n != v[q]->n || p[q]->width != 4)
+      throw Error(ORQG_E_ARG, at + "perm_out must be a width-4 vector of the same size");
+    if (b[q] < 1 || b[q] > 8 * v[q]->width) throw Error(ORQG_E_ARG, at + "bits");
+    if (v[q]->n >= (1ull << 31)) throw Error(ORQG_E_ARG, at + "at most 2^31-1 rows");
+  }
+  radix_sort_batch(cs, v, o, b, p);
+  for (size_t q = 0; q < m; q++) p[q]->enc = ORQG_ASHARED;
+  API_END(c0)
 }
 
 size_t orqg_msglog_count(orqg_ctx *c) { return c ? c->log.size() : 0; }

@@ -235,6 +235,9 @@ void check_swaps(orqg_ctx *c);
 void apply_sharded(orqg_ctx *c, orqg_vec *x, const ShardedPerm &pi, bool inverse);
 void apply_elementwise(orqg_ctx *c, orqg_vec *x, const orqg_vec *perm);
 void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out);
+void radix_sort_batch(const std::vector<orqg_ctx *> &cs, const std::vector<orqg_vec *> &vs,
+                      const std::vector<int> &orders, const std::vector<int> &bits,
+                      const std::vector<orqg_vec *> &perms);
 void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, orqg_vec *gt,
              bool full, int neg_eq, int neg_gt);
 void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);

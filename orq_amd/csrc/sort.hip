@@ -30,6 +30,10 @@
 
 #include <cstring>
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "internal.h"
 #include "kern_common.h"
@@ -211,9 +215,10 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a(const uint32_t *__restrict__ t0
   }
 }
 
-// exclusive scan of [nb][3] block sums in place; totals to tot[3]
-__global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restrict__ bsum,
-                                                uint32_t *__restrict__ tot) {
+// exclusive scan of [nb][3] block sums in place; totals to tot[3] (one
+// workgroup of 1024 threads)
+__device__ inline void scan3_body(uint32_t nb, uint32_t *__restrict__ bsum,
+                                  uint32_t *__restrict__ tot) {
   __shared__ uint32_t s_w[1024 / 64];
   __shared__ uint32_t s_carry[3];
   if (threadIdx.x < 3) s_carry[threadIdx.x] = 0;
@@ -233,6 +238,11 @@ __global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restric
     }
   }
   if (threadIdx.x < 3) tot[threadIdx.x] = s_carry[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restrict__ bsum,
+                                                uint32_t *__restrict__ tot) {
+  scan3_body(nb, bsum, tot);
 }
 
 // s0_k(e) = one_k*e - F_k(e), s1_k(e) = one_k*(n-1) - Ftot_k + F_k(e) with F the
@@ -530,8 +540,143 @@ __global__ void k_unpad(uint64_t n, bool desc, const uint32_t *__restrict__ P, K
   }
 }
 
-// open a [n][3] uint32 vector (XOR or sum) and write it as the public sharing
-// (o, 0, 0) into out (open_shares_* then public_share)
+// ---- segmented bit kernels: one bit iteration of many sorts per launch -------------
+// Each sort (a "segment": its own context, size, buffers, keys, nonces and
+// permutation) is described by a SegBit per bit.  The superblocks (k_bit_b2a,
+// k_bit_perm) and wave steps (rounds) of all segments are numbered flat; a
+// workgroup (wave, for the rounds) finds its segment by binary search over
+// the segments' first flat indices and runs the unsegmented kernel's body on
+// it.  Which workgroup takes which item changes nothing in the results, so
+// the launch shapes are free to tune.
+struct SegBit {
+  const uint32_t *rk_pair;    // pair keys (3 x 60 words)
+  const uint32_t *rk_grp[3];  // group keys
+  uint32_t *cur, *other;      // record buffers: the bit reads cur, its scatter writes other
+  uint32_t *F, *bsum, *tot, *err;
+  const int32_t *idx[3];      // pi_g^{-1}
+  NonceTriple nb, nb2;        // b2a_bit, multiply_a
+  CallSched s_b2a, s_perm, sv, sp;
+  uint64_t nv[3], np[3];      // round g: v masks, perm masks
+  uint64_t st0;               // first flat wave step
+  uint32_t sb0, nsb;          // first flat superblock, superblocks
+  uint32_t flip;
+};
+
+// The descriptors are read through the constant address space (they do not
+// change during a launch), so that a wave-uniform index gives scalar loads and
+// the fields live in scalar registers, as kernel arguments do.
+typedef const __attribute__((address_space(4))) SegBit *SegPtr;
+
+__device__ inline SegPtr seg_ptr(const SegBit *segs) { return (SegPtr)segs; }
+
+// field-wise copies out of the constant address space
+__device__ inline CallSched ld_sched(const __attribute__((address_space(4))) CallSched &x) {
+  CallSched r;
+  r.n = x.n;
+  r.bs = x.bs;
+  r.cfull = x.cfull;
+  r.tsz = x.tsz;
+  r.calls = x.calls;
+  return r;
+}
+__device__ inline NonceTriple ld_nonces(const __attribute__((address_space(4))) NonceTriple &x) {
+  NonceTriple r;
+  r.v[0] = x.v[0];
+  r.v[1] = x.v[1];
+  r.v[2] = x.v[2];
+  return r;
+}
+
+__device__ inline uint32_t seg_of_sb(SegPtr segs, uint32_t nseg, uint32_t f) {
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].sb0 <= f) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint32_t seg_of_step(SegPtr segs, uint32_t nseg, uint64_t w) {
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].st0 <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int L>
+__global__ void __launch_bounds__(kBT) k_bit_b2a_seg(const uint32_t *__restrict__ t0,
+                                                     const SegBit *__restrict__ segs, uint32_t nseg,
+                                                     uint32_t total, int word, int bit) {
+  AES_LDS;
+  __shared__ uint32_t s_w[3 * (kBT / 64)];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const SegPtr sp = seg_ptr(segs);
+  for (uint32_t f = blockIdx.x; f < total; f += gridDim.x) {
+    const SegPtr S = sp + seg_of_sb(sp, nseg, f);
+    const CallSched sch = ld_sched(S->s_b2a);
+    const NonceTriple nb = ld_nonces(S->nb);
+    const uint32_t sb = f - S->sb0;
+    const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
+    uint32_t raw[4][3], bits[4][3];
+    bit_load_raw<L>(S->cur, sch.n, word, e0, raw);
+    bit_extract(raw, e0, sch.n, bit, S->flip, bits);
+    bit_b2a_block(A, S->rk_pair, nb, sch, e0, bits, S->F, S->bsum + sb * 3, s_w);
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_scan3_seg(const SegBit *__restrict__ segs) {
+  const SegPtr S = seg_ptr(segs) + blockIdx.x;
+  scan3_body(S->nsb, S->bsum, S->tot);
+}
+
+template <int L>
+__global__ void __launch_bounds__(kBT) k_bit_perm_seg(const uint32_t *__restrict__ t0,
+                                                      const SegBit *__restrict__ segs, uint32_t nseg,
+                                                      uint32_t total) {
+  AES_LDS;
+  __shared__ uint32_t s_w[3 * (kBT / 64)];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const SegPtr sp = seg_ptr(segs);
+  for (uint32_t f = blockIdx.x; f < total; f += gridDim.x) {
+    const SegPtr S = sp + seg_of_sb(sp, nseg, f);
+    const CallSched sch = ld_sched(S->s_perm);
+    const NonceTriple nb = ld_nonces(S->nb2);
+    const uint32_t sb = f - S->sb0;
+    bit_perm_block(A, S->rk_pair, nb, sch, (uint64_t)sb * kEPB + threadIdx.x * 4, S->F,
+                   S->bsum + sb * 3, S->tot, S->cur, s_w);
+  }
+}
+
+// round g: 0 reads cur and writes other, 1 the reverse, 2 (scatter) cur -> other
+template <int L, bool SCATTER>
+__global__ void __launch_bounds__(kBT) k_round2_seg(const uint32_t *__restrict__ t0,
+                                                    const SegBit *__restrict__ segs, uint32_t nseg,
+                                                    uint64_t total, int g) {
+  AES_LDS;
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const SegPtr segp = seg_ptr(segs);
+  const uint64_t wstride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  for (uint64_t w = w0; w < total; w += wstride) {
+    // w is the same across the wave (total < 2^32 steps): the search and the
+    // segment's fields then use scalar loads
+    const uint32_t wu = __builtin_amdgcn_readfirstlane((uint32_t)w);
+    const SegPtr S = segp + seg_of_step(segp, nseg, wu);
+    const CallSched sv = ld_sched(S->sv), sp = ld_sched(S->sp);
+    const uint32_t *rin = g == 1 ? S->other : S->cur;
+    uint32_t *rout = g == 1 ? S->cur : S->other;
+    round2_step<L, SCATTER>(A, S->rk_grp[g], S->nv[g], sv, S->np[g], sp, S->idx[g], rin, rout,
+                            S->err, wu - S->st0);
+  }
+}
+
 // Returns the record buffer that holds the sorted vector (P or the second one).
 template <int L>
 uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_width) {
@@ -615,29 +760,15 @@ uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_wi
 
 }  // namespace
 
-void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out) {
+namespace {
+
+// remove_padding (into v and perm_out) and the permutation's b2a, negate
+// (DESC) and invert, on c's stream; P holds the sorted records
+void sort_finish(orqg_ctx *c, orqg_vec *v, uint32_t *P, bool desc, orqg_vec *perm_out) {
   const uint64_t n = v->n;
-  const bool desc = order == ORQG_DESC;
-  const bool full_width = bits == 8 * v->width;
-  if (n == 0) return;
-  // 1 permutation + (bits + 1) pairs; pairs are single permutations in 3PC
-  // (radixsort.h:121-128, permutation_manager.h:108-111)
-  reserve_perms(c, n, (size_t)bits + 2);
-  const int L = v->width == 8 ? 3 : 2;
-  uint32_t *P = (uint32_t *)c->scratch_get("rs_P", n * kRec * 4 + 64);
   const unsigned grid_flat = grid_for(n, 256, c->num_cu * 16);
-  if (L == 3) {
-    hipLaunchKernelGGL((k_pad<3, uint64_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc,
-                       (const uint64_t *)v->d, P);
-    P = sort_body<3>(c, P, n, bits, full_width);
-  } else {
-    hipLaunchKernelGGL((k_pad<2, uint32_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc,
-                       (const uint32_t *)v->d, P);
-    P = sort_body<2>(c, P, n, bits, full_width);
-  }
-  // remove_padding
   uint32_t *pb = (uint32_t *)perm_out->d;
-  if (L == 3)
+  if (v->width == 8)
     hipLaunchKernelGGL((k_unpad<3, uint64_t>), dim3(grid_flat), dim3(256), 0, c->stream, n, desc, P,
                        (uint64_t *)v->d, pb);
   else
@@ -652,7 +783,389 @@ void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_ou
   perm_b2a(c, perm_out);
   if (desc) perm_negate(c, perm_out);
   perm_invert(c, perm_out);
+}
+
+void pad_records(orqg_ctx *c, hipStream_t st, const orqg_vec *v, bool desc, uint32_t *P) {
+  const uint64_t n = v->n;
+  const unsigned grid_flat = grid_for(n, 256, c->num_cu * 16);
+  if (v->width == 8)
+    hipLaunchKernelGGL((k_pad<3, uint64_t>), dim3(grid_flat), dim3(256), 0, st, n, desc,
+                       (const uint64_t *)v->d, P);
+  else
+    hipLaunchKernelGGL((k_pad<2, uint32_t>), dim3(grid_flat), dim3(256), 0, st, n, desc,
+                       (const uint32_t *)v->d, P);
+  ORQG_HIP(hipGetLastError());
+}
+
+// ---- launch shapes of the segmented kernels, tuned online ------------------------
+// The workgroups of a launch are a free choice (results do not depend on it).
+// For each kernel kind, record layout and size class (log2 of the flat items)
+// the candidates are tried on live bit iterations -- one launch each, timed
+// with events read after the call's final synchronisation -- and the fastest
+// is kept for the rest of the process.  ORQG_TUNE=0 keeps the defaults.
+enum TuneKind { kTuneB2a = 0, kTunePerm = 1, kTuneRound = 2 };
+
+struct Tuner {
+  std::mutex mu;
+  std::map<uint64_t, unsigned> best;
+  std::map<uint64_t, std::vector<unsigned>> cands;
+  std::map<uint64_t, std::map<unsigned, std::pair<float, int>>> tried;  // (fastest ms, samples)
+};
+Tuner &tuner() {
+  static Tuner t;
+  return t;
+}
+bool tune_on() {
+  const char *e = std::getenv("ORQG_TUNE");
+  return !(e && e[0] == '0');
+}
+uint64_t tune_key(int kind, int L, uint64_t items) {
+  uint64_t b = 0;
+  while ((1ull << b) < items) b++;
+  return ((uint64_t)kind << 48) | ((uint64_t)L << 40) | b;
+}
+// candidate workgroup counts for `units` workgroup-sized pieces of work
+std::vector<unsigned> grid_candidates(uint64_t units, int num_cu) {
+  std::vector<unsigned> out;
+  for (unsigned g : {(unsigned)num_cu / 4, (unsigned)num_cu / 2, (unsigned)num_cu,
+                     2u * (unsigned)num_cu}) {
+    const unsigned x = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, units));
+    if (std::find(out.begin(), out.end(), x) == out.end()) out.push_back(x);
+  }
+  return out;
+}
+constexpr int kTuneSamples = 2;  // timed launches per candidate (the fastest counts)
+
+struct TuneTrial {
+  uint64_t key;
+  unsigned grid;
+  hipEvent_t a, b;
+};
+
+// the launch shape for this launch: settled, a candidate still short of
+// samples (then timed), or the default (the first candidate >= min(units,
+// num_cu), the unsegmented kernels' choice)
+unsigned tune_pick(orqg_ctx *c, int kind, int L, uint64_t units, std::vector<TuneTrial> &trials,
+                   bool &timed) {
+  const std::vector<unsigned> cand = grid_candidates(units, c->num_cu);
+  unsigned dflt = cand.back();
+  for (unsigned g : cand)
+    if (g >= (unsigned)std::min<uint64_t>(units, (uint64_t)c->num_cu)) {
+      dflt = g;
+      break;
+    }
+  timed = false;
+  if (!tune_on() || cand.size() == 1) return dflt;
+  const uint64_t key = tune_key(kind, L, units);
+  Tuner &T = tuner();
+  std::lock_guard<std::mutex> lk(T.mu);
+  auto it = T.best.find(key);
+  if (it != T.best.end()) return std::min<unsigned>(it->second, (unsigned)units);
+  // a size class's candidates are those of the first size seen in it
+  auto ci = T.cands.find(key);
+  if (ci == T.cands.end()) ci = T.cands.emplace(key, cand).first;
+  auto &done = T.tried[key];
+  for (unsigned g : ci->second) {
+    int samples = 0;
+    auto d = done.find(g);
+    if (d != done.end()) samples = d->second.second;
+    for (const TuneTrial &t : trials) samples += t.key == key && t.grid == g;
+    if (samples < kTuneSamples) {
+      timed = true;
+      return std::min<unsigned>(g, (unsigned)units);
+    }
+  }
+  return dflt;
+}
+
+// fold the call's timed launches into the tuner (after the final
+// synchronisation); a size class whose candidates all have their samples
+// settles on the fastest
+void tune_settle(std::vector<TuneTrial> &trials) {
+  Tuner &T = tuner();
+  std::lock_guard<std::mutex> lk(T.mu);
+  for (TuneTrial &t : trials) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      auto &e = T.tried[t.key][t.grid];
+      e.first = e.second ? std::min(e.first, ms) : ms;
+      e.second++;
+    }
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (TuneTrial &t : trials) {
+    if (T.best.count(t.key)) continue;
+    const auto &cand = T.cands[t.key];
+    const auto &done = T.tried[t.key];
+    unsigned pick = 0;
+    float fastest = 0;
+    bool complete = true;
+    for (unsigned g : cand) {
+      auto d = done.find(g);
+      if (d == done.end() || d->second.second < kTuneSamples) {
+        complete = false;
+        break;
+      }
+      if (!pick || d->second.first < fastest) {
+        pick = g;
+        fastest = d->second.first;
+      }
+    }
+    if (complete) T.best[t.key] = pick;
+  }
+  trials.clear();
+}
+
+struct BatchReq {
+  orqg_ctx *c;
+  orqg_vec *v, *perm;
+  bool desc, full_width;
+  int bits;
+  uint32_t *P, *R1, *cur, *other, *F, *bsum, *tot, *err;
+};
+
+// The bit iterations of the requests in `rq` (all with record layout L), on
+// lead's stream; returns after every request's tail ran on its own stream.
+template <int L>
+void batch_body(orqg_ctx *lead, std::vector<BatchReq> &rq) {
+  const uint32_t tszv = L == 3 ? 16 : 8;  // PadWidth element size
+  int maxbits = 0;
+  for (BatchReq &r : rq) maxbits = std::max(maxbits, r.bits);
+  // descriptors of every bit, built with radix_sort's own nonce, queue and
+  // message-log bookkeeping per request
+  std::vector<SegBit> all;
+  std::vector<size_t> first((size_t)maxbits + 1);
+  std::vector<uint32_t> tot_sb(maxbits);
+  std::vector<uint64_t> tot_st(maxbits);
+  std::vector<ShardedPerm> used;
+  auto release = [&]() {
+    for (ShardedPerm &p : used) free_perm(p);
+    used.clear();
+  };
+  try {
+    for (int i = 0; i < maxbits; i++) {
+      first[i] = all.size();
+      uint32_t sb = 0;
+      uint64_t st = 0;
+      for (BatchReq &r : rq) {
+        if (i >= r.bits) continue;
+        orqg_ctx *c = r.c;
+        const uint64_t n = r.v->n;
+        SegBit S;
+        std::memset(&S, 0, sizeof(S));
+        S.rk_pair = c->rk(kPair0);
+        for (int g = 0; g < 3; g++) S.rk_grp[g] = c->rk(kGroup0 + g);
+        S.cur = r.cur;
+        S.other = r.other;
+        S.F = r.F;
+        S.bsum = r.bsum;
+        S.tot = r.tot;
+        S.err = r.err;
+        S.flip = (r.full_width && i == r.bits - 1) ? 1u : 0u;  // inplace_invert on the MSB
+        for (int k = 0; k < 3; k++) S.nb.v[k] = c->nonce[kPair0 + k];
+        S.s_b2a = make_sched(n, c->batch_size, 4, 3);
+        uint64_t u = sched_nonces(S.s_b2a);
+        for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += u;
+        for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * n * 4);
+        for (int k = 0; k < 3; k++) S.nb2.v[k] = c->nonce[kPair0 + k];
+        S.s_perm = make_sched(n, c->batch_size, 4, 1);
+        u = sched_nonces(S.s_perm);
+        for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += u;
+        for (int p = 0; p < 3; p++) c->log_msg("mul_a", p, (p + 2) % 3, n * 4);
+        // oblivious_apply_elementwise_perm(v, perm): pi from the queue
+        if (c->queue.empty() || c->stored_size != n)
+          throw Error(ORQG_E_PARITY, "radix_sort_batch: permutation queue out of step");
+        ShardedPerm pi = c->queue.front();
+        c->queue.pop_front();
+        if (pi.ready) ORQG_HIP(hipStreamWaitEvent(lead->stream, pi.ready, 0));
+        used.push_back(pi);
+        for (int g = 0; g < 3; g++) S.idx[g] = pi.inv[g];
+        S.sv = make_sched(n, c->batch_size, tszv, 2);
+        S.sp = make_sched(n, c->batch_size, 4, 2);
+        const uint64_t uv = sched_nonces(S.sv), up = sched_nonces(S.sp);
+        for (int g = 0; g < 3; g++) {
+          S.nv[g] = c->nonce[kGroup0 + g];
+          S.np[g] = S.nv[g] + uv;
+          c->nonce[kGroup0 + g] += uv + up;
+        }
+        for (int g = 0; g < 3; g++) log_reshare(c, g, n * tszv);
+        for (int g = 0; g < 3; g++) log_reshare(c, g, n * 4);
+        for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, n * 4);
+        S.nsb = (uint32_t)((n + kEPB - 1) / kEPB);
+        S.sb0 = sb;
+        S.st0 = st;
+        sb += S.nsb;
+        st += (n + 127) / 128;
+        all.push_back(S);
+        std::swap(r.cur, r.other);
+      }
+      tot_sb[i] = sb;
+      tot_st[i] = st;
+    }
+    first[maxbits] = all.size();
+    if (first[maxbits] == 0) return;
+    SegBit *dseg = (SegBit *)lead->scratch_get("rsb_seg", all.size() * sizeof(SegBit) + 64);
+    ORQG_HIP(hipMemcpyAsync(dseg, all.data(), all.size() * sizeof(SegBit), hipMemcpyHostToDevice,
+                            lead->stream));
+    std::vector<TuneTrial> trials;
+    auto launch_timed = [&](int kind, uint64_t units, auto &&fn) {
+      bool timed = false;
+      const unsigned grid = tune_pick(lead, kind, L, units, trials, timed);
+      TuneTrial t{tune_key(kind, L, units), grid, nullptr, nullptr};
+      if (timed) {
+        ORQG_HIP(hipEventCreate(&t.a));
+        ORQG_HIP(hipEventCreate(&t.b));
+        ORQG_HIP(hipEventRecord(t.a, lead->stream));
+      }
+      fn(grid);
+      if (timed) {
+        ORQG_HIP(hipEventRecord(t.b, lead->stream));
+        trials.push_back(t);
+      }
+    };
+    for (int i = 0; i < maxbits; i++) {
+      const SegBit *d = dseg + first[i];
+      const uint32_t nseg = (uint32_t)(first[i + 1] - first[i]);
+      const int bitpos = 32 + i;
+      const uint32_t tsb = tot_sb[i];
+      const uint64_t tst = tot_st[i];
+      {
+        TimedRegion tr(lead, "sortb_b2a", 2);
+        launch_timed(kTuneB2a, tsb, [&](unsigned grid) {
+          hipLaunchKernelGGL(k_bit_b2a_seg<L>, dim3(grid), dim3(kBT), 0, lead->stream, lead->d_t0, d,
+                             nseg, tsb, bitpos / 32, bitpos % 32);
+        });
+        hipLaunchKernelGGL(k_scan3_seg, dim3(nseg), dim3(1024), 0, lead->stream, d);
+      }
+      {
+        TimedRegion tr(lead, "sortb_perm");
+        launch_timed(kTunePerm, tsb, [&](unsigned grid) {
+          hipLaunchKernelGGL(k_bit_perm_seg<L>, dim3(grid), dim3(kBT), 0, lead->stream, lead->d_t0, d,
+                             nseg, tsb);
+        });
+      }
+      {
+        TimedRegion tr(lead, "sortb_round", 3);
+        // a workgroup takes 16 wave steps at a time
+        launch_timed(kTuneRound, (tst + 15) / 16, [&](unsigned grid) {
+          hipLaunchKernelGGL((k_round2_seg<L, false>), dim3(grid), dim3(kBT), 0, lead->stream,
+                             lead->d_t0, d, nseg, tst, 0);
+          hipLaunchKernelGGL((k_round2_seg<L, false>), dim3(grid), dim3(kBT), 0, lead->stream,
+                             lead->d_t0, d, nseg, tst, 1);
+          hipLaunchKernelGGL((k_round2_seg<L, true>), dim3(grid), dim3(kBT), 0, lead->stream,
+                             lead->d_t0, d, nseg, tst, 2);
+        });
+      }
+      ORQG_HIP(hipGetLastError());
+    }
+    // the requests' opened-permutation checks, then each request's tail on
+    // its own stream
+    std::vector<uint32_t> herr(rq.size(), 0);
+    for (size_t q = 0; q < rq.size(); q++)
+      ORQG_HIP(hipMemcpyAsync(&herr[q], rq[q].err, 4, hipMemcpyDeviceToHost, lead->stream));
+    ORQG_HIP(hipStreamSynchronize(lead->stream));
+    tune_settle(trials);
+    release();
+    for (size_t q = 0; q < rq.size(); q++)
+      if (herr[q])
+        throw Error(ORQG_E_PARITY, "radix_sort_batch: request " + std::to_string(q) +
+                                       ": opened bit permutation out of range");
+  } catch (...) {
+    (void)hipStreamSynchronize(lead->stream);
+    release();
+    throw;
+  }
+}
+
+}  // namespace
+
+void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out) {
+  const uint64_t n = v->n;
+  const bool desc = order == ORQG_DESC;
+  const bool full_width = bits == 8 * v->width;
+  if (n == 0) return;
+  // 1 permutation + (bits + 1) pairs; pairs are single permutations in 3PC
+  // (radixsort.h:121-128, permutation_manager.h:108-111)
+  reserve_perms(c, n, (size_t)bits + 2);
+  uint32_t *P = (uint32_t *)c->scratch_get("rs_P", n * kRec * 4 + 64);
+  pad_records(c, c->stream, v, desc, P);
+  if (v->width == 8)
+    P = sort_body<3>(c, P, n, bits, full_width);
+  else
+    P = sort_body<2>(c, P, n, bits, full_width);
+  sort_finish(c, v, P, desc, perm_out);
   ORQG_HIP(hipStreamSynchronize(c->stream));
+}
+
+// radix_sort of m requests, each on its own context, with the bit iterations
+// of all of them in shared launches: every request gets the shares, nonces,
+// permutation queue and message log that radix_sort on its context gives.
+void radix_sort_batch(const std::vector<orqg_ctx *> &cs, const std::vector<orqg_vec *> &vs,
+                      const std::vector<int> &orders, const std::vector<int> &bits,
+                      const std::vector<orqg_vec *> &perms) {
+  std::vector<size_t> live;
+  for (size_t q = 0; q < cs.size(); q++)
+    if (vs[q]->n > 0) live.push_back(q);
+  if (live.empty()) return;
+  {
+    std::vector<orqg_ctx *> rc;
+    std::vector<size_t> rn, rk;
+    for (size_t q : live) {
+      rc.push_back(cs[q]);
+      rn.push_back(vs[q]->n);
+      rk.push_back((size_t)bits[q] + 2);
+    }
+    reserve_perms_multi(rc, rn, rk);
+  }
+  orqg_ctx *lead = cs[live[0]];
+  for (int L : {3, 2}) {
+    std::vector<BatchReq> rq;
+    for (size_t q : live) {
+      if ((vs[q]->width == 8 ? 3 : 2) != L) continue;
+      BatchReq r;
+      orqg_ctx *c = cs[q];
+      const uint64_t n = vs[q]->n;
+      r.c = c;
+      r.v = vs[q];
+      r.perm = perms[q];
+      r.desc = orders[q] == ORQG_DESC;
+      r.bits = bits[q];
+      r.full_width = bits[q] == 8 * vs[q]->width;
+      r.P = (uint32_t *)c->scratch_get("rs_P", n * kRec * 4 + 64);
+      r.R1 = (uint32_t *)c->scratch_get("rs_R1", n * kRec * 4 + 64);
+      r.F = (uint32_t *)c->scratch_get("rs_F", n * 12 + 64);
+      r.bsum = (uint32_t *)c->scratch_get("rs_bsum", (n + kEPB - 1) / kEPB * 12 + 64);
+      r.tot = (uint32_t *)c->scratch_get("rs_tot", 64);
+      r.err = (uint32_t *)c->scratch_get("rs_err", 16);
+      r.cur = r.P;
+      r.other = r.R1;
+      rq.push_back(r);
+    }
+    if (rq.empty()) continue;
+    // every request's earlier work (its reserve, the producers of its
+    // vector) precedes the shared launches
+    for (BatchReq &r : rq)
+      if (r.c != lead) {
+        ORQG_HIP(hipEventRecord(r.c->ev_aux, r.c->stream));
+        ORQG_HIP(hipStreamWaitEvent(lead->stream, r.c->ev_aux, 0));
+      }
+    for (BatchReq &r : rq) {
+      ORQG_HIP(hipMemsetAsync(r.err, 0, 4, lead->stream));
+      pad_records(r.c, lead->stream, r.v, r.desc, r.P);
+    }
+    if (L == 3)
+      batch_body<3>(lead, rq);
+    else
+      batch_body<2>(lead, rq);
+    // (batch_body synchronised lead's stream) each request's tail on its own
+    // stream, then its swap-resolution checks, as radix_sort ends
+    for (BatchReq &r : rq) sort_finish(r.c, r.v, r.cur, r.desc, r.perm);
+    for (BatchReq &r : rq) {
+      ORQG_HIP(hipStreamSynchronize(r.c->stream));
+      check_swaps(r.c);
+    }
+  }
 }
 
 }  // namespace orqg
