@@ -320,15 +320,16 @@ def make_requests(count, seed=0x0A11CE05, lo=10, hi=22, width=8):
     return jobs
 
 
-def run_stream(jobs, slots, rate, seed=7):
+def run_stream(jobs, slots, rate, seed=7, batch_rows=0):
     """Submit the jobs at Poisson arrival times (rate per second).  Returns a
     dict: rows/s and p50/p99 latency over the requests that completed OK,
-    and every failed request with its status, slot and error text."""
+    and every failed request with its status, slot and error text.
+    batch_rows > 0: the slots' small radix sorts run as shared batches."""
     import threading
 
     from orq_amd.serve import Server
 
-    srv = Server(slots=slots)
+    srv = Server(slots=slots, batch_rows=batch_rows)
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(jobs))
     t_start = time.perf_counter()
@@ -365,51 +366,57 @@ def run_stream(jobs, slots, rate, seed=7):
 
 
 def bench_serve(args):
-    """Served mixed-size stream: continuous batching against the plain
-    one-request-at-a-time path on the same requests and arrival times, at a
-    ladder of Poisson arrival rates.  For each mode the headline takes the
-    highest rate whose p99 latency stays within the declared target, and the
-    rows/s sustained there.  Any failed request makes the line invalid
-    (value null) and the exit status 1."""
-    jobs = make_requests(args.requests)
+    """Served mixed-size stream: continuous batching (slots), continuous
+    batching with the slots' small radix sorts run as shared segmented
+    batches, and the plain one-request-at-a-time path, on the same requests
+    and arrival times, at a ladder of Poisson arrival rates.  For each mode
+    the headline takes the highest rate whose p99 latency stays within the
+    declared target, and the rows/s sustained there.  Any failed request
+    makes the line invalid (value null) and the exit status 1."""
+    jobs = make_requests(args.requests, lo=args.lo, hi=args.hi)
     rows = sum(j.n for j in jobs)
     res = {}
     n_failed = 0
+    modes = [("one_at_a_time", 1, 0), ("slots", args.slots, 0),
+             ("slots_batched_sorts", args.batch_slots, args.batch_rows)]
     for rate in args.rates:
-        for slots in (1, args.slots):
-            r = run_stream(jobs, slots, rate)
+        for mode, slots, brows in modes:
+            r = run_stream(jobs, slots, rate, batch_rows=brows)
             st = r["stats"]
             n_failed += len(r["failed"])
-            res[f"rate{rate}_slots{slots}"] = {
-                "rate_per_s": rate, "slots": slots,
+            res[f"rate{rate}_{mode}"] = {
+                "mode": mode, "rate_per_s": rate, "slots": slots, "batch_rows": brows,
                 "rows_per_s": round(r["rows_per_s"], 1), "p50_ms": round(r["p50_ms"], 1),
                 "p99_ms": round(r["p99_ms"], 1), "wall_s": round(r["wall_s"], 2),
                 "max_running": st["max_running"], "failed": len(r["failed"]),
+                "sort_batches": st["sort_batches"], "sorts_batched": st["sorts_batched"],
                 "hipMalloc_calls": st["mallocs"], "pool_trims": st["pool_trims"]}
-            print(f"# rate {rate}/s slots {slots}: {r['rows_per_s'] / 1e6:.3f} Mrows/s "
-                  f"p50 {r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms failed {len(r['failed'])}",
-                  flush=True)
+            print(f"# rate {rate}/s {mode} ({slots} slots): {r['rows_per_s'] / 1e6:.3f} Mrows/s "
+                  f"p50 {r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms failed {len(r['failed'])} "
+                  f"sort batches {st['sort_batches']} ({st['sorts_batched']} sorts)", flush=True)
     target = args.p99_target_ms
 
-    def best(slots):
-        ok = [v for v in res.values() if v["slots"] == slots and v["p99_ms"] <= target]
+    def best(mode):
+        ok = [v for v in res.values() if v["mode"] == mode and v["p99_ms"] <= target]
         return max(ok, key=lambda v: v["rate_per_s"]) if ok else None
 
-    batched, plain = best(args.slots), best(1)
+    keys = ("rate_per_s", "rows_per_s", "p50_ms", "p99_ms")
+    bests = {m: best(m) for m, _, _ in modes}
+    head = bests["slots_batched_sorts"]
     valid = n_failed == 0
-    line = {"metric": "served-query p50/p99 latency and Mrows/s (mixed 2^10..2^22 stream)",
-            "value": round(batched["rows_per_s"] / 1e6, 4) if batched and valid else None,
+    line = {"metric": "served-query p50/p99 latency and Mrows/s "
+                      f"(mixed 2^{args.lo}..2^{args.hi} stream)",
+            "value": round(head["rows_per_s"] / 1e6, 4) if head and valid else None,
             "unit": f"Mrows/s at the highest arrival rate with p99 <= {target:.0f} ms",
             "n_gpus": 1, "steps": len(jobs), "higher_is_better": True, "dtype": "int64",
             "data": "synthetic: log-uniform sizes, Zipf(1.1) keys over 2^20, uint32 values, "
                     "Poisson arrivals",
             "config": {"workload": f"{len(jobs)} requests ({rows} rows), sort / GROUP BY SUM "
-                                   f"50/50, {args.slots} slots vs one at a time",
+                                   f"50/50; one at a time vs {args.slots} slots vs "
+                                   f"{args.batch_slots} slots with radix sorts of <= "
+                                   f"{args.batch_rows} rows batched",
                        "rates_per_s": args.rates, "p99_target_ms": target},
-            "batched": {k: batched[k] for k in ("rate_per_s", "rows_per_s", "p50_ms", "p99_ms")}
-            if batched else None,
-            "one_at_a_time": {k: plain[k] for k in ("rate_per_s", "rows_per_s", "p50_ms", "p99_ms")}
-            if plain else None,
+            "modes": {m: ({k: b[k] for k in keys} if b else None) for m, b in bests.items()},
             "failed_requests": n_failed,
             "runs": res}
     print(json.dumps(line), flush=True)
@@ -433,6 +440,12 @@ def main():
     ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--rates", type=float, nargs="+", default=[4.0, 6.0, 8.0, 12.0, 16.0])
+    ap.add_argument("--lo", type=int, default=10, help="serve: smallest request 2^lo rows")
+    ap.add_argument("--hi", type=int, default=22, help="serve: largest request 2^hi rows")
+    ap.add_argument("--batch-slots", type=int, default=8,
+                    help="serve: slots of the mode whose small radix sorts are batched")
+    ap.add_argument("--batch-rows", type=int, default=1 << 16,
+                    help="serve: radix sorts up to this many rows join shared batches")
     ap.add_argument("--p99-target-ms", type=float, default=1000.0,
                     help="serve workload: the p99 latency both modes are held to")
     args = ap.parse_args()

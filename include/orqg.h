@@ -384,12 +384,17 @@ typedef struct {
   int slots;                         /* concurrent requests, 1..64 */
   size_t hbm_budget;                 /* admission budget in bytes (0: 200 GiB) */
   long batch_size;                   /* runtime batch size as orqg_config */
+  size_t batch_rows;                 /* radix sorts of at most this many rows from
+                                        concurrent requests run as one
+                                        orqg_radix_sort_batch (0: each alone) */
+  int batch_window_us;               /* longest wait for more sorts to join a batch */
 } orqg_server_config;
 typedef struct {
   uint64_t submitted, completed, rejected, max_running, peak_reserved, peak_live_bytes;
   uint64_t failed;      /* completed with a status other than ORQG_OK */
   uint64_t mallocs;     /* hipMalloc calls of the process's allocator */
   uint64_t pool_trims;  /* pooled blocks it released (pool cap, or memory pressure) */
+  uint64_t sort_batches, sorts_batched; /* shared radix-sort batches, the sorts in them */
 } orqg_server_stats;
 typedef struct orqg_server orqg_server;
 int orqg_server_create(const orqg_server_config *cfg, orqg_server **out);

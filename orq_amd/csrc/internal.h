@@ -3,9 +3,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <condition_variable>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -121,6 +124,8 @@ struct Timer {
   uint64_t launches = 0;
 };
 
+struct SortBatcher;
+
 }  // namespace orqg
 
 struct orqg_ctx {
@@ -140,6 +145,9 @@ struct orqg_ctx {
   std::map<std::string, orqg::Scratch> scratch;
   orqg::Pool pool;  // free device blocks of this context
   std::deque<orqg::ShardedPerm> queue;
+  // a served context's small radix sorts join those of the server's other
+  // slots (null: the context sorts on its own)
+  orqg::SortBatcher *batcher = nullptr;
   uint32_t *fy_err = nullptr;       // swap-resolution check flags, read by check_swaps
   bool fy_pending = false;
   size_t stored_size = 0;
@@ -238,6 +246,31 @@ void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_ou
 void radix_sort_batch(const std::vector<orqg_ctx *> &cs, const std::vector<orqg_vec *> &vs,
                       const std::vector<int> &orders, const std::vector<int> &bits,
                       const std::vector<orqg_vec *> &perms);
+
+// Rendezvous of concurrent radix sorts (the serving loop's slots): a sort of
+// at most max_rows rows waits until every active request is waiting here or
+// window_us has passed since the batch's first sort arrived, and the batch
+// then runs as one radix_sort_batch (each request's results unchanged).
+struct SortBatcher {
+  size_t max_rows = 0;
+  int window_us = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  int active = 0;  // requests running on the slots
+  struct Job {
+    orqg_ctx *c;
+    orqg_vec *v, *perm;
+    int order, bits;
+    bool done = false;
+    std::exception_ptr err;
+  };
+  std::vector<Job *> waiting;
+  bool leading = false;
+  uint64_t batches = 0, batched = 0;
+  void enter();
+  void leave();
+  void sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out);
+};
 void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, orqg_vec *gt,
              bool full, int neg_eq, int neg_gt);
 void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y);

@@ -8,7 +8,10 @@
 // slots' streams, so small requests fill the CUs a large one leaves idle and
 // one request's host-side waits overlap another's kernels.  Every request
 // re-keys its slot's context with the request's own keys, so its result
-// shares depend only on the request: the same alone and in any mix.
+// shares depend only on the request: the same alone and in any mix.  With
+// batch_rows set, the small radix sorts of concurrent requests meet in a
+// rendezvous and run as one batched sort (segmented launches across the
+// requests' contexts), which leaves every request's results unchanged.
 //
 // The request operations follow EncodedTable::sort (encoded_table.h:900-998,
 // radix table sort over all columns) and EncodedTable::aggregate with its
@@ -30,6 +33,7 @@
 struct orqg_server {
   orqg_server_config cfg;
   std::vector<orqg_ctx *> ctxs;
+  orqg::SortBatcher batcher;
   std::vector<std::thread> workers;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -124,6 +128,7 @@ void run_request(orqg_ctx *c, const orqg_request &r) {
 
 void worker(orqg_server *s, int slot) {
   orqg_ctx *c = s->ctxs[slot];
+  (void)hipSetDevice(c->device);
   for (;;) {
     orqg_server::Pending p;
     {
@@ -149,6 +154,7 @@ void worker(orqg_server *s, int slot) {
     done.t_submit_ms = p.t_submit;
     done.t_start_ms = s->now_ms();
     auto note = [&](const char *m) { std::snprintf(done.msg, sizeof(done.msg), "%s", m); };
+    s->batcher.enter();
     try {
       run_request(c, p.req);
       done.status = ORQG_OK;
@@ -162,6 +168,7 @@ void worker(orqg_server *s, int slot) {
       done.status = ORQG_E_DEVICE;
       note("unknown exception");
     }
+    s->batcher.leave();
     if (done.status != ORQG_OK) {
       // leave the slot's context clean for the next request: wait for both
       // streams and read (and so clear) a pending swap-resolution check, so
@@ -197,6 +204,8 @@ int orqg_server_create(const orqg_server_config *cfg, orqg_server **out) {
   s->cfg = *cfg;
   if (s->cfg.hbm_budget == 0) s->cfg.hbm_budget = (size_t)200 << 30;
   s->t0 = std::chrono::steady_clock::now();
+  s->batcher.max_rows = cfg->batch_rows;
+  s->batcher.window_us = std::max(0, cfg->batch_window_us);
   for (int i = 0; i < cfg->slots; i++) {
     orqg_config cc{};
     cc.device = cfg->device;
@@ -208,6 +217,7 @@ int orqg_server_create(const orqg_server_config *cfg, orqg_server **out) {
       delete s;
       return rc;
     }
+    if (cfg->batch_rows) c->batcher = &s->batcher;
     s->ctxs.push_back(c);
   }
   for (int i = 0; i < cfg->slots; i++) s->workers.emplace_back(orqg::worker, s, i);
@@ -269,6 +279,11 @@ int orqg_server_get_stats(orqg_server *s, orqg_server_stats *st) {
   std::lock_guard<std::mutex> lk(s->mu);
   *st = s->stats;
   orqg::alloc_counters(&st->mallocs, &st->pool_trims);
+  {
+    std::lock_guard<std::mutex> lb(s->batcher.mu);
+    st->sort_batches = s->batcher.batches;
+    st->sorts_batched = s->batcher.batched;
+  }
   return ORQG_OK;
 }
 

@@ -30,6 +30,7 @@
 
 #include <cstring>
 #include <cmath>
+#include <chrono>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -1085,6 +1086,10 @@ void radix_sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_ou
   const bool desc = order == ORQG_DESC;
   const bool full_width = bits == 8 * v->width;
   if (n == 0) return;
+  if (c->batcher && n <= c->batcher->max_rows) {
+    c->batcher->sort(c, v, order, bits, perm_out);
+    return;
+  }
   // 1 permutation + (bits + 1) pairs; pairs are single permutations in 3PC
   // (radixsort.h:121-128, permutation_manager.h:108-111)
   reserve_perms(c, n, (size_t)bits + 2);
@@ -1166,6 +1171,70 @@ void radix_sort_batch(const std::vector<orqg_ctx *> &cs, const std::vector<orqg_
       check_swaps(r.c);
     }
   }
+}
+
+void SortBatcher::enter() {
+  std::lock_guard<std::mutex> lk(mu);
+  active++;
+}
+
+void SortBatcher::leave() {
+  std::lock_guard<std::mutex> lk(mu);
+  active--;
+  cv.notify_all();  // a gathering leader may now have everyone
+}
+
+void SortBatcher::sort(orqg_ctx *c, orqg_vec *v, int order, int bits, orqg_vec *perm_out) {
+  Job j;
+  j.c = c;
+  j.v = v;
+  j.perm = perm_out;
+  j.order = order;
+  j.bits = bits;
+  std::unique_lock<std::mutex> lk(mu);
+  waiting.push_back(&j);
+  cv.notify_all();
+  while (!j.done) {
+    if (leading) {
+      cv.wait(lk, [&] { return j.done || !leading; });
+      continue;
+    }
+    // lead: gather, then run the batch outside the lock (the other jobs'
+    // threads wait, so their contexts are not in use meanwhile)
+    leading = true;
+    const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(window_us);
+    cv.wait_until(lk, until, [&] { return (int)waiting.size() >= active; });
+    std::vector<Job *> take;
+    take.swap(waiting);
+    lk.unlock();
+    std::exception_ptr err;
+    try {
+      std::vector<orqg_ctx *> cs;
+      std::vector<orqg_vec *> vs, ps;
+      std::vector<int> os, bs;
+      for (Job *t : take) {
+        cs.push_back(t->c);
+        vs.push_back(t->v);
+        ps.push_back(t->perm);
+        os.push_back(t->order);
+        bs.push_back(t->bits);
+      }
+      radix_sort_batch(cs, vs, os, bs, ps);
+    } catch (...) {
+      err = std::current_exception();
+    }
+    lk.lock();
+    for (Job *t : take) {
+      t->err = err;
+      t->done = true;
+    }
+    batches++;
+    batched += take.size();
+    leading = false;
+    cv.notify_all();
+  }
+  lk.unlock();
+  if (j.err) std::rethrow_exception(j.err);
 }
 
 }  // namespace orqg

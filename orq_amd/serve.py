@@ -25,7 +25,8 @@ class Completion(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
-                ("batch_size", ctypes.c_long)]
+                ("batch_size", ctypes.c_long), ("batch_rows", ctypes.c_size_t),
+                ("batch_window_us", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -33,7 +34,8 @@ class ServerStats(ctypes.Structure):
                 ("rejected", ctypes.c_uint64), ("max_running", ctypes.c_uint64),
                 ("peak_reserved", ctypes.c_uint64), ("peak_live_bytes", ctypes.c_uint64),
                 ("failed", ctypes.c_uint64), ("mallocs", ctypes.c_uint64),
-                ("pool_trims", ctypes.c_uint64)]
+                ("pool_trims", ctypes.c_uint64), ("sort_batches", ctypes.c_uint64),
+                ("sorts_batched", ctypes.c_uint64)]
 
 
 class Job:
@@ -61,8 +63,12 @@ class Job:
 
 
 class Server:
-    def __init__(self, slots=4, hbm_budget=0, device=0, batch_size=-12):
-        cfg = ServerConfig(device, slots, hbm_budget, batch_size)
+    """batch_rows > 0: radix sorts of at most that many rows from concurrent
+    requests run as one batched sort (gathered for up to batch_window_us)."""
+
+    def __init__(self, slots=4, hbm_budget=0, device=0, batch_size=-12, batch_rows=0,
+                 batch_window_us=2000):
+        cfg = ServerConfig(device, slots, hbm_budget, batch_size, batch_rows, batch_window_us)
         self.h = ctypes.c_void_p()
         rc = lib().orqg_server_create(ctypes.byref(cfg), ctypes.byref(self.h))
         if rc:

@@ -184,3 +184,36 @@ def test_many_overlapping_small_requests(G):
         for x, y in ((a.out_key, b.out_key), (a.out_val, b.out_val), (a.out_valid, b.out_valid)):
             np.testing.assert_array_equal(x, y)
         _check_plain(a, op, k, v)
+
+
+@pytest.mark.parametrize("slots", [4, 12])
+def test_batched_sorts_across_slots(G, slots):
+    """Served with batch_rows set: the small radix sorts of concurrent
+    requests meet and run as shared batches (a batch holds several sorts),
+    a large request's sorts run on their own, and every result equals the
+    same request served alone, unbatched."""
+    from orq_amd.serve import Server
+
+    rng = np.random.default_rng(12)
+    sizes = [int(x) for x in rng.integers(1 << 10, 1 << 14, 40)] + [1 << 17]
+    mix, plain = _make_jobs(G, sizes, seed=6)
+    alone, _ = _make_jobs(G, sizes, seed=6)
+    srv = Server(slots=slots, batch_rows=1 << 16, batch_window_us=3000)
+    for j in mix:
+        srv.submit(j)
+    done = srv.drain()
+    st = srv.stats()
+    srv.close()
+    bad = [(j.id, j.completion[0], j.error) for j in done if j.completion[0] != 0]
+    assert not bad, bad
+    assert st["sort_batches"] > 0 and st["sorts_batched"] > st["sort_batches"]
+    srv1 = Server(slots=1)
+    for j in alone:
+        srv1.submit(j)
+    srv1.drain()
+    srv1.close()
+    for a, b, (op, k, v) in zip(mix, alone, plain):
+        assert b.completion[0] == 0, b.error
+        for x, y in ((a.out_key, b.out_key), (a.out_val, b.out_val), (a.out_valid, b.out_valid)):
+            np.testing.assert_array_equal(x, y, err_msg=f"request {a.id} ({a.n} rows)")
+        _check_plain(a, op, k, v)
