@@ -276,7 +276,7 @@ int orqg_table_sort(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, const in
                     const int *single_bit, orqg_vec *const *a_cols, size_t na,
                     orqg_vec *const *b_cols, size_t nb);
 /* SortingProtocol (sorting.h:15-22); QUICKSORT is the reference's DEFAULT   */
-enum { ORQG_RADIXSORT = 0, ORQG_QUICKSORT = 1 };
+enum { ORQG_RADIXSORT = 0, ORQG_QUICKSORT = 1, ORQG_BITONICSORT = 2, ORQG_SORT_DEFAULT = 3 };
 /* operators::table_sort with an explicit protocol: RADIXSORT as above, or
  * QUICKSORT for the multi-bit sort columns (32-bit columns only; single-bit
  * columns use 1-bit radix sort under either), with the reserve counts of
@@ -352,6 +352,65 @@ int orqg_timer_enable(orqg_ctx *ctx, int enable);
 int orqg_timer_get(orqg_ctx *ctx, const char *family, double *ms, uint64_t *launches);
 int orqg_timer_reset(orqg_ctx *ctx);
 
+
+/* ---- EncodedTable (include/core/containers/tabular/encoded_table.h) ---------
+ * A table of named shared columns of one width, ordered by name (std::map,
+ * encoded_table.h:147).  A name containing '[' is boolean-shared, any other
+ * arithmetic-shared (isBShared, encoded_table.h:1672); every table has the
+ * valid column "[##V]" (configureValid, encoded_table.h:274-285); the sort
+ * keys "[##V]" and "[##TID]" are sorted on one bit (encoded_table.h:915-935).
+ * Columns are owned by the table; orqg_et_column lends one until the table
+ * changes it or is freed.  The protocols of orqg_et_sort are RADIXSORT,
+ * QUICKSORT (32-bit tables), BITONICSORT and SORT_DEFAULT (the reference's
+ * DEFAULT, quicksort: RADIXSORT for 64-bit tables here).                    */
+typedef struct orqg_et orqg_et;
+typedef struct {
+  const char *data;    /* input column */
+  const char *result;  /* result column (may be `data`) */
+  int func;            /* ORQG_AGG_* */
+} orqg_et_agg;
+/* EncodedTable(name, schema, rows) (encoded_table.h:244-249): zero columns
+ * plus the valid column set to ones                                         */
+int orqg_et_create(orqg_ctx *ctx, int width, size_t rows, const char *const *names, size_t ncols,
+                   orqg_et **out);
+int orqg_et_free(orqg_et *t);
+size_t orqg_et_rows(const orqg_et *t);
+size_t orqg_et_ncols(const orqg_et *t);
+/* name of column i in schema order (valid until the next call on t) */
+const char *orqg_et_column_name(orqg_et *t, size_t i);
+orqg_vec *orqg_et_column(orqg_et *t, const char *name);
+/* copy v's shares into column `name` (created if absent; its encoding follows the name) */
+int orqg_et_set_column(orqg_et *t, const char *name, const orqg_vec *v);
+/* addColumns / deleteColumns / project (encoded_table.h:724-803)           */
+int orqg_et_add_columns(orqg_et *t, const char *const *names, size_t n);
+int orqg_et_delete_columns(orqg_et *t, const char *const *names, size_t n);
+int orqg_et_project(orqg_et *t, const char *const *names, size_t n);
+/* filter (encoded_table.h:298-314): [##V] &= column                         */
+int orqg_et_filter(orqg_et *t, const char *name);
+/* resize / tail (encoded_table.h:1407-1436)                                 */
+int orqg_et_resize(orqg_et *t, size_t n);
+int orqg_et_tail(orqg_et *t, size_t n);
+/* shuffle (encoded_table.h:1003-1018)                                       */
+int orqg_et_shuffle(orqg_et *t);
+/* sort (encoded_table.h:900-998): every column by the keys, keys[0] most
+ * significant; orders NULL = all ascending                                   */
+int orqg_et_sort(orqg_et *t, const char *const *keys, const int *orders, size_t nk, int protocol);
+/* distinct (encoded_table.h:1199-1237): sort on [##V] + keys, keep the first
+ * row of each key group valid                                               */
+int orqg_et_distinct(orqg_et *t, const char *const *keys, size_t nk);
+/* aggregate (encoded_table.h:1059-1188) with its options: reverse direction,
+ * do_sort (on [##V] + keys first), mark_valid (keep one row per group valid),
+ * table_id (the join selector column, or NULL)                              */
+int orqg_et_aggregate(orqg_et *t, const char *const *keys, size_t nk, const orqg_et_agg *spec,
+                      size_t ns, int reverse, int do_sort, int mark_valid, const char *table_id);
+/* concatenate (encoded_table.h:1548-1616): a's rows then b's, with the table
+ * id column "[##TID]" (0 for a, a sharing of 1 for b)                       */
+int orqg_et_concatenate(orqg_et *a, orqg_et *b, orqg_et **out);
+/* operators::inner_join (operators/join.h:31-233): left (unique keys) JOIN
+ * right ON keys, with aggregations `spec` (e.g. copying left's columns down);
+ * trim_invalid drops rows that cannot be valid when that pays (join.h)      */
+int orqg_et_inner_join(orqg_et *left, orqg_et *right, const char *const *keys, size_t nk,
+                       const orqg_et_agg *spec, size_t ns, int trim_invalid, orqg_et **out);
 
 /* ---- serving loop (no counterpart in the reference: its batched/served form) --
  * Secret-shared query requests on one GPU with continuous batching: `slots`

@@ -146,12 +146,90 @@ inline void table_sort(Context &g, const std::vector<Vec *> &keys, const std::ve
                           b.data(), b.size()));
 }
 
+// EncodedTable (include/core/containers/tabular/encoded_table.h) in the
+// library: named columns ('[' in a name = boolean-shared), the valid column
+// "[##V]", and the relational methods, freed with the object.
+class Table {
+ public:
+  Table(Context &g, int width, size_t rows, const std::vector<std::string> &names) : g_(&g) {
+    std::vector<const char *> p;
+    for (const std::string &x : names) p.push_back(x.c_str());
+    g.check(orqg_et_create(g.get(), width, rows, p.data(), p.size(), &t_));
+  }
+  ~Table() {
+    if (t_) orqg_et_free(t_);
+  }
+  Table(const Table &) = delete;
+  Table &operator=(const Table &) = delete;
+  Table(Table &&o) noexcept : g_(o.g_), t_(o.t_) { o.t_ = nullptr; }
+
+  orqg_et *get() const { return t_; }
+  size_t rows() const { return orqg_et_rows(t_); }
+  // the column (lent: valid until the table changes)
+  orqg_vec *column(const std::string &name) const {
+    orqg_vec *v = orqg_et_column(t_, name.c_str());
+    if (!v) throw Error(ORQG_E_ARG, "Table: no column " + name);
+    return v;
+  }
+  // host [3][n] shares into / out of a column
+  template <typename T>
+  void write(const std::string &name, const std::vector<T> &host) {
+    if (host.size() != 3 * rows()) throw Error(ORQG_E_ARG, "Table::write: host buffer shape");
+    g_->check(orqg_vec_write(g_->get(), column(name), host.data()));
+  }
+  template <typename T>
+  std::vector<T> read(const std::string &name) const {
+    std::vector<T> host(3 * rows());
+    g_->check(orqg_vec_read(g_->get(), column(name), host.data()));
+    return host;
+  }
+  void add_columns(const std::vector<std::string> &names) {
+    std::vector<const char *> p = c_names(names);
+    g_->check(orqg_et_add_columns(t_, p.data(), p.size()));
+  }
+  void sort(const std::vector<std::string> &keys, const std::vector<int> &orders = {},
+            int protocol = ORQG_SORT_DEFAULT) {
+    std::vector<const char *> p = c_names(keys);
+    g_->check(orqg_et_sort(t_, p.data(), orders.empty() ? nullptr : orders.data(), p.size(), protocol));
+  }
+  // aggregate with its defaults (sort on valid + keys, one valid row per group)
+  void aggregate(const std::vector<std::string> &keys, const std::vector<orqg_et_agg> &spec,
+                 bool reverse = false) {
+    std::vector<const char *> p = c_names(keys);
+    g_->check(orqg_et_aggregate(t_, p.data(), p.size(), spec.data(), spec.size(), reverse, 1, 1,
+                                nullptr));
+  }
+  // operators::inner_join (operators/join.h:31-233): this (unique keys) JOIN right
+  Table inner_join(Table &right, const std::vector<std::string> &keys,
+                   const std::vector<orqg_et_agg> &spec, bool trim_invalid = true) {
+    std::vector<const char *> p = c_names(keys);
+    orqg_et *out = nullptr;
+    g_->check(orqg_et_inner_join(t_, right.t_, p.data(), p.size(), spec.data(), spec.size(),
+                                 trim_invalid, &out));
+    return Table(*g_, out);
+  }
+
+ private:
+  Table(Context &g, orqg_et *t) : g_(&g), t_(t) {}
+  static std::vector<const char *> c_names(const std::vector<std::string> &names) {
+    std::vector<const char *> p;
+    for (const std::string &x : names) p.push_back(x.c_str());
+    return p;
+  }
+  Context *g_;
+  orqg_et *t_ = nullptr;
+};
+
 // open_shares_b / _a (replicated_3pc.h:310-330)
 template <typename T>
-std::vector<T> open(Context &g, const Vec &v) {
-  std::vector<T> out(v.size());
-  g.check(orqg_open(g.get(), v.get(), out.data()));
+std::vector<T> open(Context &g, const orqg_vec *v) {
+  std::vector<T> out(orqg_vec_size(v));
+  g.check(orqg_open(g.get(), v, out.data()));
   return out;
+}
+template <typename T>
+std::vector<T> open(Context &g, const Vec &v) {
+  return open<T>(g, v.get());
 }
 
 }  // namespace orqg_adapter

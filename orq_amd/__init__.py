@@ -60,6 +60,11 @@ EXPORTS = [
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
     "orqg_server_destroy",
     "orqg_reserve_perms", "orqg_perm_queue_size", "orqg_peek_perm", "orqg_apply_sharded_perm",
+    "orqg_et_create", "orqg_et_free", "orqg_et_rows", "orqg_et_ncols", "orqg_et_column_name",
+    "orqg_et_column", "orqg_et_set_column", "orqg_et_add_columns", "orqg_et_delete_columns",
+    "orqg_et_project", "orqg_et_filter", "orqg_et_resize", "orqg_et_tail", "orqg_et_shuffle",
+    "orqg_et_sort", "orqg_et_distinct", "orqg_et_aggregate", "orqg_et_concatenate",
+    "orqg_et_inner_join",
     "orqg_apply_elementwise_perm", "orqg_radix_sort", "orqg_radix_sort_batch",
     "orqg_msglog_count", "orqg_msglog_get",
     "orqg_msglog_clear", "orqg_timer_enable", "orqg_timer_get", "orqg_timer_reset",
@@ -154,6 +159,25 @@ def lib():
             "orqg_apply_sharded_perm": (i32, [vp, vp, i32]),
             "orqg_apply_elementwise_perm": (i32, [vp, vp, vp]),
             "orqg_radix_sort": (i32, [vp, vp, i32, i32, vp]),
+            "orqg_et_create": (i32, [vp, i32, sz, vp, sz, ctypes.POINTER(vp)]),
+            "orqg_et_free": (i32, [vp]),
+            "orqg_et_rows": (sz, [vp]),
+            "orqg_et_ncols": (sz, [vp]),
+            "orqg_et_column_name": (ctypes.c_char_p, [vp, sz]),
+            "orqg_et_column": (vp, [vp, ctypes.c_char_p]),
+            "orqg_et_set_column": (i32, [vp, ctypes.c_char_p, vp]),
+            "orqg_et_add_columns": (i32, [vp, vp, sz]),
+            "orqg_et_delete_columns": (i32, [vp, vp, sz]),
+            "orqg_et_project": (i32, [vp, vp, sz]),
+            "orqg_et_filter": (i32, [vp, ctypes.c_char_p]),
+            "orqg_et_resize": (i32, [vp, sz]),
+            "orqg_et_tail": (i32, [vp, sz]),
+            "orqg_et_shuffle": (i32, [vp]),
+            "orqg_et_sort": (i32, [vp, vp, vp, sz, i32]),
+            "orqg_et_distinct": (i32, [vp, vp, sz]),
+            "orqg_et_aggregate": (i32, [vp, vp, sz, vp, sz, i32, i32, i32, ctypes.c_char_p]),
+            "orqg_et_concatenate": (i32, [vp, vp, ctypes.POINTER(vp)]),
+            "orqg_et_inner_join": (i32, [vp, vp, vp, sz, vp, sz, i32, ctypes.POINTER(vp)]),
             "orqg_radix_sort_batch": (i32, [ctypes.POINTER(vp), ctypes.POINTER(vp),
                                             ctypes.POINTER(i32), ctypes.POINTER(i32),
                                             ctypes.POINTER(vp), sz]),

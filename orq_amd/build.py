@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liborqg.so")
-SOURCES = ["api.hip", "prim.hip", "cmp.hip", "genperm.hip", "perm.hip", "sort.hip", "table.hip", "bitonic.hip", "agg.hip", "adder.hip", "serve.hip", "vecops.hip", "quicksort.hip", "csv.hip"]
+SOURCES = ["api.hip", "prim.hip", "cmp.hip", "genperm.hip", "perm.hip", "sort.hip", "table.hip", "bitonic.hip", "agg.hip", "adder.hip", "serve.hip", "vecops.hip", "quicksort.hip", "csv.hip", "etable.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

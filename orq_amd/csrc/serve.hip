@@ -13,11 +13,10 @@
 // rendezvous and run as one batched sort (segmented launches across the
 // requests' contexts), which leaves every request's results unchanged.
 //
-// The request operations follow EncodedTable::sort (encoded_table.h:900-998,
-// radix table sort over all columns) and EncodedTable::aggregate with its
-// defaults (encoded_table.h:1059-1188: sort on valid + key, pad to a power of
-// two, distinct, forward aggregation network, unpad, filter on the distinct
-// bits).
+// The request operations are the library's EncodedTable::sort
+// (encoded_table.h:900-998) and EncodedTable::aggregate with its defaults
+// (encoded_table.h:1059-1188: sort on valid + key, pad to a power of two,
+// distinct, forward aggregation network, unpad, filter on the distinct bits).
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -67,17 +66,10 @@ size_t request_bytes(const orqg_request &r) {
   return n * 1800 + pad * 6 * r.width * 3 + (64u << 20);
 }
 
-struct VecGuard {
-  std::vector<orqg_vec *> v;
-  ~VecGuard() {
-    for (orqg_vec *x : v) orqg_vec_free(x);
-  }
-  orqg_vec *make(orqg_ctx *c, size_t n, int w, int enc) {
-    orqg_vec *x = nullptr;
-    int rc = orqg_vec_create(c, n, w, enc, &x);
-    if (rc) throw Error(rc, orqg_last_error(c));
-    v.push_back(x);
-    return x;
+struct TableGuard {
+  orqg_et *t = nullptr;
+  ~TableGuard() {
+    if (t) orqg_et_free(t);
   }
 };
 
@@ -87,43 +79,28 @@ struct VecGuard {
     if (rc_) throw ::orqg::Error(rc_, orqg_last_error(c));      \
   } while (0)
 
+// The request as the reference's table calls make it: a table {[K], V} with
+// its valid column, then EncodedTable::sort on [K] or EncodedTable::aggregate
+// on [K] with V summed (both with their default options and protocol).
 void run_request(orqg_ctx *c, const orqg_request &r) {
   ORQG_CALL(orqg_ctx_rekey(c, r.keys));
-  const size_t n = r.n;
-  const int w = r.width;
-  VecGuard g;
-  orqg_vec *key = g.make(c, n, w, ORQG_BSHARED);
-  orqg_vec *val = g.make(c, n, w, ORQG_ASHARED);
-  orqg_vec *valid = g.make(c, n, w, ORQG_BSHARED);
-  ORQG_CALL(orqg_vec_write(c, key, r.key_shares));
-  ORQG_CALL(orqg_vec_write(c, val, r.val_shares));
-  ORQG_CALL(orqg_public_fill(c, valid, 1, ORQG_BSHARED));  // configureValid
+  const char *names[2] = {"[K]", "V"};
+  TableGuard g;
+  ORQG_CALL(orqg_et_create(c, r.width, r.n, names, 2, &g.t));
+  ORQG_CALL(orqg_vec_write(c, orqg_et_column(g.t, "[K]"), r.key_shares));
+  ORQG_CALL(orqg_vec_write(c, orqg_et_column(g.t, "V"), r.val_shares));
+  const char *keys[1] = {"[K]"};
   if (r.op == ORQG_REQ_SORT) {
-    // schema {"V", "[##V]", "[K]"}: key [K], data V (A) and [##V] (B)
-    orqg_vec *keys[1] = {key}, *a[1] = {val}, *b[1] = {valid};
-    int order[1] = {ORQG_ASC}, single[1] = {0};
-    ORQG_CALL(orqg_table_sort(c, keys, 1, order, single, a, 1, b, 1));
+    ORQG_CALL(orqg_et_sort(g.t, keys, nullptr, 1, ORQG_SORT_DEFAULT));
   } else if (r.op == ORQG_REQ_GROUPBY_SUM) {
-    orqg_vec *keys[2] = {valid, key}, *a[1] = {val};
-    int order[2] = {ORQG_ASC, ORQG_ASC}, single[2] = {1, 0};
-    ORQG_CALL(orqg_table_sort(c, keys, 2, order, single, a, 1, nullptr, 0));
-    size_t pad = 1;
-    while (pad < n) pad <<= 1;
-    for (orqg_vec *x : {key, val, valid}) ORQG_CALL(orqg_vec_resize(c, x, pad));
-    orqg_vec *uniq = g.make(c, pad, w, ORQG_BSHARED);
-    ORQG_CALL(orqg_distinct(c, keys, 2, uniq));
-    orqg_agg_spec sum = {val, val, ORQG_AGG_SUM};
-    ORQG_CALL(orqg_aggregate(c, keys, 2, nullptr, 0, &sum, 1, 0, nullptr));
-    for (orqg_vec *x : {key, val, valid, uniq}) ORQG_CALL(orqg_vec_resize(c, x, n));
-    orqg_vec *nv = g.make(c, n, w, ORQG_BSHARED);
-    ORQG_CALL(orqg_and_b(c, valid, uniq, nv));  // filter(UNIQ)
-    valid = nv;
+    const orqg_et_agg sum = {"V", "V", ORQG_AGG_SUM};
+    ORQG_CALL(orqg_et_aggregate(g.t, keys, 1, &sum, 1, 0, 1, 1, nullptr));
   } else {
     throw Error(ORQG_E_ARG, "request: unknown op");
   }
-  if (r.out_key) ORQG_CALL(orqg_vec_read(c, key, r.out_key));
-  if (r.out_val) ORQG_CALL(orqg_vec_read(c, val, r.out_val));
-  if (r.out_valid) ORQG_CALL(orqg_vec_read(c, valid, r.out_valid));
+  if (r.out_key) ORQG_CALL(orqg_vec_read(c, orqg_et_column(g.t, "[K]"), r.out_key));
+  if (r.out_val) ORQG_CALL(orqg_vec_read(c, orqg_et_column(g.t, "V"), r.out_val));
+  if (r.out_valid) ORQG_CALL(orqg_vec_read(c, orqg_et_column(g.t, "[##V]"), r.out_valid));
 }
 
 void worker(orqg_server *s, int slot) {

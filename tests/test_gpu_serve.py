@@ -85,10 +85,14 @@ def _table_flow(G, job):
     return [np.stack(x.shares()) for x in (t["[K]"], t["V"], t["[##V]"])]
 
 
-def test_server_matches_table_layer(G):
+@pytest.mark.parametrize("width", [4, 8])
+def test_server_matches_table_layer(G, width):
+    """Every request equals the table layer's EncodedTable::sort / aggregate
+    with their default options and protocol (quicksort on 32-bit tables,
+    radix sort on 64-bit ones)."""
     from orq_amd.serve import Server
 
-    jobs, plain = _make_jobs(G, [1024, 3000, 4096, 777], seed=1)
+    jobs, plain = _make_jobs(G, [1024, 3000, 4096, 777], width=width, seed=1)
     srv = Server(slots=2)
     for j in jobs:
         srv.submit(j)
