@@ -34,6 +34,12 @@ def lib():
         L.oc_destroy.argtypes = [vp]
         L.oc_prg_nonce.restype = u64
         L.oc_prg_nonce.argtypes = [vp, i32, i32]
+        L.oc_create_threads.restype = vp
+        L.oc_create_threads.argtypes = [ctypes.c_char_p, i32, ctypes.c_long, i32]
+        L.oc_prg_nonce_thread.restype = u64
+        L.oc_prg_nonce_thread.argtypes = [vp, i32, i32, i32]
+        L.oc_thread_bounds.restype = None
+        L.oc_thread_bounds.argtypes = [sz, ctypes.c_long, i32, vp]
         L.oc_keystream.restype = u64
         L.oc_keystream.argtypes = [ctypes.c_char_p, u64, vp, sz]
         L.oc_gen_perm.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), vp, sz, ctypes.POINTER(u64)]
@@ -119,6 +125,13 @@ def batches(n, batch_size=-12):
     return [int(x) for x in out[:k]]
 
 
+def thread_bounds(n, batch_size, threads):
+    """The threads' element ranges of one runtime call over n elements."""
+    out = np.zeros(2 * threads, dtype=np.uint64)
+    lib().oc_thread_bounds(n, batch_size, threads, _ptr(out))
+    return [(int(out[2 * t]), int(out[2 * t + 1])) for t in range(threads)]
+
+
 def _check(rc, what):
     if rc != 0:
         raise RuntimeError(f"oracle {what} failed")
@@ -157,17 +170,29 @@ class Vec:
 
 
 class Ctx:
-    def __init__(self, keys: bytes, batch_size=-12, gen_everyone=False):
+    """The three parties of one ORQ run.  threads > 1: that many worker
+    threads, thread 0 with `keys` and thread t with thread_keys[t - 1]."""
+
+    def __init__(self, keys: bytes, batch_size=-12, gen_everyone=False, threads=1,
+                 thread_keys=()):
         assert len(keys) == 320
-        self.h = lib().oc_create(keys, batch_size, int(gen_everyone))
+        self.threads = threads
+        if threads == 1:
+            self.h = lib().oc_create(keys, batch_size, int(gen_everyone))
+        else:
+            assert len(thread_keys) == threads - 1 and all(len(k) == 320 for k in thread_keys)
+            self.h = lib().oc_create_threads(keys + b"".join(thread_keys), threads, batch_size,
+                                             int(gen_everyone))
+            if not self.h:
+                raise RuntimeError("oracle oc_create_threads: thread count")
 
     def __del__(self):
         if getattr(self, "h", None):
             lib().oc_destroy(self.h)
             self.h = None
 
-    def nonce(self, party, role):
-        return lib().oc_prg_nonce(self.h, party, role)
+    def nonce(self, party, role, thread=0):
+        return lib().oc_prg_nonce_thread(self.h, thread, party, role)
 
     def secret_share(self, data, width, arith=False):
         a = np.ascontiguousarray(data, dtype=_DT[width])

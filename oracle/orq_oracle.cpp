@@ -10,9 +10,11 @@
 // (s_p, s_{p+1}) of every shared vector and its own copies of every PRG object,
 // exactly as one reference process would.  Messages are copies between party
 // structs and are logged with their byte counts.  Runtime batching follows the
-// reference's single-worker schedule (threads = 1, batch_size < 0 => |b| equal
-// batches of 256-multiples), because every batch makes its own PRG calls and
-// therefore owns its own nonces.
+// reference's schedule: a call over n elements is split into contiguous
+// thread ranges (T worker threads, each with its own PRG objects), and each
+// range into batches (batch_size < 0 => |b| equal batches of 256-multiples),
+// because every batch makes its own PRG calls and therefore owns its own
+// nonces.
 //
 // Parity pins (see tests/test_oracle.py):
 //   AES keystream ........ AES-GCM spec test case 14 + a second KAT; OpenSSL CTR
@@ -24,6 +26,7 @@
 #include <openssl/evp.h>
 
 #include <algorithm>
+#include <array>
 #include <cassert>
 #include <cstdint>
 #include <cstdio>
@@ -117,26 +120,55 @@ struct Prg {
 };
 
 // ------------------------------------------------------- batch schedule ---
-// RunTime::getThreadBatchBoundaries (include/backend/common/runtime.h:198-247)
-// with one worker gives the whole range to thread 0; Task splits it into
-// batches of compute_equal_batches(...) elements when batch_size < 0
+// RunTime::getThreadBatchBoundaries (include/backend/common/runtime.h:198-247):
+// whole batches of the call's batch size (256 elements when it is negative)
+// are dealt to the T threads in contiguous runs, the first threads taking one
+// extra when they do not divide evenly, and the last thread also takes the
+// partial batch at the end.
+std::vector<std::pair<size_t, size_t>> thread_bounds(size_t n, long batch_size, int threads) {
+  const size_t B = batch_size < 0 ? 256 : (size_t)batch_size;
+  const size_t whole = n / B, per = whole / (size_t)threads;
+  size_t extra = whole % (size_t)threads;
+  std::vector<std::pair<size_t, size_t>> out;
+  size_t start = 0;
+  for (int t = 0; t < threads; t++) {
+    size_t m = per * B;
+    if (extra > 0) {
+      m += B;
+      extra--;
+    }
+    out.push_back({start, start + m});
+    start += m;
+  }
+  out.back().second += n % B;
+  return out;
+}
+
+// Each thread's Task splits its range into batches of
+// compute_equal_batches(...) elements when batch_size < 0, else batch_size
 // (include/backend/common/task.h:34-44, 89-94).
 struct Batch {
   size_t s, e;
+  int t;  // the worker thread whose PRG objects the batch draws from
 };
 
-std::vector<Batch> batches_for(size_t n, long batch_size) {
+std::vector<Batch> batches_for(size_t n, long batch_size, int threads = 1) {
   std::vector<Batch> out;
   if (n == 0) return out;
-  size_t bs;
-  if (batch_size < 0) {
-    size_t nb = (size_t)(-batch_size);
-    size_t chunks = (n + 255) / 256;
-    bs = (chunks + nb - 1) / nb * 256;
-  } else {
-    bs = (size_t)batch_size;
+  const auto bounds = thread_bounds(n, batch_size, threads);
+  for (int t = 0; t < threads; t++) {
+    const size_t s0 = bounds[t].first, e0 = bounds[t].second;
+    if (e0 == s0) continue;
+    size_t bs;
+    if (batch_size < 0) {
+      size_t nb = (size_t)(-batch_size);
+      size_t chunks = (e0 - s0 + 255) / 256;
+      bs = (chunks + nb - 1) / nb * 256;
+    } else {
+      bs = (size_t)batch_size;
+    }
+    for (size_t i = s0; i < e0; i += bs) out.push_back({i, std::min(i + bs, e0), t});
   }
-  for (size_t i = 0; i < n; i += bs) out.push_back({i, std::min(i + bs, n)});
   return out;
 }
 
@@ -175,8 +207,13 @@ struct Party {
 //   keys[6..9]  group keys {0,1},{0,2},{1,2},{0,1,2}
 }  // namespace
 
+constexpr int kMaxThreads = 64;
+
 struct oc_ctx {
-  Party P[3];
+  // the parties' PRG objects of each worker thread (setup_random_generation
+  // per thread, include/backend/common/setup.h:48-57)
+  std::vector<std::array<Party, 3>> TP;
+  int threads = 1;
   long batch_size = -12;
   int gen_everyone = 0;  // also run gen_perm for the unused {0,1,2} group
   std::vector<Msg> log;
@@ -254,16 +291,16 @@ void zero_arith(Party &pt, T *out, size_t n) {
 // does (runtime.h:690-702).  z may alias x or y.
 template <typename T, bool kArith>
 void mult_batch(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z, size_t s, size_t e,
-                const char *op) {
+                const char *op, int t) {
   size_t nb = e - s;
   std::vector<T> local[3];
   PARTIES_PARALLEL
   for (int p = 0; p < 3; p++) {
     local[p].resize(nb);
     if (kArith)
-      zero_arith<T>(c->P[p], local[p].data(), nb);
+      zero_arith<T>(c->TP[t][p], local[p].data(), nb);
     else
-      zero_binary<T>(c->P[p], local[p].data(), nb);
+      zero_binary<T>(c->TP[t][p], local[p].data(), nb);
     const T *x0 = S<T>(x, p, 0) + s, *x1 = S<T>(x, p, 1) + s;
     const T *y0 = S<T>(y, p, 0) + s, *y1 = S<T>(y, p, 1) + s;
     for (size_t i = 0; i < nb; i++) {
@@ -283,12 +320,13 @@ void mult_batch(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z, size_t s, size_t e,
 
 template <typename T, bool kArith>
 void mult(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *z, const char *op) {
-  for (auto b : batches_for(x->n, c->batch_size)) mult_batch<T, kArith>(c, x, y, z, b.s, b.e, op);
+  for (auto b : batches_for(x->n, c->batch_size, c->threads))
+    mult_batch<T, kArith>(c, x, y, z, b.s, b.e, op, b.t);
 }
 
 // Replicated_3PC::b2a_bit (replicated_3pc.h:220-251), per batch.
 template <typename T>
-void b2a_bit_batch(oc_ctx *c, oc_vec *x, oc_vec *y, size_t s, size_t e) {
+void b2a_bit_batch(oc_ctx *c, oc_vec *x, oc_vec *y, size_t s, size_t e, int t) {
   size_t nb = e - s;
   oc_vec *s0 = new_vec(nb, sizeof(T)), *s1 = new_vec(nb, sizeof(T));
   std::vector<T> send0[3], send1[3];
@@ -302,8 +340,8 @@ void b2a_bit_batch(oc_ctx *c, oc_vec *x, oc_vec *y, size_t s, size_t e) {
       lsb1[i] = x1[i] & 1;
       if (p == 0) lsb1[i] ^= lsb0[i];
     }
-    zero_arith<T>(c->P[p], a0, nb);
-    zero_arith<T>(c->P[p], b0, nb);
+    zero_arith<T>(c->TP[t][p], a0, nb);
+    zero_arith<T>(c->TP[t][p], b0, nb);
     for (size_t i = 0; i < nb; i++) {
       if (p == 0) a0[i] += lsb1[i];
       if (p == 1) b0[i] += lsb1[i];
@@ -324,7 +362,7 @@ void b2a_bit_batch(oc_ctx *c, oc_vec *x, oc_vec *y, size_t s, size_t e) {
     }
   // y = s0 * s0 (multiply_a on the same batch)
   oc_vec *out = new_vec(nb, sizeof(T));
-  mult_batch<T, true>(c, s0, s0, out, 0, nb, "b2a_bit");
+  mult_batch<T, true>(c, s0, s0, out, 0, nb, "b2a_bit", t);
   for (int p = 0; p < 3; p++)
     for (int k = 0; k < 2; k++)
       std::memcpy(S<T>(y, p, k) + s, S<T>(out, p, k), nb * sizeof(T));
@@ -335,14 +373,14 @@ void b2a_bit_batch(oc_ctx *c, oc_vec *x, oc_vec *y, size_t s, size_t e) {
 
 template <typename T>
 void b2a_bit(oc_ctx *c, oc_vec *x, oc_vec *y) {
-  for (auto b : batches_for(x->n, c->batch_size)) b2a_bit_batch<T>(c, x, y, b.s, b.e);
+  for (auto b : batches_for(x->n, c->batch_size, c->threads)) b2a_bit_batch<T>(c, x, y, b.s, b.e, b.t);
 }
 
 // open_shares_a / open_shares_b (replicated_3pc.h:310-330)
 template <typename T, bool kArith>
 std::vector<T> open_vec(oc_ctx *c, oc_vec *x) {
   std::vector<T> out(x->n);
-  for (auto b : batches_for(x->n, c->batch_size)) {
+  for (auto b : batches_for(x->n, c->batch_size, c->threads)) {
     for (int p = 0; p < 3; p++) log_msg(c, "open", p, (p + 2) % 3, (b.e - b.s) * sizeof(T));
   }
   // every party reconstructs the same value; compute party 0's view and
@@ -373,11 +411,11 @@ void public_share(oc_vec *v, const T *x) {
 // batched by eval_protocol_1arg_alloc (runtime.h:584-603).
 template <typename T, bool kArith>
 void secret_share(oc_ctx *c, const T *data, oc_vec *v, int owner = 0) {
-  for (auto b : batches_for(v->n, c->batch_size)) {
+  for (auto b : batches_for(v->n, c->batch_size, c->threads)) {
     size_t nb = b.e - b.s;
     std::vector<T> s1(nb), s2(nb), s3(nb);
-    c->P[owner].local.getNext(s1.data(), nb);
-    c->P[owner].local.getNext(s2.data(), nb);
+    c->TP[b.t][owner].local.getNext(s1.data(), nb);
+    c->TP[b.t][owner].local.getNext(s2.data(), nb);
     for (size_t i = 0; i < nb; i++)
       s3[i] = kArith ? (T)(data[b.s + i] - s1[i] - s2[i]) : (T)(data[b.s + i] ^ s1[i] ^ s2[i]);
     // the owner keeps (s1, s2), its successor gets (s2, s3), its predecessor (s3, s1)
@@ -400,7 +438,7 @@ void reshare(oc_ctx *c, oc_vec *v, int g) {
   int outsider = -1;
   for (int p = 0; p < 3; p++)
     if (!in_group(g, p)) outsider = p;
-  for (auto b : batches_for(v->n, c->batch_size)) {
+  for (auto b : batches_for(v->n, c->batch_size, c->threads)) {
     size_t nb = b.e - b.s;
     PARTIES_PARALLEL
     for (int p = 0; p < 3; p++) {
@@ -408,7 +446,7 @@ void reshare(oc_ctx *c, oc_vec *v, int g) {
       std::vector<T> r[3];
       for (int k = 0; k < 3; k++) r[k].assign(nb, 0);
       for (int k = 0; k < 2; k++) {
-        c->P[p].group[g].getNext(r[k].data(), nb);
+        c->TP[b.t][p].group[g].getNext(r[k].data(), nb);
         for (size_t i = 0; i < nb; i++) {
           if (kArith)
             r[2][i] -= r[k][i];
@@ -461,16 +499,26 @@ void gen_perm(std::vector<int32_t> &perm, Prg &g, uint64_t *draws_out) {
   if (draws_out) *draws_out = draws;
 }
 
-// PermutationManager::reserve -> RunTime::generate_permutations ->
-// HMShardedPermutationGenerator::generateBatch (hm_sharded_permutation_generator.h:178-194):
-// for each permutation, for each group containing the party (in group order),
-// gen_perm from that group's PRG.
+// PermutationManager::reserve -> RunTime::generate_permutations
+// (runtime.h:749-788) -> HMShardedPermutationGenerator::generateBatch
+// (hm_sharded_permutation_generator.h:178-194): the new permutations are
+// dealt to the threads in contiguous runs (count / T each, the first
+// count % T threads one more), queue order thread by thread; thread t makes
+// each of its permutations from its own PRG objects, for each group
+// containing the party (in group order), with gen_perm.
 void reserve(oc_ctx *c, size_t n, size_t count) {
   if (!c->queue.empty() && c->stored_size != n) c->queue.clear();
   c->stored_size = n;
   if (count <= c->queue.size()) return;
   count -= c->queue.size();
   std::vector<oc_ctx::Sharded> fresh(count);
+  std::vector<int> owner(count);
+  {
+    const size_t T = (size_t)c->threads, base = count / T, rem = count % T;
+    size_t k = 0;
+    for (size_t t = 0; t < T; t++)
+      for (size_t j = 0; j < base + (t < rem ? 1 : 0); j++) owner[k++] = (int)t;
+  }
   // each party generates its groups' permutations in queue order from its
   // own group PRGs
   PARTIES_PARALLEL
@@ -482,7 +530,7 @@ void reserve(oc_ctx *c, size_t n, size_t count) {
         if (!in_group(g, p)) continue;
         if (g == 3 && !c->gen_everyone) continue;
         sp.perm[p][g].resize(n);
-        gen_perm(sp.perm[p][g], c->P[p].group[g], nullptr);
+        gen_perm(sp.perm[p][g], c->TP[owner[k]][p].group[g], nullptr);
       }
     }
   for (auto &sp : fresh) c->queue.push_back(std::move(sp));
@@ -1782,26 +1830,48 @@ static void prefix_sum_op(oc_vec *v) {
 
 extern "C" {
 
-oc_ctx *oc_create(const uint8_t *keys /* 10 x 32 bytes */, long batch_size, int gen_everyone) {
+// `threads` worker threads, thread t with the ten keys at keys + 320 t
+oc_ctx *oc_create_threads(const uint8_t *keys /* threads x 10 x 32 bytes */, int threads,
+                          long batch_size, int gen_everyone) {
+  if (threads < 1 || threads > kMaxThreads) return nullptr;
   oc_ctx *c = new oc_ctx;
   c->batch_size = batch_size;
   c->gen_everyone = gen_everyone;
-  for (int p = 0; p < 3; p++) {
-    Party &pt = c->P[p];
-    pt.id = p;
-    pt.local = Prg(keys + 32 * p);
-    pt.next = Prg(keys + 32 * (3 + p));
-    pt.prev = Prg(keys + 32 * (3 + (p + 2) % 3));
-    for (int g = 0; g < 4; g++) pt.group[g] = Prg(keys + 32 * (6 + g));
+  c->threads = threads;
+  c->TP.resize(threads);
+  for (int t = 0; t < threads; t++) {
+    const uint8_t *k = keys + 320 * (size_t)t;
+    for (int p = 0; p < 3; p++) {
+      Party &pt = c->TP[t][p];
+      pt.id = p;
+      pt.local = Prg(k + 32 * p);
+      pt.next = Prg(k + 32 * (3 + p));
+      pt.prev = Prg(k + 32 * (3 + (p + 2) % 3));
+      for (int g = 0; g < 4; g++) pt.group[g] = Prg(k + 32 * (6 + g));
+    }
   }
   return c;
 }
 
+oc_ctx *oc_create(const uint8_t *keys /* 10 x 32 bytes */, long batch_size, int gen_everyone) {
+  return oc_create_threads(keys, 1, batch_size, gen_everyone);
+}
+
 void oc_destroy(oc_ctx *c) { delete c; }
 
-// nonce of one PRG object: role 0 local, 1 prev, 2 next, 3+g group g
+// nonce of one PRG object of worker thread t: role 0 local, 1 prev, 2 next,
+// 3+g group g
+uint64_t oc_prg_nonce_thread(oc_ctx *c, int t, int party, int role) {
+  Party &pt = c->TP[t][party];
+  if (role == 0) return pt.local.nonce;
+  if (role == 1) return pt.prev.nonce;
+  if (role == 2) return pt.next.nonce;
+  return pt.group[role - 3].nonce;
+}
+
+// nonce of one PRG object of thread 0
 uint64_t oc_prg_nonce(oc_ctx *c, int party, int role) {
-  Party &pt = c->P[party];
+  Party &pt = c->TP[0][party];
   if (role == 0) return pt.local.nonce;
   if (role == 1) return pt.prev.nonce;
   if (role == 2) return pt.next.nonce;
@@ -2034,6 +2104,15 @@ int oc_radix_sort(oc_ctx *c, oc_vec *v, int bits, int desc, oc_vec *perm_out) {
     else
       throw std::runtime_error("radix_sort: width");
   })
+}
+
+// thread ranges of one runtime call over n elements: out[2t], out[2t+1]
+void oc_thread_bounds(size_t n, long batch_size, int threads, uint64_t *out) {
+  auto b = thread_bounds(n, batch_size, threads);
+  for (int t = 0; t < threads; t++) {
+    out[2 * t] = b[t].first;
+    out[2 * t + 1] = b[t].second;
+  }
 }
 
 // batch sizes of one runtime call over n elements (returns the batch count)
