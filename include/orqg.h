@@ -50,9 +50,19 @@ enum { ORQG_EQ = 0, ORQG_NE = 1, ORQG_LT = 2, ORQG_GT = 3, ORQG_LE = 4, ORQG_GE 
 
 typedef struct {
   int device;                        /* HIP device ordinal */
-  uint8_t keys[ORQG_NUM_KEYS][32];   /* see roles above */
+  uint8_t keys[ORQG_NUM_KEYS][32];   /* see roles above (worker thread 0) */
   long batch_size;                   /* runtime batch size; <0 = |b| equal batches (ref default -12) */
   size_t hbm_budget;                 /* bytes; 0 = no limit */
+  /* worker threads of the reference's RunTime (0 or 1: one).  With T > 1, a
+   * call over n elements is split into the threads' ranges
+   * (runtime.h:198-247), each batched on its own and drawing from its own
+   * thread's keys, and reserved permutations are dealt to the threads in
+   * runs (runtime.h:749-788).  Supported on the radix-sort path: secret
+   * sharing, open, and_b / multiply_a, b2a_bit, reshare, sharded and
+   * elementwise permutations and their ElementwisePermutation ops, radix and
+   * table sort (RADIXSORT); other operations return ORQG_E_ARG.            */
+  int threads;
+  const uint8_t *thread_keys;        /* T > 1: (T-1) x ORQG_NUM_KEYS x 32 bytes, threads 1.. */
 } orqg_config;
 
 typedef struct orqg_ctx orqg_ctx;
@@ -74,6 +84,8 @@ void *orqg_stream(orqg_ctx *ctx);
 /* current nonce of PRG `role` (0..9, see above): AESPRGAlgorithm::nonce
  * (include/core/random/prg/prg_algorithm.h:200-207)                         */
 int orqg_prg_nonce(orqg_ctx *ctx, int role, uint64_t *nonce);
+/* the nonce of key `role` of worker thread `thread` */
+int orqg_prg_nonce_thread(orqg_ctx *ctx, int thread, int role, uint64_t *nonce);
 
 /* ---- raw device buffers (plumbing for FFI callers and tests) ---------- */
 int orqg_dev_alloc(orqg_ctx *ctx, size_t bytes, void **dev_out);

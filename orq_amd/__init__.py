@@ -33,6 +33,8 @@ class _Config(ctypes.Structure):
         ("keys", (ctypes.c_uint8 * 32) * 10),
         ("batch_size", ctypes.c_long),
         ("hbm_budget", ctypes.c_size_t),
+        ("threads", ctypes.c_int),
+        ("thread_keys", ctypes.c_char_p),
     ]
 
 
@@ -43,7 +45,7 @@ class _Msg(ctypes.Structure):
 
 EXPORTS = [
     "orqg_ctx_create", "orqg_ctx_destroy", "orqg_last_error", "orqg_sync", "orqg_stream",
-    "orqg_prg_nonce", "orqg_dev_alloc", "orqg_dev_free", "orqg_dev_to_host", "orqg_vec_create", "orqg_vec_free", "orqg_vec_size", "orqg_vec_width",
+    "orqg_prg_nonce", "orqg_prg_nonce_thread", "orqg_dev_alloc", "orqg_dev_free", "orqg_dev_to_host", "orqg_vec_create", "orqg_vec_free", "orqg_vec_size", "orqg_vec_width",
     "orqg_vec_data", "orqg_vec_write", "orqg_vec_read", "orqg_vec_copy", "orqg_prg_keystream",
     "orqg_secret_share", "orqg_open", "orqg_public_share", "orqg_xor_b", "orqg_add_a",
     "orqg_sub_a", "orqg_not_b", "orqg_and_b", "orqg_mul_a", "orqg_b2a_bit", "orqg_reshare",
@@ -86,6 +88,7 @@ def lib():
             "orqg_sync": (i32, [vp]),
             "orqg_stream": (vp, [vp]),
             "orqg_prg_nonce": (i32, [vp, i32, ctypes.POINTER(u64)]),
+            "orqg_prg_nonce_thread": (i32, [vp, i32, i32, ctypes.POINTER(u64)]),
             "orqg_dev_alloc": (i32, [vp, sz, ctypes.POINTER(vp)]),
             "orqg_dev_free": (i32, [vp]),
             "orqg_dev_to_host": (i32, [vp, vp, vp, sz]),
@@ -251,12 +254,23 @@ class Vec:
 
 
 class Ctx:
-    def __init__(self, keys: bytes, batch_size=-12, device=0, hbm_budget=0):
+    """The three parties on one GPU.  threads > 1: that many worker threads of
+    the reference's RunTime, thread 0 with `keys` and thread t with
+    thread_keys[t - 1] (320 bytes each)."""
+
+    def __init__(self, keys: bytes, batch_size=-12, device=0, hbm_budget=0, threads=1,
+                 thread_keys=()):
         cfg = _Config()
         cfg.device = device
         ctypes.memmove(ctypes.addressof(cfg.keys), keys, 320)
         cfg.batch_size = batch_size
         cfg.hbm_budget = hbm_budget
+        cfg.threads = threads
+        if threads > 1:
+            if len(thread_keys) != threads - 1 or any(len(k) != 320 for k in thread_keys):
+                raise OrqgError("Ctx: threads - 1 key sets of 320 bytes are needed")
+            self._thread_keys = b"".join(thread_keys)
+            cfg.thread_keys = self._thread_keys
         h = ctypes.c_void_p()
         rc = lib().orqg_ctx_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -287,9 +301,10 @@ class Ctx:
     def sync(self):
         self._check(lib().orqg_sync(self.h), "sync")
 
-    def nonce(self, role):
+    def nonce(self, role, thread=0):
         v = ctypes.c_uint64()
-        self._check(lib().orqg_prg_nonce(self.h, role, ctypes.byref(v)), "prg_nonce")
+        self._check(lib().orqg_prg_nonce_thread(self.h, thread, role, ctypes.byref(v)),
+                    "prg_nonce")
         return v.value
 
     def vec(self, n, width, encoding=ORQG_BSHARED):

@@ -251,6 +251,7 @@ __global__ void __launch_bounds__(kAesThreads)
 
 // op: 0 RCA add, 1 RCA subtract, 2 PPA add, 3 PPA subtract, 4 rca_compare
 void adder(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
+  require_one_thread(c, "adder");
   const uint64_t n = x->n;
   if (n == 0) return;
   const int w = x->width, B = 8 * w;
@@ -295,6 +296,7 @@ void adder(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *
 }
 
 void a2b(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
+  require_one_thread(c, "a2b");
   const uint64_t n = x->n;
   if (n == 0) return;
   const int w = x->width;

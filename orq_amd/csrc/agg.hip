@@ -532,6 +532,7 @@ void distinct_t(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res, T 
 
 void aggregate(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const orqg_agg_spec *bs, size_t nbs,
                const orqg_agg_spec *as, size_t nas, int reverse, const orqg_vec *sel) {
+  require_one_thread(c, "aggregate");
   const orqg_vec *first = nk ? keys[0] : nbs ? bs[0].in : nas ? as[0].in : nullptr;
   if (!first) throw Error(ORQG_E_ARG, "Empty aggregation!");
   const uint64_t n = first->n;
@@ -563,6 +564,7 @@ void aggregate(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const orqg_agg_spe
 }
 
 void distinct(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res) {
+  require_one_thread(c, "distinct");
   for (size_t k = 0; k < nk; k++)
     if (!keys[k] || keys[k]->n != res->n || keys[k]->width != res->width)
       throw Error(ORQG_E_ARG, "distinct: columns differ in size or width");
@@ -598,6 +600,7 @@ void noalloc_setup(int device) {
 }
 
 void distinct_noalloc(orqg_ctx *c, orqg_vec *const *keys, size_t nk, orqg_vec *res) {
+  require_one_thread(c, "distinct_noalloc");
   for (size_t k = 0; k < nk; k++)
     if (!keys[k] || keys[k]->n != res->n || keys[k]->width != res->width)
       throw Error(ORQG_E_ARG, "distinct: columns differ in size or width");

@@ -273,17 +273,34 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
     ORQG_HIP(hipGetDeviceProperties(&prop, c->device));
     c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     aes_make_tables(&c->tabs);
-    uint32_t rk[ORQG_NUM_KEYS * 60];
-    for (int k = 0; k < ORQG_NUM_KEYS; k++) aes256_expand(c->tabs, c->keys[k], rk + 60 * k);
-    c->d_rk = (uint32_t *)dev_alloc(c, sizeof(rk));
+    // the round keys of every worker thread: thread 0's keys, then those of
+    // threads 1.. from thread_keys
+    const int T = cfg->threads > 1 ? cfg->threads : 1;
+    if (T > 64) throw Error(ORQG_E_ARG, "ctx_create: at most 64 worker threads");
+    if (T > 1 && !cfg->thread_keys) throw Error(ORQG_E_ARG, "ctx_create: thread_keys missing");
+    c->threads = T;
+    c->tnonce.assign(T, std::array<uint64_t, ORQG_NUM_KEYS>{});
+    std::vector<uint32_t> rk((size_t)T * ORQG_NUM_KEYS * 60);
+    for (int t = 0; t < T; t++)
+      for (int k = 0; k < ORQG_NUM_KEYS; k++)
+        aes256_expand(c->tabs,
+                      t == 0 ? c->keys[k] : cfg->thread_keys + ((size_t)(t - 1) * ORQG_NUM_KEYS + k) * 32,
+                      rk.data() + ((size_t)t * ORQG_NUM_KEYS + k) * 60);
+    c->d_rk_all = (uint32_t *)dev_alloc(c, rk.size() * 4);
+    c->d_rk = c->d_rk_all;
     c->d_t0 = (uint32_t *)dev_alloc(c, sizeof(c->tabs.t0));  // T0; kernels replicate it in LDS
     // copies on the context's own stream, never the legacy null stream: a
     // null-stream operation would be ordered against every other context's
     // streams (and is refused while any of them is capturing a graph)
-    ORQG_HIP(hipMemcpyAsync(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice, c->stream));
+    ORQG_HIP(hipMemcpyAsync(c->d_rk_all, rk.data(), rk.size() * 4, hipMemcpyHostToDevice, c->stream));
     ORQG_HIP(hipMemcpyAsync(c->d_t0, c->tabs.t0, sizeof(c->tabs.t0), hipMemcpyHostToDevice,
                             c->stream));
     ORQG_HIP(hipStreamSynchronize(c->stream));
+  } catch (const Error &e) {
+    std::fprintf(stderr, "orqg_ctx_create: %s\n", e.what());
+    unregister_ctx(c);
+    delete c;
+    return e.code;
   } catch (const std::exception &e) {
     std::fprintf(stderr, "orqg_ctx_create: %s\n", e.what());
     unregister_ctx(c);
@@ -300,7 +317,7 @@ int orqg_ctx_destroy(orqg_ctx *c) {
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto &kv : c->scratch) dev_free(kv.second.p);
   for (auto &p : c->queue) free_perm(p);
-  dev_free(c->d_rk);
+  dev_free(c->d_rk_all);
   dev_free(c->d_t0);
   unregister_ctx(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -317,13 +334,14 @@ int orqg_ctx_destroy(orqg_ctx *c) {
 int orqg_ctx_rekey(orqg_ctx *c, const uint8_t keys[ORQG_NUM_KEYS][32]) {
   API_BEGIN(c)
   if (!keys) throw Error(ORQG_E_ARG, "ctx_rekey: NULL keys");
+  require_one_thread(c, "ctx_rekey");
   ORQG_HIP(hipStreamSynchronize(c->stream));
   check_swaps(c);
   ORQG_HIP(hipStreamSynchronize(c->stream2));
   std::memcpy(c->keys, keys, sizeof(c->keys));
   uint32_t rk[ORQG_NUM_KEYS * 60];
   for (int k = 0; k < ORQG_NUM_KEYS; k++) aes256_expand(c->tabs, c->keys[k], rk + 60 * k);
-  ORQG_HIP(hipMemcpyAsync(c->d_rk, rk, sizeof(rk), hipMemcpyHostToDevice, c->stream));
+  ORQG_HIP(hipMemcpyAsync(c->d_rk_all, rk, sizeof(rk), hipMemcpyHostToDevice, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));
   for (int k = 0; k < ORQG_NUM_KEYS; k++) c->nonce[k] = 0;
   for (auto &p : c->queue) free_perm(p);
@@ -350,6 +368,13 @@ void *orqg_stream(orqg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 int orqg_prg_nonce(orqg_ctx *c, int role, uint64_t *nonce) {
   if (!c || role < 0 || role >= ORQG_NUM_KEYS || !nonce) return ORQG_E_ARG;
   *nonce = c->nonce[role];
+  return ORQG_OK;
+}
+
+int orqg_prg_nonce_thread(orqg_ctx *c, int thread, int role, uint64_t *nonce) {
+  if (!c || thread < 0 || thread >= c->threads || role < 0 || role >= ORQG_NUM_KEYS || !nonce)
+    return ORQG_E_ARG;
+  *nonce = thread == c->cur_thread ? c->nonce[role] : c->tnonce[thread][role];
   return ORQG_OK;
 }
 

@@ -264,6 +264,7 @@ __global__ void __launch_bounds__(kAesThreads)
 // B-shared data columns; n must be a power of two.
 void bitonic_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nk, const int *orders,
                   orqg_vec *const *a_cols, size_t na, orqg_vec *const *b_cols, size_t nb) {
+  require_one_thread(c, "bitonic_sort");
   if (nk == 0) throw Error(ORQG_E_ARG, "bitonic_sort: no sort keys");
   if (nk > (size_t)kMaxCols || na > (size_t)kMaxCols || nb > (size_t)kMaxCols)
     throw Error(ORQG_E_ARG, "bitonic_sort: at most 8 key, 8 A-shared and 8 B-shared columns");

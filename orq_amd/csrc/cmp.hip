@@ -50,6 +50,7 @@ __global__ void __launch_bounds__(kAesThreads)
 
 void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, orqg_vec *gt,
              bool full, int neg_eq, int neg_gt) {
+  require_one_thread(c, "compare");
   const uint64_t n = x->n;
   if (n == 0) return;
   const int B = 8 * x->width;
@@ -139,6 +140,7 @@ __global__ void __launch_bounds__(kAesThreads)
 }
 
 void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
+  require_one_thread(c, "b2a");
   const uint64_t n = x->n;
   if (n == 0) return;
   const int B = 8 * x->width;

@@ -1319,6 +1319,21 @@ void free_perm(ShardedPerm &p) {
 
 namespace {
 
+// RunTime::generate_permutations (runtime.h:749-788): the new permutations
+// dealt to the worker threads in contiguous runs, count / T each and one more
+// for the first count % T threads.  Returns (first, count) per thread.
+std::vector<std::pair<uint32_t, uint32_t>> thread_runs(const orqg_ctx *c, uint32_t count) {
+  const uint32_t T = (uint32_t)c->threads, base = count / T, rem = count % T;
+  std::vector<std::pair<uint32_t, uint32_t>> out;
+  uint32_t k = 0;
+  for (uint32_t t = 0; t < T; t++) {
+    const uint32_t m = base + (t < rem ? 1 : 0);
+    out.push_back({k, m});
+    k += m;
+  }
+  return out;
+}
+
 // One context's share of a reserve: the permutations it still needs.
 struct Reserve {
   orqg_ctx *c = nullptr;
@@ -1360,8 +1375,14 @@ bool reserve_begin(orqg_ctx *c, size_t n_, size_t count_, Reserve &R) {
     }
   }
   if (n == 0) {
-    // gen_perm still fills its buffer once: 4 nonces per local permutation
-    for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] += 4ull * count;
+    // gen_perm still fills its buffer once: 4 nonces per local permutation,
+    // on the group PRGs of the thread that makes it
+    const auto runs = thread_runs(c, count);
+    for (int t = 0; t < c->threads; t++) {
+      if (!runs[t].second) continue;
+      ThreadScope ts(c, t);
+      for (int g = 0; g < 3; g++) c->nonce[kGroup0 + g] += 4ull * runs[t].second;
+    }
     for (auto &p : R.fresh) c->queue.push_back(p);
     return false;
   }
@@ -1735,10 +1756,35 @@ void gen_small(orqg_ctx *lead, const std::vector<Reserve *> &rs) {
 void reserve_perms(orqg_ctx *c, size_t n, size_t count) {
   Reserve R;
   if (!reserve_begin(c, n, count, R)) return;
-  if (R.n <= small_max())
-    gen_small(c, {&R});
-  else
-    gen_windows(c, R);
+  if (c->threads <= 1) {
+    if (R.n <= small_max())
+      gen_small(c, {&R});
+    else
+      gen_windows(c, R);
+    reserve_end(R);
+    return;
+  }
+  // each worker thread makes its run of the permutations from its own group
+  // keys, in queue order
+  const auto runs = thread_runs(c, R.count);
+  for (int t = 0; t < c->threads; t++) {
+    if (!runs[t].second) continue;
+    check_swaps(c);  // the previous run's swap resolution (its scratch is reused)
+    Reserve Rt;
+    Rt.c = c;
+    Rt.n = R.n;
+    Rt.count = runs[t].second;
+    Rt.fresh.assign(R.fresh.begin() + runs[t].first,
+                    R.fresh.begin() + runs[t].first + runs[t].second);
+    {
+      ThreadScope ts(c, t);
+      if (Rt.n <= small_max())
+        gen_small(c, {&Rt});
+      else
+        gen_windows(c, Rt);
+    }
+    std::copy(Rt.fresh.begin(), Rt.fresh.end(), R.fresh.begin() + runs[t].first);
+  }
   reserve_end(R);
 }
 
@@ -1746,6 +1792,7 @@ void reserve_perms(orqg_ctx *c, size_t n, size_t count) {
 // share one launch sequence on the first one's stream.
 void reserve_perms_multi(const std::vector<orqg_ctx *> &cs, const std::vector<size_t> &ns,
                          const std::vector<size_t> &counts) {
+  for (orqg_ctx *c : cs) require_one_thread(c, "batched reserve");
   std::vector<Reserve> R(cs.size());
   std::vector<Reserve *> small;
   for (size_t q = 0; q < cs.size(); q++) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <condition_variable>
 #include <deque>
 #include <exception>
@@ -138,7 +139,15 @@ struct orqg_ctx {
   uint64_t nonce[ORQG_NUM_KEYS] = {0};
   uint8_t keys[ORQG_NUM_KEYS][32];
   orqg::AesTables tabs;
-  uint32_t *d_rk = nullptr;  // ORQG_NUM_KEYS x 60 round-key words
+  uint32_t *d_rk = nullptr;  // ORQG_NUM_KEYS x 60 round-key words of the current worker thread
+  // worker threads (the reference's RunTime threads): the round keys of all
+  // of them, and the nonces of those not current (`nonce` holds the current
+  // thread's; outside a thread scope that is thread 0)
+  int threads = 1;
+  uint32_t *d_rk_all = nullptr;
+  std::vector<std::array<uint64_t, ORQG_NUM_KEYS>> tnonce;
+  int cur_thread = 0;
+  int scope_depth = 0;
   uint32_t *d_t0 = nullptr;  // 256 words
   std::string err;
   std::vector<orqg_msg> log;
@@ -180,6 +189,14 @@ struct orqg_vec {
   void *d = nullptr;  // [n][3] elements of `width` bytes
 };
 
+// rows [off, off + m) of v, as a vector of its own
+inline orqg_vec vec_rows(const orqg_vec *v, uint64_t off, uint64_t m) {
+  orqg_vec s = *v;
+  s.n = m;
+  s.d = (char *)v->d + off * 3 * (uint64_t)v->width;
+  return s;
+}
+
 namespace orqg {
 
 // Allocation that counts against the HBM budget (a caching allocator per context).
@@ -212,6 +229,82 @@ struct TimedRegion {
     }
   }
 };
+
+// ---- worker threads ------------------------------------------------------------
+// RunTime::getThreadBatchBoundaries (include/backend/common/runtime.h:198-247):
+// whole batches of the batch size (256 elements when it is negative) dealt to
+// the threads in contiguous runs, the first threads one extra when they do not
+// divide evenly, the last thread also the partial batch at the end.  Each
+// thread's Task then batches its own range (task.h:34-44,89-94), which is
+// make_sched on the range's length.
+inline std::vector<std::pair<uint64_t, uint64_t>> thread_bounds(uint64_t n, long batch_size,
+                                                                int threads) {
+  const uint64_t B = batch_size < 0 ? 256 : (uint64_t)batch_size;
+  const uint64_t whole = n / B, per = whole / (uint64_t)threads;
+  uint64_t extra = whole % (uint64_t)threads, start = 0;
+  std::vector<std::pair<uint64_t, uint64_t>> out;
+  for (int t = 0; t < threads; t++) {
+    uint64_t m = per * B;
+    if (extra > 0) {
+      m += B;
+      extra--;
+    }
+    out.push_back({start, start + m});
+    start += m;
+  }
+  out.back().second += n % B;
+  return out;
+}
+
+// Makes worker thread t current: its nonces in c->nonce, its round keys at c->d_rk.
+struct ThreadScope {
+  orqg_ctx *c;
+  int prev;
+  ThreadScope(orqg_ctx *c_, int t) : c(c_), prev(c_->cur_thread) {
+    c->scope_depth++;
+    if (t == prev) return;
+    for (int k = 0; k < ORQG_NUM_KEYS; k++) c->tnonce[prev][k] = c->nonce[k];
+    for (int k = 0; k < ORQG_NUM_KEYS; k++) c->nonce[k] = c->tnonce[t][k];
+    c->d_rk = c->d_rk_all + (size_t)t * ORQG_NUM_KEYS * 60;
+    c->cur_thread = t;
+  }
+  ~ThreadScope() {
+    c->scope_depth--;
+    const int t = c->cur_thread;
+    if (t == prev) return;
+    for (int k = 0; k < ORQG_NUM_KEYS; k++) c->tnonce[t][k] = c->nonce[k];
+    for (int k = 0; k < ORQG_NUM_KEYS; k++) c->nonce[k] = c->tnonce[prev][k];
+    c->d_rk = c->d_rk_all + (size_t)prev * ORQG_NUM_KEYS * 60;
+    c->cur_thread = prev;
+  }
+};
+
+// f(offset, count) once per worker thread with a non-empty range of a runtime
+// call over n elements, with that thread current (one call, f(0, n), for a
+// single thread)
+template <class F>
+void for_threads(orqg_ctx *c, uint64_t n, F &&f) {
+  if (n == 0) return;
+  if (c->threads <= 1) {
+    f((uint64_t)0, n);
+    return;
+  }
+  if (c->scope_depth) throw Error(ORQG_E_PARITY, "internal: nested worker-thread ranges");
+  const auto b = thread_bounds(n, c->batch_size, c->threads);
+  for (int t = 0; t < c->threads; t++) {
+    if (b[t].second == b[t].first) continue;
+    ThreadScope ts(c, t);
+    f(b[t].first, b[t].second - b[t].first);
+  }
+}
+
+inline void require_one_thread(const orqg_ctx *c, const char *what) {
+  if (c->threads > 1)
+    throw Error(ORQG_E_ARG, std::string(what) +
+                                ": more than one worker thread is supported on the radix-sort "
+                                "path only (secret sharing, and_b, multiply_a, b2a_bit, reshare, "
+                                "permutations, radix and table sort)");
+}
 
 inline unsigned grid_for(uint64_t items, unsigned block, unsigned cap) {
   uint64_t g = (items + block - 1) / block;

@@ -41,16 +41,22 @@ __global__ void __launch_bounds__(kAesThreads) k_round(const uint32_t *__restric
   }
 }
 
+// The reshare after local_apply_perm is a runtime call over the permuted
+// vector: each worker thread masks its range of destination rows with its own
+// group keys.
 template <typename T>
 void launch_round(orqg_ctx *c, bool arith, int g, const int32_t *idx, const T *in, T *out,
                   uint64_t n) {
   int role = kGroup0 + g;
-  CallSched s = make_sched(n, c->batch_size, sizeof(T), 2);
-  unsigned grid = grid_for((n * sizeof(T) + 15) / 16, kAesThreads, c->num_cu);
-  hipLaunchKernelGGL((arith ? k_round<T, true> : k_round<T, false>), dim3(grid), dim3(kAesThreads), 0,
-                     c->stream, c->d_t0, c->rk(role), c->nonce[role], s, idx, in, out);
-  ORQG_HIP(hipGetLastError());
-  c->nonce[role] += sched_nonces(s);
+  for_threads(c, n, [&](uint64_t off, uint64_t m) {
+    CallSched s = make_sched(m, c->batch_size, sizeof(T), 2);
+    unsigned grid = grid_for((m * sizeof(T) + 15) / 16, kAesThreads, c->num_cu);
+    hipLaunchKernelGGL((arith ? k_round<T, true> : k_round<T, false>), dim3(grid), dim3(kAesThreads),
+                       0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s, idx + off, in,
+                       out + 3 * off);
+    ORQG_HIP(hipGetLastError());
+    c->nonce[role] += sched_nonces(s);
+  });
   log_reshare(c, g, n * sizeof(T));
 }
 template void launch_round<uint32_t>(orqg_ctx *, bool, int, const int32_t *, const uint32_t *,

@@ -91,21 +91,26 @@ void secret_share(orqg_ctx *c, const void *host, orqg_vec *v, int owner) {
   if (v->n == 0) return;
   void *tmp = c->scratch_get("plain_in", bytes);
   ORQG_HIP(hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, c->stream));
-  CallSched s = make_sched(v->n, c->batch_size, v->width, 2);
-  unsigned grid = grid_for((v->n + 3) / 4, kAesThreads, c->num_cu);
   int role = kLocal0 + owner;  // the data owner's local PRG
   bool arith = v->enc == ORQG_ASHARED;
-  if (v->width == 4) {
-    auto k = arith ? k_secret_share<uint32_t, true> : k_secret_share<uint32_t, false>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
-                       c->nonce[role], s, (const uint32_t *)tmp, (uint32_t *)v->d, owner);
-  } else {
-    auto k = arith ? k_secret_share<uint64_t, true> : k_secret_share<uint64_t, false>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
-                       c->nonce[role], s, (const uint64_t *)tmp, (uint64_t *)v->d, owner);
-  }
-  ORQG_HIP(hipGetLastError());
-  c->nonce[role] += sched_nonces(s);
+  // each worker thread shares its range with its own keys
+  for_threads(c, v->n, [&](uint64_t off, uint64_t m) {
+    CallSched s = make_sched(m, c->batch_size, v->width, 2);
+    unsigned grid = grid_for((m + 3) / 4, kAesThreads, c->num_cu);
+    const char *in = (const char *)tmp + off * v->width;
+    char *out = (char *)v->d + off * 3 * v->width;
+    if (v->width == 4) {
+      auto k = arith ? k_secret_share<uint32_t, true> : k_secret_share<uint32_t, false>;
+      hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
+                         c->nonce[role], s, (const uint32_t *)in, (uint32_t *)out, owner);
+    } else {
+      auto k = arith ? k_secret_share<uint64_t, true> : k_secret_share<uint64_t, false>;
+      hipLaunchKernelGGL(k, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role),
+                         c->nonce[role], s, (const uint64_t *)in, (uint64_t *)out, owner);
+    }
+    ORQG_HIP(hipGetLastError());
+    c->nonce[role] += sched_nonces(s);
+  });
   c->log_msg("secret_share", owner, (owner + 1) % 3, 2 * bytes);
   c->log_msg("secret_share", owner, (owner + 2) % 3, 2 * bytes);
 }
@@ -229,21 +234,25 @@ __global__ void __launch_bounds__(kAesThreads) k_mult(const uint32_t *__restrict
 
 void mult(orqg_ctx *c, bool arith, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
   if (x->n == 0) return;
-  CallSched s = make_sched(x->n, c->batch_size, x->width, 1);
-  NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
-  unsigned grid = grid_for((x->n * x->width + 15) / 16, kAesThreads, c->num_cu);
   TimedRegion tr(c, "mult");
-  if (x->width == 4)
-    hipLaunchKernelGGL((arith ? k_mult<uint32_t, true> : k_mult<uint32_t, false>), dim3(grid),
-                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
-                       (const uint32_t *)x->d, (const uint32_t *)y->d, (uint32_t *)z->d);
-  else
-    hipLaunchKernelGGL((arith ? k_mult<uint64_t, true> : k_mult<uint64_t, false>), dim3(grid),
-                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
-                       (const uint64_t *)x->d, (const uint64_t *)y->d, (uint64_t *)z->d);
-  ORQG_HIP(hipGetLastError());
-  uint64_t used = sched_nonces(s);
-  for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+  // each worker thread's range with its own pair keys and batch schedule
+  for_threads(c, x->n, [&](uint64_t off, uint64_t m) {
+    const orqg_vec xs = vec_rows(x, off, m), ys = vec_rows(y, off, m), zs = vec_rows(z, off, m);
+    CallSched s = make_sched(m, c->batch_size, x->width, 1);
+    NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+    unsigned grid = grid_for((m * x->width + 15) / 16, kAesThreads, c->num_cu);
+    if (x->width == 4)
+      hipLaunchKernelGGL((arith ? k_mult<uint32_t, true> : k_mult<uint32_t, false>), dim3(grid),
+                         dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
+                         (const uint32_t *)xs.d, (const uint32_t *)ys.d, (uint32_t *)zs.d);
+    else
+      hipLaunchKernelGGL((arith ? k_mult<uint64_t, true> : k_mult<uint64_t, false>), dim3(grid),
+                         dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s,
+                         (const uint64_t *)xs.d, (const uint64_t *)ys.d, (uint64_t *)zs.d);
+    ORQG_HIP(hipGetLastError());
+    uint64_t used = sched_nonces(s);
+    for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+  });
   const char *op = arith ? "mul_a" : "and_b";
   for (int p = 0; p < 3; p++) c->log_msg(op, p, (p + 2) % 3, x->n * (uint64_t)x->width);
 }
@@ -276,18 +285,21 @@ __global__ void __launch_bounds__(kAesThreads) k_b2a_bit(const uint32_t *__restr
 
 void b2a_bit(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
   if (x->n == 0) return;
-  CallSched s = make_sched(x->n, c->batch_size, x->width, 3);
-  NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
-  unsigned grid = grid_for((x->n * x->width + 15) / 16, kAesThreads, c->num_cu);
-  if (x->width == 4)
-    hipLaunchKernelGGL(k_b2a_bit<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                       c->rk(kPair0), nb, s, (const uint32_t *)x->d, (uint32_t *)y->d);
-  else
-    hipLaunchKernelGGL(k_b2a_bit<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                       c->rk(kPair0), nb, s, (const uint64_t *)x->d, (uint64_t *)y->d);
-  ORQG_HIP(hipGetLastError());
-  uint64_t used = sched_nonces(s);
-  for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+  for_threads(c, x->n, [&](uint64_t off, uint64_t m) {
+    const orqg_vec xs = vec_rows(x, off, m), ys = vec_rows(y, off, m);
+    CallSched s = make_sched(m, c->batch_size, x->width, 3);
+    NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+    unsigned grid = grid_for((m * x->width + 15) / 16, kAesThreads, c->num_cu);
+    if (x->width == 4)
+      hipLaunchKernelGGL(k_b2a_bit<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
+                         c->rk(kPair0), nb, s, (const uint32_t *)xs.d, (uint32_t *)ys.d);
+    else
+      hipLaunchKernelGGL(k_b2a_bit<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
+                         c->rk(kPair0), nb, s, (const uint64_t *)xs.d, (uint64_t *)ys.d);
+    ORQG_HIP(hipGetLastError());
+    uint64_t used = sched_nonces(s);
+    for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+  });
   for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * x->n * (uint64_t)x->width);
 }
 
@@ -324,19 +336,22 @@ void reshare(orqg_ctx *c, orqg_vec *v, int group) {
   if (group < 0 || group > 2) throw Error(ORQG_E_ARG, "reshare: group");
   if (v->n == 0) return;
   int role = kGroup0 + group;
-  CallSched s = make_sched(v->n, c->batch_size, v->width, 2);
-  unsigned grid = grid_for((v->n * v->width + 15) / 16, kAesThreads, c->num_cu);
   bool arith = v->enc == ORQG_ASHARED;
-  if (v->width == 4)
-    hipLaunchKernelGGL((arith ? k_reshare<uint32_t, true> : k_reshare<uint32_t, false>), dim3(grid),
-                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
-                       (uint32_t *)v->d);
-  else
-    hipLaunchKernelGGL((arith ? k_reshare<uint64_t, true> : k_reshare<uint64_t, false>), dim3(grid),
-                       dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
-                       (uint64_t *)v->d);
-  ORQG_HIP(hipGetLastError());
-  c->nonce[role] += sched_nonces(s);
+  for_threads(c, v->n, [&](uint64_t off, uint64_t m) {
+    const orqg_vec vs = vec_rows(v, off, m);
+    CallSched s = make_sched(m, c->batch_size, v->width, 2);
+    unsigned grid = grid_for((m * v->width + 15) / 16, kAesThreads, c->num_cu);
+    if (v->width == 4)
+      hipLaunchKernelGGL((arith ? k_reshare<uint32_t, true> : k_reshare<uint32_t, false>), dim3(grid),
+                         dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
+                         (uint32_t *)vs.d);
+    else
+      hipLaunchKernelGGL((arith ? k_reshare<uint64_t, true> : k_reshare<uint64_t, false>), dim3(grid),
+                         dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(role), c->nonce[role], s,
+                         (uint64_t *)vs.d);
+    ORQG_HIP(hipGetLastError());
+    c->nonce[role] += sched_nonces(s);
+  });
   log_reshare(c, group, v->n * (uint64_t)v->width);
 }
 

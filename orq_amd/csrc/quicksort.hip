@@ -172,6 +172,7 @@ uint64_t *quicksort_rounds(orqg_ctx *c, uint64_t *P, uint64_t *Q, uint64_t n) {
 }  // namespace
 
 void quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out) {
+  require_one_thread(c, "quicksort");
   if (!v || !perm_out || v->width != 4 || perm_out->width != 4 || perm_out->n != v->n)
     throw Error(ORQG_E_ARG, "quicksort: 32-bit keys and a 32-bit permutation of the same size");
   const uint64_t n = v->n;

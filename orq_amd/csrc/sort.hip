@@ -249,12 +249,15 @@ __global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restric
 // s0_k(e) = one_k*e - F_k(e), s1_k(e) = one_k*(n-1) - Ftot_k + F_k(e) with F the
 // inclusive prefix of f1 and one = (1,0,0); perm = s0 + f1*(s1 - s0).  One
 // superblock: boff3 = its exclusive offsets, tot = the totals (3 words each).
+// The rows F, R and the schedule s cover start at global row row0 of a
+// vector of ntot rows (one worker thread's range; 0 and s.n for one thread).
 __device__ inline void bit_perm_block(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
                                       const NonceTriple &nb, const CallSched &s, uint64_t e0,
                                       const uint32_t *__restrict__ F,
                                       const uint32_t *__restrict__ boff3,
                                       const uint32_t *__restrict__ tot, uint32_t *__restrict__ R,
-                                      uint32_t *s_w) {
+                                      uint32_t *s_w, uint64_t row0 = 0, uint64_t ntot = 0) {
+  if (ntot == 0) ntot = s.n;
   uint32_t f[4][3];
   uint32_t loc[3] = {0, 0, 0};
   if (e0 + 4 <= s.n) {
@@ -291,7 +294,7 @@ __device__ inline void bit_perm_block(const AesCtx<kCopies> &A, const uint32_t *
       unpack_block<uint32_t>(blk, S[k]);
     }
   }
-  const uint32_t n1 = (uint32_t)(s.n - 1);
+  const uint32_t n1 = (uint32_t)(ntot - 1);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     uint32_t s0[3], y[3];
@@ -299,7 +302,7 @@ __device__ inline void bit_perm_block(const AesCtx<kCopies> &A, const uint32_t *
     for (int k = 0; k < 3; k++) {
       pre[k] += f[i][k];  // inclusive
       uint32_t one = k == 0 ? 1u : 0u;
-      s0[k] = one * (uint32_t)(e0 + i) - pre[k];
+      s0[k] = one * (uint32_t)(row0 + e0 + i) - pre[k];
       uint32_t s1 = one * n1 - tot[k] + pre[k];
       y[k] = s1 - s0[k];
     }
@@ -330,7 +333,8 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
                                                   const uint32_t *__restrict__ F,
                                                   const uint32_t *__restrict__ boff,
                                                   const uint32_t *__restrict__ tot,
-                                                  uint32_t *__restrict__ R) {
+                                                  uint32_t *__restrict__ R, uint64_t row0,
+                                                  uint64_t ntot) {
   AES_LDS;
   __shared__ uint32_t s_w[3 * (kBT / 64)];
   aes_load_tables<kCopies>(lds, t0);
@@ -338,7 +342,7 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
   const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
   for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x)
     bit_perm_block(A, rk_pair, nb, s, (uint64_t)sb * kEPB + threadIdx.x * 4, F, boff + sb * 3, tot,
-                   R, s_w);
+                   R, s_w, row0, ntot);
 }
 
 // ---- sharded round on element records: gather through idx, then add the
@@ -381,15 +385,20 @@ __device__ inline void quad_transpose(uint4 (&X)[4], uint32_t s) {
 }
 
 // One wave step of a sharded round: rows [128 w, 128 w + 128) of the n = sv.n
-// rows (every wave of the launch takes its own steps).
+// rows (every wave of the launch takes its own steps).  Those rows are global
+// rows row0.. of a vector of ntot rows (one worker thread's destination
+// range; 0 and n for one thread): idx is indexed from row0, a gathered row
+// and an opened destination are global.
 template <int L, bool SCATTER>
 __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk,
                                    uint64_t nv, const CallSched &sv, uint64_t np,
                                    const CallSched &sp, const int32_t *__restrict__ idx,
                                    const uint32_t *__restrict__ Rin, uint32_t *__restrict__ Rout,
-                                   uint32_t *__restrict__ err, uint64_t w) {
+                                   uint32_t *__restrict__ err, uint64_t w, uint64_t row0 = 0,
+                                   uint64_t ntot = 0) {
   constexpr int kVq = (3 * L + 3) / 4;  // 16-byte pieces holding v
   const uint64_t n = sv.n;
+  if (ntot == 0) ntot = n;
   const uint32_t lane = threadIdx.x & 63, s = lane & 3;
   const uint64_t nstep = (n + 127) / 128;
   // source row of this lane's element i of step ww, or -1
@@ -397,7 +406,7 @@ __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__r
 #pragma unroll
     for (int i = 0; i < 2; i++) {
       const uint64_t e = ww * 128 + 64 * i + lane;
-      ix[i] = ww < nstep && e < n ? idx[e] : -1;
+      ix[i] = ww < nstep && e < n ? idx[row0 + e] : -1;
     }
   };
   auto load_rec = [&](const int32_t (&ix)[2], uint4 (&pv)[2][kVq], uint4 (&pq)[2]) {
@@ -476,11 +485,11 @@ __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__r
     wd[kRecPerm + 1] = cq[i].y + m1;
     wd[kRecPerm + 2] = cq[i].z + m2;
     // destination row: the element itself, or (scatter) its opened perm
-    uint32_t dst = (uint32_t)e;
+    uint32_t dst = (uint32_t)(row0 + e);
     bool ok = live;
     if (SCATTER) {
       dst = wd[kRecPerm] + wd[kRecPerm + 1] + wd[kRecPerm + 2];
-      if (live && dst >= n) {
+      if (live && dst >= ntot) {
         atomicOr(err, 1u);
         ok = false;
       }
@@ -512,14 +521,15 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
                                                 const int32_t *__restrict__ idx,
                                                 const uint32_t *__restrict__ Rin,
                                                 uint32_t *__restrict__ Rout,
-                                                uint32_t *__restrict__ err) {
+                                                uint32_t *__restrict__ err, uint64_t row0,
+                                                uint64_t ntot) {
   AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   const uint64_t nstep = (sv.n + 127) / 128;
   const uint64_t wstride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nstep; w += wstride)
-    round2_step<L, SCATTER>(A, rk, nv, sv, np, sp, idx, Rin, Rout, err, w);
+    round2_step<L, SCATTER>(A, rk, nv, sv, np, sp, idx, Rin, Rout, err, w, row0, ntot);
 }
 
 // remove_padding (sorting.h:475-495): perm_b = low 32 bits (B-shared int32),
@@ -684,62 +694,76 @@ uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_wi
   const uint32_t tszv = L == 3 ? 16 : 8;  // PadWidth element size
   uint32_t *F = (uint32_t *)c->scratch_get("rs_F", n * 12 + 64);
   uint32_t *R1 = (uint32_t *)c->scratch_get("rs_R1", n * kRec * 4 + 64);
-  const uint32_t nblk = (uint32_t)((n + kEPB - 1) / kEPB);
-  uint32_t *bsum = (uint32_t *)c->scratch_get("rs_bsum", (size_t)nblk * 12 + 64);
+  // superblocks are counted per worker-thread range (one range per thread),
+  // their sums laid out range after range
+  const uint32_t nblk_max = (uint32_t)((n + kEPB - 1) / kEPB) + (uint32_t)c->threads;
+  uint32_t *bsum = (uint32_t *)c->scratch_get("rs_bsum", (size_t)nblk_max * 12 + 64);
   uint32_t *tot = (uint32_t *)c->scratch_get("rs_tot", 64);
   uint32_t *err = (uint32_t *)c->scratch_get("rs_err", 16);
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
-  const unsigned grid_rounds = grid_for((n + 3) / 4, kBT, c->num_cu);
-  const unsigned grid_pers = std::min<unsigned>(nblk, (unsigned)c->num_cu);
   uint32_t *cur = P, *other = R1;
 
   for (int i = 0; i < bits; i++) {
     const int bitpos = 32 + i;
     const uint32_t flip = (full_width && i == bits - 1) ? 1u : 0u;  // inplace_invert on the MSB
-    NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
     {
       TimedRegion tr(c, "sort_b2a");
-      CallSched s = make_sched(n, c->batch_size, 4, 3);
-      hipLaunchKernelGGL(k_bit_b2a<L>, dim3(grid_pers), dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0),
-                         nb, s, cur, bitpos / 32, bitpos % 32, flip, F, bsum);
-      uint64_t used = sched_nonces(s);
-      for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+      uint32_t sb0 = 0;
+      for_threads(c, n, [&](uint64_t off, uint64_t m) {
+        NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+        CallSched s = make_sched(m, c->batch_size, 4, 3);
+        const uint32_t nsb = (uint32_t)((m + kEPB - 1) / kEPB);
+        hipLaunchKernelGGL(k_bit_b2a<L>, dim3(std::min<unsigned>(nsb, (unsigned)c->num_cu)),
+                           dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s, cur + off * kRec,
+                           bitpos / 32, bitpos % 32, flip, F + off * 3, bsum + (size_t)sb0 * 3);
+        uint64_t used = sched_nonces(s);
+        for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+        sb0 += nsb;
+      });
       for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * n * 4);
-      hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c->stream, nblk, bsum, tot);
+      hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c->stream, sb0, bsum, tot);
     }
     {
       TimedRegion tr(c, "sort_perm");
-      NonceTriple nb2 = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
-      CallSched s = make_sched(n, c->batch_size, 4, 1);
-      hipLaunchKernelGGL(k_bit_perm<L>, dim3(grid_pers), dim3(kBT), 0, c->stream, c->d_t0,
-                         c->rk(kPair0), nb2, s, F, bsum, tot, cur);
-      uint64_t used = sched_nonces(s);
-      for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+      uint32_t sb0 = 0;
+      for_threads(c, n, [&](uint64_t off, uint64_t m) {
+        NonceTriple nb2 = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+        CallSched s = make_sched(m, c->batch_size, 4, 1);
+        const uint32_t nsb = (uint32_t)((m + kEPB - 1) / kEPB);
+        hipLaunchKernelGGL(k_bit_perm<L>, dim3(std::min<unsigned>(nsb, (unsigned)c->num_cu)),
+                           dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0), nb2, s, F + off * 3,
+                           bsum + (size_t)sb0 * 3, tot, cur + off * kRec, off, n);
+        uint64_t used = sched_nonces(s);
+        for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+        sb0 += nsb;
+      });
       for (int p = 0; p < 3; p++) c->log_msg("mul_a", p, (p + 2) % 3, n * 4);
     }
     // oblivious_apply_elementwise_perm(v, perm): pi from the queue
     ShardedPerm pi = next_perm(c, n);
     {
       TimedRegion tr(c, "sort_round");
-      // key G_g is used first by v's round g, then by perm's round g
+      // key G_g is used first by v's round g, then by perm's round g (each
+      // worker thread's own G_g, over its range of destination rows)
       // cur -> other -> cur -> other (scattered)
       uint32_t *rin = cur, *rout = other;
-      CallSched sv = make_sched(n, c->batch_size, tszv, 2);
-      CallSched sp = make_sched(n, c->batch_size, 4, 2);
-      const uint64_t uv = sched_nonces(sv), up = sched_nonces(sp);
       for (int g = 0; g < 3; g++) {
         const int role = kGroup0 + g;
-        const uint64_t nv = c->nonce[role], np = nv + uv;
-        {
-          TimedRegion tk(c, g < 2 ? "k_round2" : "k_round2_scatter");
+        TimedRegion tk(c, g < 2 ? "k_round2" : "k_round2_scatter");
+        for_threads(c, n, [&](uint64_t off, uint64_t m) {
+          CallSched sv = make_sched(m, c->batch_size, tszv, 2);
+          CallSched sp = make_sched(m, c->batch_size, 4, 2);
+          const uint64_t uv = sched_nonces(sv), up = sched_nonces(sp);
+          const uint64_t nv = c->nonce[role], np = nv + uv;
+          const unsigned grid = grid_for((m + 3) / 4, kBT, c->num_cu);
           if (g < 2)
-            hipLaunchKernelGGL((k_round2<L, false>), dim3(grid_rounds), dim3(kBT), 0, c->stream,
-                               c->d_t0, c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err);
+            hipLaunchKernelGGL((k_round2<L, false>), dim3(grid), dim3(kBT), 0, c->stream, c->d_t0,
+                               c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err, off, n);
           else
-            hipLaunchKernelGGL((k_round2<L, true>), dim3(grid_rounds), dim3(kBT), 0, c->stream,
-                               c->d_t0, c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err);
-        }
-        c->nonce[role] += uv + up;
+            hipLaunchKernelGGL((k_round2<L, true>), dim3(grid), dim3(kBT), 0, c->stream, c->d_t0,
+                               c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err, off, n);
+          c->nonce[role] += uv + up;
+        });
         std::swap(rin, rout);
       }
       // log in the reference's order: three rounds on v, then three on perm

@@ -17,8 +17,8 @@ def _tag(v, enc):
 
 
 class OracleCtx:
-    def __init__(self, keys, batch_size=-12):
-        self.c = O.Ctx(keys, batch_size)
+    def __init__(self, keys, batch_size=-12, threads=1, thread_keys=()):
+        self.c = O.Ctx(keys, batch_size, threads=threads, thread_keys=thread_keys)
 
     # vectors
     def vec(self, n, width, encoding=B):
