@@ -293,6 +293,63 @@ def bench_join(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_batch(args):
+    """The batched form: --batch-requests independent radix sorts of 2^--lg
+    int64 rows, each on its own context, as one orqg_radix_sort_batch against
+    orqg_radix_sort on each context in turn (same keys, same inputs; every
+    request's result is checked against the plaintext sort).  Rows/s of the
+    whole batch, best of --steps after --warmup."""
+    import orq_amd
+
+    m, n = args.batch_requests, 1 << args.lg
+    rng = np.random.default_rng(args.lg)
+    keys = [rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64).view(np.uint64) for _ in range(m)]
+    ctxs = [orq_amd.Ctx(orq_amd.keys_from_seed(0xBA7C0000 + i)) for i in range(m)]
+    base = [c.secret_share(k, 8) for c, k in zip(ctxs, keys)]
+    work = [c.vec(n, 8) for c in ctxs]
+
+    def reset():
+        for c, b, w in zip(ctxs, base, work):
+            orq_amd.lib().orqg_vec_copy(c.h, b.h, w.h)
+        for c in ctxs:
+            c.sync()
+
+    def batched():
+        reset()
+        t0 = time.perf_counter()
+        perms = orq_amd.radix_sort_batch(work)
+        t = time.perf_counter() - t0
+        for p in perms:
+            p.free()
+        return t
+
+    def one_by_one():
+        reset()
+        t0 = time.perf_counter()
+        for c, w in zip(ctxs, work):
+            c.radix_sort(w).free()
+        for c in ctxs:
+            c.sync()
+        return time.perf_counter() - t0
+
+    for _ in range(args.warmup):
+        batched()
+        one_by_one()
+    tb = min(batched() for _ in range(args.steps))
+    ok = all(np.array_equal(c.open(w).view(np.int64), np.sort(k.view(np.int64)))
+             for c, w, k in zip(ctxs, work, keys))
+    ts = min(one_by_one() for _ in range(args.steps))
+    line = {"metric": "batched radix sort Mrows/s (independent requests, one launch sequence)",
+            "value": round(m * n / tb / 1e6, 4), "unit": "Mrows/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(tb * 1e3, 2),
+            "higher_is_better": True, "dtype": "int64", "data": "synthetic uniform int64 keys",
+            "config": {"workload": f"{m} requests x 2^{args.lg} rows, int64 keys, own context "
+                                   "each, incl. gen_perm"},
+            "one_by_one_ms": round(ts * 1e3, 2), "one_by_one_Mrows_s": round(m * n / ts / 1e6, 4),
+            "speedup": round(ts / tb, 2), "correct": ok}
+    print(json.dumps(line), flush=True)
+
+
 def make_requests(count, seed=0x0A11CE05, lo=10, hi=22, width=8):
     """The served stream's requests: row counts 2^U, U ~ Uniform[lo, hi];
     Zipf(1.1) int64 keys over [0, 2^20) and one uint32 value column, shared
@@ -435,11 +492,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gen-keystream", action="store_true",
                     help="skip the extra reserve that measures gen_perm's keystream rate")
-    ap.add_argument("--workload", choices=["radix", "primitives", "table", "join", "serve"],
+    ap.add_argument("--workload", choices=["radix", "primitives", "table", "join", "serve", "batch"],
                     default="radix")
     ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--rates", type=float, nargs="+", default=[4.0, 6.0, 8.0, 12.0, 16.0])
+    ap.add_argument("--batch-requests", type=int, default=64, help="batch: requests")
+    ap.add_argument("--lg", type=int, default=10, help="batch: rows per request, 2^lg")
     ap.add_argument("--lo", type=int, default=10, help="serve: smallest request 2^lo rows")
     ap.add_argument("--hi", type=int, default=22, help="serve: largest request 2^hi rows")
     ap.add_argument("--batch-slots", type=int, default=8,
@@ -461,6 +520,8 @@ def main():
         return bench_join(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import orq_amd
 

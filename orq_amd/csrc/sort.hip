@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <string>
 
 #include "internal.h"
@@ -1188,12 +1189,32 @@ void radix_sort_batch(const std::vector<orqg_ctx *> &cs, const std::vector<orqg_
     else
       batch_body<2>(lead, rq);
     // (batch_body synchronised lead's stream) each request's tail on its own
-    // stream, then its swap-resolution checks, as radix_sort ends
-    for (BatchReq &r : rq) sort_finish(r.c, r.v, r.cur, r.desc, r.perm);
-    for (BatchReq &r : rq) {
-      ORQG_HIP(hipStreamSynchronize(r.c->stream));
-      check_swaps(r.c);
+    // stream, then its swap-resolution checks, as radix_sort ends.  The
+    // tails are many small launches with host waits between them; the
+    // requests' contexts are independent, so a few host threads issue them
+    // side by side.
+    const size_t nthr = std::min<size_t>(rq.size(), 8);
+    std::vector<std::exception_ptr> errs(rq.size());
+    auto tail = [&](size_t first) {
+      for (size_t q = first; q < rq.size(); q += nthr) {
+        try {
+          ORQG_HIP(hipSetDevice(rq[q].c->device));
+          sort_finish(rq[q].c, rq[q].v, rq[q].cur, rq[q].desc, rq[q].perm);
+          ORQG_HIP(hipStreamSynchronize(rq[q].c->stream));
+          check_swaps(rq[q].c);
+        } catch (...) {
+          errs[q] = std::current_exception();
+        }
+      }
+    };
+    {
+      std::vector<std::thread> pool;
+      for (size_t k = 1; k < nthr; k++) pool.emplace_back(tail, k);
+      tail(0);
+      for (std::thread &t : pool) t.join();
     }
+    for (std::exception_ptr &e : errs)
+      if (e) std::rethrow_exception(e);
   }
 }
 
