@@ -120,8 +120,9 @@ def gen_perm(key: bytes, nonce: int, n: int):
 
 
 def batches(n, batch_size=-12):
-    out = np.zeros(4096, dtype=np.uint64)
-    k = lib().oc_batches(n, batch_size, _ptr(out), out.size)
+    k = lib().oc_batches(n, batch_size, None, 0)
+    out = np.zeros(max(k, 1), dtype=np.uint64)
+    lib().oc_batches(n, batch_size, _ptr(out), out.size)
     return [int(x) for x in out[:k]]
 
 

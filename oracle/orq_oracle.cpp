@@ -23,6 +23,11 @@
 //   gen_perm ............. brute-force enumeration on tiny n; bijection
 //   sharded perms ........ apply then inverse = identity, reshare keeps values
 //   radix_sort ........... plaintext stable sort (signed order); perm applies
+// and, beyond these, against the reference's own code compiled in place
+// (ref_driver.cpp, ref_party.cpp; tests/test_ref_oracle.py): keystream,
+// gen_perm, sharded permutations, batching, every primitive's shares, radix
+// sort, quicksort, the comparison and adder circuits and the table operators
+// share for share, with the bytes each party sends.
 #include <openssl/evp.h>
 
 #include <algorithm>
