@@ -174,43 +174,48 @@ BITONICSORT_ID, QUICKSORT_ID, RADIXSORT_ID = 0, 1, 2
 
 
 def run_parties(keys: bytes, scenario: int, params, payload: bytes, batch_size=-12, timeout=900,
-                default_sort="QUICKSORT"):
+                default_sort="QUICKSORT", thread_keys=()):
     """Run `scenario` of orq_ref_party as parties 0, 1, 2 connected by a
-    socket pair per pair of parties; returns each party's output bytes.
-    default_sort: the reference's DEFAULT sort protocol, QUICKSORT (as it
-    ships) or RADIXSORT."""
+    socket pair per pair of parties and worker thread; returns each party's
+    output bytes.  default_sort: the reference's DEFAULT sort protocol,
+    QUICKSORT (as it ships) or RADIXSORT.  thread_keys: the key blocks of
+    worker threads 1, 2, ... (thread 0 has `keys`)."""
     import socket
     import tempfile
 
     if not available():
         raise RuntimeError("the reference's party program is not built (needs /root/reference)")
-    assert len(keys) == 320 and len(params) <= 8
+    threads = 1 + len(thread_keys)
+    assert len(keys) == 320 and all(len(k) == 320 for k in thread_keys) and len(params) <= 8
     prm = list(params) + [0] * (8 - len(params))
     hdr = np.array([batch_size, scenario] + prm, dtype=np.int64).tobytes()
     with tempfile.TemporaryDirectory() as d:
         inp = os.path.join(d, "in")
         with open(inp, "wb") as f:
-            f.write(keys + hdr + payload)
-        pairs = {(0, 1): socket.socketpair(), (0, 2): socket.socketpair(),
-                 (1, 2): socket.socketpair()}
+            f.write(np.int64(threads).tobytes() + keys + b"".join(thread_keys) + hdr + payload)
+        pairs = [{(0, 1): socket.socketpair(), (0, 2): socket.socketpair(),
+                  (1, 2): socket.socketpair()} for _ in range(threads)]
 
-        def end(p, q):  # party p's end of the connection to party q
+        def end(t, p, q):  # thread t's end at party p of its connection to party q
             a, b = min(p, q), max(p, q)
-            return pairs[(a, b)][0 if p == a else 1].fileno()
+            return pairs[t][(a, b)][0 if p == a else 1].fileno()
 
+        exe = {"QUICKSORT": _PARTY, "RADIXSORT": _PARTY_RADIX}[default_sort]
         procs, outs = [], []
         try:
             for p in range(3):
-                fn, fp = end(p, (p + 1) % 3), end(p, (p + 2) % 3)
+                fds = []
+                for t in range(threads):
+                    fds += [end(t, p, (p + 1) % 3), end(t, p, (p + 2) % 3)]
                 out = os.path.join(d, f"out{p}")
                 outs.append(out)
-                exe = {"QUICKSORT": _PARTY, "RADIXSORT": _PARTY_RADIX}[default_sort]
-                procs.append(subprocess.Popen([exe, str(p), str(fn), str(fp), inp, out],
-                                              pass_fds=(fn, fp), stderr=subprocess.PIPE))
+                procs.append(subprocess.Popen([exe, str(p), ",".join(map(str, fds)), inp, out],
+                                              pass_fds=fds, stderr=subprocess.PIPE))
         finally:
-            for s in pairs.values():
-                s[0].close()
-                s[1].close()
+            for ps in pairs:
+                for s in ps.values():
+                    s[0].close()
+                    s[1].close()
         errs = []
         for p, proc in enumerate(procs):
             try:
@@ -251,14 +256,14 @@ class Reader:
         return self.off == len(self.b)
 
 
-def radix_sort(keys, data, bits=None, desc=False, batch_size=-12):
+def radix_sort(keys, data, bits=None, desc=False, batch_size=-12, thread_keys=()):
     """secret_share_b(data, 0), then radix_sort: ([party][idx][n] shares of
     the sorted vector, of the permutation, bytes sent per party)."""
     d = np.ascontiguousarray(data)
     w = d.itemsize
     bits = 8 * w if bits is None else bits
     outs = run_parties(keys, RADIX, [w, bits, int(desc), 0, 0, 0, 0, d.size], d.tobytes(),
-                       batch_size)
+                       batch_size, thread_keys=thread_keys)
     return _sorted_outputs(outs, d.dtype, d.size)
 
 
@@ -306,11 +311,11 @@ def _payload(cols, width):
     return b"".join(np.asarray(c).astype(np.int64).astype(dt).tobytes() for c in cols)
 
 
-def table_sort(keys, cols, width, nkeys, protocol, desc=False, batch_size=-12):
+def table_sort(keys, cols, width, nkeys, protocol, desc=False, batch_size=-12, thread_keys=()):
     """{[K1], ([K2],) [P], A} from `cols`, then sort on the key columns."""
     n = len(cols[0])
     outs = run_parties(keys, TABLE_SORT, [width, nkeys, protocol, int(desc), 0, 0, 0, n],
-                       _payload(cols, width), batch_size)
+                       _payload(cols, width), batch_size, thread_keys=thread_keys)
     return _tables(outs, width)
 
 

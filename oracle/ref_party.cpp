@@ -21,9 +21,11 @@
 // L_0..L_2, K_0..K_2, G_{01}, G_{02}, G_{12}, G_{012}) instead of keys drawn
 // by randombytes and exchanged.
 //
-// Usage: orq_ref_party <party> <fd to party+1> <fd to party+2> <in> <out>
-//   in:  keys[320] | i64 batch_size | i64 scenario | i64 params[8] | payload
-//   out: the scenario's arrays, then u64 bytes sent
+// Usage: orq_ref_party <party> <fds> <in> <out>
+//   fds: per worker thread, the sockets to party+1 and party+2 ("n0,p0,n1,p1")
+//   in:  i64 threads | keys[320 x threads] | i64 batch_size | i64 scenario |
+//        i64 params[8] | payload
+//   out: the scenario's arrays, then u64 bytes sent (all threads)
 #include <sys/socket.h>
 #include <unistd.h>
 
@@ -443,37 +445,52 @@ ORQ_REF_BY_WIDTH(circuits)
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 6) {
-    std::fprintf(stderr, "usage: %s party fd_next fd_prev in out\n", argv[0]);
+  if (argc != 5) {
+    std::fprintf(stderr, "usage: %s party fds in out\n", argv[0]);
     return 2;
   }
   const int party = std::atoi(argv[1]);
-  const int fd_next = std::atoi(argv[2]), fd_prev = std::atoi(argv[3]);
-  std::ifstream in(argv[4], std::ios::binary);
+  // per worker thread t: the sockets to party+1 and party+2, "n0,p0,n1,p1,..."
+  std::vector<int> fds;
+  for (const char *f = argv[2]; *f;) {
+    fds.push_back(std::atoi(f));
+    while (*f && *f != ',') f++;
+    if (*f == ',') f++;
+  }
+  std::ifstream in(argv[3], std::ios::binary);
   std::vector<uint8_t> buf((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-  if (buf.size() < 320 + 8 * 10) return 2;
-  const uint8_t *keys = buf.data();
+  if (buf.size() < 8) return 2;
+  int64_t threads;
+  std::memcpy(&threads, buf.data(), 8);
+  if (threads < 1 || (size_t)threads * 2 != fds.size() ||
+      buf.size() < 8 + 320 * (size_t)threads + 8 * 10)
+    return 2;
+  const uint8_t *keys = buf.data() + 8;  // thread t's key block at keys + 320 t
   int64_t hdr[10];
-  std::memcpy(hdr, buf.data() + 320, sizeof(hdr));
+  std::memcpy(hdr, keys + 320 * threads, sizeof(hdr));
   const int64_t batch_size = hdr[0], scenario = hdr[1];
   const int64_t *prm = hdr + 2;
-  const uint8_t *payload = buf.data() + 320 + sizeof(hdr);
+  const uint8_t *payload = keys + 320 * threads + sizeof(hdr);
   const size_t n = (size_t)prm[7];
   if (sodium_init() < 0) return 3;
 
   try {
-    // orq_runtime_init (setup.h:8-57) with one worker thread
-    runTime = std::make_unique<RunTime>(batch_size, 1, true);
+    // orq_runtime_init (setup.h:8-57): every worker thread gets its own
+    // communicator, randomness and protocol objects
+    runTime = std::make_unique<RunTime>(batch_size, (int)threads, true);
     runTime->setup_workers(party);
-    runTime->workers[0].attach_comm(std::make_unique<SocketComm>(party, fd_next, fd_prev));
     R3::ProtocolFactory protocolFactory(party, 3);
     auto groups = orq::ProtocolBase::generateRandomnessGroups(3, 2, 1);
-    runTime->workers[0].attach_rand(make_randomness(party, keys, groups));
-    runTime->workers[0].init_proto<int8_t>(protocolFactory);
-    runTime->workers[0].init_proto<int16_t>(protocolFactory);
-    runTime->workers[0].init_proto<int32_t>(protocolFactory);
-    runTime->workers[0].init_proto<int64_t>(protocolFactory);
-    runTime->workers[0].init_proto<__int128_t>(protocolFactory);
+    for (int t = 0; t < threads; t++) {
+      auto &w = runTime->workers[t];
+      w.attach_comm(std::make_unique<SocketComm>(party, fds[2 * t], fds[2 * t + 1]));
+      w.attach_rand(make_randomness(party, keys + 320 * t, groups));
+      w.init_proto<int8_t>(protocolFactory);
+      w.init_proto<int16_t>(protocolFactory);
+      w.init_proto<int32_t>(protocolFactory);
+      w.init_proto<int64_t>(protocolFactory);
+      w.init_proto<__int128_t>(protocolFactory);
+    }
 
     Out out;
     switch (scenario) {
@@ -489,8 +506,10 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "unknown scenario %ld\n", (long)scenario);
         return 2;
     }
-    out.put_u64(runTime->workers[0].getCommunicator()->getBytesSent());
-    std::ofstream o(argv[5], std::ios::binary);
+    uint64_t sent = 0;
+    for (int t = 0; t < threads; t++) sent += runTime->workers[t].getCommunicator()->getBytesSent();
+    out.put_u64(sent);
+    std::ofstream o(argv[4], std::ios::binary);
     o.write(reinterpret_cast<const char *>(out.bytes.data()), (std::streamsize)out.bytes.size());
     o.close();
   } catch (const std::exception &e) {

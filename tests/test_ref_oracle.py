@@ -339,3 +339,46 @@ def test_circuits_vs_reference(width, n):
     for k in R.CIRCUIT_OUTPUTS:
         np.testing.assert_array_equal(faithful(got[k]), res[k], err_msg=k)
     assert _sent(oc) == sent
+
+
+# ---- the reference's runtime with T worker threads ---------------------------
+
+def _thread_keys(T):
+    from orq_amd import keys_from_seed
+
+    return keys_from_seed(0x7E5D0000), [keys_from_seed(0x7E5D0000 + t) for t in range(1, T)]
+
+
+@pytest.mark.parametrize("T,n,width,batch_size", [(2, 5000, 4, -12), (3, 5000, 8, -12),
+                                                  (4, 300, 4, -12), (2, 70_001, 4, -12),
+                                                  (3, 3000, 8, 512)])
+def test_radix_sort_threads_vs_reference(T, n, width, batch_size):
+    """The runtime's thread ranges (runtime.h:198-247), each thread's own
+    randomness (setup.h:48-57) and the reserved permutations dealt to threads
+    (runtime.h:749-788), through the reference's own RunTime."""
+    k0, extra = _thread_keys(T)
+    dt = _DT[width]
+    x = np.random.default_rng(T * n).integers(0, np.iinfo(dt).max, n, dtype=dt, endpoint=True)
+    v, perm, sent = R.radix_sort(k0, x, batch_size=batch_size, thread_keys=extra)
+    oc = O.Ctx(k0, batch_size, threads=T, thread_keys=extra)
+    ov = oc.secret_share(x, width)
+    op = oc.radix_sort(ov)
+    np.testing.assert_array_equal(faithful(ov), v)
+    np.testing.assert_array_equal(faithful(op), perm)
+    assert _sent(oc) == sent
+
+
+@pytest.mark.parametrize("T", [2, 3])
+def test_table_sort_threads_vs_reference(T):
+    OracleCtx, Tb = _host()
+    k0, extra = _thread_keys(T)
+    n = 999
+    rng = np.random.default_rng(T)
+    cols = [rng.integers(-20, 20, n), rng.integers(-20, 20, n), rng.integers(0, 9, n),
+            rng.integers(0, 9, n)]
+    ref_table, sent = R.table_sort(k0, cols, 8, 2, R.RADIXSORT_ID, thread_keys=extra)
+    oc = OracleCtx(k0, threads=T, thread_keys=extra)
+    ot = Tb.EncodedTable.secret_share(oc, cols, ["[K1]", "[K2]", "[P]", "A"], 8)
+    ot.sort(["[K1]", "[K2]"], protocol=Tb.RADIXSORT)
+    _same_table(ref_table, ot)
+    assert _sent(oc.c) == sent
