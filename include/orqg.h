@@ -77,6 +77,13 @@ int orqg_ctx_destroy(orqg_ctx *ctx);
  * queue emptied, the message log and timers cleared (a new session of the
  * three parties, as RandomnessManager setup does in rand_setup.h:21-146).   */
 int orqg_ctx_rekey(orqg_ctx *ctx, const uint8_t keys[ORQG_NUM_KEYS][32]);
+/* The protocol that SortingProtocol::DEFAULT stands for in this context's
+ * table sorts (EncodedTable::sort, aggregate, distinct, inner_join): the
+ * reference's compile-time DEFAULT_SORT_PROTO (operators/sorting.h:6-9),
+ * QUICKSORT unless set (ORQG_QUICKSORT or ORQG_RADIXSORT; the ORQG_*SORT
+ * values are below).                                                        */
+int orqg_ctx_set_default_sort(orqg_ctx *ctx, int protocol);
+int orqg_ctx_default_sort(orqg_ctx *ctx);
 const char *orqg_last_error(orqg_ctx *ctx);
 int orqg_sync(orqg_ctx *ctx);
 /* the context's HIP stream (hipStream_t), for callers that order their own work */
@@ -290,15 +297,16 @@ int orqg_table_sort(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys, const in
 /* SortingProtocol (sorting.h:15-22); QUICKSORT is the reference's DEFAULT   */
 enum { ORQG_RADIXSORT = 0, ORQG_QUICKSORT = 1, ORQG_BITONICSORT = 2, ORQG_SORT_DEFAULT = 3 };
 /* operators::table_sort with an explicit protocol: RADIXSORT as above, or
- * QUICKSORT for the multi-bit sort columns (32-bit columns only; single-bit
- * columns use 1-bit radix sort under either), with the reserve counts of
- * sorting.h:531-550 for that protocol.                                      */
+ * QUICKSORT for the multi-bit sort columns (single-bit columns use 1-bit
+ * radix sort under either), with the reserve counts of sorting.h:531-550
+ * for that protocol.                                                        */
 int orqg_table_sort_protocol(orqg_ctx *ctx, orqg_vec *const *keys, size_t nkeys,
                              const int *orders, const int *single_bit, orqg_vec *const *a_cols,
                              size_t na, orqg_vec *const *b_cols, size_t nb, int protocol);
 /* operators::quicksort (include/core/operators/quicksort.h:212-244) of a
- * B-shared 32-bit vector in place (signed order; stable, through the index
- * padding): shuffle, rounds of pivot comparisons whose bits are opened,
+ * B-shared 32- or 64-bit vector in place (signed order; stable, through the
+ * index padding to 64 or 128 bits): shuffle, rounds of pivot comparisons
+ * whose bits are opened,
  * public partitions; perm_out (width 4) gets the A-shared sorting
  * permutation.  Consumes 3 permutations (reserved if the queue lacks them). */
 int orqg_quicksort(orqg_ctx *ctx, orqg_vec *v, int order, orqg_vec *perm_out);
@@ -373,8 +381,8 @@ int orqg_timer_reset(orqg_ctx *ctx);
  * keys "[##V]" and "[##TID]" are sorted on one bit (encoded_table.h:915-935).
  * Columns are owned by the table; orqg_et_column lends one until the table
  * changes it or is freed.  The protocols of orqg_et_sort are RADIXSORT,
- * QUICKSORT (32-bit tables), BITONICSORT and SORT_DEFAULT (the reference's
- * DEFAULT, quicksort: RADIXSORT for 64-bit tables here).                    */
+ * QUICKSORT, BITONICSORT and SORT_DEFAULT (the context's default sort
+ * protocol, orqg_ctx_set_default_sort: QUICKSORT, as in the reference).     */
 typedef struct orqg_et orqg_et;
 typedef struct {
   const char *data;    /* input column */
@@ -430,6 +438,9 @@ int orqg_et_inner_join(orqg_et *left, orqg_et *right, const char *const *keys, s
  * in arrival order when a slot is free and its estimated device footprint fits
  * hbm_budget beside the running ones.  Each request re-keys its slot with its
  * own keys, so its result shares equal those of the same request run alone.
+ * The slot contexts sort with radix sort as their DEFAULT protocol (the
+ * reference built with DEFAULT_SORT_PROTO=RADIXSORT, sorting.h:6-9), so that
+ * the small sorts of concurrent requests can share batched launches.
  * Host buffers named by a request must stay valid until its completion.      */
 enum { ORQG_REQ_SORT = 0,        /* EncodedTable::sort on [K], data V and [##V]  */
        ORQG_REQ_GROUPBY_SUM = 1  /* EncodedTable::aggregate([K], {V, V, sum})     */ };

@@ -658,6 +658,17 @@ void reverse_vec(oc_vec *v) {
 }
 
 // ----------------------------------------------------- comparison circuits
+typedef unsigned __int128 u128;
+
+// std::popcount of a share word of any width (bit_xor's parity,
+// mapping_access_vector.h:430-439)
+template <typename T>
+int popc(T x) {
+  int c = 0;
+  for (unsigned s = 0; s < 8 * sizeof(T); s += 64) c += __builtin_popcountll((unsigned long long)(x >> s));
+  return c;
+}
+
 template <typename T>
 void not_vec(oc_vec *v) {
   for (int p = 0; p < 3; p++)
@@ -714,7 +725,7 @@ void compare(oc_ctx *c, oc_vec *x, oc_vec *y, oc_vec *eq, oc_vec *gt) {
   for (int p = 0; p < 3; p++)
     for (int k = 0; k < 2; k++) {
       T *d = S<T>(gt, p, k);
-      for (size_t i = 0; i < n; i++) d[i] = (T)(__builtin_popcountll((unsigned long long)d[i]) & 1);
+      for (size_t i = 0; i < n; i++) d[i] = (T)(popc<T>(d[i]) & 1);
     }
   // signed inputs: fix the sign bit on MSB-packed words
   oc_vec *s1 = new_vec(nc, sizeof(T)), *s2 = new_vec(nc, sizeof(T)), *r = new_vec(nc, sizeof(T));
@@ -929,47 +940,50 @@ void radix_sort(oc_ctx *c, oc_vec *v, int bits, bool desc, oc_vec *perm_out) {
 }
 
 // ---------------------------------------------------------- quicksort ---
-// quicksort (include/core/operators/quicksort.h:212-244) of 32-bit keys:
-// pad_input to int64 (sorting.h:431-462), quicksort_body
-// (quicksort.h:62-198), remove_padding (sorting.h:474-495) and invert.  The
-// comparison bits are opened each round and the partition is public, as in
-// the reference.
+// quicksort (include/core/operators/quicksort.h:212-244): pad_input
+// (sorting.h:431-462) to PadWidth -- int64 for 32-bit keys, __int128 for
+// 64-bit keys --, quicksort_body (quicksort.h:62-198), remove_padding
+// (sorting.h:474-495) and invert.  The comparison bits are opened each round
+// and the partition is public, as in the reference.
 
 template <typename T>
 oc_vec *gather_rows(const oc_vec *v, const std::vector<size_t> &idx);
 
 // swap rows a and b in every party's share copies
-void swap_rows64(oc_vec *v, size_t a, size_t b) {
+template <typename P>
+void swap_rows(oc_vec *v, size_t a, size_t b) {
   for (int p = 0; p < 3; p++)
-    for (int k = 0; k < 2; k++) std::swap(S<uint64_t>(v, p, k)[a], S<uint64_t>(v, p, k)[b]);
+    for (int k = 0; k < 2; k++) std::swap(S<P>(v, p, k)[a], S<P>(v, p, k)[b]);
 }
 
 // partition (quicksort.h:28-50): Lomuto over [start, end] with the element
 // at `start` as pivot; cmp[j] == 0 (not above the pivot) moves j left.  The
 // pivot lands at i, which is marked, and i + 1 heads the right part.
-void qs_partition(oc_vec *v, const std::vector<uint64_t> &cmp, long start, long end,
+template <typename P>
+void qs_partition(oc_vec *v, const std::vector<uint8_t> &cmp, long start, long end,
                   std::vector<long> &pivots) {
   long i = start;
   for (long j = start + 1; j <= end; j++)
     if (cmp[j] == 0) {
       i++;
-      swap_rows64(v, (size_t)j, (size_t)i);
+      swap_rows<P>(v, (size_t)j, (size_t)i);
     }
-  swap_rows64(v, (size_t)start, (size_t)i);
+  swap_rows<P>(v, (size_t)start, (size_t)i);
   pivots[i] = i;
   if (i + 1 <= end) pivots[i + 1] = i + 1;
 }
 
 // quicksort_body (quicksort.h:62-198) on the padded (unique) values
+template <typename P>
 void quicksort_body(oc_ctx *c, oc_vec *v) {
   const size_t N = v->n;
   {  // v.shuffle(): one sharded permutation (shuffle.h:752-757)
     oc_ctx::Sharded sp = next_perm(c, N);
-    apply_sharded<uint64_t, false>(c, v, sp, false);
+    apply_sharded<P, false>(c, v, sp, false);
   }
   std::vector<long> pivots(N, -1);
   pivots[0] = 0;
-  std::vector<uint64_t> exp_cmp(N, 0);
+  std::vector<uint8_t> exp_cmp(N, 0);
   for (;;) {
     // non-pivots, and the pivot of each: the last marked position before it
     // (pivot_temp = prefix max of pivots)
@@ -983,11 +997,11 @@ void quicksort_body(oc_ctx *c, oc_vec *v) {
       }
     }
     const size_t size = red_idx.size();
-    oc_vec *red = gather_rows<uint64_t>(v, red_idx), *piv = gather_rows<uint64_t>(v, piv_idx);
-    oc_vec *eq = new_vec(size, 8), *gt = new_vec(size, 8);
-    compare<uint64_t>(c, red, piv, eq, gt);  // reduced_vec._compare(pivot_vec, temp, comparisons)
-    std::vector<uint64_t> cmp = open_vec<uint64_t, false>(c, gt);
-    for (size_t k = 0; k < size; k++) exp_cmp[red_idx[k]] = cmp[k];
+    oc_vec *red = gather_rows<P>(v, red_idx), *piv = gather_rows<P>(v, piv_idx);
+    oc_vec *eq = new_vec(size, sizeof(P)), *gt = new_vec(size, sizeof(P));
+    compare<P>(c, red, piv, eq, gt);  // reduced_vec._compare(pivot_vec, temp, comparisons)
+    std::vector<P> cmp = open_vec<P, false>(c, gt);
+    for (size_t k = 0; k < size; k++) exp_cmp[red_idx[k]] = (uint8_t)(cmp[k] & 1);
     delete red;
     delete piv;
     delete eq;
@@ -999,43 +1013,49 @@ void quicksort_body(oc_ctx *c, oc_vec *v) {
         index++;
         continue;
       }
-      qs_partition(v, exp_cmp, index, p - 1, pivots);
+      qs_partition<P>(v, exp_cmp, index, p - 1, pivots);
       index = p;
     }
-    qs_partition(v, exp_cmp, index, (long)N - 1, pivots);
+    qs_partition<P>(v, exp_cmp, index, (long)N - 1, pivots);
     size_t num_pivots = 0;
     for (long x : pivots) num_pivots += x >= 0;
     if (num_pivots >= N) return;
   }
 }
 
-void quicksort32(oc_ctx *c, oc_vec *v, bool desc, oc_vec *perm_out) {
+// K: the key's unsigned type (32 or 64 bits); P: its PadWidth, twice as wide.
+// Padded share = sign-extending copy-cast of the key share, << 32; the public
+// index (i, or -1 - i for DESC) sits in the low 32 bits of share 0.
+template <typename K, typename P>
+void quicksort_any(oc_ctx *c, oc_vec *v, bool desc, oc_vec *perm_out) {
+  typedef typename std::make_signed<K>::type SK;
+  typedef typename std::make_signed<P>::type SP;
   const size_t n = v->n;
   reserve(c, n, 3);  // 1 for the shuffle, 1 for remove_padding's b2a, 1 pair for invert
   if (n == 0) return;
-  oc_vec *pad = new_vec(n, 8);
+  oc_vec *pad = new_vec(n, sizeof(P));
   for (int p = 0; p < 3; p++)
     for (int k = 0; k < 2; k++) {
-      const uint32_t *src = S<uint32_t>(v, p, k);
-      uint64_t *dst = S<uint64_t>(pad, p, k);
-      for (size_t i = 0; i < n; i++) dst[i] = (uint64_t)(int64_t)(int32_t)src[i] << 32;
+      const K *src = S<K>(v, p, k);
+      P *dst = S<P>(pad, p, k);
+      for (size_t i = 0; i < n; i++) dst[i] = (P)((SP)(SK)src[i]) << 32;
     }
   for (size_t i = 0; i < n; i++) {
     uint32_t idx = desc ? (uint32_t)(-1 - (int64_t)i) : (uint32_t)i;
-    S<uint64_t>(pad, 0, 0)[i] |= idx;
-    S<uint64_t>(pad, 2, 1)[i] |= idx;
+    S<P>(pad, 0, 0)[i] |= idx;
+    S<P>(pad, 2, 1)[i] |= idx;
   }
   if (desc) reverse_vec(pad);
-  quicksort_body(c, pad);
+  quicksort_body<P>(c, pad);
   if (desc) reverse_vec(pad);
   for (int p = 0; p < 3; p++)
     for (int k = 0; k < 2; k++) {
-      const uint64_t *src = S<uint64_t>(pad, p, k);
+      const P *src = S<P>(pad, p, k);
       uint32_t *pd = S<uint32_t>(perm_out, p, k);
-      uint32_t *vd = S<uint32_t>(v, p, k);
+      K *vd = S<K>(v, p, k);
       for (size_t j = 0; j < n; j++) {
         pd[j] = (uint32_t)src[j];
-        vd[j] = (uint32_t)(uint64_t)((int64_t)src[j] >> 32);
+        vd[j] = (K)(P)((SP)src[j] >> 32);
       }
     }
   delete pad;
@@ -1139,8 +1159,10 @@ void table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, const
     if (single_bit && single_bit[col]) {
       radix_any(c, keys[col], 1, desc, perm_out);
     } else if (protocol == 1) {
-      if (keys[col]->width != 4) throw std::runtime_error("quicksort: 32-bit keys only");
-      quicksort32(c, keys[col], desc, perm_out);
+      if (keys[col]->width == 4)
+        quicksort_any<uint32_t, uint64_t>(c, keys[col], desc, perm_out);
+      else
+        quicksort_any<uint64_t, u128>(c, keys[col], desc, perm_out);
     } else {
       radix_any(c, keys[col], 8 * keys[col]->width, desc, perm_out);
     }
@@ -2195,8 +2217,12 @@ int oc_table_sort(oc_ctx *c, oc_vec **keys, size_t nkeys, const int *orders, con
 
 int oc_quicksort(oc_ctx *c, oc_vec *v, int desc, oc_vec *perm_out) {
   OC_TRY({
-    if (v->width != 4) throw std::runtime_error("quicksort: 32-bit keys only");
-    quicksort32(c, v, desc != 0, perm_out);
+    if (v->width == 4)
+      quicksort_any<uint32_t, uint64_t>(c, v, desc != 0, perm_out);
+    else if (v->width == 8)
+      quicksort_any<uint64_t, u128>(c, v, desc != 0, perm_out);
+    else
+      throw std::runtime_error("quicksort: 32- or 64-bit keys");
   })
 }
 

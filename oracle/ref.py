@@ -268,9 +268,10 @@ def radix_sort(keys, data, bits=None, desc=False, batch_size=-12, thread_keys=()
 
 
 def quicksort(keys, data, desc=False, batch_size=-12):
-    d = np.ascontiguousarray(data, dtype=np.uint32)
-    outs = run_parties(keys, QUICKSORT, [4, 32, int(desc), 0, 0, 0, 0, d.size], d.tobytes(),
-                       batch_size)
+    """secret_share_b(data, 0), then quicksort (32- or 64-bit keys)."""
+    d = np.ascontiguousarray(data)
+    outs = run_parties(keys, QUICKSORT, [d.itemsize, 8 * d.itemsize, int(desc), 0, 0, 0, 0, d.size],
+                       d.tobytes(), batch_size)
     return _sorted_outputs(outs, d.dtype, d.size)
 
 

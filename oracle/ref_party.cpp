@@ -134,9 +134,10 @@ class SocketComm : public orq::Communicator {
     for (int q = 0; q < 3; q++)
       if (q != id) peers[q].th = std::thread(&SocketComm::sender, this, &peers[q]);
   }
-  ~SocketComm() {
+  // every queued message sent: the sender threads drain their queues and end
+  void flush() {
     for (int q = 0; q < 3; q++) {
-      if (q == currentId) continue;
+      if (q == currentId || !peers[q].th.joinable()) continue;
       {
         std::lock_guard<std::mutex> g(peers[q].mu);
         peers[q].stop = true;
@@ -145,6 +146,7 @@ class SocketComm : public orq::Communicator {
       peers[q].th.join();
     }
   }
+  ~SocketComm() { flush(); }
 
 #define ORQ_REF_COMM_TYPE(T)                                                                      \
   void sendShare(T e, orq::PartyID id) override { send_one(e, id); }                              \
@@ -271,10 +273,12 @@ void radix(const int64_t *prm, const uint8_t *payload, size_t n, Out &out) {
   out.put_ev(perm.sharedVector.vector);
 }
 
-// scenario 2: quicksort of 32-bit keys (quicksort.h:28-246), same outputs
+// scenario 2: quicksort (quicksort.h:28-246), 32- or 64-bit keys, same
+// outputs.  params: width, -, desc
+template <typename T>
 void quick(const int64_t *prm, const uint8_t *payload, size_t n, Out &out) {
-  auto data = vec_from<int32_t>(payload, n);
-  R3::BSharedVector<int32_t> v = R3::secret_share_b(data, 0);
+  auto data = vec_from<T>(payload, n);
+  R3::BSharedVector<T> v = R3::secret_share_b(data, 0);
   auto order = prm[2] ? SortOrder::DESC : SortOrder::ASC;
   auto perm = quicksort(v, order);
   out.put_ev(v.vector);
@@ -434,6 +438,7 @@ void circuits(const int64_t *, const uint8_t *payload, size_t n, Out &out) {
       fn<int64_t>(prm, payload, n, out);                                                    \
   }
 ORQ_REF_BY_WIDTH(radix)
+ORQ_REF_BY_WIDTH(quick)
 ORQ_REF_BY_WIDTH(table_sort)
 ORQ_REF_BY_WIDTH(join_groupby)
 ORQ_REF_BY_WIDTH(table_distinct)
@@ -495,7 +500,7 @@ int main(int argc, char **argv) {
     Out out;
     switch (scenario) {
       case 1: radix_any(prm, payload, n, out); break;
-      case 2: quick(prm, payload, n, out); break;
+      case 2: quick_any(prm, payload, n, out); break;
       case 3: table_sort_any(prm, payload, n, out); break;
       case 4: join_groupby_any(prm, payload, n, out); break;
       case 5: table_distinct_any(prm, payload, n, out); break;
@@ -507,7 +512,12 @@ int main(int argc, char **argv) {
         return 2;
     }
     uint64_t sent = 0;
-    for (int t = 0; t < threads; t++) sent += runTime->workers[t].getCommunicator()->getBytesSent();
+    for (int t = 0; t < threads; t++) {
+      auto *comm = runTime->workers[t].getCommunicator();
+      sent += comm->getBytesSent();
+      // the last messages must reach the peers before this process ends
+      static_cast<SocketComm *>(comm)->flush();
+    }
     out.put_u64(sent);
     std::ofstream o(argv[4], std::ios::binary);
     o.write(reinterpret_cast<const char *>(out.bytes.data()), (std::streamsize)out.bytes.size());

@@ -57,7 +57,8 @@ EXPORTS = [
     "orqg_prefix_sum", "orqg_quicksort", "orqg_table_sort_protocol",
     "orqg_vec_read_share_files", "orqg_vec_write_share_files", "orqg_table_read_share_csv",
     "orqg_table_write_share_csv", "orqg_table_read_csv_data", "orqg_public_fill", "orqg_vec_copy_rows",
-    "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
+    "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_ctx_set_default_sort",
+    "orqg_ctx_default_sort", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
     "orqg_a2b", "orqg_secret_share_from", "orqg_server_create", "orqg_server_submit",
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
     "orqg_server_destroy",
@@ -143,6 +144,8 @@ def lib():
             "orqg_vec_broadcast_row": (i32, [vp, vp, sz, vp, sz, sz]),
             "orqg_add_b": (i32, [vp, vp, vp, vp]),
             "orqg_ctx_rekey": (i32, [vp, ctypes.c_char_p]),
+            "orqg_ctx_set_default_sort": (i32, [vp, i32]),
+            "orqg_ctx_default_sort": (i32, [vp]),
             "orqg_server_create": (i32, [vp, vp]),
             "orqg_server_submit": (i32, [vp, vp]),
             "orqg_server_poll": (i32, [vp, vp, i32, vp, i32]),
@@ -300,6 +303,20 @@ class Ctx:
 
     def sync(self):
         self._check(lib().orqg_sync(self.h), "sync")
+
+    SORT_PROTOCOLS = {"RADIXSORT": 0, "QUICKSORT": 1}
+
+    @property
+    def default_sort(self):
+        """The protocol SortingProtocol::DEFAULT stands for in table sorts (the
+        reference's DEFAULT_SORT_PROTO, operators/sorting.h:6-9)."""
+        code = lib().orqg_ctx_default_sort(self.h)
+        return {v: k for k, v in self.SORT_PROTOCOLS.items()}[code]
+
+    @default_sort.setter
+    def default_sort(self, protocol):
+        self._check(lib().orqg_ctx_set_default_sort(self.h, self.SORT_PROTOCOLS[protocol]),
+                    "ctx_set_default_sort")
 
     def nonce(self, role, thread=0):
         v = ctypes.c_uint64()

@@ -353,6 +353,16 @@ int orqg_ctx_rekey(orqg_ctx *c, const uint8_t keys[ORQG_NUM_KEYS][32]) {
   API_END(c)
 }
 
+int orqg_ctx_set_default_sort(orqg_ctx *c, int protocol) {
+  API_BEGIN(c)
+  if (protocol != ORQG_QUICKSORT && protocol != ORQG_RADIXSORT)
+    throw Error(ORQG_E_ARG, "ctx_set_default_sort: QUICKSORT or RADIXSORT");
+  c->default_sort = protocol;
+  API_END(c)
+}
+
+int orqg_ctx_default_sort(orqg_ctx *c) { return c ? c->default_sort : ORQG_E_ARG; }
+
 const char *orqg_last_error(orqg_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 int orqg_sync(orqg_ctx *c) {

@@ -54,7 +54,7 @@ void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, or
   const uint64_t n = x->n;
   if (n == 0) return;
   const int B = 8 * x->width;
-  const int L = x->width == 4 ? 5 : 6;
+  const int L = x->width == 4 ? 5 : x->width == 8 ? 6 : 7;
   CallSched s = make_sched(n, c->batch_size, x->width, 1);
   const uint64_t nwords = n / B + (n % B > 0);
   CallSched sp = make_sched(nwords, c->batch_size, x->width, 1);
@@ -69,8 +69,10 @@ void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, or
                      gt ? (T *)gt->d : (T *)nullptr, neg_eq, neg_gt)
   if (x->width == 4)
     ORQG_CMP_LAUNCH(uint32_t);
-  else
+  else if (x->width == 8)
     ORQG_CMP_LAUNCH(uint64_t);
+  else
+    ORQG_CMP_LAUNCH(u128);  // quicksort's padded 64-bit keys
 #undef ORQG_CMP_LAUNCH
   ORQG_HIP(hipGetLastError());
   uint64_t used = (uint64_t)L * stride;

@@ -10,7 +10,7 @@ namespace orqg {
 // (b_shared_vector.h:164-178, LOG_MAX_BITS_NUMBER).
 template <typename T>
 constexpr int same_levels() {
-  return sizeof(T) == 4 ? 5 : 6;
+  return sizeof(T) == 4 ? 5 : sizeof(T) == 8 ? 6 : 7;
 }
 
 // z = x & y for the three parties (replicated_3pc.h:166-191) with the pair
@@ -80,11 +80,13 @@ __device__ inline void compare_group(const AesCtx<kCopies> &A, const uint32_t *_
                                      int lane, const T (&X)[16 / sizeof(T)][3],
                                      const T (&Y)[16 / sizeof(T)][3], T (&EQ)[16 / sizeof(T)][3],
                                      T (&GT)[16 / sizeof(T)][3]) {
-  typedef typename std::make_signed<T>::type ST;
+  typedef typename SignedOf<T>::type ST;
   constexpr int E = 16 / sizeof(T);
   constexpr int B = 8 * sizeof(T);
   constexpr int L = same_levels<T>();
-  constexpr int kBlkPerWave = 64 / B;  // AES blocks of packed words per wave
+  // AES blocks of packed words per wave: a block packs 128 elements' bits and
+  // a wave holds 64 E of them (for 16-byte words, half a block)
+  constexpr int kBlkPerWave = 64 / B > 0 ? 64 / B : 1;
   const uint64_t nwords = sp.n;
   const uint64_t e0 = (gw + lane) * E;
   // lanes past the end run the same arithmetic on zeros; the caller stores nothing
@@ -129,12 +131,12 @@ __device__ inline void compare_group(const AesCtx<kCopies> &A, const uint32_t *_
       for (int k = 0; k < 3; k++) a[k] = (T)((ST)e[i][k] >> 1) ^ e[i][k];
       and3<T>(a, X[i], ks, z);
 #pragma unroll
-      for (int k = 0; k < 3; k++) r[i][k] = (T)(__popcll((unsigned long long)z[k]) & 1);
+      for (int k = 0; k < 3; k++) r[i][k] = (T)(popc<T>(z[k]) & 1);
     }
 
     // keystream of the packed and_b: lane j < 3*kBlkPerWave computes key
     // j%3 of the wave's AES block j/3
-    const uint64_t wave_e0 = gw * E;  // multiple of 64*E, hence of B
+    const uint64_t wave_e0 = gw * E;  // multiple of 64*E: of 128 unless E = 1
     uint4 kb = make_uint4(0, 0, 0, 0);
     if (lane < 3 * kBlkPerWave) {
       const int key = lane % 3, blk = lane / 3;
@@ -148,8 +150,9 @@ __device__ inline void compare_group(const AesCtx<kCopies> &A, const uint32_t *_
       }
     }
     // this lane's elements sit in dword D of the wave's packed keystream,
-    // at bits b0 .. b0+E-1
-    const int rel = lane * E;
+    // at bits b0 .. b0+E-1 (a wave of 16-byte words starts 0 or 64 elements
+    // into its block)
+    const int rel = (int)(wave_e0 % 128) + lane * E;
     const int D = rel / 32, b0 = rel % 32;
     uint32_t kw[3];
 #pragma unroll

@@ -139,9 +139,9 @@ void et_pad_power_of_two(orqg_et *t) {
 void et_sort(orqg_et *t, const std::vector<std::string> &keys, const std::vector<int> &orders,
           int protocol) {
   if (protocol == ORQG_SORT_DEFAULT)
-    // SortingProtocol::DEFAULT is QUICKSORT (sorting.h:5-22), which takes
-    // 32-bit columns here; 64-bit tables sort with RADIXSORT
-    protocol = t->width == 4 ? ORQG_QUICKSORT : ORQG_RADIXSORT;
+    // SortingProtocol::DEFAULT: the context's DEFAULT_SORT_PROTO (QUICKSORT
+    // unless set, sorting.h:6-9)
+    protocol = t->ctx->default_sort;
   if (protocol != ORQG_RADIXSORT && protocol != ORQG_QUICKSORT && protocol != ORQG_BITONICSORT)
     throw Error(ORQG_E_ARG, "table sort: unknown protocol");
   std::vector<orqg_vec *> kv, a, b;

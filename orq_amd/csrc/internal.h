@@ -131,6 +131,7 @@ struct SortBatcher;
 
 struct orqg_ctx {
   int device = 0;
+  int default_sort = ORQG_QUICKSORT;  // SortingProtocol::DEFAULT (sorting.h:6-9)
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // auxiliary stream (same default priority)
   hipEvent_t ev_aux = nullptr;

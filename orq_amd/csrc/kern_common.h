@@ -1,6 +1,8 @@
 // Device helpers shared by the kernels: share-group loads/stores, keystream
 // unpacking and the group-PRG resharing masks.
 #pragma once
+#include <type_traits>
+
 #include "internal.h"
 
 namespace orqg {
@@ -13,6 +15,29 @@ constexpr int kAesThreads = 1024; // threads per AES workgroup (one workgroup pe
 struct NonceTriple {
   uint64_t v[3];
 };
+
+// 16-byte share words: quicksort's padded values for 64-bit keys
+// (PadWidth __int128, sorting.h:409-416)
+typedef unsigned __int128 u128;
+
+// the signed type of a share word, for arithmetic shifts
+template <typename T>
+struct SignedOf {
+  typedef typename std::make_signed<T>::type type;
+};
+template <>
+struct SignedOf<u128> {
+  typedef __int128 type;
+};
+
+// population count of a share word of any width
+template <typename T>
+__device__ inline int popc(T x) {
+  int c = 0;
+#pragma unroll
+  for (unsigned s = 0; s < 8 * sizeof(T); s += 64) c += __popcll((unsigned long long)(x >> s));
+  return c;
+}
 
 // Split a 16-byte keystream block into little-endian elements of type T.
 template <typename T>
@@ -28,6 +53,11 @@ template <>
 __device__ inline void unpack_block<uint64_t>(uint4 b, uint64_t (&out)[2]) {
   out[0] = (uint64_t)b.x | ((uint64_t)b.y << 32);
   out[1] = (uint64_t)b.z | ((uint64_t)b.w << 32);
+}
+template <>
+__device__ inline void unpack_block<u128>(uint4 b, u128 (&out)[1]) {
+  out[0] = (u128)((uint64_t)b.x | ((uint64_t)b.y << 32)) |
+           ((u128)((uint64_t)b.z | ((uint64_t)b.w << 32)) << 64);
 }
 
 // E = 16/sizeof(T) consecutive elements of an [n][3] share array.

@@ -63,6 +63,8 @@ template void launch_round<uint32_t>(orqg_ctx *, bool, int, const int32_t *, con
                                      uint32_t *, uint64_t);
 template void launch_round<uint64_t>(orqg_ctx *, bool, int, const int32_t *, const uint64_t *,
                                      uint64_t *, uint64_t);
+template void launch_round<u128>(orqg_ctx *, bool, int, const int32_t *, const u128 *, u128 *,
+                                 uint64_t);
 
 // hm_oblivious_apply_sharded_perm / _inverse_ (shuffle.h:349-412)
 void apply_sharded(orqg_ctx *c, orqg_vec *x, const ShardedPerm &pi, bool inverse) {
@@ -76,8 +78,10 @@ void apply_sharded(orqg_ctx *c, orqg_vec *x, const ShardedPerm &pi, bool inverse
     const int32_t *idx = inverse ? pi.fwd[g] : pi.inv[g];
     if (x->width == 4)
       launch_round<uint32_t>(c, arith, g, idx, (const uint32_t *)cur, (uint32_t *)nxt, x->n);
-    else
+    else if (x->width == 8)
       launch_round<uint64_t>(c, arith, g, idx, (const uint64_t *)cur, (uint64_t *)nxt, x->n);
+    else
+      launch_round<u128>(c, arith, g, idx, (const u128 *)cur, (u128 *)nxt, x->n);
     std::swap(cur, nxt);
   }
   // three rounds: result is in tmp

@@ -135,10 +135,13 @@ void open_vec(orqg_ctx *c, const orqg_vec *v, void *host_out) {
     hipLaunchKernelGGL((arith ? k_open<uint32_t, true> : k_open<uint32_t, false>), dim3(grid),
                        dim3(256), 0, c->stream, (uint64_t)v->n, (const uint32_t *)v->d,
                        (uint32_t *)tmp);
-  else
+  else if (v->width == 8)
     hipLaunchKernelGGL((arith ? k_open<uint64_t, true> : k_open<uint64_t, false>), dim3(grid),
                        dim3(256), 0, c->stream, (uint64_t)v->n, (const uint64_t *)v->d,
                        (uint64_t *)tmp);
+  else  // 16-byte words (quicksort's padded 64-bit keys)
+    hipLaunchKernelGGL((arith ? k_open<u128, true> : k_open<u128, false>), dim3(grid), dim3(256),
+                       0, c->stream, (uint64_t)v->n, (const u128 *)v->d, (u128 *)tmp);
   ORQG_HIP(hipGetLastError());
   ORQG_HIP(hipMemcpyAsync(host_out, tmp, bytes, hipMemcpyDeviceToHost, c->stream));
   ORQG_HIP(hipStreamSynchronize(c->stream));

@@ -1,7 +1,8 @@
 // operators::quicksort (include/core/operators/quicksort.h:212-244) for
-// 32-bit keys, the reference's default sort protocol (sorting.h:5-22).
+// 32- and 64-bit keys, the reference's default sort protocol (sorting.h:5-22).
 //
-// The keys are padded to 64 bits with their index in the low half
+// The keys are padded to PadWidth -- 64 bits for 32-bit keys, 128 for 64-bit
+// keys (sorting.h:409-416) -- with their index in the low 32 bits
 // (pad_input, sorting.h:431-462), so every padded value is unique, and
 // shuffled with one sharded permutation (shuffle.h:752-757).  Then, round by
 // round (quicksort_body, quicksort.h:62-198): every element that is not yet a
@@ -23,44 +24,52 @@
 #include <vector>
 
 #include "internal.h"
+#include "kern_common.h"
 
 namespace orqg {
 namespace {
 
-// pad_input for int32 keys: padded share k = share k << 32 (the sign
-// extension of the copy-cast is shifted out), and the public index -- i, or
-// -1 - i for DESC -- ORed into the low half of s_0 (public_share holds a
-// public value in s_0 only).  DESC reverses the rows afterwards.
-__global__ void k_qs_pad(uint64_t n, int desc, const uint32_t *__restrict__ v,
-                         uint64_t *__restrict__ pad) {
+// pad_input: padded share k = the sign-extending copy-cast of key share k to
+// P (bit-sliced sign extension, as the reference's copy-cast of each share),
+// shifted left by 32, and the public index -- i, or -1 - i for DESC -- ORed
+// into the low 32 bits of s_0 (public_share holds a public value in s_0
+// only).  DESC reverses the rows afterwards.
+template <typename K, typename P>
+__global__ void k_qs_pad(uint64_t n, int desc, const K *__restrict__ v, P *__restrict__ pad) {
+  typedef typename SignedOf<K>::type SK;
+  typedef typename SignedOf<P>::type SP;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t idx = desc ? (uint32_t)(-1 - (int64_t)i) : (uint32_t)i;
     const uint64_t r = desc ? n - 1 - i : i;
-    pad[3 * r] = ((uint64_t)v[3 * i] << 32) | idx;
-    pad[3 * r + 1] = (uint64_t)v[3 * i + 1] << 32;
-    pad[3 * r + 2] = (uint64_t)v[3 * i + 2] << 32;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const P x = (P)(SP)(SK)v[3 * i + k] << 32;
+      pad[3 * r + k] = k == 0 ? (P)(x | idx) : x;
+    }
   }
 }
 
-// remove_padding after the (DESC) reversal: low halves -> the B-shared
-// permutation, high halves (padded >> 32, copy-cast to 32 bits) -> v
-__global__ void k_qs_unpad(uint64_t n, int desc, const uint64_t *__restrict__ pad,
-                           uint32_t *__restrict__ v, uint32_t *__restrict__ perm) {
+// remove_padding after the (DESC) reversal: the low 32 bits -> the B-shared
+// permutation, padded >> 32 copy-cast to the key width -> v
+template <typename K, typename P>
+__global__ void k_qs_unpad(uint64_t n, int desc, const P *__restrict__ pad, K *__restrict__ v,
+                           uint32_t *__restrict__ perm) {
   for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n;
        r += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t src = desc ? n - 1 - r : r;
     for (int k = 0; k < 3; k++) {
-      const uint64_t x = pad[3 * src + k];
+      const P x = pad[3 * src + k];
       perm[3 * r + k] = (uint32_t)x;
-      v[3 * r + k] = (uint32_t)(x >> 32);
+      v[3 * r + k] = (K)(x >> 32);
     }
   }
 }
 
 // dst row i = src row idx[i], all three shares
+template <typename P>
 __global__ void k_qs_gather(uint64_t m, const uint32_t *__restrict__ idx,
-                            const uint64_t *__restrict__ src, uint64_t *__restrict__ dst) {
+                            const P *__restrict__ src, P *__restrict__ dst) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t j = idx[i];
@@ -95,20 +104,22 @@ void partition(std::vector<uint32_t> &src, const std::vector<uint8_t> &cmp, int6
   if (i + 1 <= end) pivots[i + 1] = (int32_t)(i + 1);
 }
 
-// quicksort_body on the padded, shuffled vector P (n rows of 3 x uint64);
-// returns the buffer that holds the result (P or its twin Q)
-uint64_t *quicksort_rounds(orqg_ctx *c, uint64_t *P, uint64_t *Q, uint64_t n) {
+// quicksort_body on the padded, shuffled vector P (n rows of 3 words of type
+// W); returns the buffer that holds the result (P or its twin Q)
+template <typename W>
+W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
+  constexpr int kW = (int)sizeof(W);
   std::vector<int32_t> pivots(n, -1);
   pivots[0] = 0;
   std::vector<uint8_t> cmp(n, 0);  // exp_cmp_plaintext: the last opened bit of each row
   std::vector<uint32_t> idx, src(n);
-  std::vector<uint64_t> opened;
+  std::vector<W> opened;
   uint32_t *d_idx = (uint32_t *)c->scratch_get("qs_idx", 8 * n + 64);
   uint32_t *d_src = (uint32_t *)c->scratch_get("qs_src", 4 * n + 64);
-  uint64_t *red = (uint64_t *)c->scratch_get("qs_red", 24 * n + 64);
-  uint64_t *piv = (uint64_t *)c->scratch_get("qs_piv", 24 * n + 64);
-  uint64_t *eq = (uint64_t *)c->scratch_get("qs_eq", 24 * n + 64);
-  uint64_t *gt = (uint64_t *)c->scratch_get("qs_gt", 24 * n + 64);
+  W *red = (W *)c->scratch_get("qs_red", 3 * kW * n + 64);
+  W *piv = (W *)c->scratch_get("qs_piv", 3 * kW * n + 64);
+  W *eq = (W *)c->scratch_get("qs_eq", 3 * kW * n + 64);
+  W *gt = (W *)c->scratch_get("qs_gt", 3 * kW * n + 64);
   uint64_t num_pivots = 1;
   for (;;) {
     // the non-pivot rows, then the pivot of each (prefix max of pivots)
@@ -127,13 +138,13 @@ uint64_t *quicksort_rounds(orqg_ctx *c, uint64_t *P, uint64_t *Q, uint64_t n) {
     if (size) {
       ORQG_HIP(hipMemcpyAsync(d_idx, idx.data(), 8 * size, hipMemcpyHostToDevice, c->stream));
       const unsigned g = grid_for(size, 256, c->num_cu * 16);
-      hipLaunchKernelGGL(k_qs_gather, dim3(g), dim3(256), 0, c->stream, size, d_idx, P, red);
-      hipLaunchKernelGGL(k_qs_gather, dim3(g), dim3(256), 0, c->stream, size, d_idx + size, P,
-                         piv);
+      hipLaunchKernelGGL(k_qs_gather<W>, dim3(g), dim3(256), 0, c->stream, size, d_idx, P, red);
+      hipLaunchKernelGGL(k_qs_gather<W>, dim3(g), dim3(256), 0, c->stream, size, d_idx + size,
+                         P, piv);
       ORQG_HIP(hipGetLastError());
       // reduced_vec._compare(pivot_vec, temp, comparisons): gt = reduced > pivot
-      orqg_vec vr = view(c, red, size, 8, ORQG_BSHARED), vp = view(c, piv, size, 8, ORQG_BSHARED);
-      orqg_vec ve = view(c, eq, size, 8, ORQG_BSHARED), vg = view(c, gt, size, 8, ORQG_BSHARED);
+      orqg_vec vr = view(c, red, size, kW, ORQG_BSHARED), vp = view(c, piv, size, kW, ORQG_BSHARED);
+      orqg_vec ve = view(c, eq, size, kW, ORQG_BSHARED), vg = view(c, gt, size, kW, ORQG_BSHARED);
       compare(c, &vr, &vp, &ve, &vg, true, 0, 0);
       opened.resize(size);
       open_vec(c, &vg, opened.data());  // comparisons.open()
@@ -156,7 +167,7 @@ uint64_t *quicksort_rounds(orqg_ctx *c, uint64_t *P, uint64_t *Q, uint64_t n) {
     // cmp[j] before any swap has touched position j, and the next round
     // rewrites every non-pivot position before reading it)
     ORQG_HIP(hipMemcpyAsync(d_src, src.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_qs_gather, dim3(grid_for(n, 256, c->num_cu * 16)), dim3(256), 0,
+    hipLaunchKernelGGL(k_qs_gather<W>, dim3(grid_for(n, 256, c->num_cu * 16)), dim3(256), 0,
                        c->stream, n, d_src, P, Q);
     ORQG_HIP(hipGetLastError());
     std::swap(P, Q);
@@ -169,28 +180,19 @@ uint64_t *quicksort_rounds(orqg_ctx *c, uint64_t *P, uint64_t *Q, uint64_t n) {
   }
 }
 
-}  // namespace
-
-void quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out) {
-  require_one_thread(c, "quicksort");
-  if (!v || !perm_out || v->width != 4 || perm_out->width != 4 || perm_out->n != v->n)
-    throw Error(ORQG_E_ARG, "quicksort: 32-bit keys and a 32-bit permutation of the same size");
+// pad, shuffle, sort rounds and unpad for keys of type K padded to P
+template <typename K, typename P>
+void quicksort_padded(orqg_ctx *c, orqg_vec *v, bool desc, orqg_vec *perm_out) {
   const uint64_t n = v->n;
-  if (n >= (1ull << 31)) throw Error(ORQG_E_ARG, "quicksort: n must be < 2^31");
-  const bool desc = order == ORQG_DESC;
-  // 1 permutation for the shuffle, 1 for remove_padding's b2a, 1 pair for
-  // the inversion; pairs are single permutations in 3PC (quicksort.h:215-224)
-  reserve_perms(c, n, 3);
-  if (n == 0) return;
-  TimedRegion tr(c, "quicksort");
-  uint64_t *P = (uint64_t *)c->scratch_get("qs_P", 24 * n + 64);
-  uint64_t *Q = (uint64_t *)c->scratch_get("qs_Q", 24 * n + 64);
+  constexpr int kP = (int)sizeof(P);
+  P *Pb = (P *)c->scratch_get("qs_P", 3 * kP * n + 64);
+  P *Qb = (P *)c->scratch_get("qs_Q", 3 * kP * n + 64);
   const unsigned grid = grid_for(n, 256, c->num_cu * 16);
-  hipLaunchKernelGGL(k_qs_pad, dim3(grid), dim3(256), 0, c->stream, n, (int)desc,
-                     (const uint32_t *)v->d, P);
+  hipLaunchKernelGGL((k_qs_pad<K, P>), dim3(grid), dim3(256), 0, c->stream, n, (int)desc,
+                     (const K *)v->d, Pb);
   ORQG_HIP(hipGetLastError());
   {  // v.shuffle()
-    orqg_vec pv = view(c, P, n, 8, ORQG_BSHARED);
+    orqg_vec pv = view(c, Pb, n, kP, ORQG_BSHARED);
     ShardedPerm pi = next_perm(c, n);
     try {
       apply_sharded(c, &pv, pi, false);
@@ -202,10 +204,32 @@ void quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out) {
     ORQG_HIP(hipStreamSynchronize(c->stream));
     free_perm(pi);
   }
-  P = quicksort_rounds(c, P, Q, n);
-  hipLaunchKernelGGL(k_qs_unpad, dim3(grid), dim3(256), 0, c->stream, n, (int)desc, P,
-                     (uint32_t *)v->d, (uint32_t *)perm_out->d);
+  Pb = quicksort_rounds<P>(c, Pb, Qb, n);
+  hipLaunchKernelGGL((k_qs_unpad<K, P>), dim3(grid), dim3(256), 0, c->stream, n, (int)desc, Pb,
+                     (K *)v->d, (uint32_t *)perm_out->d);
   ORQG_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void quicksort(orqg_ctx *c, orqg_vec *v, int order, orqg_vec *perm_out) {
+  require_one_thread(c, "quicksort");
+  if (!v || !perm_out || (v->width != 4 && v->width != 8) || perm_out->width != 4 ||
+      perm_out->n != v->n)
+    throw Error(ORQG_E_ARG,
+                "quicksort: 32- or 64-bit keys and a 32-bit permutation of the same size");
+  const uint64_t n = v->n;
+  if (n >= (1ull << 31)) throw Error(ORQG_E_ARG, "quicksort: n must be < 2^31");
+  const bool desc = order == ORQG_DESC;
+  // 1 permutation for the shuffle, 1 for remove_padding's b2a, 1 pair for
+  // the inversion; pairs are single permutations in 3PC (quicksort.h:215-224)
+  reserve_perms(c, n, 3);
+  if (n == 0) return;
+  TimedRegion tr(c, "quicksort");
+  if (v->width == 4)
+    quicksort_padded<uint32_t, uint64_t>(c, v, desc, perm_out);
+  else
+    quicksort_padded<uint64_t, u128>(c, v, desc, perm_out);
   perm_out->enc = ORQG_BSHARED;
   perm_b2a(c, perm_out);
   if (desc) perm_negate(c, perm_out);

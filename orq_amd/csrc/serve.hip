@@ -81,7 +81,8 @@ struct TableGuard {
 
 // The request as the reference's table calls make it: a table {[K], V} with
 // its valid column, then EncodedTable::sort on [K] or EncodedTable::aggregate
-// on [K] with V summed (both with their default options and protocol).
+// on [K] with V summed (both with their default options and the DEFAULT
+// protocol, radix sort on the server's contexts).
 void run_request(orqg_ctx *c, const orqg_request &r) {
   ORQG_CALL(orqg_ctx_rekey(c, r.keys));
   const char *names[2] = {"[K]", "V"};
@@ -195,6 +196,9 @@ int orqg_server_create(const orqg_server_config *cfg, orqg_server **out) {
       return rc;
     }
     if (cfg->batch_rows) c->batcher = &s->batcher;
+    // DEFAULT_SORT_PROTO=RADIXSORT (sorting.h:6-9): small radix sorts of
+    // concurrent requests can then share batched launches
+    c->default_sort = ORQG_RADIXSORT;
     s->ctxs.push_back(c);
   }
   for (int i = 0; i < cfg->slots; i++) s->workers.emplace_back(orqg::worker, s, i);

@@ -254,10 +254,6 @@ void table_sort(orqg_ctx *c, orqg_vec *const *keys, size_t nkeys, const int *ord
   const int nk = (int)nkeys - ns;
   const int nc = (int)(na + nb);
   const int L = 8 * keys[0]->width;
-  if (protocol == ORQG_QUICKSORT)
-    for (size_t j = 0; j < nkeys; j++)
-      if (!(single_bit && single_bit[j]) && keys[j]->width != 4)
-        throw Error(ORQG_E_ARG, "table_sort: QUICKSORT takes 32-bit sort columns");
   // sorting.h:531-550 for 3PC (pairs are single permutations): quicksort
   // needs one more permutation per multi-bit key (its shuffle), radix sort
   // one more pair per key bit

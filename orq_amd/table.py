@@ -24,9 +24,9 @@ NON_AGGREGATIONS = ("copy", "valid")  # AggregationSelector::isAggregation
 RADIXSORT = "RADIXSORT"
 BITONICSORT = "BITONICSORT"
 QUICKSORT = "QUICKSORT"
-# SortingProtocol::DEFAULT is QUICKSORT (sorting.h:5-22).  Quicksort pads a
-# 32-bit key to 64 bits; for 64-bit tables (128-bit padded values) DEFAULT
-# resolves to RADIXSORT here.
+# SortingProtocol::DEFAULT: the backend context's `default_sort`, the
+# reference's compile-time DEFAULT_SORT_PROTO (sorting.h:6-9), QUICKSORT
+# unless set.
 DEFAULT = "DEFAULT"
 
 
@@ -129,7 +129,7 @@ class EncodedTable:
         """EncodedTable::sort (encoded_table.h:900-998).  `spec` is a list of
         column names (all ascending) or of (name, order) pairs."""
         if protocol == DEFAULT:
-            protocol = QUICKSORT if self.width == 4 else RADIXSORT
+            protocol = getattr(self.ctx, "default_sort", QUICKSORT)
         if protocol not in (RADIXSORT, BITONICSORT, QUICKSORT):
             raise ValueError(f"sorting protocol {protocol!r} is not provided")
         spec = [(s, ASC) if isinstance(s, str) else (s[0], s[1]) for s in spec]

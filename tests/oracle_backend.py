@@ -17,8 +17,10 @@ def _tag(v, enc):
 
 
 class OracleCtx:
-    def __init__(self, keys, batch_size=-12, threads=1, thread_keys=()):
+    def __init__(self, keys, batch_size=-12, threads=1, thread_keys=(), default_sort="QUICKSORT"):
         self.c = O.Ctx(keys, batch_size, threads=threads, thread_keys=thread_keys)
+        # the protocol SortingProtocol::DEFAULT stands for (DEFAULT_SORT_PROTO)
+        self.default_sort = default_sort
 
     # vectors
     def vec(self, n, width, encoding=B):

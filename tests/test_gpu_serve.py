@@ -72,6 +72,7 @@ def _table_flow(G, job):
     from orq_amd.table import EncodedTable
 
     c = G.Ctx(bytes(job.req.keys))
+    c.default_sort = "RADIXSORT"  # the server's contexts: DEFAULT_SORT_PROTO=RADIXSORT
     w = job.key_in.dtype.itemsize
     K = c.vec(job.n, w, G.ORQG_BSHARED)
     K.set_shares(list(job.key_in))
@@ -88,8 +89,8 @@ def _table_flow(G, job):
 @pytest.mark.parametrize("width", [4, 8])
 def test_server_matches_table_layer(G, width):
     """Every request equals the table layer's EncodedTable::sort / aggregate
-    with their default options and protocol (quicksort on 32-bit tables,
-    radix sort on 64-bit ones)."""
+    with their default options and the server's DEFAULT protocol (radix
+    sort, the reference built with DEFAULT_SORT_PROTO=RADIXSORT)."""
     from orq_amd.serve import Server
 
     jobs, plain = _make_jobs(G, [1024, 3000, 4096, 777], width=width, seed=1)

@@ -131,7 +131,8 @@ def test_reference_table_sort_multi(O, G, keys, tiebreak, protocol):
 
 
 def test_table_sort_valid_bit_and_int64_payload(O, G, keys):
-    """[##V] as a single-bit key (DESC) beside an int64 key and payloads."""
+    """[##V] as a single-bit key (DESC) beside an int64 key and payloads,
+    with the DEFAULT protocol (quicksort for the int64 key)."""
     n = 4099
     rng = np.random.default_rng(10)
     key = rng.integers(-50, 50, n)
@@ -148,7 +149,7 @@ def test_table_sort_valid_bit_and_int64_payload(O, G, keys):
     gt = EncodedTable.secret_share(gc, [key, pay, valid], schema, width=8)
     spec = [("[##V]", "DESC"), ("[K]", "ASC")]
     a, b = _split(ocols, keys=("[##V]", "[K]"))
-    oc.table_sort([ocols["[##V]"], ocols["[K]"]], ["DESC", "ASC"], [1, 0], a, b)
+    oc.table_sort([ocols["[##V]"], ocols["[K]"]], ["DESC", "ASC"], [1, 0], a, b, "QUICKSORT")
     gt.sort(spec)
     for name in schema:
         _same_shares(ocols[name], gt[name])

@@ -180,15 +180,16 @@ def test_radix_sort_vs_reference(width, bits, desc, n):
     assert _sent(oc) == sent
 
 
+@pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("desc", [False, True])
 @pytest.mark.parametrize("n", [1, 2, 3, 1000, 5000])
-def test_quicksort_vs_reference(desc, n):
+def test_quicksort_vs_reference(desc, n, width):
     rng = np.random.default_rng(n)
-    x = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    x = rng.integers(0, np.iinfo(_DT[width]).max, n, dtype=_DT[width], endpoint=True)
     x[::7] = x[n // 2]  # ties
     v, perm, sent = R.quicksort(KEYS, x, desc=desc)
     oc = O.Ctx(KEYS)
-    ov = oc.secret_share(x, 4)
+    ov = oc.secret_share(x, width)
     op = oc.quicksort(ov, desc=desc)
     np.testing.assert_array_equal(faithful(ov), v)
     np.testing.assert_array_equal(faithful(op), perm)
@@ -211,7 +212,8 @@ def _host():
 
 @pytest.mark.parametrize("width,nkeys,proto,desc", [
     (4, 1, "QUICKSORT", False), (4, 2, "QUICKSORT", True), (4, 1, "RADIXSORT", False),
-    (4, 2, "RADIXSORT", False), (8, 1, "RADIXSORT", True), (4, 1, "BITONICSORT", False),
+    (4, 2, "RADIXSORT", False), (8, 1, "RADIXSORT", True), (8, 1, "QUICKSORT", False),
+    (8, 2, "QUICKSORT", True), (4, 1, "BITONICSORT", False),
     (4, 2, "BITONICSORT", True), (8, 2, "BITONICSORT", False)])
 def test_table_sort_vs_reference(width, nkeys, proto, desc):
     OracleCtx, T = _host()
@@ -231,52 +233,54 @@ def test_table_sort_vs_reference(width, nkeys, proto, desc):
     assert _sent(oc.c) == sent
 
 
-# The reference's DEFAULT sort protocol is quicksort (operators/sorting.h:6-9);
-# the table layer's DEFAULT takes quicksort for 32-bit tables and radix sort
-# for 64-bit ones, so 64-bit tables are checked against the reference built
-# with radix sort as its default (the same compile-time option).
-_DEFAULT_SORT = {4: "QUICKSORT", 8: "RADIXSORT"}
+# The reference's DEFAULT sort protocol is quicksort (operators/sorting.h:6-9)
+# unless it is built with DEFAULT_SORT_PROTO set; the table layer's DEFAULT is
+# its context's default_sort.  Both settings are checked, against the shipped
+# build and against the build with radix sort as its default.
+_SORT_DEFAULTS = ["QUICKSORT", "RADIXSORT"]
 
 
+@pytest.mark.parametrize("default_sort", _SORT_DEFAULTS)
 @pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("nr,nl,dom", [(8, 32, 12), (40, 160, 64)])
-def test_join_groupby_vs_reference(width, nr, nl, dom):
+def test_join_groupby_vs_reference(width, nr, nl, dom, default_sort):
     OracleCtx, _ = _host()
     from test_oracle_join import relational_inputs, run_pipeline
 
     rkey, attr, lkey, value = relational_inputs(nr, nl, dom, nr + nl + width)
     ref_table, sent = R.join_groupby(KEYS, rkey, attr, lkey, value, width,
-                                     default_sort=_DEFAULT_SORT[width])
-    oc = OracleCtx(KEYS)
+                                     default_sort=default_sort)
+    oc = OracleCtx(KEYS, default_sort=default_sort)
     ot = run_pipeline(oc, rkey, attr, lkey, value, width)
     _same_table(ref_table, ot)
     assert _sent(oc.c) == sent
 
 
+@pytest.mark.parametrize("default_sort", _SORT_DEFAULTS)
 @pytest.mark.parametrize("width", [4, 8])
-def test_distinct_vs_reference(width):
+def test_distinct_vs_reference(width, default_sort):
     OracleCtx, T = _host()
     rng = np.random.default_rng(3)
     k, v = rng.integers(0, 6, 200), rng.integers(0, 2, 200)
-    ref_table, sent = R.table_distinct(KEYS, k, v, width, default_sort=_DEFAULT_SORT[width])
-    oc = OracleCtx(KEYS)
+    ref_table, sent = R.table_distinct(KEYS, k, v, width, default_sort=default_sort)
+    oc = OracleCtx(KEYS, default_sort=default_sort)
     ot = T.EncodedTable.secret_share(oc, [k, v], ["[K]", "[V2]"], width)
     ot.distinct(["[K]", "[V2]"])
     _same_table(ref_table, ot)
     assert _sent(oc.c) == sent
 
 
+@pytest.mark.parametrize("default_sort", _SORT_DEFAULTS)
 @pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("reverse", [False, True])
-def test_aggregate_vs_reference(width, reverse):
+def test_aggregate_vs_reference(width, reverse, default_sort):
     OracleCtx, T = _host()
     rng = np.random.default_rng(7 + width)
     n = 150
     cols = [rng.integers(0, 9, n), rng.integers(0, 1000, n), np.zeros(n, np.int64),
             rng.integers(-50, 1000, n), rng.integers(-50, 1000, n)]
-    ref_table, sent = R.table_aggregate(KEYS, cols, width, reverse,
-                                        default_sort=_DEFAULT_SORT[width])
-    oc = OracleCtx(KEYS)
+    ref_table, sent = R.table_aggregate(KEYS, cols, width, reverse, default_sort=default_sort)
+    oc = OracleCtx(KEYS, default_sort=default_sort)
     ot = T.EncodedTable.secret_share(oc, cols, ["[K]", "A", "C", "[B]", "[M]"], width)
     ot.aggregate(["[K]"], [("A", "A", "sum"), ("A", "C", "count"), ("[B]", "[B]", "max"),
                            ("[M]", "[M]", "min")], reverse=reverse)
