@@ -44,6 +44,14 @@ SORT_MODEL_BYTES_PER_ROW = 64 * (48 * SORT_MODEL_W + 304) + 6 * SORT_MODEL_W + 5
 ONLINE_KEYSTREAM_PER_ROW_BIT = 168
 
 
+
+def _with_build(line):
+    """The source hash of the liborqg.so that produced the line."""
+    import orq_amd
+
+    line["build"] = orq_amd.build_info()
+    return line
+
 def zipf_keys(n, s=1.1, domain=1 << 20, seed=0x0A11CE03):
     from orq_amd import splitmix64
 
@@ -186,7 +194,7 @@ def bench_primitives(args):
             "dtype": "u64", "data": "synthetic: splitmix64 seeds 0x0A11CE02 / 0x0A11CE12",
             "config": {"workload": f"xor/and/==/</b2a/keystream over {n} boolean-shared uint64"},
             "ops": res, "correct": ok}
-    print(json.dumps(line), flush=True)
+    print(json.dumps(_with_build(line)), flush=True)
 
 
 def bench_table(args):
@@ -226,7 +234,7 @@ def bench_table(args):
             "config": {"workload": f"EncodedTable {{[K],[P1],[P2],[##V]}} x {n} rows: shuffle, "
                                    "then sort on [K] (radix; bitonic beside it)"},
             "protocols": res}
-    print(json.dumps(line), flush=True)
+    print(json.dumps(_with_build(line)), flush=True)
 
 
 def bench_join(args):
@@ -290,7 +298,7 @@ def bench_join(args):
                                    "GROUP BY [key] SUM(value), COUNT, MAX([vmax]); radix sorts"},
             "join_ms": round((t1 - t0) * 1e3, 1), "groupby_ms": round((t2 - t1) * 1e3, 1),
             "families_ms": fams, "groups": int(len(keys_u)), "correct": ok}
-    print(json.dumps(line), flush=True)
+    print(json.dumps(_with_build(line)), flush=True)
 
 
 def bench_batch(args):
@@ -347,7 +355,7 @@ def bench_batch(args):
                                    "each, incl. gen_perm"},
             "one_by_one_ms": round(ts * 1e3, 2), "one_by_one_Mrows_s": round(m * n / ts / 1e6, 4),
             "speedup": round(ts / tb, 2), "correct": ok}
-    print(json.dumps(line), flush=True)
+    print(json.dumps(_with_build(line)), flush=True)
 
 
 def make_requests(count, seed=0x0A11CE05, lo=10, hi=22, width=8):
@@ -476,7 +484,7 @@ def bench_serve(args):
             "modes": {m: ({k: b[k] for k in keys} if b else None) for m, b in bests.items()},
             "failed_requests": n_failed,
             "runs": res}
-    print(json.dumps(line), flush=True)
+    print(json.dumps(_with_build(line)), flush=True)
     if not valid:
         sys.exit(1)
 
@@ -654,7 +662,7 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_cap)
-    print(json.dumps(line), flush=True)
+    print(json.dumps(_with_build(line)), flush=True)
 
 
 if __name__ == "__main__":

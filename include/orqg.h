@@ -85,6 +85,10 @@ int orqg_ctx_rekey(orqg_ctx *ctx, const uint8_t keys[ORQG_NUM_KEYS][32]);
 int orqg_ctx_set_default_sort(orqg_ctx *ctx, int protocol);
 int orqg_ctx_default_sort(orqg_ctx *ctx);
 const char *orqg_last_error(orqg_ctx *ctx);
+/* The hash of the sources the library was built from (SHA-256 prefix over
+ * orq_amd/csrc, this header and the compile flags; orq_amd/build.py).  No
+ * counterpart in the reference.                                              */
+const char *orqg_build_info(void);
 int orqg_sync(orqg_ctx *ctx);
 /* the context's HIP stream (hipStream_t), for callers that order their own work */
 void *orqg_stream(orqg_ctx *ctx);

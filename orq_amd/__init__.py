@@ -58,7 +58,7 @@ EXPORTS = [
     "orqg_vec_read_share_files", "orqg_vec_write_share_files", "orqg_table_read_share_csv",
     "orqg_table_write_share_csv", "orqg_table_read_csv_data", "orqg_public_fill", "orqg_vec_copy_rows",
     "orqg_vec_broadcast_row", "orqg_ctx_rekey", "orqg_ctx_set_default_sort",
-    "orqg_ctx_default_sort", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
+    "orqg_ctx_default_sort", "orqg_build_info", "orqg_add_b", "orqg_sub_b", "orqg_ppa", "orqg_rca_compare",
     "orqg_a2b", "orqg_secret_share_from", "orqg_server_create", "orqg_server_submit",
     "orqg_server_poll", "orqg_server_pending", "orqg_server_get_stats", "orqg_server_now_ms",
     "orqg_server_destroy",
@@ -146,6 +146,7 @@ def lib():
             "orqg_ctx_rekey": (i32, [vp, ctypes.c_char_p]),
             "orqg_ctx_set_default_sort": (i32, [vp, i32]),
             "orqg_ctx_default_sort": (i32, [vp]),
+            "orqg_build_info": (ctypes.c_char_p, []),
             "orqg_server_create": (i32, [vp, vp]),
             "orqg_server_submit": (i32, [vp, vp]),
             "orqg_server_poll": (i32, [vp, vp, i32, vp, i32]),
@@ -201,6 +202,11 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def build_info():
+    """The source hash compiled into liborqg.so (orqg_build_info)."""
+    return lib().orqg_build_info().decode()
 
 
 def _ptr(a):

@@ -365,6 +365,11 @@ int orqg_ctx_default_sort(orqg_ctx *c) { return c ? c->default_sort : ORQG_E_ARG
 
 const char *orqg_last_error(orqg_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+#ifndef ORQG_SOURCE_HASH
+#define ORQG_SOURCE_HASH "unknown"
+#endif
+const char *orqg_build_info(void) { return ORQG_SOURCE_HASH; }
+
 int orqg_sync(orqg_ctx *c) {
   API_BEGIN(c)
   ORQG_HIP(hipStreamSynchronize(c->stream));

@@ -1,6 +1,8 @@
 """The C ABI: liborqg.so loads without a GPU and exports every symbol that
 include/orqg.h declares (no compute calls here)."""
 import ctypes
+
+import pytest
 import os
 import re
 
@@ -43,3 +45,22 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         assert "no CPU fallback" in str(e)
     else:
         raise AssertionError("expected OrqgError")
+
+
+def _library_matches_sources():
+    import orq_amd
+    from orq_amd import build
+
+    assert orq_amd.build_info() == build.source_hash(), (
+        "liborqg.so was built from other sources: rebuild it")
+
+
+def test_library_matches_sources():
+    """The library under test was built from these sources (a hash of
+    orq_amd/csrc, include/orqg.h and the flags, compiled in)."""
+    _library_matches_sources()
+
+
+@pytest.mark.gpu
+def test_gpu_library_matches_sources():
+    _library_matches_sources()
