@@ -72,7 +72,69 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(cap_s, sizes=(1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)):
+def _size_ladder(run_one, cap_s, sizes):
+    """Run run_one(n) -> dict with 's' over the sizes while the next size's
+    projected time fits the cap; later sizes are reported as cut."""
+    runs, spent, last = [], 0.0, None
+    for n in sizes:
+        if last is not None:
+            est = last[1] * n / last[0] * 1.6  # per-row cost grows with n (caches)
+            if spent + est > cap_s:
+                runs.append({"rows": n, "run": False,
+                             "why": f"projected {est:.0f} s would pass the {cap_s:.0f} s cap"})
+                continue
+        r = run_one(n)
+        spent += r["s"]
+        last = (n, r["s"])
+        runs.append(r)
+    return runs
+
+
+def reference_baseline(cap_s, sizes=(1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)):
+    """The reference's own radix_sort (its RunTime, Replicated_3PC and
+    operators, compiled from its sources into oracle/_ref) on the host CPU:
+    three party processes, T worker threads each with 3 (T + 1) <= nproc,
+    batch size -12, int64 Zipf keys, timed as bench/micro/micro_sorting.cpp
+    times it (permutation generation included; online-only beside it)."""
+    from oracle import ref as R
+    from orq_amd import keys_from_seed
+
+    nproc = os.cpu_count() or 3
+    T = max(1, nproc // 3 - 1)
+    k0 = keys_from_seed(0x0A11CE03)
+    extra = [keys_from_seed(0x0A11CE03 + 0x100 * t) for t in range(1, T)]
+
+    def one(n):
+        keys = zipf_keys(n)
+        v, t_res, t_all = R.radix_bench(k0, keys.view(np.uint64), thread_keys=extra)
+        ok = bool(np.array_equal((v[0, 0] ^ v[0, 1] ^ v[1, 1]).view(np.int64),
+                                 np.sort(keys.view(np.int64))))
+        return {"rows": n, "run": True, "s": round(t_all, 3),
+                "Mrows_per_s": round(n / t_all / 1e6, 4),
+                "online_Mrows_per_s": round(n / (t_all - t_res) / 1e6, 4),
+                "reserve_s": round(t_res, 3), "correct": ok}
+
+    runs = _size_ladder(one, cap_s, sizes)
+    top = [r for r in runs if r["run"]][-1]
+    return {
+        "value": top["Mrows_per_s"],
+        "unit": "Mrows/s",
+        "cores": 3 * T,
+        "kind": "reference",
+        "sample": f"the reference's radix_sort of {top['rows']} int64 Zipf keys incl. "
+                  f"permutation generation ({top['s']:.2f} s); 3 party processes x {T} "
+                  f"worker threads",
+        "threads_per_party": T,
+        "batch_size": -12,
+        "nproc": nproc,
+        "cpu_model": _cpu_model(),
+        "cap_s": cap_s,
+        "online_Mrows_per_s": top["online_Mrows_per_s"],
+        "sizes": runs,
+    }
+
+
+def port_baseline(cap_s, sizes=(1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)):
     """The oracle's 3PC radix sort of int64 Zipf keys on the host CPU: the
     three parties on three threads, one worker thread per party (the
     reference's default), the reference's batch schedule (batch_size -12).
@@ -83,14 +145,7 @@ def cpu_baseline(cap_s, sizes=(1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)):
     from oracle import oracle as O
     from orq_amd import keys_from_seed
 
-    runs, spent, last = [], 0.0, None
-    for n in sizes:
-        if last is not None:
-            est = last[1] * n / last[0] * 1.6  # per-row cost grows with n (caches)
-            if spent + est > cap_s:
-                runs.append({"rows": n, "run": False,
-                             "why": f"projected {est:.0f} s would pass the {cap_s:.0f} s cap"})
-                continue
+    def one(n):
         keys = zipf_keys(n)
         ctx = O.Ctx(keys_from_seed(0x0A11CE03), -12)
         v = ctx.secret_share(keys, 8)
@@ -100,15 +155,13 @@ def cpu_baseline(cap_s, sizes=(1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)):
         ctx.radix_sort(v)
         t2 = time.perf_counter()
         ok = bool(np.array_equal(ctx.open(v).view(np.int64), np.sort(keys.view(np.int64))))
-        spent += t2 - t0
-        last = (n, t2 - t0)
-        runs.append({"rows": n, "run": True, "s": round(t2 - t0, 3),
-                     "Mrows_per_s": round(n / (t2 - t0) / 1e6, 4),
-                     "online_Mrows_per_s": round(n / (t2 - t1) / 1e6, 4),
-                     "reserve_s": round(t1 - t0, 3), "correct": ok})
-        del ctx, v
-    done = [r for r in runs if r["run"]]
-    top = done[-1]
+        return {"rows": n, "run": True, "s": round(t2 - t0, 3),
+                "Mrows_per_s": round(n / (t2 - t0) / 1e6, 4),
+                "online_Mrows_per_s": round(n / (t2 - t1) / 1e6, 4),
+                "reserve_s": round(t1 - t0, 3), "correct": ok}
+
+    runs = _size_ladder(one, cap_s, sizes)
+    top = [r for r in runs if r["run"]][-1]
     return {
         "value": top["Mrows_per_s"],
         "unit": "Mrows/s",
@@ -124,6 +177,22 @@ def cpu_baseline(cap_s, sizes=(1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)):
         "online_Mrows_per_s": top["online_Mrows_per_s"],
         "sizes": runs,
     }
+
+
+def cpu_baseline(cap_s):
+    """The reference itself where its compiled party programs are present
+    (oracle/_ref, built with the oracle), with the oracle's restatement
+    beside it on a 2^18-row sample; the restatement alone otherwise."""
+    from oracle import ref as R
+
+    if R.party_built():
+        try:
+            base = reference_baseline(cap_s)
+            base["port"] = port_baseline(cap_s, sizes=(1 << 18,))
+            return base
+        except Exception as e:  # the programs need the host's libsodium
+            sys.stderr.write(f"reference CPU baseline unavailable ({e}); using the oracle\n")
+    return port_baseline(cap_s)
 
 
 # Algorithmic bytes per row of one k_round2 launch (one sharded-permutation

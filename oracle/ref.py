@@ -27,6 +27,12 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
 
 
+def party_built():
+    """The party programs are present (they travel with the tree; no sources
+    are needed to run them)."""
+    return os.path.exists(_PARTY) and os.path.exists(_PARTY_RADIX)
+
+
 def available():
     built = lambda: all(os.path.exists(f) for f in (_LIB_PATH, _PARTY, _PARTY_RADIX))
     if built():
@@ -168,7 +174,7 @@ class Ctx:
 
 # ---- the reference's runtime and operators, three party processes ----------
 
-RADIX, QUICKSORT, TABLE_SORT, JOIN_GROUPBY, DISTINCT, AGGREGATE, SHUFFLE, CIRCUITS = range(1, 9)
+RADIX, QUICKSORT, TABLE_SORT, JOIN_GROUPBY, DISTINCT, AGGREGATE, SHUFFLE, CIRCUITS, RADIX_BENCH = range(1, 10)
 # orq::SortingProtocol (operators/sorting.h:15-21)
 BITONICSORT_ID, QUICKSORT_ID, RADIXSORT_ID = 0, 1, 2
 
@@ -183,7 +189,7 @@ def run_parties(keys: bytes, scenario: int, params, payload: bytes, batch_size=-
     import socket
     import tempfile
 
-    if not available():
+    if not party_built():
         raise RuntimeError("the reference's party program is not built (needs /root/reference)")
     threads = 1 + len(thread_keys)
     assert len(keys) == 320 and all(len(k) == 320 for k in thread_keys) and len(params) <= 8
@@ -363,3 +369,21 @@ def circuits(keys, x, y, batch_size=-12):
         sent.append(r.u64())
         assert r.done()
     return {k: np.stack(v) for k, v in res.items()}, sent
+
+
+def radix_bench(keys, data, thread_keys=(), batch_size=-12, timeout=900):
+    """The reference's radix_sort timed in its three party processes: (the
+    sorted shares [party][idx][n], reserve seconds, total seconds), the times
+    the slowest party's."""
+    d = np.ascontiguousarray(data)
+    outs = run_parties(keys, RADIX_BENCH, [d.itemsize, 8 * d.itemsize, 0, 0, 0, 0, 0, d.size],
+                       d.tobytes(), batch_size, timeout=timeout, thread_keys=thread_keys)
+    v, t_res, t_all = [], 0.0, 0.0
+    for b in outs:
+        r = Reader(b)
+        v.append(r.pair(d.dtype, d.size))
+        t_res = max(t_res, float(r.arr(np.float64, 1)[0]))
+        t_all = max(t_all, float(r.arr(np.float64, 1)[0]))
+        r.u64()
+        assert r.done()
+    return np.stack(v), t_res, t_all

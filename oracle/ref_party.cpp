@@ -9,7 +9,8 @@
 // DefaultProtocolFactory, HMShardedPermutationGenerator behind the
 // PermutationManager, and the operators (radix_sort, quicksort, table sort,
 // aggregate, distinct, inner join, the comparison and adder circuits) called
-// as the reference's tests call them.
+// as the reference's tests call them, and radix sort timed as its
+// micro-benchmark times it.
 //
 // What this file supplies is what orq_runtime_init (backend/common/setup.h:8-57)
 // gets from its launcher: the party id, a Communicator, and the randomness
@@ -29,6 +30,7 @@
 #include <sys/socket.h>
 #include <unistd.h>
 
+#include <chrono>
 #include <condition_variable>
 #include <cstdio>
 #include <deque>
@@ -430,6 +432,31 @@ void circuits(const int64_t *, const uint8_t *payload, size_t n, Out &out) {
   out.put_ev((*ax.a2b()).vector);
 }
 
+// scenario 9: the CPU baseline.  v = secret_share_b(data, 0) (not timed),
+// then radix_sort(v) timed as bench/micro/micro_sorting.cpp:31-38 times it,
+// with its permutations reserved first (radixsort.h:121-128) and timed apart,
+// so that the online part can be reported too.  params: width, bits.
+// out: v's two shares, then f64 seconds for the reserve and for the whole.
+template <typename T>
+void radix_bench(const int64_t *prm, const uint8_t *payload, size_t n, Out &out) {
+  auto data = vec_from<T>(payload, n);
+  R3::BSharedVector<T> v = R3::secret_share_b(data, 0);
+  const size_t bits = (size_t)prm[1];
+  const auto t0 = std::chrono::steady_clock::now();
+  orq::random::PermutationManager::get()->reserve(n, 1, bits + 1);
+  const auto t1 = std::chrono::steady_clock::now();
+  auto perm = radix_sort(v, SortOrder::ASC, bits);
+  const auto t2 = std::chrono::steady_clock::now();
+  (void)perm;
+  out.put_ev(v.vector);
+  const double s_res = std::chrono::duration<double>(t1 - t0).count();
+  const double s_all = std::chrono::duration<double>(t2 - t0).count();
+  out.bytes.insert(out.bytes.end(), reinterpret_cast<const uint8_t *>(&s_res),
+                   reinterpret_cast<const uint8_t *>(&s_res) + 8);
+  out.bytes.insert(out.bytes.end(), reinterpret_cast<const uint8_t *>(&s_all),
+                   reinterpret_cast<const uint8_t *>(&s_all) + 8);
+}
+
 #define ORQ_REF_BY_WIDTH(fn)                                                             \
   void fn##_any(const int64_t *prm, const uint8_t *payload, size_t n, Out &out) {           \
     if (prm[0] == 4)                                                                        \
@@ -445,6 +472,7 @@ ORQ_REF_BY_WIDTH(table_distinct)
 ORQ_REF_BY_WIDTH(table_aggregate)
 ORQ_REF_BY_WIDTH(table_shuffle)
 ORQ_REF_BY_WIDTH(circuits)
+ORQ_REF_BY_WIDTH(radix_bench)
 #undef ORQ_REF_BY_WIDTH
 
 }  // namespace
@@ -507,6 +535,7 @@ int main(int argc, char **argv) {
       case 6: table_aggregate_any(prm, payload, n, out); break;
       case 7: table_shuffle_any(prm, payload, n, out); break;
       case 8: circuits_any(prm, payload, n, out); break;
+      case 9: radix_bench_any(prm, payload, n, out); break;
       default:
         std::fprintf(stderr, "unknown scenario %ld\n", (long)scenario);
         return 2;
