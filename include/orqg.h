@@ -51,16 +51,20 @@ enum { ORQG_EQ = 0, ORQG_NE = 1, ORQG_LT = 2, ORQG_GT = 3, ORQG_LE = 4, ORQG_GE 
 typedef struct {
   int device;                        /* HIP device ordinal */
   uint8_t keys[ORQG_NUM_KEYS][32];   /* see roles above (worker thread 0) */
-  long batch_size;                   /* runtime batch size; <0 = |b| equal batches (ref default -12) */
+  long batch_size;                   /* runtime batch size; <0 = |b| equal batches (ref default -12);
+                                        a positive one must be a multiple of 256 (the kernels
+                                        keep a batch's elements in whole 16-byte keystream
+                                        blocks and whole packed words) */
   size_t hbm_budget;                 /* bytes; 0 = no limit */
   /* worker threads of the reference's RunTime (0 or 1: one).  With T > 1, a
    * call over n elements is split into the threads' ranges
    * (runtime.h:198-247), each batched on its own and drawing from its own
    * thread's keys, and reserved permutations are dealt to the threads in
-   * runs (runtime.h:749-788).  Supported on the radix-sort path: secret
-   * sharing, open, and_b / multiply_a, b2a_bit, reshare, sharded and
-   * elementwise permutations and their ElementwisePermutation ops, radix and
-   * table sort (RADIXSORT); other operations return ORQG_E_ARG.            */
+   * runs (runtime.h:749-788).  Supported: secret sharing, open, and_b /
+   * multiply_a, b2a_bit, reshare, sharded and elementwise permutations and
+   * their ElementwisePermutation ops, radix and table sort (RADIXSORT), the
+   * comparisons (==, <, ...), full b2a, the adders and a2b; the sorting
+   * networks, quicksort, aggregation and distinct return ORQG_E_ARG.        */
   int threads;
   const uint8_t *thread_keys;        /* T > 1: (T-1) x ORQG_NUM_KEYS x 32 bytes, threads 1.. */
 } orqg_config;

@@ -355,12 +355,12 @@ def table_shuffle(keys, cols, width, batch_size=-12):
 CIRCUIT_OUTPUTS = ["==", "!=", "<", ">", "<=", ">=", "b2a", "+", "-", "a2b"]
 
 
-def circuits(keys, x, y, batch_size=-12):
+def circuits(keys, x, y, batch_size=-12, thread_keys=()):
     """x, y boolean-shared and x arithmetic-shared from party 0, then the
     outputs CIRCUIT_OUTPUTS in order: {name: [party][idx][n]}, bytes sent."""
     x, y = np.ascontiguousarray(x), np.ascontiguousarray(y, dtype=x.dtype)
     outs = run_parties(keys, CIRCUITS, [x.itemsize, 0, 0, 0, 0, 0, 0, x.size],
-                       x.tobytes() + y.tobytes(), batch_size)
+                       x.tobytes() + y.tobytes(), batch_size, thread_keys=thread_keys)
     res, sent = {k: [] for k in CIRCUIT_OUTPUTS}, []
     for b in outs:
         r = Reader(b)

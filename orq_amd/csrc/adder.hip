@@ -8,6 +8,8 @@
 // its bit of each packed call's keystream.  A wave covers whole packed words,
 // so all 3*(B-1) AES blocks a wave needs are computed up front, spread over
 // its 64 lanes, and each step reads its block with cross-lane shuffles.
+#include <algorithm>
+
 #include "internal.h"
 #include "kern_common.h"
 #include "cmp_core.h"
@@ -251,75 +253,89 @@ __global__ void __launch_bounds__(kAesThreads)
 
 // op: 0 RCA add, 1 RCA subtract, 2 PPA add, 3 PPA subtract, 4 rca_compare
 void adder(orqg_ctx *c, int op, const orqg_vec *x, const orqg_vec *y, orqg_vec *z) {
-  require_one_thread(c, "adder");
   const uint64_t n = x->n;
   if (n == 0) return;
   const int w = x->width, B = 8 * w;
-  NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
   const bool sub = op == 1 || op == 3;
   TimedRegion tr(c, "adder");
   if (op == 2 || op == 3) {
-    CallSched s = make_sched(n, c->batch_size, w, 1);
-    const uint64_t N1 = sched_nonces(s);
+    // every call is over the n elements: each worker thread runs the whole
+    // adder over its element range with its own keys and nonces
     const int LOG = w == 4 ? 5 : 6;
-    unsigned grid = grid_for((n * w + 15) / 16, kAesThreads, c->num_cu);
-    if (w == 4)
-      hipLaunchKernelGGL(k_ppa<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                         c->rk(kPair0), nb, s, N1, (const uint32_t *)x->d, (const uint32_t *)y->d,
-                         sub ? 1 : 0, sub ? 1 : 0, (uint32_t *)z->d);
-    else
-      hipLaunchKernelGGL(k_ppa<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                         c->rk(kPair0), nb, s, N1, (const uint64_t *)x->d, (const uint64_t *)y->d,
-                         sub ? 1 : 0, sub ? 1 : 0, (uint64_t *)z->d);
-    ORQG_HIP(hipGetLastError());
     const uint64_t calls = 1 + 3 * (uint64_t)LOG;
-    for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += calls * N1;
+    for_threads(c, n, [&](uint64_t off, uint64_t m) {
+      NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+      CallSched s = make_sched(m, c->batch_size, w, 1);
+      const uint64_t N1 = sched_nonces(s);
+      unsigned grid = grid_for((m * w + 15) / 16, kAesThreads, c->num_cu);
+      if (w == 4)
+        hipLaunchKernelGGL(k_ppa<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
+                           c->rk(kPair0), nb, s, N1, (const uint32_t *)x->d + 3 * off,
+                           (const uint32_t *)y->d + 3 * off, sub ? 1 : 0, sub ? 1 : 0,
+                           (uint32_t *)z->d + 3 * off);
+      else
+        hipLaunchKernelGGL(k_ppa<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
+                           c->rk(kPair0), nb, s, N1, (const uint64_t *)x->d + 3 * off,
+                           (const uint64_t *)y->d + 3 * off, sub ? 1 : 0, sub ? 1 : 0,
+                           (uint64_t *)z->d + 3 * off);
+      ORQG_HIP(hipGetLastError());
+      for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += calls * N1;
+    });
     for (int p = 0; p < 3; p++) c->log_msg("and_b", p, (p + 2) % 3, calls * n * w);
     return;
   }
+  // the ripple-carry adder's calls are over the packed words (B elements
+  // each): a worker thread's range of words is a range of elements
   const uint64_t nwords = n / B + (n % B > 0);
-  CallSched sp = make_sched(nwords, c->batch_size, w, 1);
-  const uint64_t Np = sched_nonces(sp);
-  unsigned grid = grid_for((n + 63) / 64 * 64, kAesThreads, c->num_cu);
   const int negy = (op == 1 || op == 4) ? 1 : 0, cin = negy, mode = op == 4 ? 1 : 0;
-  if (w == 4)
-    hipLaunchKernelGGL(k_rca<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                       c->rk(kPair0), nb, sp, Np, n, (const uint32_t *)x->d,
-                       (const uint32_t *)y->d, negy, cin, mode, (uint32_t *)z->d);
-  else
-    hipLaunchKernelGGL(k_rca<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                       c->rk(kPair0), nb, sp, Np, n, (const uint64_t *)x->d,
-                       (const uint64_t *)y->d, negy, cin, mode, (uint64_t *)z->d);
-  ORQG_HIP(hipGetLastError());
-  for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += (uint64_t)(B - 1) * Np;
+  for_threads(c, nwords, [&](uint64_t woff, uint64_t wm) {
+    NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+    CallSched sp = make_sched(wm, c->batch_size, w, 1);
+    const uint64_t Np = sched_nonces(sp);
+    const uint64_t off = woff * B, m = std::min<uint64_t>(n - off, wm * B);
+    unsigned grid = grid_for((m + 63) / 64 * 64, kAesThreads, c->num_cu);
+    if (w == 4)
+      hipLaunchKernelGGL(k_rca<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
+                         c->rk(kPair0), nb, sp, Np, m, (const uint32_t *)x->d + 3 * off,
+                         (const uint32_t *)y->d + 3 * off, negy, cin, mode,
+                         (uint32_t *)z->d + 3 * off);
+    else
+      hipLaunchKernelGGL(k_rca<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
+                         c->rk(kPair0), nb, sp, Np, m, (const uint64_t *)x->d + 3 * off,
+                         (const uint64_t *)y->d + 3 * off, negy, cin, mode,
+                         (uint64_t *)z->d + 3 * off);
+    ORQG_HIP(hipGetLastError());
+    for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += (uint64_t)(B - 1) * Np;
+  });
   for (int p = 0; p < 3; p++) c->log_msg("and_b", p, (p + 2) % 3, (uint64_t)(B - 1) * nwords * w);
 }
 
 void a2b(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
-  require_one_thread(c, "a2b");
   const uint64_t n = x->n;
   if (n == 0) return;
   const int w = x->width;
-  CallSched s = make_sched(n, c->batch_size, w, 2);
   orqg_vec f = *x, sv = *x;
   f.enc = sv.enc = ORQG_BSHARED;
   f.d = c->scratch_get("a2b_f", 3 * n * w + 64);
   sv.d = c->scratch_get("a2b_s", 3 * n * w + 64);
-  unsigned grid = grid_for((n * w + 15) / 16, kAesThreads, c->num_cu);
-  if (w == 4)
-    hipLaunchKernelGGL(k_redistribute<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream,
-                       c->d_t0, c->rk(kLocal0), c->rk(kLocal0 + 2), c->nonce[kLocal0],
-                       c->nonce[kLocal0 + 2], s, (const uint32_t *)x->d, (uint32_t *)f.d,
-                       (uint32_t *)sv.d);
-  else
-    hipLaunchKernelGGL(k_redistribute<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream,
-                       c->d_t0, c->rk(kLocal0), c->rk(kLocal0 + 2), c->nonce[kLocal0],
-                       c->nonce[kLocal0 + 2], s, (const uint64_t *)x->d, (uint64_t *)f.d,
-                       (uint64_t *)sv.d);
-  ORQG_HIP(hipGetLastError());
-  const uint64_t used = sched_nonces(s);
-  c->nonce[kLocal0] += used;
-  c->nonce[kLocal0 + 2] += used;
+  for_threads(c, n, [&](uint64_t off, uint64_t m) {
+    CallSched s = make_sched(m, c->batch_size, w, 2);
+    unsigned grid = grid_for((m * w + 15) / 16, kAesThreads, c->num_cu);
+    if (w == 4)
+      hipLaunchKernelGGL(k_redistribute<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream,
+                         c->d_t0, c->rk(kLocal0), c->rk(kLocal0 + 2), c->nonce[kLocal0],
+                         c->nonce[kLocal0 + 2], s, (const uint32_t *)x->d + 3 * off,
+                         (uint32_t *)f.d + 3 * off, (uint32_t *)sv.d + 3 * off);
+    else
+      hipLaunchKernelGGL(k_redistribute<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream,
+                         c->d_t0, c->rk(kLocal0), c->rk(kLocal0 + 2), c->nonce[kLocal0],
+                         c->nonce[kLocal0 + 2], s, (const uint64_t *)x->d + 3 * off,
+                         (uint64_t *)f.d + 3 * off, (uint64_t *)sv.d + 3 * off);
+    ORQG_HIP(hipGetLastError());
+    const uint64_t used = sched_nonces(s);
+    c->nonce[kLocal0] += used;
+    c->nonce[kLocal0 + 2] += used;
+  });
   const uint64_t bytes = 2 * n * (uint64_t)w;
   c->log_msg("secret_share", 0, 1, bytes);
   c->log_msg("secret_share", 0, 2, bytes);

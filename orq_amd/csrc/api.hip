@@ -261,6 +261,8 @@ int orqg_ctx_create(const orqg_config *cfg, orqg_ctx **out) {
   try {
     c->device = cfg->device;
     c->batch_size = cfg->batch_size == 0 ? -12 : cfg->batch_size;
+    if (c->batch_size > 0 && c->batch_size % 256)
+      throw Error(ORQG_E_ARG, "ctx_create: a positive batch size must be a multiple of 256");
     c->budget = cfg->hbm_budget;
     std::memcpy(c->keys, cfg->keys, sizeof(c->keys));
     ORQG_HIP(hipSetDevice(c->device));

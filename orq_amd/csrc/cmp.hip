@@ -5,6 +5,9 @@
 // round still draws its randomness from the keystream positions that the
 // round's own runtime call would use, so the shares, nonces and message
 // counts are those of the reference's sequence of calls.
+#include <cstring>
+#include <vector>
+
 #include "internal.h"
 #include "kern_common.h"
 #include "cmp_core.h"
@@ -19,7 +22,8 @@ __global__ void __launch_bounds__(kAesThreads)
     k_compare(const uint32_t *__restrict__ t0, const uint32_t *__restrict__ rk_pair,
               NonceTriple nb, uint64_t stride, CallSched s, CallSched sp,
               const T *__restrict__ x, const T *__restrict__ y, T *__restrict__ eq_out,
-              T *__restrict__ gt_out, int neg_eq, int neg_gt) {
+              T *__restrict__ gt_out, int neg_eq, int neg_gt,
+              const PackedThreads *__restrict__ pt, uint64_t woff) {
   constexpr int E = 16 / sizeof(T);
   AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
@@ -33,7 +37,8 @@ __global__ void __launch_bounds__(kAesThreads)
     Grp3<T> X, Y, EQ, GT;
     X.load(x, e0, s.n);
     Y.load(y, e0, s.n);
-    compare_group<T, CMP>(A, rk_pair, nb, 0, stride, s, sp, gw, lane, X.v, Y.v, EQ.v, GT.v);
+    compare_group<T, CMP>(A, rk_pair, nb, 0, stride, s, sp, gw, lane, X.v, Y.v, EQ.v, GT.v, pt,
+                          woff);
 #pragma unroll
     for (int i = 0; i < E; i++)
 #pragma unroll
@@ -50,36 +55,76 @@ __global__ void __launch_bounds__(kAesThreads)
 
 void compare(orqg_ctx *c, const orqg_vec *x, const orqg_vec *y, orqg_vec *eq, orqg_vec *gt,
              bool full, int neg_eq, int neg_gt) {
-  require_one_thread(c, "compare");
   const uint64_t n = x->n;
   if (n == 0) return;
-  const int B = 8 * x->width;
-  const int L = x->width == 4 ? 5 : x->width == 8 ? 6 : 7;
-  CallSched s = make_sched(n, c->batch_size, x->width, 1);
+  const int w = x->width;
+  const int B = 8 * w;
+  const int L = w == 4 ? 5 : w == 8 ? 6 : 7;
   const uint64_t nwords = n / B + (n % B > 0);
-  CallSched sp = make_sched(nwords, c->batch_size, x->width, 1);
-  const uint64_t stride = sched_nonces(s);
-  NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
-  unsigned grid = grid_for((n * x->width + 15) / 16, kAesThreads, c->num_cu);
   TimedRegion tr(c, full ? "compare" : "equal");
-#define ORQG_CMP_LAUNCH(T)                                                                     \
-  hipLaunchKernelGGL((full ? k_compare<T, true> : k_compare<T, false>), dim3(grid),           \
+  // With worker threads, each thread evaluates the element-wise calls over
+  // its range of elements with its keys; the packed call (the sign fix) is
+  // split by ranges of words, which the kernels look up in a table.
+  const PackedThreads *dpt = nullptr;
+  std::vector<std::pair<uint64_t, uint64_t>> eb(1, {0, n}), pb(1, {0, nwords});
+  if (c->threads > 1) {
+    const int T = c->threads;
+    eb = thread_bounds(n, c->batch_size, T);
+    pb = thread_bounds(nwords, c->batch_size, T);
+    for (int t = 0; t < T; t++)
+      if (eb[t].first % B) throw Error(ORQG_E_ARG, "compare: thread range not word-aligned");
+    PackedThreads h;
+    std::memset(&h, 0, sizeof(h));
+    h.T = (uint32_t)T;
+    for (int t = 0; t < T; t++) {
+      ThreadScope ts(c, t);
+      const uint64_t stride = sched_nonces(make_sched(eb[t].second - eb[t].first, c->batch_size, w, 1));
+      PackedThread &P = h.t[t];
+      P.lo = pb[t].first;
+      P.hi = pb[t].second;
+      for (int k = 0; k < 3; k++) P.nonce[k] = c->nonce[kPair0 + k] + (uint64_t)(L + 1) * stride;
+      P.s = make_sched(P.hi - P.lo, c->batch_size, w, 1);
+      P.rk = c->rk(kPair0);
+    }
+    PackedThreads *d = (PackedThreads *)c->scratch_get("cmp_threads", sizeof(PackedThreads));
+    ORQG_HIP(hipMemcpyAsync(d, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+    ORQG_HIP(hipStreamSynchronize(c->stream));  // h is on the host stack
+    dpt = d;
+  }
+  const char *xd = (const char *)x->d, *yd = (const char *)y->d;
+  char *eqd = eq ? (char *)eq->d : nullptr, *gtd = gt ? (char *)gt->d : nullptr;
+  for_threads(c, n, [&](uint64_t off, uint64_t m) {
+    CallSched s = make_sched(m, c->batch_size, w, 1);
+    CallSched sp = make_sched(m / B + (m % B > 0), c->batch_size, w, 1);
+    const uint64_t stride = sched_nonces(s);
+    NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+    unsigned grid = grid_for((m * w + 15) / 16, kAesThreads, c->num_cu);
+    const uint64_t boff = 3 * off * (uint64_t)w;
+#define ORQG_CMP_LAUNCH(T)                                                                      \
+  hipLaunchKernelGGL((full ? k_compare<T, true> : k_compare<T, false>), dim3(grid),            \
                      dim3(kAesThreads), 0, c->stream, c->d_t0, c->rk(kPair0), nb, stride, s, sp, \
-                     (const T *)x->d, (const T *)y->d, eq ? (T *)eq->d : (T *)nullptr,        \
-                     gt ? (T *)gt->d : (T *)nullptr, neg_eq, neg_gt)
-  if (x->width == 4)
-    ORQG_CMP_LAUNCH(uint32_t);
-  else if (x->width == 8)
-    ORQG_CMP_LAUNCH(uint64_t);
-  else
-    ORQG_CMP_LAUNCH(u128);  // quicksort's padded 64-bit keys
+                     (const T *)(xd + boff), (const T *)(yd + boff),                            \
+                     eqd ? (T *)(eqd + boff) : (T *)nullptr, gtd ? (T *)(gtd + boff) : (T *)nullptr, \
+                     neg_eq, neg_gt, dpt, off / B)
+    if (w == 4)
+      ORQG_CMP_LAUNCH(uint32_t);
+    else if (w == 8)
+      ORQG_CMP_LAUNCH(uint64_t);
+    else
+      ORQG_CMP_LAUNCH(u128);  // quicksort's padded 64-bit keys
 #undef ORQG_CMP_LAUNCH
-  ORQG_HIP(hipGetLastError());
-  uint64_t used = (uint64_t)L * stride;
-  if (full) used += stride + sched_nonces(sp);
-  for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
-  const uint64_t bytes = (uint64_t)(full ? L + 1 : L) * n * x->width +
-                         (full ? nwords * (uint64_t)x->width : 0);
+    ORQG_HIP(hipGetLastError());
+  });
+  // each thread's pair PRGs: its element-wise calls, then its packed call
+  for (int t = 0; t < (int)eb.size(); t++) {
+    ThreadScope ts(c, c->threads > 1 ? t : c->cur_thread);
+    const uint64_t stride =
+        sched_nonces(make_sched(eb[t].second - eb[t].first, c->batch_size, w, 1));
+    uint64_t used = (uint64_t)L * stride;
+    if (full) used += stride + sched_nonces(make_sched(pb[t].second - pb[t].first, c->batch_size, w, 1));
+    for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+  }
+  const uint64_t bytes = (uint64_t)(full ? L + 1 : L) * n * w + (full ? nwords * (uint64_t)w : 0);
   for (int p = 0; p < 3; p++) c->log_msg("and_b", p, (p + 2) % 3, bytes);
 }
 
@@ -142,27 +187,30 @@ __global__ void __launch_bounds__(kAesThreads)
 }
 
 void b2a_full(orqg_ctx *c, const orqg_vec *x, orqg_vec *y) {
-  require_one_thread(c, "b2a");
   const uint64_t n = x->n;
   if (n == 0) return;
   const int B = 8 * x->width;
-  CallSched s3 = make_sched(n, c->batch_size, x->width, 3);
-  CallSched s1 = make_sched(n, c->batch_size, x->width, 1);
-  const uint64_t n3 = sched_nonces(s3), n1 = sched_nonces(s1);
-  NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
-  unsigned grid = grid_for((n * x->width + 15) / 16, kAesThreads, c->num_cu);
   TimedRegion tr(c, "b2a");
-  if (x->width == 4)
-    hipLaunchKernelGGL(k_b2a_full<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                       c->rk(kPair0), nb, n3, n1, s3, s1, (const uint32_t *)x->d,
-                       (uint32_t *)y->d);
-  else
-    hipLaunchKernelGGL(k_b2a_full<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream, c->d_t0,
-                       c->rk(kPair0), nb, n3, n1, s3, s1, (const uint64_t *)x->d,
-                       (uint64_t *)y->d);
-  ORQG_HIP(hipGetLastError());
-  const uint64_t used = (uint64_t)B * n3 + (uint64_t)(B - 1) * n1;
-  for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+  // every call is over the n elements, so each worker thread runs the whole
+  // conversion over its range with its own keys and nonces
+  for_threads(c, n, [&](uint64_t off, uint64_t m) {
+    CallSched s3 = make_sched(m, c->batch_size, x->width, 3);
+    CallSched s1 = make_sched(m, c->batch_size, x->width, 1);
+    const uint64_t n3 = sched_nonces(s3), n1 = sched_nonces(s1);
+    NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+    unsigned grid = grid_for((m * x->width + 15) / 16, kAesThreads, c->num_cu);
+    if (x->width == 4)
+      hipLaunchKernelGGL(k_b2a_full<uint32_t>, dim3(grid), dim3(kAesThreads), 0, c->stream,
+                         c->d_t0, c->rk(kPair0), nb, n3, n1, s3, s1,
+                         (const uint32_t *)x->d + 3 * off, (uint32_t *)y->d + 3 * off);
+    else
+      hipLaunchKernelGGL(k_b2a_full<uint64_t>, dim3(grid), dim3(kAesThreads), 0, c->stream,
+                         c->d_t0, c->rk(kPair0), nb, n3, n1, s3, s1,
+                         (const uint64_t *)x->d + 3 * off, (uint64_t *)y->d + 3 * off);
+    ORQG_HIP(hipGetLastError());
+    const uint64_t used = (uint64_t)B * n3 + (uint64_t)(B - 1) * n1;
+    for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used;
+  });
   const uint64_t nb_bytes = n * (uint64_t)x->width;
   for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * nb_bytes * B);
   for (int p = 0; p < 3; p++) c->log_msg("mul_a", p, (p + 2) % 3, nb_bytes * (B - 1));

@@ -68,18 +68,38 @@ __device__ inline void pair_stream(const AesCtx<kCopies> &A, const uint32_t *__r
 // the others read with cross-lane shuffles.  The loop is wave-uniform so the
 // shuffles always run with the whole wave.
 //
+// With more than one worker thread the packed call is split into the
+// threads' ranges of packed words, which need not match their ranges of
+// elements: each thread's word range, its pair keys and its nonces at the
+// start of its packed call.
+constexpr int kMaxWorkerThreads = 64;
+struct PackedThread {
+  uint64_t lo, hi;    // packed words [lo, hi) of the whole call
+  uint64_t nonce[3];  // the thread's pair-key nonces where its packed call starts
+  CallSched s;        // schedule of its hi - lo words
+  const uint32_t *rk; // its three pair keys (3 x 60 words)
+};
+struct PackedThreads {
+  uint32_t T;
+  PackedThread t[kMaxWorkerThreads];
+};
+
 // compare_group: the circuit for the E elements of one thread; X, Y hold
 // their three shares.  The runtime calls start at nonce offset `noff`; `s`
 // schedules the element-wise calls and `sp` the packed one; `gw` is the
 // first group index of the calling wave (the loop around it must be
 // wave-uniform).  EQ / GT receive the masked boolean shares (GT only with CMP).
+// pt (worker threads, else null): the packed call's thread table, the
+// elements being those of a range that starts at packed word woff.
 template <typename T, bool CMP>
 __device__ inline void compare_group(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
                                      const NonceTriple &nb, uint64_t noff, uint64_t stride,
                                      const CallSched &s, const CallSched &sp, uint64_t gw,
                                      int lane, const T (&X)[16 / sizeof(T)][3],
                                      const T (&Y)[16 / sizeof(T)][3], T (&EQ)[16 / sizeof(T)][3],
-                                     T (&GT)[16 / sizeof(T)][3]) {
+                                     T (&GT)[16 / sizeof(T)][3],
+                                     const PackedThreads *__restrict__ pt = nullptr,
+                                     uint64_t woff = 0) {
   typedef typename SignedOf<T>::type ST;
   constexpr int E = 16 / sizeof(T);
   constexpr int B = 8 * sizeof(T);
@@ -144,9 +164,20 @@ __device__ inline void compare_group(const AesCtx<kCopies> &A, const uint32_t *_
       if (w < nwords) {
         uint64_t poff;
         uint32_t pctr, pbib;
-        sched_locate(sp, w, 0, poff, pctr, pbib);
-        kb = aes_ctr_block<kCopies>(A, rk_pair + 60 * key,
-                                  nb.v[key] + noff + (L + 1) * stride + poff, pctr);
+        if (pt) {
+          // the thread whose word range holds word woff + w (ranges are
+          // whole batches, so a block's words share one thread)
+          const uint64_t g = woff + w;
+          uint32_t t = 0;
+          while (t + 1 < pt->T && g >= pt->t[t].hi) t++;
+          const PackedThread &P = pt->t[t];
+          sched_locate(P.s, g - P.lo, 0, poff, pctr, pbib);
+          kb = aes_ctr_block<kCopies>(A, P.rk + 60 * key, P.nonce[key] + poff, pctr);
+        } else {
+          sched_locate(sp, w, 0, poff, pctr, pbib);
+          kb = aes_ctr_block<kCopies>(A, rk_pair + 60 * key,
+                                    nb.v[key] + noff + (L + 1) * stride + poff, pctr);
+        }
       }
     }
     // this lane's elements sit in dword D of the wave's packed keystream,

@@ -302,9 +302,10 @@ void for_threads(orqg_ctx *c, uint64_t n, F &&f) {
 inline void require_one_thread(const orqg_ctx *c, const char *what) {
   if (c->threads > 1)
     throw Error(ORQG_E_ARG, std::string(what) +
-                                ": more than one worker thread is supported on the radix-sort "
-                                "path only (secret sharing, and_b, multiply_a, b2a_bit, reshare, "
-                                "permutations, radix and table sort)");
+                                ": more than one worker thread is not supported here (it is "
+                                "for secret sharing, and_b, multiply_a, b2a_bit, reshare, "
+                                "permutations, radix and table sort, comparisons, b2a, the "
+                                "adders and a2b)");
 }
 
 inline unsigned grid_for(uint64_t items, unsigned block, unsigned cap) {

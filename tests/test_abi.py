@@ -64,3 +64,14 @@ def test_library_matches_sources():
 @pytest.mark.gpu
 def test_gpu_library_matches_sources():
     _library_matches_sources()
+
+
+def test_positive_batch_size_must_be_a_multiple_of_256():
+    """The kernels keep a batch in whole keystream blocks and packed words; a
+    positive batch size that is not a multiple of 256 is refused before any
+    device work (no GPU needed)."""
+    import orq_amd
+
+    keys = orq_amd.keys_from_seed(1)
+    with pytest.raises(orq_amd.OrqgError, match=r"orqg_ctx_create failed \(-1\)"):
+        orq_amd.Ctx(keys, batch_size=100)

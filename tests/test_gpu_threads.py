@@ -1,7 +1,7 @@
 """More than one worker thread (the reference's RunTime with T threads, each
 with its own keys): a T-thread GPU context against the T-thread oracle on the
 same keys, share for share, every thread's nonces and the message log, for
-the operations on the radix-sort path; the others refuse."""
+the operations that support it; the others refuse."""
 import numpy as np
 import pytest
 
@@ -156,11 +156,41 @@ def test_table_sort_with_threads(O, G):
     np.testing.assert_array_equal(got["[K]"], data[0][order])
 
 
+@pytest.mark.parametrize("T,n", [(2, 5000), (3, 3001), (4, 300), (3, 70001)])
+@pytest.mark.parametrize("width", [4, 8])
+def test_circuits_with_threads(O, G, T, n, width):
+    """The comparisons (element-wise calls split by element ranges, the packed
+    sign-fix call by word ranges), full b2a, the ripple-carry and
+    parallel-prefix adders, rca_compare and a2b, each as the T-thread oracle
+    gives them."""
+    rng = np.random.default_rng(T * n + 3 * width)
+    dt = np.uint32 if width == 4 else np.uint64
+    x = rng.integers(0, 2**(8 * width), n, dtype=np.uint64).astype(dt)
+    y = rng.integers(0, 2**(8 * width), n, dtype=np.uint64).astype(dt)
+    y[::7] = x[::7]
+    oc, gc = _pair(O, G, T)
+    ox, gx = oc.secret_share(x, width), gc.secret_share(x, width)
+    oy, gy = oc.secret_share(y, width), gc.secret_share(y, width)
+    for op in ["==", "!=", "<", ">", "<=", ">="]:
+        _same_shares(oc.cmp(op, ox, oy), gc.cmp(op, gx, gy))
+    _same_shares(oc.b2a(ox), gc.b2a(gx))
+    for op in ["add", "sub", "ppa_add", "ppa_sub", "rca_lt"]:
+        _same_shares(oc.adder(op, ox, oy), gc.adder(op, gx, gy))
+    oa, ga = oc.secret_share(x, width, arith=True), gc.secret_share(x, width, arith=True)
+    _same_shares(oc.adder("a2b", oa), gc.adder("a2b", ga))
+    _same_nonces(oc, gc, T)
+    _same_log(oc, gc)
+    sdt = np.int32 if width == 4 else np.int64
+    got = gc.open(gc.cmp("<", gx, gy))
+    np.testing.assert_array_equal(got, (x.view(sdt) < y.view(sdt)).astype(dt))
+    with np.errstate(over="ignore"):
+        np.testing.assert_array_equal(gc.open(gc.adder("add", gx, gy)), x + y)
+
+
 def test_other_operations_refuse_threads(G):
     gc = G.Ctx(_keys(0), threads=2, thread_keys=[_keys(1)])
     x = gc.secret_share(np.arange(600, dtype=np.uint64), 8)
-    y = gc.secret_share(np.arange(600, dtype=np.uint64), 8)
     with pytest.raises(G.OrqgError, match="worker thread"):
-        gc.compare(x, y)
+        gc.quicksort(x)
     with pytest.raises(G.OrqgError, match="worker thread"):
         gc.bitonic_sort([x])

@@ -372,6 +372,33 @@ def test_radix_sort_threads_vs_reference(T, n, width, batch_size):
     assert _sent(oc) == sent
 
 
+@pytest.mark.parametrize("T,n,width,batch_size", [(2, 5000, 4, -12), (3, 3001, 8, -12),
+                                                  (4, 70_001, 4, -12), (3, 2000, 8, 512)])
+def test_circuits_threads_vs_reference(T, n, width, batch_size):
+    """The comparison circuits, full b2a, the adders and a2b under the
+    reference's RunTime with T worker threads: every runtime call (over n
+    elements, or over the packed words of the sign fix and the ripple-carry
+    adder) split into the threads' ranges, each with its thread's randomness."""
+    k0, extra = _thread_keys(T)
+    dt = _DT[width]
+    rng = np.random.default_rng(T * n + width)
+    x = rng.integers(0, np.iinfo(dt).max, n, dtype=dt, endpoint=True)
+    y = rng.integers(0, np.iinfo(dt).max, n, dtype=dt, endpoint=True)
+    y[::7] = x[::7]
+    res, sent = R.circuits(k0, x, y, batch_size=batch_size, thread_keys=extra)
+    oc = O.Ctx(k0, batch_size, threads=T, thread_keys=extra)
+    ox, oy = oc.secret_share(x, width), oc.secret_share(y, width)
+    oa = oc.secret_share(x, width, arith=True)
+    got = {op: oc.cmp(op, ox, oy) for op in ["==", "!=", "<", ">", "<=", ">="]}
+    got["b2a"] = oc.b2a(ox)
+    got["+"] = oc.adder("add", ox, oy)
+    got["-"] = oc.adder("sub", ox, oy)
+    got["a2b"] = oc.adder("a2b", oa)
+    for k in R.CIRCUIT_OUTPUTS:
+        np.testing.assert_array_equal(faithful(got[k]), res[k], err_msg=k)
+    assert _sent(oc) == sent
+
+
 @pytest.mark.parametrize("T", [2, 3])
 def test_table_sort_threads_vs_reference(T):
     OracleCtx, Tb = _host()
