@@ -248,42 +248,20 @@ __global__ void __launch_bounds__(1024) k_scan3(uint32_t nb, uint32_t *__restric
 }
 
 // s0_k(e) = one_k*e - F_k(e), s1_k(e) = one_k*(n-1) - Ftot_k + F_k(e) with F the
-// inclusive prefix of f1 and one = (1,0,0); perm = s0 + f1*(s1 - s0).  One
-// superblock: boff3 = its exclusive offsets, tot = the totals (3 words each).
-// The rows F, R and the schedule s cover start at global row row0 of a
-// vector of ntot rows (one worker thread's range; 0 and s.n for one thread).
-__device__ inline void bit_perm_block(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
-                                      const NonceTriple &nb, const CallSched &s, uint64_t e0,
-                                      const uint32_t *__restrict__ F,
-                                      const uint32_t *__restrict__ boff3,
-                                      const uint32_t *__restrict__ tot, uint32_t *__restrict__ R,
-                                      uint32_t *s_w, uint64_t row0 = 0, uint64_t ntot = 0) {
-  if (ntot == 0) ntot = s.n;
-  uint32_t f[4][3];
-  uint32_t loc[3] = {0, 0, 0};
-  if (e0 + 4 <= s.n) {
-    const uint4 *src = reinterpret_cast<const uint4 *>(F + e0 * 3);
-    const uint4 a = src[0], b = src[1], c = src[2];
-    f[0][0] = a.x; f[0][1] = a.y; f[0][2] = a.z; f[1][0] = a.w;
-    f[1][1] = b.x; f[1][2] = b.y; f[2][0] = b.z; f[2][1] = b.w;
-    f[2][2] = c.x; f[3][0] = c.y; f[3][1] = c.z; f[3][2] = c.w;
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int k = 0; k < 3; k++) f[i][k] = (e0 + i < s.n) ? F[(e0 + i) * 3 + k] : 0u;
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int k = 0; k < 3; k++) loc[k] += f[i][k];
-  uint32_t pre[3];
-  {
-    uint32_t t[3];
-    bexcl3(loc, s_w, pre, t);
-#pragma unroll
-    for (int k = 0; k < 3; k++) pre[k] += boff3[k];
-  }
+// inclusive prefix of f1 and one = (1,0,0); perm = s0 + f1*(s1 - s0), with
+// multiply_a's zero sharing from the pair PRGs at nb.  Rows e0..e0+3 of one
+// superblock: f = their f1 shares, pre = F's exclusive prefix at row e0,
+// tot = Ftot.  The rows R and the schedule s cover start at global row row0
+// of a vector of ntot rows (one worker thread's range; 0 and s.n for one
+// thread).  The result goes into the rows' records as one 32-byte piece
+// (words 8..15): word 8 is v's last word for L = 3 (kept; 0 for L = 2),
+// words 9..11 take fw (f1's shares, or zeros), 12..14 the permutation.
+template <int L>
+__device__ inline void bit_perm_rows(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
+                                     const NonceTriple &nb, const CallSched &s, uint64_t e0,
+                                     const uint32_t (&f)[4][3], uint32_t (&pre)[3],
+                                     const uint32_t (&tot)[3], uint32_t *R, uint64_t row0,
+                                     uint64_t ntot, bool keep_f) {
   uint32_t S[3][4];
   {
     uint64_t off;
@@ -315,16 +293,110 @@ __device__ inline void bit_perm_block(const AesCtx<kCopies> &A, const uint32_t *
         uint32_t z = (S[pm][i] - S[p][i]) + f[i][p] * y[p] + f[i][p] * y[q] + f[i][q] * y[p];
         pr[p] = s0[p] + z;
       }
-      // into the element record's perm words, as one whole 32-byte piece
-      // (words 8..15: word 8 is v's last word for L = 3, 9..11 are zero) so
-      // that no partial write needs a read-modify-write
+      // one whole 32-byte piece, so that no partial write needs a
+      // read-modify-write
       uint32_t *rec = R + (e0 + i) * kRec;
-      const uint32_t w8 = rec[8];
+      const uint32_t w8 = L == 3 ? rec[8] : 0u;
       uint4 *o = reinterpret_cast<uint4 *>(rec + 8);
-      o[0] = make_uint4(w8, 0u, 0u, 0u);
+      o[0] = keep_f ? make_uint4(w8, f[i][0], f[i][1], f[i][2]) : make_uint4(w8, 0u, 0u, 0u);
       o[1] = make_uint4(pr[0], pr[1], pr[2], 0u);
     }
   }
+}
+
+// bit_perm_rows from f1 stored in F and the superblock's exclusive offsets
+// boff3 (3 words)
+template <int L>
+__device__ inline void bit_perm_block(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk_pair,
+                                      const NonceTriple &nb, const CallSched &s, uint64_t e0,
+                                      const uint32_t *__restrict__ F,
+                                      const uint32_t *__restrict__ boff3,
+                                      const uint32_t *__restrict__ tot, uint32_t *__restrict__ R,
+                                      uint32_t *s_w, uint64_t row0 = 0, uint64_t ntot = 0) {
+  if (ntot == 0) ntot = s.n;
+  uint32_t f[4][3];
+  uint32_t loc[3] = {0, 0, 0};
+  if (e0 + 4 <= s.n) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(F + e0 * 3);
+    const uint4 a = src[0], b = src[1], c = src[2];
+    f[0][0] = a.x; f[0][1] = a.y; f[0][2] = a.z; f[1][0] = a.w;
+    f[1][1] = b.x; f[1][2] = b.y; f[2][0] = b.z; f[2][1] = b.w;
+    f[2][2] = c.x; f[3][0] = c.y; f[3][1] = c.z; f[3][2] = c.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) f[i][k] = (e0 + i < s.n) ? F[(e0 + i) * 3 + k] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) loc[k] += f[i][k];
+  uint32_t pre[3], t3[3] = {tot[0], tot[1], tot[2]};
+  {
+    uint32_t t[3];
+    bexcl3(loc, s_w, pre, t);
+#pragma unroll
+    for (int k = 0; k < 3; k++) pre[k] += boff3[k];
+  }
+  bit_perm_rows<L>(A, rk_pair, nb, s, e0, f, pre, t3, R, row0, ntot, false);
+}
+
+// ---- one bit in one pass (one worker thread): b2a_bit, then the permutation
+// with F's superblock offset B and total T left out.  Both enter the shares
+// linearly: with D = 2B - T, perm_p gains -B_p + f_p*(D_p + D_q) + f_q*D_p
+// (q = p+1 mod 3), which the first sharded round adds to each gathered row
+// from the row's f1 shares (record words 9..11) and its superblock's (B, D).
+template <int L>
+__global__ void __launch_bounds__(kBT) k_bit_fused(const uint32_t *__restrict__ t0,
+                                                   const uint32_t *__restrict__ rk_pair,
+                                                   NonceTriple nb, CallSched s, NonceTriple nb2,
+                                                   CallSched s2, uint32_t *P, int word, int bit,
+                                                   uint32_t flip, uint32_t *__restrict__ bsum) {
+  AES_LDS;
+  __shared__ uint32_t s_w[3 * (kBT / 64)];
+  aes_load_tables<kCopies>(lds, t0);
+  AesCtx<kCopies> A(lds);
+  const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
+  uint32_t raw[4][3];
+  if (blockIdx.x < nsb) bit_load_raw<L>(P, s.n, word, (uint64_t)blockIdx.x * kEPB + threadIdx.x * 4, raw);
+  for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
+    const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
+    uint32_t bits[4][3];
+    bit_extract(raw, e0, s.n, bit, flip, bits);
+    // the next superblock's words are in flight during this one's work; its
+    // rows are not the ones this superblock writes
+    const uint32_t nx = sb + gridDim.x < nsb ? sb + gridDim.x : sb;
+    bit_load_raw<L>(P, s.n, word, (uint64_t)nx * kEPB + threadIdx.x * 4, raw);
+    uint32_t f[4][3];
+    b2a_bit_group<uint32_t>(A, rk_pair, nb, s, e0, bits, f);
+    uint32_t loc[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (e0 + i >= s.n) f[i][k] = 0u;
+        loc[k] += f[i][k];
+      }
+    uint32_t pre[3], btot[3];
+    bexcl3(loc, s_w, pre, btot);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int k = 0; k < 3; k++) bsum[sb * 3 + k] = btot[k];
+    const uint32_t zero3[3] = {0u, 0u, 0u};
+    bit_perm_rows<L>(A, rk_pair, nb2, s2, e0, f, pre, zero3, P, 0, s.n, true);
+  }
+}
+
+// the exclusive scan of k_bit_fused's superblock sums into C[sb] = (B0, B1,
+// B2, 0), one 16-byte entry per superblock; the totals T to tot
+__global__ void __launch_bounds__(1024) k_scan3_corr(uint32_t nb, uint32_t *__restrict__ bsum,
+                                                     uint32_t *__restrict__ tot,
+                                                     uint4 *__restrict__ C) {
+  scan3_body(nb, bsum, tot);
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x)
+    C[j] = make_uint4(bsum[j * 3], bsum[j * 3 + 1], bsum[j * 3 + 2], 0u);
 }
 
 template <int L>
@@ -342,7 +414,7 @@ __global__ void __launch_bounds__(kBT) k_bit_perm(const uint32_t *__restrict__ t
   AesCtx<kCopies> A(lds);
   const uint32_t nsb = (uint32_t)((s.n + kEPB - 1) / kEPB);
   for (uint32_t sb = blockIdx.x; sb < nsb; sb += gridDim.x)
-    bit_perm_block(A, rk_pair, nb, s, (uint64_t)sb * kEPB + threadIdx.x * 4, F, boff + sb * 3, tot,
+    bit_perm_block<L>(A, rk_pair, nb, s, (uint64_t)sb * kEPB + threadIdx.x * 4, F, boff + sb * 3, tot,
                    R, s_w, row0, ntot);
 }
 
@@ -390,13 +462,19 @@ __device__ inline void quad_transpose(uint4 (&X)[4], uint32_t s) {
 // rows row0.. of a vector of ntot rows (one worker thread's destination
 // range; 0 and n for one thread): idx is indexed from row0, a gathered row
 // and an opened destination are global.
-template <int L, bool SCATTER>
+//
+// CORR (the first round after k_bit_fused): each gathered row's permutation
+// shares first gain the terms of F's superblock offset B and total T that
+// k_bit_fused left out, from the row's f1 shares (record words 9..11),
+// C[row / kEPB] = B and T = tot[0..2].
+template <int L, bool SCATTER, bool CORR = false>
 __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__restrict__ rk,
                                    uint64_t nv, const CallSched &sv, uint64_t np,
                                    const CallSched &sp, const int32_t *__restrict__ idx,
                                    const uint32_t *__restrict__ Rin, uint32_t *__restrict__ Rout,
                                    uint32_t *__restrict__ err, uint64_t w, uint64_t row0 = 0,
-                                   uint64_t ntot = 0) {
+                                   uint64_t ntot = 0, const uint4 *__restrict__ C = nullptr,
+                                   const uint32_t *__restrict__ tot = nullptr) {
   constexpr int kVq = (3 * L + 3) / 4;  // 16-byte pieces holding v
   const uint64_t n = sv.n;
   if (ntot == 0) ntot = n;
@@ -410,7 +488,11 @@ __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__r
       ix[i] = ww < nstep && e < n ? idx[row0 + e] : -1;
     }
   };
-  auto load_rec = [&](const int32_t (&ix)[2], uint4 (&pv)[2][kVq], uint4 (&pq)[2]) {
+  // CORR: pf = record words 8..11 (for L = 3 that is pv's last piece), pc
+  // = the row's B
+  constexpr int kF = CORR && L != 3 ? 1 : 0, kC = CORR ? 1 : 0;
+  auto load_rec = [&](const int32_t (&ix)[2], uint4 (&pv)[2][kVq], uint4 (&pq)[2],
+                      uint4 (&pf)[2][kF + 1], uint4 (&pc)[2][kC + 1]) {
 #pragma unroll
     for (int i = 0; i < 2; i++) {
       if (ix[i] >= 0) {
@@ -418,6 +500,8 @@ __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__r
 #pragma unroll
         for (int q = 0; q < kVq; q++) pv[i][q] = rs[q];
         pq[i] = rs[kRecPerm / 4];
+        if (kF) pf[i][0] = rs[2];
+        if (kC) pc[i][0] = C[(uint32_t)ix[i] / (uint32_t)kEPB];
       }
     }
   };
@@ -425,9 +509,9 @@ __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__r
   // the step's records, in flight during its AES work (loading a step
   // ahead measured slower: the registers it holds cost more than it hides)
   int32_t ci[2];
-  uint4 cv[2][kVq], cq[2];
+  uint4 cv[2][kVq], cq[2], cf[2][kF + 1], cc[2][kC + 1];
   load_idx(w, ci);
-  load_rec(ci, cv, cq);
+  load_rec(ci, cv, cq, cf, cc);
   // this lane's perm block: call lane & 1 of the group of block (lane >> 1) & 1
   uint4 pb;
   {
@@ -482,9 +566,21 @@ __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__r
       wd[1 * L + q] ^= r[1][q];
       wd[2 * L + q] ^= r[0][q] ^ r[1][q];
     }
-    wd[kRecPerm + 0] = cq[i].x + m0;
-    wd[kRecPerm + 1] = cq[i].y + m1;
-    wd[kRecPerm + 2] = cq[i].z + m2;
+    uint32_t pq3[3] = {cq[i].x, cq[i].y, cq[i].z};
+    if (CORR) {
+      const uint4 fw = kF ? cf[i][0] : cv[i][kVq - 1];  // words 8..11
+      const uint32_t f[3] = {fw.y, fw.z, fw.w};
+      const uint32_t B[3] = {cc[i][0].x, cc[i][0].y, cc[i][0].z};
+      const uint32_t D[3] = {2u * B[0] - tot[0], 2u * B[1] - tot[1], 2u * B[2] - tot[2]};
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        const int q = (p + 1) % 3;
+        pq3[p] += f[p] * (D[p] + D[q]) + f[q] * D[p] - B[p];
+      }
+    }
+    wd[kRecPerm + 0] = pq3[0] + m0;
+    wd[kRecPerm + 1] = pq3[1] + m1;
+    wd[kRecPerm + 2] = pq3[2] + m2;
     // destination row: the element itself, or (scatter) its opened perm
     uint32_t dst = (uint32_t)(row0 + e);
     bool ok = live;
@@ -515,7 +611,7 @@ __device__ inline void round2_step(const AesCtx<kCopies> &A, const uint32_t *__r
   }
 }
 
-template <int L, bool SCATTER>
+template <int L, bool SCATTER, bool CORR = false>
 __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
                                                 const uint32_t *__restrict__ rk, uint64_t nv,
                                                 CallSched sv, uint64_t np, CallSched sp,
@@ -523,14 +619,16 @@ __global__ void __launch_bounds__(kBT) k_round2(const uint32_t *__restrict__ t0,
                                                 const uint32_t *__restrict__ Rin,
                                                 uint32_t *__restrict__ Rout,
                                                 uint32_t *__restrict__ err, uint64_t row0,
-                                                uint64_t ntot) {
+                                                uint64_t ntot, const uint4 *__restrict__ C = nullptr,
+                                                const uint32_t *__restrict__ tot = nullptr) {
   AES_LDS;
   aes_load_tables<kCopies>(lds, t0);
   AesCtx<kCopies> A(lds);
   const uint64_t nstep = (sv.n + 127) / 128;
   const uint64_t wstride = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < nstep; w += wstride)
-    round2_step<L, SCATTER>(A, rk, nv, sv, np, sp, idx, Rin, Rout, err, w, row0, ntot);
+    round2_step<L, SCATTER, CORR>(A, rk, nv, sv, np, sp, idx, Rin, Rout, err, w, row0, ntot, C,
+                                  tot);
 }
 
 // remove_padding (sorting.h:475-495): perm_b = low 32 bits (B-shared int32),
@@ -660,7 +758,7 @@ __global__ void __launch_bounds__(kBT) k_bit_perm_seg(const uint32_t *__restrict
     const CallSched sch = ld_sched(S->s_perm);
     const NonceTriple nb = ld_nonces(S->nb2);
     const uint32_t sb = f - S->sb0;
-    bit_perm_block(A, S->rk_pair, nb, sch, (uint64_t)sb * kEPB + threadIdx.x * 4, S->F,
+    bit_perm_block<L>(A, S->rk_pair, nb, sch, (uint64_t)sb * kEPB + threadIdx.x * 4, S->F,
                    S->bsum + sb * 3, S->tot, S->cur, s_w);
   }
 }
@@ -703,11 +801,31 @@ uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_wi
   uint32_t *err = (uint32_t *)c->scratch_get("rs_err", 16);
   ORQG_HIP(hipMemsetAsync(err, 0, 4, c->stream));
   uint32_t *cur = P, *other = R1;
+  // one worker thread: b2a_bit and the permutation in one pass (k_bit_fused),
+  // F's offsets and total added by the first round from (B, D) per superblock
+  const bool fused = c->threads <= 1;
+  const uint32_t nsb1 = (uint32_t)((n + kEPB - 1) / kEPB);
+  uint4 *corr = fused ? (uint4 *)c->scratch_get("rs_corr", (size_t)nsb1 * 16 + 64) : nullptr;
 
   for (int i = 0; i < bits; i++) {
     const int bitpos = 32 + i;
     const uint32_t flip = (full_width && i == bits - 1) ? 1u : 0u;  // inplace_invert on the MSB
-    {
+    if (fused) {
+      TimedRegion tr(c, "sort_b2a");
+      NonceTriple nb = {{c->nonce[kPair0], c->nonce[kPair0 + 1], c->nonce[kPair0 + 2]}};
+      CallSched s = make_sched(n, c->batch_size, 4, 3);
+      const uint64_t used = sched_nonces(s);
+      NonceTriple nb2 = {{nb.v[0] + used, nb.v[1] + used, nb.v[2] + used}};
+      CallSched s2 = make_sched(n, c->batch_size, 4, 1);
+      const uint64_t used2 = sched_nonces(s2);
+      hipLaunchKernelGGL(k_bit_fused<L>, dim3(std::min<unsigned>(nsb1, (unsigned)c->num_cu)),
+                         dim3(kBT), 0, c->stream, c->d_t0, c->rk(kPair0), nb, s, nb2, s2, cur,
+                         bitpos / 32, bitpos % 32, flip, bsum);
+      hipLaunchKernelGGL(k_scan3_corr, dim3(1), dim3(1024), 0, c->stream, nsb1, bsum, tot, corr);
+      for (int k = 0; k < 3; k++) c->nonce[kPair0 + k] += used + used2;
+      for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * n * 4);
+      for (int p = 0; p < 3; p++) c->log_msg("mul_a", p, (p + 2) % 3, n * 4);
+    } else {
       TimedRegion tr(c, "sort_b2a");
       uint32_t sb0 = 0;
       for_threads(c, n, [&](uint64_t off, uint64_t m) {
@@ -724,7 +842,7 @@ uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_wi
       for (int p = 0; p < 3; p++) c->log_msg("b2a_bit", p, (p + 2) % 3, 3 * n * 4);
       hipLaunchKernelGGL(k_scan3, dim3(1), dim3(1024), 0, c->stream, sb0, bsum, tot);
     }
-    {
+    if (!fused) {
       TimedRegion tr(c, "sort_perm");
       uint32_t sb0 = 0;
       for_threads(c, n, [&](uint64_t off, uint64_t m) {
@@ -757,7 +875,11 @@ uint32_t *sort_body(orqg_ctx *c, uint32_t *P, uint64_t n, int bits, bool full_wi
           const uint64_t uv = sched_nonces(sv), up = sched_nonces(sp);
           const uint64_t nv = c->nonce[role], np = nv + uv;
           const unsigned grid = grid_for((m + 3) / 4, kBT, c->num_cu);
-          if (g < 2)
+          if (g == 0 && fused)
+            hipLaunchKernelGGL((k_round2<L, false, true>), dim3(grid), dim3(kBT), 0, c->stream,
+                               c->d_t0, c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err,
+                               off, n, (const uint4 *)corr, (const uint32_t *)tot);
+          else if (g < 2)
             hipLaunchKernelGGL((k_round2<L, false>), dim3(grid), dim3(kBT), 0, c->stream, c->d_t0,
                                c->rk(role), nv, sv, np, sp, pi.inv[g], rin, rout, err, off, n);
           else
