@@ -206,10 +206,11 @@ ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
 # HBM bytes one k_round2 launch at 2^24 rows actually moves, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the factors a plain
 # 384 MiB device copy gives (2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE):
-# (1080446.1 * 2.0 + 1048966.9 * 1.0) KiB; profiles/radix_sort_2p24_int64_kernels.txt.
+# (1080549.5 * 2.0 + 1049169.8 * 1.0) KiB, the mean of the two gather-only rounds;
+# profiles/radix_sort_2p24_int64_kernels.txt.
 # The gather reads one 128-byte line per 64-byte element record; the writes
 # are the 64-byte records exactly.
-ROUND_TRAFFIC_2P24 = int((1080446.1 * 2.0 + 1048966.9 * 1.0) * 1024)
+ROUND_TRAFFIC_2P24 = int((1080549.5 * 2.0 + 1049169.8 * 1.0) * 1024)
 # HBM bytes per k_gp_step launch averaged over the 3225 launches of one 2^24
 # sort, the same way: (3400.4 * 2.0 + 10460.2 * 1.0) KiB
 # (profiles/gen_perm_k_gp_step_2p24_counters.txt)
