@@ -390,13 +390,18 @@ __global__ void __launch_bounds__(kBT) k_bit_fused(const uint32_t *__restrict__ 
 
 // the exclusive scan of k_bit_fused's superblock sums into C[sb] = (B0, B1,
 // B2, 0), one 16-byte entry per superblock; the totals T to tot
-__global__ void __launch_bounds__(1024) k_scan3_corr(uint32_t nb, uint32_t *__restrict__ bsum,
-                                                     uint32_t *__restrict__ tot,
-                                                     uint4 *__restrict__ C) {
+__device__ inline void scan3_corr_body(uint32_t nb, uint32_t *__restrict__ bsum,
+                                       uint32_t *__restrict__ tot, uint4 *__restrict__ C) {
   scan3_body(nb, bsum, tot);
   __syncthreads();
   for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x)
     C[j] = make_uint4(bsum[j * 3], bsum[j * 3 + 1], bsum[j * 3 + 2], 0u);
+}
+
+__global__ void __launch_bounds__(1024) k_scan3_corr(uint32_t nb, uint32_t *__restrict__ bsum,
+                                                     uint32_t *__restrict__ tot,
+                                                     uint4 *__restrict__ C) {
+  scan3_corr_body(nb, bsum, tot, C);
 }
 
 template <int L>
@@ -717,10 +722,15 @@ __device__ inline uint32_t seg_of_step(SegPtr segs, uint32_t nseg, uint64_t w) {
   return lo;
 }
 
+// k_bit_fused over the superblocks of every request (numbered flat); a
+// request's superblock prefix entries C go to its F buffer, which the fused
+// bit does not otherwise use (16 bytes per superblock of 4096 rows fit in the
+// 12 bytes per row F holds)
 template <int L>
-__global__ void __launch_bounds__(kBT) k_bit_b2a_seg(const uint32_t *__restrict__ t0,
-                                                     const SegBit *__restrict__ segs, uint32_t nseg,
-                                                     uint32_t total, int word, int bit) {
+__global__ void __launch_bounds__(kBT) k_bit_fused_seg(const uint32_t *__restrict__ t0,
+                                                       const SegBit *__restrict__ segs,
+                                                       uint32_t nseg, uint32_t total, int word,
+                                                       int bit) {
   AES_LDS;
   __shared__ uint32_t s_w[3 * (kBT / 64)];
   aes_load_tables<kCopies>(lds, t0);
@@ -728,43 +738,37 @@ __global__ void __launch_bounds__(kBT) k_bit_b2a_seg(const uint32_t *__restrict_
   const SegPtr sp = seg_ptr(segs);
   for (uint32_t f = blockIdx.x; f < total; f += gridDim.x) {
     const SegPtr S = sp + seg_of_sb(sp, nseg, f);
-    const CallSched sch = ld_sched(S->s_b2a);
-    const NonceTriple nb = ld_nonces(S->nb);
+    const CallSched sch = ld_sched(S->s_b2a), sch2 = ld_sched(S->s_perm);
+    const NonceTriple nb = ld_nonces(S->nb), nb2 = ld_nonces(S->nb2);
     const uint32_t sb = f - S->sb0;
     const uint64_t e0 = (uint64_t)sb * kEPB + threadIdx.x * 4;
     uint32_t raw[4][3], bits[4][3];
     bit_load_raw<L>(S->cur, sch.n, word, e0, raw);
     bit_extract(raw, e0, sch.n, bit, S->flip, bits);
-    bit_b2a_block(A, S->rk_pair, nb, sch, e0, bits, S->F, S->bsum + sb * 3, s_w);
+    uint32_t fv[4][3];
+    b2a_bit_group<uint32_t>(A, S->rk_pair, nb, sch, e0, bits, fv);
+    uint32_t loc[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (e0 + i >= sch.n) fv[i][k] = 0u;
+        loc[k] += fv[i][k];
+      }
+    uint32_t pre[3], btot[3];
+    bexcl3(loc, s_w, pre, btot);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int k = 0; k < 3; k++) S->bsum[sb * 3 + k] = btot[k];
+    const uint32_t zero3[3] = {0u, 0u, 0u};
+    bit_perm_rows<L>(A, S->rk_pair, nb2, sch2, e0, fv, pre, zero3, S->cur, 0, sch.n, true);
   }
 }
-
-__global__ void __launch_bounds__(1024) k_scan3_seg(const SegBit *__restrict__ segs) {
+__global__ void __launch_bounds__(1024) k_scan3_corr_seg(const SegBit *__restrict__ segs) {
   const SegPtr S = seg_ptr(segs) + blockIdx.x;
-  scan3_body(S->nsb, S->bsum, S->tot);
+  scan3_corr_body(S->nsb, S->bsum, S->tot, reinterpret_cast<uint4 *>(S->F));
 }
-
-template <int L>
-__global__ void __launch_bounds__(kBT) k_bit_perm_seg(const uint32_t *__restrict__ t0,
-                                                      const SegBit *__restrict__ segs, uint32_t nseg,
-                                                      uint32_t total) {
-  AES_LDS;
-  __shared__ uint32_t s_w[3 * (kBT / 64)];
-  aes_load_tables<kCopies>(lds, t0);
-  AesCtx<kCopies> A(lds);
-  const SegPtr sp = seg_ptr(segs);
-  for (uint32_t f = blockIdx.x; f < total; f += gridDim.x) {
-    const SegPtr S = sp + seg_of_sb(sp, nseg, f);
-    const CallSched sch = ld_sched(S->s_perm);
-    const NonceTriple nb = ld_nonces(S->nb2);
-    const uint32_t sb = f - S->sb0;
-    bit_perm_block<L>(A, S->rk_pair, nb, sch, (uint64_t)sb * kEPB + threadIdx.x * 4, S->F,
-                   S->bsum + sb * 3, S->tot, S->cur, s_w);
-  }
-}
-
-// round g: 0 reads cur and writes other, 1 the reverse, 2 (scatter) cur -> other
-template <int L, bool SCATTER>
+template <int L, bool SCATTER, bool CORR = false>
 __global__ void __launch_bounds__(kBT) k_round2_seg(const uint32_t *__restrict__ t0,
                                                     const SegBit *__restrict__ segs, uint32_t nseg,
                                                     uint64_t total, int g) {
@@ -782,8 +786,9 @@ __global__ void __launch_bounds__(kBT) k_round2_seg(const uint32_t *__restrict__
     const CallSched sv = ld_sched(S->sv), sp = ld_sched(S->sp);
     const uint32_t *rin = g == 1 ? S->other : S->cur;
     uint32_t *rout = g == 1 ? S->cur : S->other;
-    round2_step<L, SCATTER>(A, S->rk_grp[g], S->nv[g], sv, S->np[g], sp, S->idx[g], rin, rout,
-                            S->err, wu - S->st0);
+    round2_step<L, SCATTER, CORR>(A, S->rk_grp[g], S->nv[g], sv, S->np[g], sp, S->idx[g], rin,
+                                  rout, S->err, wu - S->st0, 0, 0,
+                                  reinterpret_cast<const uint4 *>(S->F), S->tot);
   }
 }
 
@@ -951,7 +956,7 @@ void pad_records(orqg_ctx *c, hipStream_t st, const orqg_vec *v, bool desc, uint
 // the candidates are tried on live bit iterations -- one launch each, timed
 // with events read after the call's final synchronisation -- and the fastest
 // is kept for the rest of the process.  ORQG_TUNE=0 keeps the defaults.
-enum TuneKind { kTuneB2a = 0, kTunePerm = 1, kTuneRound = 2 };
+enum TuneKind { kTuneB2a = 0, kTuneRound = 2 };
 
 struct Tuner {
   std::mutex mu;
@@ -1179,26 +1184,21 @@ void batch_body(orqg_ctx *lead, std::vector<BatchReq> &rq) {
       const uint32_t tsb = tot_sb[i];
       const uint64_t tst = tot_st[i];
       {
+        // b2a_bit and the bit permutation in one pass (k_bit_fused), the
+        // prefix's superblock offsets and totals added by the first round
         TimedRegion tr(lead, "sortb_b2a", 2);
         launch_timed(kTuneB2a, tsb, [&](unsigned grid) {
-          hipLaunchKernelGGL(k_bit_b2a_seg<L>, dim3(grid), dim3(kBT), 0, lead->stream, lead->d_t0, d,
-                             nseg, tsb, bitpos / 32, bitpos % 32);
+          hipLaunchKernelGGL(k_bit_fused_seg<L>, dim3(grid), dim3(kBT), 0, lead->stream,
+                             lead->d_t0, d, nseg, tsb, bitpos / 32, bitpos % 32);
         });
-        hipLaunchKernelGGL(k_scan3_seg, dim3(nseg), dim3(1024), 0, lead->stream, d);
-      }
-      {
-        TimedRegion tr(lead, "sortb_perm");
-        launch_timed(kTunePerm, tsb, [&](unsigned grid) {
-          hipLaunchKernelGGL(k_bit_perm_seg<L>, dim3(grid), dim3(kBT), 0, lead->stream, lead->d_t0, d,
-                             nseg, tsb);
-        });
+        hipLaunchKernelGGL(k_scan3_corr_seg, dim3(nseg), dim3(1024), 0, lead->stream, d);
       }
       {
         TimedRegion tr(lead, "sortb_round", 3);
         // a workgroup takes 16 wave steps at a time
         launch_timed(kTuneRound, (tst + 15) / 16, [&](unsigned grid) {
-          hipLaunchKernelGGL((k_round2_seg<L, false>), dim3(grid), dim3(kBT), 0, lead->stream,
-                             lead->d_t0, d, nseg, tst, 0);
+          hipLaunchKernelGGL((k_round2_seg<L, false, true>), dim3(grid), dim3(kBT), 0,
+                             lead->stream, lead->d_t0, d, nseg, tst, 0);
           hipLaunchKernelGGL((k_round2_seg<L, false>), dim3(grid), dim3(kBT), 0, lead->stream,
                              lead->d_t0, d, nseg, tst, 1);
           hipLaunchKernelGGL((k_round2_seg<L, true>), dim3(grid), dim3(kBT), 0, lead->stream,
