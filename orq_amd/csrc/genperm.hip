@@ -206,6 +206,34 @@ __device__ inline WinDesc plan_window(const ChainBufs &B, int chain, uint32_t s,
   return w;
 }
 
+// When the three windows are dense and their tiles fill r whole rounds over
+// the nwork drawing workgroups plus a last round under a quarter full, the
+// last round's tiles come off the windows with the most tiles (one at a time,
+// the lowest chain first on ties), so that the step runs r rounds.  Any
+// window size is exact (the next window starts where this one ends); only
+// windows of whole tiles of at least two tiles are shortened.
+__device__ inline void trim_last_round(WinDesc (&p)[3], uint32_t nwork) {
+  uint32_t T = 0;
+  for (int c = 0; c < 3; c++) {
+    if (!p[c].active || p[c].nt == 0) continue;
+    if (p[c].sparse || p[c].W % kTile != 0) return;
+    T += p[c].nt;
+  }
+  if (T <= nwork) return;
+  const uint32_t extra = T % nwork;
+  if (extra == 0 || extra * 4 > nwork) return;
+  for (uint32_t e = 0; e < extra; e++) {
+    int best = -1;
+    for (int c = 0; c < 3; c++)
+      if (p[c].active && p[c].nt > 1 && (best < 0 || p[c].nt > p[best].nt)) best = c;
+    if (best < 0) break;
+    p[best].nt--;
+    p[best].W -= kTile;
+  }
+  for (int c = 0; c < 3; c++)
+    if (p[c].active) p[c].can_end = p[c].W >= p[c].klo;
+}
+
 // exclusive block scan of one value per thread (blockDim multiple of 64)
 template <typename T>
 __device__ inline T block_scan(T x, T *s_w, T &total) {
@@ -479,20 +507,28 @@ __global__ void __launch_bounds__(kGpThreads) k_gp_step(const uint32_t *__restri
     }
     s_plan[c] = w;
     s_nonce[c] = rs_prev[c].nonce;
-    if (draws_on && blockIdx.x == nres) {
-      WinDesc &o = B.wd[(step & 1) * 3 + c];
-      o.d0 = w.d0;
-      o.active = w.active;
-      o.W = w.W;
-      o.klo = w.klo;
-      o.khi = w.khi;
-      o.sparse = w.sparse;
-      o.nt = w.nt;
-      o.seq = w.seq;
-      o.can_end = w.can_end;
-    }
   }
   __syncthreads();
+  // the three windows' dense tiles as whole rounds over the drawing
+  // workgroups, where the last round would be under a quarter full: its
+  // tiles are taken off the largest windows (every workgroup computes the
+  // same, from the same plans)
+  if (threadIdx.x == 0 && draws_on) trim_last_round(s_plan, nwork);
+  __syncthreads();
+  if (threadIdx.x < 3 && draws_on && blockIdx.x == nres) {
+    const int c = threadIdx.x;
+    const WinDesc &w = s_plan[c];
+    WinDesc &o = B.wd[(step & 1) * 3 + c];
+    o.d0 = w.d0;
+    o.active = w.active;
+    o.W = w.W;
+    o.klo = w.klo;
+    o.khi = w.khi;
+    o.sparse = w.sparse;
+    o.nt = w.nt;
+    o.seq = w.seq;
+    o.can_end = w.can_end;
+  }
   // tasks, in ticket order: the tiles of the current windows, then commit
   // chunks (16 blocks of 1024 ranks, one per wave) of the windows resolved
   // at the previous step, which workgroups without a tile (or done with
