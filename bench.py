@@ -212,9 +212,9 @@ ROUND_KEYSTREAM_PER_ROW = 2 * 16 + 2 * 4
 # are the 64-byte records exactly.
 ROUND_TRAFFIC_2P24 = int((1080549.5 * 2.0 + 1049169.8 * 1.0) * 1024)
 # HBM bytes per k_gp_step launch averaged over the 3225 launches of one 2^24
-# sort, the same way: (3400.4 * 2.0 + 10460.2 * 1.0) KiB
+# sort, the same way: (3290.5 * 2.0 + 10446.1 * 1.0) KiB
 # (profiles/gen_perm_k_gp_step_2p24_counters.txt)
-GP_STEP_TRAFFIC_2P24 = int((3400.4 * 2.0 + 10460.2 * 1.0) * 1024)
+GP_STEP_TRAFFIC_2P24 = int((3290.5 * 2.0 + 10446.1 * 1.0) * 1024)
 
 
 def _timed(ctx, fn, reps):
