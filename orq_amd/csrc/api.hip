@@ -321,6 +321,7 @@ int orqg_ctx_destroy(orqg_ctx *c) {
   for (auto &p : c->queue) free_perm(p);
   dev_free(c->d_rk_all);
   dev_free(c->d_t0);
+  if (c->host_pinned) (void)hipHostFree(c->host_pinned);
   unregister_ctx(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev_aux) (void)hipEventDestroy(c->ev_aux);

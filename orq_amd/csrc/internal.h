@@ -135,6 +135,10 @@ struct orqg_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // auxiliary stream (same default priority)
   hipEvent_t ev_aux = nullptr;
+  // pinned host staging of the quicksort rounds (hipHostMalloc; grown on
+  // demand, freed with the context)
+  void *host_pinned = nullptr;
+  size_t host_pinned_bytes = 0;
   long batch_size = -12;
   size_t budget = 0;
   uint64_t nonce[ORQG_NUM_KEYS] = {0};

@@ -79,6 +79,14 @@ __global__ void k_qs_gather(uint64_t m, const uint32_t *__restrict__ idx,
   }
 }
 
+// bit 0 of the opened B-shared rows (s0 ^ s1 ^ s2), one byte per row
+template <typename P>
+__global__ void k_qs_open_bit(uint64_t m, const P *__restrict__ v, uint8_t *__restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint8_t)((v[3 * i] ^ v[3 * i + 1] ^ v[3 * i + 2]) & 1u);
+}
+
 orqg_vec view(orqg_ctx *c, void *d, uint64_t n, int width, int enc) {
   orqg_vec v;
   v.ctx = c;
@@ -91,8 +99,8 @@ orqg_vec view(orqg_ctx *c, void *d, uint64_t n, int width, int enc) {
 
 // partition (quicksort.h:28-50) of rows [start, end] around the row at
 // start, applied to the row map `src` (src[pos] = row now at pos)
-void partition(std::vector<uint32_t> &src, const std::vector<uint8_t> &cmp, int64_t start,
-               int64_t end, std::vector<int32_t> &pivots) {
+void partition(uint32_t *src, const std::vector<uint8_t> &cmp, int64_t start, int64_t end,
+               std::vector<int32_t> &pivots) {
   int64_t i = start;
   for (int64_t j = start + 1; j <= end; j++)
     if (cmp[j] == 0) {
@@ -112,8 +120,20 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
   std::vector<int32_t> pivots(n, -1);
   pivots[0] = 0;
   std::vector<uint8_t> cmp(n, 0);  // exp_cmp_plaintext: the last opened bit of each row
-  std::vector<uint32_t> idx, src(n);
-  std::vector<W> opened;
+  // host staging in pinned memory: idx (2n words), src (n words), the opened
+  // comparison bits (n bytes)
+  const size_t pin_bytes = 13 * (size_t)n + 64;
+  if (c->host_pinned_bytes < pin_bytes) {
+    if (c->host_pinned) ORQG_HIP(hipHostFree(c->host_pinned));
+    c->host_pinned = nullptr;
+    c->host_pinned_bytes = 0;
+    ORQG_HIP(hipHostMalloc(&c->host_pinned, pin_bytes, hipHostMallocDefault));
+    c->host_pinned_bytes = pin_bytes;
+  }
+  uint32_t *idx = (uint32_t *)c->host_pinned;
+  uint32_t *src = idx + 2 * n;
+  uint8_t *bits = (uint8_t *)(src + n);
+  uint8_t *d_bits = (uint8_t *)c->scratch_get("qs_bits", n + 64);
   uint32_t *d_idx = (uint32_t *)c->scratch_get("qs_idx", 8 * n + 64);
   uint32_t *d_src = (uint32_t *)c->scratch_get("qs_src", 4 * n + 64);
   W *red = (W *)c->scratch_get("qs_red", 3 * kW * n + 64);
@@ -124,7 +144,6 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
   for (;;) {
     // the non-pivot rows, then the pivot of each (prefix max of pivots)
     const uint64_t size = n - num_pivots;
-    idx.resize(2 * size);
     uint64_t k = 0;
     int32_t run = -1;
     for (uint64_t j = 0; j < n; j++) {
@@ -136,7 +155,9 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
       }
     }
     if (size) {
-      ORQG_HIP(hipMemcpyAsync(d_idx, idx.data(), 8 * size, hipMemcpyHostToDevice, c->stream));
+      ORQG_HIP(hipMemcpyAsync(d_idx, idx, 4 * size, hipMemcpyHostToDevice, c->stream));
+      ORQG_HIP(hipMemcpyAsync(d_idx + size, idx + size, 4 * size, hipMemcpyHostToDevice,
+                              c->stream));
       const unsigned g = grid_for(size, 256, c->num_cu * 16);
       hipLaunchKernelGGL(k_qs_gather<W>, dim3(g), dim3(256), 0, c->stream, size, d_idx, P, red);
       hipLaunchKernelGGL(k_qs_gather<W>, dim3(g), dim3(256), 0, c->stream, size, d_idx + size,
@@ -146,9 +167,14 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
       orqg_vec vr = view(c, red, size, kW, ORQG_BSHARED), vp = view(c, piv, size, kW, ORQG_BSHARED);
       orqg_vec ve = view(c, eq, size, kW, ORQG_BSHARED), vg = view(c, gt, size, kW, ORQG_BSHARED);
       compare(c, &vr, &vp, &ve, &vg, true, 0, 0);
-      opened.resize(size);
-      open_vec(c, &vg, opened.data());  // comparisons.open()
-      for (uint64_t t = 0; t < size; t++) cmp[idx[t]] = (uint8_t)(opened[t] & 1);
+      // comparisons.open(): only bit 0 of each opened row is used, so only
+      // it leaves the device; the message log carries the whole rows
+      hipLaunchKernelGGL(k_qs_open_bit<W>, dim3(g), dim3(256), 0, c->stream, size, gt, d_bits);
+      ORQG_HIP(hipGetLastError());
+      ORQG_HIP(hipMemcpyAsync(bits, d_bits, size, hipMemcpyDeviceToHost, c->stream));
+      ORQG_HIP(hipStreamSynchronize(c->stream));
+      for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, size * kW);
+      for (uint64_t t = 0; t < size; t++) cmp[idx[t]] = bits[t];
     }
     // partition every segment between marked positions (quicksort.h:149-165)
     for (uint64_t j = 0; j < n; j++) src[j] = (uint32_t)j;
@@ -166,7 +192,7 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
     // (cmp stays indexed by position, as in the reference: a partition reads
     // cmp[j] before any swap has touched position j, and the next round
     // rewrites every non-pivot position before reading it)
-    ORQG_HIP(hipMemcpyAsync(d_src, src.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+    ORQG_HIP(hipMemcpyAsync(d_src, src, 4 * n, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_qs_gather<W>, dim3(grid_for(n, 256, c->num_cu * 16)), dim3(256), 0,
                        c->stream, n, d_src, P, Q);
     ORQG_HIP(hipGetLastError());
