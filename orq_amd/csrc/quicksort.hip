@@ -23,6 +23,9 @@
 #include <algorithm>
 #include <vector>
 
+#include <thread>
+#include <utility>
+
 #include "internal.h"
 #include "kern_common.h"
 
@@ -97,6 +100,8 @@ orqg_vec view(orqg_ctx *c, void *d, uint64_t n, int width, int enc) {
   return v;
 }
 
+constexpr int kHostThreads = 8;  // host threads for the partitions of one round
+
 // partition (quicksort.h:28-50) of rows [start, end] around the row at
 // start, applied to the row map `src` (src[pos] = row now at pos)
 void partition(uint32_t *src, const std::vector<uint8_t> &cmp, int64_t start, int64_t end,
@@ -141,6 +146,7 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
   W *eq = (W *)c->scratch_get("qs_eq", 3 * kW * n + 64);
   W *gt = (W *)c->scratch_get("qs_gt", 3 * kW * n + 64);
   uint64_t num_pivots = 1;
+  std::vector<std::pair<int64_t, int64_t>> segs;
   for (;;) {
     // the non-pivot rows, then the pivot of each (prefix max of pivots)
     const uint64_t size = n - num_pivots;
@@ -176,8 +182,12 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
       for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, size * kW);
       for (uint64_t t = 0; t < size; t++) cmp[idx[t]] = bits[t];
     }
-    // partition every segment between marked positions (quicksort.h:149-165)
+    // partition every segment between marked positions (quicksort.h:149-165).
+    // The segments are found first (a partition only marks positions inside
+    // its own segment), then partitioned on host threads: each segment's rows
+    // of src, cmp and pivots are its own.
     for (uint64_t j = 0; j < n; j++) src[j] = (uint32_t)j;
+    segs.clear();
     int64_t index = -1;
     for (int64_t p = 0; p < (int64_t)n; p++) {
       if (pivots[p] < 0) continue;
@@ -185,10 +195,29 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
         index++;
         continue;
       }
-      partition(src, cmp, index, p - 1, pivots);
+      segs.emplace_back(index, p - 1);
       index = p;
     }
-    partition(src, cmp, index, (int64_t)n - 1, pivots);
+    segs.emplace_back(index, (int64_t)n - 1);
+    const int nthr = n >= (1u << 16) && segs.size() > 1
+                         ? (int)std::min<size_t>(segs.size(), kHostThreads)
+                         : 1;
+    if (nthr == 1) {
+      for (const auto &sg : segs) partition(src, cmp, sg.first, sg.second, pivots);
+    } else {
+      // contiguous runs of segments with about n / nthr rows each
+      std::vector<size_t> cut(nthr + 1, segs.size());
+      cut[0] = 0;
+      for (size_t q = 0, t = 1; q < segs.size() && t < (size_t)nthr; q++)
+        if ((uint64_t)(segs[q].second + 1) * nthr >= (uint64_t)t * n) cut[t++] = q + 1;
+      std::vector<std::thread> th;
+      for (int t = 0; t < nthr; t++)
+        th.emplace_back([&, t] {
+          for (size_t q = cut[t]; q < cut[t + 1]; q++)
+            partition(src, cmp, segs[q].first, segs[q].second, pivots);
+        });
+      for (auto &x : th) x.join();
+    }
     // (cmp stays indexed by position, as in the reference: a partition reads
     // cmp[j] before any swap has touched position j, and the next round
     // rewrites every non-pivot position before reading it)
