@@ -104,8 +104,8 @@ constexpr int kHostThreads = 8;  // host threads for the partitions of one round
 
 // partition (quicksort.h:28-50) of rows [start, end] around the row at
 // start, applied to the row map `src` (src[pos] = row now at pos)
-void partition(uint32_t *src, const std::vector<uint8_t> &cmp, int64_t start, int64_t end,
-               std::vector<int32_t> &pivots) {
+uint64_t partition(uint32_t *src, const std::vector<uint8_t> &cmp, int64_t start, int64_t end,
+                   std::vector<int32_t> &pivots) {
   int64_t i = start;
   for (int64_t j = start + 1; j <= end; j++)
     if (cmp[j] == 0) {
@@ -113,8 +113,30 @@ void partition(uint32_t *src, const std::vector<uint8_t> &cmp, int64_t start, in
       std::swap(src[j], src[i]);
     }
   std::swap(src[start], src[i]);
+  // position start is marked already (it is the segment's pivot position);
+  // i and i + 1 are new marks unless i is start
+  uint64_t marked = i != start ? 1 : 0;
   pivots[i] = (int32_t)i;
-  if (i + 1 <= end) pivots[i + 1] = (int32_t)(i + 1);
+  if (i + 1 <= end) {
+    pivots[i + 1] = (int32_t)(i + 1);
+    marked++;
+  }
+  return marked;
+}
+
+// f(lo, hi) over [0, m) split into up to kHostThreads ranges on host threads
+// (one range, on the caller, below 2^16)
+template <typename F>
+void host_ranges(uint64_t m, F &&f) {
+  const int T = m >= (1u << 16) ? kHostThreads : 1;
+  if (T == 1) {
+    f((uint64_t)0, m);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; t++)
+    th.emplace_back([&, t] { f(m * t / T, m * (t + 1) / T); });
+  for (auto &x : th) x.join();
 }
 
 // quicksort_body on the padded, shuffled vector P (n rows of 3 words of type
@@ -180,13 +202,17 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
       ORQG_HIP(hipMemcpyAsync(bits, d_bits, size, hipMemcpyDeviceToHost, c->stream));
       ORQG_HIP(hipStreamSynchronize(c->stream));
       for (int p = 0; p < 3; p++) c->log_msg("open", p, (p + 2) % 3, size * kW);
-      for (uint64_t t = 0; t < size; t++) cmp[idx[t]] = bits[t];
+      host_ranges(size, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t t = lo; t < hi; t++) cmp[idx[t]] = bits[t];  // distinct rows
+      });
     }
     // partition every segment between marked positions (quicksort.h:149-165).
     // The segments are found first (a partition only marks positions inside
     // its own segment), then partitioned on host threads: each segment's rows
     // of src, cmp and pivots are its own.
-    for (uint64_t j = 0; j < n; j++) src[j] = (uint32_t)j;
+    host_ranges(n, [&](uint64_t lo, uint64_t hi) {
+      for (uint64_t j = lo; j < hi; j++) src[j] = (uint32_t)j;
+    });
     segs.clear();
     int64_t index = -1;
     for (int64_t p = 0; p < (int64_t)n; p++) {
@@ -202,21 +228,24 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
     const int nthr = n >= (1u << 16) && segs.size() > 1
                          ? (int)std::min<size_t>(segs.size(), kHostThreads)
                          : 1;
+    uint64_t marked = 0;  // positions the partitions mark as pivots
     if (nthr == 1) {
-      for (const auto &sg : segs) partition(src, cmp, sg.first, sg.second, pivots);
+      for (const auto &sg : segs) marked += partition(src, cmp, sg.first, sg.second, pivots);
     } else {
       // contiguous runs of segments with about n / nthr rows each
       std::vector<size_t> cut(nthr + 1, segs.size());
       cut[0] = 0;
       for (size_t q = 0, t = 1; q < segs.size() && t < (size_t)nthr; q++)
         if ((uint64_t)(segs[q].second + 1) * nthr >= (uint64_t)t * n) cut[t++] = q + 1;
+      std::vector<uint64_t> mk(nthr, 0);
       std::vector<std::thread> th;
       for (int t = 0; t < nthr; t++)
         th.emplace_back([&, t] {
           for (size_t q = cut[t]; q < cut[t + 1]; q++)
-            partition(src, cmp, segs[q].first, segs[q].second, pivots);
+            mk[t] += partition(src, cmp, segs[q].first, segs[q].second, pivots);
         });
       for (auto &x : th) x.join();
+      for (uint64_t x : mk) marked += x;
     }
     // (cmp stays indexed by position, as in the reference: a partition reads
     // cmp[j] before any swap has touched position j, and the next round
@@ -229,8 +258,7 @@ W *quicksort_rounds(orqg_ctx *c, W *P, W *Q, uint64_t n) {
     // the host vectors above are reused next round: the copies out of them
     // must have been consumed
     ORQG_HIP(hipStreamSynchronize(c->stream));
-    num_pivots = 0;
-    for (int32_t x : pivots) num_pivots += x >= 0;
+    num_pivots += marked;
     if (num_pivots >= n) return P;
   }
 }
